@@ -1,0 +1,366 @@
+"""MI355X-native GRMHD fluid step of H-AMR (tp2872/HAMR_gamma_wind).
+
+Python host side of libhamr_hip.so (include/hamr_hip.h): a ctypes binding of
+the C ABI plus `advance_step`, the per-block call sequence of the reference's
+advance_GPU / GPU_step_ch (step_ch.c:916-1336), so a driver written against the
+reference's launch layer reads the same here.  There is no CPU fallback: every
+device call goes through the HIP library and raises if it is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhamr_hip.so")
+
+NPR = 9
+NG = 3
+RHO, UU, U1, U2, U3, B1, B2, B3, KTOT = range(9)
+FACE1, FACE2, CORN, CENT, FACE3 = range(5)
+(HAMR_P, HAMR_PS, HAMR_PH, HAMR_PSH, HAMR_F1, HAMR_F2, HAMR_F3, HAMR_EMF, HAMR_ECENT, HAMR_U0) = range(10)
+NCOMP = {HAMR_P: 9, HAMR_PS: 3, HAMR_PH: 9, HAMR_PSH: 3, HAMR_F1: 9, HAMR_F2: 9, HAMR_F3: 9, HAMR_EMF: 3,
+         HAMR_ECENT: 3, HAMR_U0: 9}
+
+
+class HamrError(RuntimeError):
+    pass
+
+
+class Desc(C.Structure):
+    _fields_ = [("bs", C.c_int * 3), ("nb", C.c_int * 3), ("gam", C.c_double), ("spin", C.c_double),
+                ("cour", C.c_double), ("rin", C.c_double), ("rout", C.c_double), ("rho_floor", C.c_double),
+                ("u_floor", C.c_double), ("minkowski", C.c_int), ("device", C.c_int)]
+
+
+class TorusDesc(C.Structure):
+    _fields_ = [("rin", C.c_double), ("rmax", C.c_double), ("kappa", C.c_double), ("beta_min", C.c_double),
+                ("aphi_cut", C.c_double), ("perturb", C.c_double), ("seed", C.c_ulonglong)]
+
+
+def make_desc(bs, nb, gam=13.0 / 9.0, spin=0.9375, cour=0.8, rin=0.0, rout=1000.0, rho_floor=1e-5,
+              u_floor=1e-7, minkowski=0, device=0):
+    d = Desc()
+    for k in range(3):
+        d.bs[k] = int(bs[k])
+        d.nb[k] = int(nb[k])
+    d.gam, d.spin, d.cour, d.rin, d.rout = gam, spin, cour, rin, rout
+    d.rho_floor, d.u_floor, d.minkowski, d.device = rho_floor, u_floor, minkowski, device
+    return d
+
+
+def torus_desc(rin=6.0, rmax=12.0, kappa=1e-3, beta_min=100.0, aphi_cut=0.2, perturb=0.04, seed=1):
+    t = TorusDesc()
+    t.rin, t.rmax, t.kappa, t.beta_min, t.aphi_cut, t.perturb, t.seed = rin, rmax, kappa, beta_min, aphi_cut, perturb, seed
+    return t
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_llp = C.POINTER(C.c_longlong)
+_lib = None
+
+
+def _ptr(a, t=_dp):
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(t)
+
+
+def load_library(path=None):
+    """Load libhamr_hip.so (built in-tree by build.py); raises if it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise HamrError("libhamr_hip.so is not built: run __graft_entry__.build() "
+                        "(hamr_gamma_wind_amd/build.py)")
+    L = C.CDLL(p)
+    D = C.POINTER(Desc)
+    ctxp = C.c_void_p
+    sig = {
+        "hamr_desc_layout": [D, _ip, _llp, _llp, _llp, _llp, _ip, _ip],
+        "hamr_desc_block_table": [D, _ip, _ip],
+        "hamr_desc_coords": [D, _dp, _dp],
+        "hamr_desc_metric_at": [D, _dp, _dp, _dp, _dp, _dp, _dp],
+        "hamr_desc_geometry": [D, C.c_int, _dp, _dp, _dp, _dp, _dp],
+        "hamr_desc_torus": [D, C.POINTER(TorusDesc), _dp, _dp],
+        "hamr_create": [D, C.POINTER(ctxp)],
+        "hamr_destroy": [ctxp],
+        "hamr_layout": [ctxp, _ip, _llp, _llp, _ip, _ip],
+        "hamr_block_row": [ctxp, C.c_int, _ip],
+        "hamr_block_order": [ctxp, _ip],
+        "hamr_set_grid": [ctxp],
+        "hamr_metric_sizes": [ctxp, _llp, _llp],
+        "hamr_upload_metric": [ctxp, C.c_int, _dp, _dp, _dp, _dp, _dp],
+        "hamr_download_metric": [ctxp, C.c_int, _dp, _dp, _dp, _dp, _dp],
+        "hamr_init_torus": [ctxp, C.POINTER(TorusDesc)],
+        "hamr_upload": [ctxp, C.c_int, _dp, _dp],
+        "hamr_download": [ctxp, C.c_int, _dp, _dp],
+        "hamr_set_array": [ctxp, C.c_int, _dp],
+        "hamr_get_array": [ctxp, C.c_int, _dp],
+        "hamr_fluxcalc": [ctxp, C.c_int, C.c_int, C.c_int],
+        "hamr_block_dt": [ctxp, C.c_int, C.c_int, _dp],
+        "hamr_ct_emf": [ctxp, C.c_int, C.c_int],
+        "hamr_ct_bound": [ctxp, C.c_double],
+        "hamr_ct_update": [ctxp, C.c_int, C.c_double, C.c_int],
+        "hamr_update": [ctxp, C.c_int, C.c_int, C.c_double],
+        "hamr_boundprim": [ctxp, C.c_int],
+        "hamr_flags": [ctxp, C.c_int, _ip, _ip],
+        "hamr_clear_flags": [ctxp],
+        "hamr_estimate_dt": [ctxp, _dp],
+        "hamr_step": [ctxp, C.c_int, _dp],
+        "hamr_step_graph": [ctxp, C.c_int, _dp],
+        "hamr_sync": [ctxp],
+        "hamr_time_stage": [ctxp, C.c_int, C.c_int, _dp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.hamr_last_error.argtypes = [ctxp]
+    L.hamr_last_error.restype = C.c_char_p
+    if path is None:
+        _lib = L
+    return L
+
+
+# ---------------------------------------------------------------------------
+# host-only set-up
+def desc_layout(desc):
+    L = load_library()
+    D = (C.c_int * 3)()
+    ncell, ksize, ncol, kcol = C.c_longlong(), C.c_longlong(), C.c_longlong(), C.c_longlong()
+    nblk, ngeo = C.c_int(), C.c_int()
+    if L.hamr_desc_layout(C.byref(desc), D, C.byref(ncell), C.byref(ksize), C.byref(ncol), C.byref(kcol),
+                          C.byref(nblk), C.byref(ngeo)):
+        raise HamrError("hamr_desc_layout: invalid descriptor")
+    return dict(D=tuple(D), ncell=ncell.value, ksize=ksize.value, ncol=ncol.value, kcol=kcol.value,
+                nblk=nblk.value, ngeo=ngeo.value)
+
+
+def desc_block_table(desc):
+    L = load_library()
+    lay = desc_layout(desc)
+    rows = np.zeros((lay["nblk"], 17), dtype=np.int32)
+    order = np.zeros(lay["nblk"], dtype=np.int32)
+    if L.hamr_desc_block_table(C.byref(desc), _ptr(rows, _ip), _ptr(order, _ip)):
+        raise HamrError("hamr_desc_block_table failed")
+    return rows, order
+
+
+def desc_coords(desc):
+    L = load_library()
+    sx = np.zeros(3)
+    dx = np.zeros(3)
+    if L.hamr_desc_coords(C.byref(desc), _ptr(sx), _ptr(dx)):
+        raise HamrError("hamr_desc_coords failed")
+    return sx, dx
+
+
+def desc_metric_at(desc, X):
+    L = load_library()
+    gcov, gcon, gdet, conn, rth = np.zeros(10), np.zeros(10), np.zeros(1), np.zeros(40), np.zeros(2)
+    X = np.ascontiguousarray(X, dtype=float)
+    if L.hamr_desc_metric_at(C.byref(desc), _ptr(X), _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn), _ptr(rth)):
+        raise HamrError("hamr_desc_metric_at failed")
+    return gcov, gcon, gdet[0], conn, rth
+
+
+def desc_geometry(desc, geo):
+    L = load_library()
+    lay = desc_layout(desc)
+    kc = lay["kcol"]
+    gcov, gcon = np.zeros(5 * 10 * kc), np.zeros(5 * 10 * kc)
+    gdet, conn, rcen = np.zeros(5 * kc), np.zeros(40 * kc), np.zeros(kc)
+    if L.hamr_desc_geometry(C.byref(desc), geo, _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn), _ptr(rcen)):
+        raise HamrError("hamr_desc_geometry failed")
+    return dict(gcov=gcov.reshape(5, 10, kc), gcon=gcon.reshape(5, 10, kc), gdet=gdet.reshape(5, kc),
+                conn=conn.reshape(40, kc), rcen=rcen)
+
+
+def desc_torus(desc, tdesc):
+    L = load_library()
+    lay = desc_layout(desc)
+    p = np.zeros((lay["nblk"], NPR) + lay["D"])
+    ps = np.zeros((lay["nblk"], 3) + lay["D"])
+    r = L.hamr_desc_torus(C.byref(desc), C.byref(tdesc), _ptr(p), _ptr(ps))
+    if r:
+        raise HamrError("hamr_desc_torus failed (%d)" % r)
+    return p, ps
+
+
+# ---------------------------------------------------------------------------
+class Context:
+    """A resident block set on one MI355X (hamr_ctx)."""
+
+    def __init__(self, desc):
+        self.L = load_library()
+        self.desc = desc
+        self.h = C.c_void_p()
+        r = self.L.hamr_create(C.byref(desc), C.byref(self.h))
+        if r:
+            msg = self.L.hamr_last_error(self.h).decode() if self.h.value else "hamr_create failed"
+            if self.h.value:
+                self.L.hamr_destroy(self.h)
+            self.h = C.c_void_p()
+            raise HamrError(msg)
+        lay = desc_layout(desc)
+        self.D = lay["D"]
+        self.ncell = lay["ncell"]
+        self.nblk = lay["nblk"]
+        self.ngeo = lay["ngeo"]
+        self.kcol = lay["kcol"]
+
+    def close(self):
+        if self.h.value:
+            self.L.hamr_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, r, what):
+        if r:
+            raise HamrError("%s: %s" % (what, self.L.hamr_last_error(self.h).decode()))
+
+    # set-up
+    def set_grid(self):
+        self._chk(self.L.hamr_set_grid(self.h), "hamr_set_grid")
+
+    def init_torus(self, tdesc):
+        self._chk(self.L.hamr_init_torus(self.h, C.byref(tdesc)), "hamr_init_torus")
+
+    def upload_metric(self, geo, m):
+        arrs = [np.ascontiguousarray(m[k], dtype=float).ravel() for k in ("gcov", "gcon", "gdet", "conn", "rcen")]
+        self._chk(self.L.hamr_upload_metric(self.h, geo, *[_ptr(a) for a in arrs]), "hamr_upload_metric")
+
+    def download_metric(self, geo):
+        kc = self.kcol
+        gcov, gcon = np.zeros(5 * 10 * kc), np.zeros(5 * 10 * kc)
+        gdet, conn, rcen = np.zeros(5 * kc), np.zeros(40 * kc), np.zeros(kc)
+        self._chk(self.L.hamr_download_metric(self.h, geo, _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn),
+                                              _ptr(rcen)), "hamr_download_metric")
+        return dict(gcov=gcov.reshape(5, 10, kc), gcon=gcon.reshape(5, 10, kc), gdet=gdet.reshape(5, kc),
+                    conn=conn.reshape(40, kc), rcen=rcen)
+
+    def block_row(self, n):
+        row = np.zeros(17, dtype=np.int32)
+        self._chk(self.L.hamr_block_row(self.h, n, _ptr(row, _ip)), "hamr_block_row")
+        return row
+
+    # arrays
+    def get(self, which):
+        a = np.zeros((self.nblk, NCOMP[which]) + self.D)
+        self._chk(self.L.hamr_get_array(self.h, which, _ptr(a)), "hamr_get_array")
+        return a
+
+    def set(self, which, a):
+        a = np.ascontiguousarray(a, dtype=float)
+        assert a.shape == (self.nblk, NCOMP[which]) + self.D
+        self._chk(self.L.hamr_set_array(self.h, which, _ptr(a)), "hamr_set_array")
+
+    def upload(self, n, p_aos, ps_aos):
+        self._chk(self.L.hamr_upload(self.h, n, _ptr(np.ascontiguousarray(p_aos, float)),
+                                     _ptr(np.ascontiguousarray(ps_aos, float))), "hamr_upload")
+
+    def download(self, n):
+        p = np.zeros(self.D + (NPR,))
+        ps = np.zeros(self.D + (3,))
+        self._chk(self.L.hamr_download(self.h, n, _ptr(p), _ptr(ps)), "hamr_download")
+        return p, ps
+
+    def flags(self, n):
+        pf = np.zeros(self.D, dtype=np.int32)
+        fi = np.zeros(4, dtype=np.int32)
+        self._chk(self.L.hamr_flags(self.h, n, _ptr(pf, _ip), _ptr(fi, _ip)), "hamr_flags")
+        return pf, fi
+
+    def all_flags(self):
+        pf = np.zeros((self.nblk,) + self.D, dtype=np.int32)
+        fi = np.zeros((self.nblk, 4), dtype=np.int32)
+        for n in range(self.nblk):
+            pf[n], fi[n] = self.flags(n)
+        return pf, fi
+
+    def clear_flags(self):
+        self._chk(self.L.hamr_clear_flags(self.h), "hamr_clear_flags")
+
+    # stages (reference names in the docstrings of include/hamr_hip.h)
+    def fluxcalc(self, d, flag, n):
+        self._chk(self.L.hamr_fluxcalc(self.h, d, flag, n), "hamr_fluxcalc")
+
+    def block_dt(self, n, d):
+        v = C.c_double()
+        self._chk(self.L.hamr_block_dt(self.h, n, d, C.byref(v)), "hamr_block_dt")
+        return v.value
+
+    def ct_emf(self, flag, n):
+        self._chk(self.L.hamr_ct_emf(self.h, flag, n), "hamr_ct_emf")
+
+    def ct_bound(self, dt):
+        self._chk(self.L.hamr_ct_bound(self.h, dt), "hamr_ct_bound")
+
+    def ct_update(self, flag, Dt, n):
+        self._chk(self.L.hamr_ct_update(self.h, flag, Dt, n), "hamr_ct_update")
+
+    def update(self, flag, n, Dt):
+        self._chk(self.L.hamr_update(self.h, flag, n, Dt), "hamr_update")
+
+    def boundprim(self, flag):
+        self._chk(self.L.hamr_boundprim(self.h, flag), "hamr_boundprim")
+
+    def sync(self):
+        self._chk(self.L.hamr_sync(self.h), "hamr_sync")
+
+    def time_stage(self, stage, nrep=10):
+        v = C.c_double()
+        self._chk(self.L.hamr_time_stage(self.h, stage, nrep, C.byref(v)), "hamr_time_stage")
+        return v.value
+
+    # batched
+    def estimate_dt(self):
+        v = C.c_double()
+        self._chk(self.L.hamr_estimate_dt(self.h, C.byref(v)), "hamr_estimate_dt")
+        return v.value
+
+    def step(self, nsteps, dt=None, graph=False):
+        v = C.c_double(dt if dt is not None else -1.0)
+        f = self.L.hamr_step_graph if graph else self.L.hamr_step
+        self._chk(f(self.h, nsteps, C.byref(v)), "hamr_step")
+        return v.value
+
+
+def advance_step(ctx, dt):
+    """One predictor + corrector step through the per-block stage API, in the
+    order of advance_GPU (step_ch.c:997-1336) with every block at timelevel 1:
+    fluxes dir 3, 2, 1; CT EMF; polar E1 average; face-B update; conserved
+    update + inversion; ghost fill.  Returns the next dt (per-block harmonic of
+    the corrector's Courant minima, minimum over blocks, clamped to 1.3 dt)."""
+    nb = ctx.nblk
+    for flag, Dt in ((0, 0.5 * dt), (1, dt)):
+        for d in (3, 2, 1):
+            for n in range(nb):
+                ctx.fluxcalc(d, flag, n)
+        if flag == 1:
+            bdt = [[ctx.block_dt(n, d) for d in (1, 2, 3)] for n in range(nb)]
+        else:
+            for n in range(nb):
+                for d in (1, 2, 3):
+                    ctx.block_dt(n, d)
+        for n in range(nb):
+            ctx.ct_emf(flag, n)
+        ctx.ct_bound(dt)
+        for n in range(nb):
+            ctx.ct_update(flag, Dt, n)
+        for n in range(nb):
+            ctx.update(flag, n, Dt)
+        ctx.boundprim(1 if flag else 0)
+    ndt = min(1.0 / (1.0 / a + 1.0 / b + 1.0 / c) for a, b, c in bdt)
+    return min(ndt, 1.3 * dt)

@@ -1,0 +1,100 @@
+// hamr_grid.h -- block table, device layout and host set-up of the step.
+//
+// Layout in HBM (all blocks resident, one slab per block):
+//   cell(i,j,z) = (i*D2 + j)*D3 + z, D = BS + 2*NG, z fastest (the reference's
+//   index_3D order); each component of each block is a slab of `ksize` cells
+//   (padded to a multiple of 64 doubles = 512 B), components of a block are
+//   contiguous:  state[((blk*ncomp) + k)*ksize + cell].
+//   Metric: one geometry slab per distinct (c1, c2) block column (the metric is
+//   axisymmetric), columns col(i,j) = i*D2 + j padded to `kcol`:
+//   gcov/gcon[((geo*NPOS + pos)*10 + s)*kcol + col], gdet[(geo*NPOS + pos)*kcol + col],
+//   conn[(geo*40 + lam*10 + s)*kcol + col], rcen[geo*kcol + col].
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace hamr {
+
+constexpr int NG = 3;
+
+// Face kinds of the ghost fill.
+enum FaceKind : int { FK_COPY = 0, FK_OUTFLOW = 1, FK_POLE = 2 };
+
+// Per-block record used by the kernels (plain data, 72 bytes).
+struct BlockDev {
+    int geo;        // geometry slab
+    int c1, c2, c3; // block coordinates on the base level
+    int pole;       // bit0: lower x2 face is the polar axis, bit1: upper x2 face
+    int nbr[6];     // source block for faces -x1,+x1,-x2,+x2,-x3,+x3 (-1: none)
+    int kind[6];    // FaceKind of each face
+    int pad;
+};
+static_assert(sizeof(BlockDev) == 72, "BlockDev layout");
+
+// Static description of a run (mirrors the reference's compile-time config).
+struct GridDesc {
+    int bs[3];      // block interior size
+    int nb[3];      // blocks per direction
+    double gam;     // adiabatic index
+    double a;       // black-hole spin
+    double cour;    // Courant factor
+    double rin;     // inner radius of the grid (<= 0: 0.9 (1 + sqrt(1-a^2)))
+    double rout;    // outer radius
+    double rho_floor, u_floor;  // floors rho_floor r^-1.5, u_floor r^-2.5
+    int minkowski;  // 1: flat periodic box (unit cube per block grid), else Kerr-Schild torus grid
+};
+
+struct Layout {
+    int D[3];
+    int64_t ncell;  // D1*D2*D3
+    int64_t ksize;  // padded slab length (cells)
+    int64_t ncol;   // D1*D2
+    int64_t kcol;   // padded metric slab length (columns)
+    int nblk, ngeo;
+};
+
+Layout make_layout(const GridDesc &d);
+
+// Block table: natural order n = (c1*NB2 + c2)*NB3 + c3 on the base level.
+// Neighbours: x1 faces outflow at the radial ends, x2 faces at the axis use the
+// pi-shifted partner block (c3 + NB3/2) with the polar mirror, x3 periodic.
+std::vector<BlockDev> build_block_table(const GridDesc &d, int *ngeo_out);
+// Z-order (Morton) traversal of the base-level blocks, the launch order.
+std::vector<int> zorder(const GridDesc &d);
+
+// Host metric generation for one geometry slab (block column c1, c2).
+// Writes into host arrays laid out as one slab of the device layout.
+struct GridCoords {
+    double startx[3], dx[3];
+    double rin, rout, fractheta;
+};
+GridCoords grid_coords(const GridDesc &d);
+void bl_coord(const GridDesc &d, const double *X, double *r, double *th);
+void metric_code(const GridDesc &d, const double *X, double gcov[4][4]);
+double gdet_of(const double g[4][4]);
+int invert4(const double a[4][4], double inv[4][4]);
+void conn_code(const GridDesc &d, const double *X, const double gcon[4][4], double conn[4][4][4]);
+// Code coordinates of position `pos` of global cell (i, j) (interior-relative,
+// may be a ghost index), following the reference's coord() mirroring.
+void coord_of(const GridDesc &d, const GridCoords &gc, int i, int j, int pos, double X[4]);
+void fill_geometry(const GridDesc &d, const Layout &L, int c1, int c2, double *gcov, double *gcon,
+                   double *gdet, double *conn, double *rcen);
+
+// Torus initial state on the whole grid, written per block in the device layout
+// (p: NPR components, ps: 3).  Returns an empty string or an error message.
+struct TorusDesc {
+    double rin, rmax, kappa;   // Fishbone-Moncrief torus (r_in, r at pressure max, K)
+    double beta_min;           // field normalisation: min over cells of p_gas / p_mag
+    double aphi_cut;           // A_phi ~ max(rho/rho_max - aphi_cut, 0)
+    double perturb;            // u *= 1 + perturb (U[0,1) - 1/2)
+    uint64_t seed;
+};
+std::string init_torus(const GridDesc &d, const Layout &L, const std::vector<BlockDev> &blocks,
+                       const TorusDesc &t, const double *gcov, const double *gcon, const double *gdet,
+                       double *p, double *ps);
+
+uint64_t hash64(uint64_t x);
+
+}  // namespace hamr

@@ -1,0 +1,41 @@
+// hamr_kernels.h -- kernel parameters and launchers of the gfx950 step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hamr_grid.h"
+#include "hamr_phys.h"
+
+namespace hamr {
+
+// Passed by value to every kernel.
+struct KParams {
+    int D[3];              // block extent including ghosts
+    int bs[3];             // block interior
+    int64_t ksize;         // cells per component slab
+    int64_t kcol;          // columns per metric slab
+    int nb3;               // blocks around the axis (pole partner shift)
+    double gam, cour;
+    double dx[3];
+    double rho_floor, u_floor;
+    const BlockDev *blk;
+    const double *gcov, *gcon, *gdet, *conn, *rcen;
+};
+
+void launch_flux(const KParams &P, int dir, const int *blist, int nb, const double *p, const double *ps, double *F,
+                 unsigned long long *dtmin, hipStream_t s);
+void launch_emf(const KParams &P, const int *blist, int nb, const double *p, const double *F1, const double *F2,
+                const double *F3, double *Ec, double *E, hipStream_t s);
+void launch_pole(const KParams &P, const int *ring, int nring, double dt_val, const double *dt_ptr, double *E,
+                 hipStream_t s);
+void launch_ct_update(const KParams &P, const int *blist, int nb, double Dt_val, const double *dt_ptr,
+                      double dt_mult, const double *psi, double *psf, const double *E, hipStream_t s);
+void launch_update(const KParams &P, const int *blist, int nb, int full, double Dt_val, const double *dt_ptr,
+                   double dt_mult, const double *pi, const double *pb, double *pf, double *U0, const double *psf,
+                   const double *F1, const double *F2, const double *F3, int *pflag, int *failcnt, hipStream_t s);
+void launch_bound(const KParams &P, int dir, const int *blist, int nb, double *p, double *ps, int *pflag,
+                  hipStream_t s);
+void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);
+void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s);
+
+}  // namespace hamr

@@ -1,0 +1,561 @@
+// hamr_kernels.hip -- gfx950 kernels of the GRMHD fluid step (FP64 path).
+//
+// Every kernel takes a list of blocks (blockIdx.y selects one) so that the same
+// code serves the reference-shaped per-block calls and the batched step over the
+// whole resident block set.  Threads run along z (the fastest index of the
+// block layout) so that every global access of a wavefront is one or two
+// contiguous 512-byte segments.  Built without FMA contraction so that the FP64
+// arithmetic matches the reference's expression by expression.
+#include "hamr_kernels.h"
+
+#include <cstdio>
+
+namespace hamr {
+
+__device__ __forceinline__ void region_index(int t, int n2, int n3, int &a, int &b, int &c)
+{
+    c = t % n3;
+    int q = t / n3;
+    b = q % n2;
+    a = q / n2;
+}
+
+__device__ __forceinline__ Geom<double> load_geom(const KParams &P, int geo, int pos, int64_t col)
+{
+    Geom<double> G;
+    const double *gc = P.gcov + ((int64_t)geo * NPOS + pos) * 10 * P.kcol + col;
+    const double *gi = P.gcon + ((int64_t)geo * NPOS + pos) * 10 * P.kcol + col;
+#pragma unroll
+    for (int s = 0; s < 10; s++) {
+        G.gcov[s] = gc[s * P.kcol];
+        G.gcon[s] = gi[s * P.kcol];
+    }
+    G.g = P.gdet[((int64_t)geo * NPOS + pos) * P.kcol + col];
+    return G;
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Fused PPM reconstruction + HLL flux through the lower dir-faces of a block:
+// fluxcalcprep (GPU_program1.cu:78-190) and fluxcalc2D2 (:791-1068) in one pass,
+// without the dq1/dq2 round trip through HBM.  Face range: [NG, NG+BS] along
+// dir, [NG-1, NG+BS] across (the transverse ghost faces feed the corner EMFs).
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux(KParams P, const int *__restrict__ blist, const double *__restrict__ p,
+                                              const double *__restrict__ ps, double *__restrict__ F,
+                                              unsigned long long *__restrict__ dtmin)
+{
+    const int n = blist[blockIdx.y];
+    const int e1 = P.bs[0] + (DIR == 1 ? 1 : 2), e2 = P.bs[1] + (DIR == 2 ? 1 : 2), e3 = P.bs[2] + (DIR == 3 ? 1 : 2);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double vdt = 1.e9;
+    if (t < e1 * e2 * e3) {
+        int a, b, c;
+        region_index(t, e2, e3, a, b, c);
+        const int i = a + (DIR == 1 ? NG : NG - 1);
+        const int j = b + (DIR == 2 ? NG : NG - 1);
+        const int z = c + (DIR == 3 ? NG : NG - 1);
+        const int64_t cell = ((int64_t)i * P.D[1] + j) * P.D[2] + z;
+        const int64_t sd = (DIR == 1) ? (int64_t)P.D[1] * P.D[2] : (DIR == 2) ? (int64_t)P.D[2] : 1;
+        const double *pb = p + (int64_t)n * NPR * P.ksize;
+        double pl[NPR], pr[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            const double *q = pb + (int64_t)k * P.ksize + cell;
+            const double ym3 = q[-3 * sd], ym2 = q[-2 * sd], ym1 = q[-sd], y0 = q[0], yp1 = q[sd], yp2 = q[2 * sd];
+            double l, r;
+            para(ym3, ym2, ym1, y0, yp1, l, r);
+            pl[k] = r;
+            para(ym2, ym1, y0, yp1, yp2, l, r);
+            pr[k] = l;
+        }
+        const double bn = ps[((int64_t)n * 3 + (DIR - 1)) * P.ksize + cell];
+        pl[B1 + DIR - 1] = bn;
+        pr[B1 + DIR - 1] = bn;
+        const BlockDev &B = P.blk[n];
+        if (DIR == 2 && (((B.pole & 1) && j == NG) || ((B.pole & 2) && j == NG + P.bs[1]))) {
+            pl[B1] = 0.;
+            pr[B1] = 0.;
+        }
+        const int64_t col = (int64_t)i * P.D[1] + j;
+        const Geom<double> G = load_geom(P, B.geo, DIR == 1 ? FACE1 : DIR == 2 ? FACE2 : FACE3, col);
+        double t1[NPR], t2[NPR], t3[NPR], t4[NPR], cmax_l = 0., cmin_l = 0., cmax_r = 0., cmin_r = 0., dum;
+        State<double> q;
+        get_state(pl, G, q);
+        primtoflux(pl, q, DIR, G, P.gam, t1, &cmax_l, &cmin_l);
+        primtoflux(pl, q, 0, G, P.gam, t2, &dum, &dum);
+        get_state(pr, G, q);
+        primtoflux(pr, q, DIR, G, P.gam, t3, &cmax_r, &cmin_r);
+        primtoflux(pr, q, 0, G, P.gam, t4, &dum, &dum);
+        const double cmax = fabs(mymax(mymax(0., cmax_l), cmax_r));
+        const double cmin = fabs(mymax(mymax(0., -cmin_l), -cmin_r));
+        const double ctop = mymax(cmax, cmin);
+        double *Fb = F + (int64_t)n * NPR * P.ksize + cell;
+#pragma unroll
+        for (int k = 0; k < NPR; k++)
+            Fb[(int64_t)k * P.ksize] = (cmax * t1[k] + cmin * t3[k] - cmax * cmin * (t4[k] - t2[k])) / (cmax + cmin + kSMALL);
+        const double v = P.cour * P.dx[DIR - 1] / ctop;
+        if (v < 1.e9 && v > 1.e-6) vdt = v;
+    }
+    if (dtmin) {
+        vdt = wave_min(vdt);
+        if ((threadIdx.x & 63) == 0 && vdt < 1.e9) atomicMin(&dtmin[n * 3 + DIR - 1], (unsigned long long)__double_as_longlong(vdt));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cell-centre EMF, consttransport1 (GPU_const_trans.h:63-119), cells [NG-1, NG+BS].
+__global__ void __launch_bounds__(256) k_emf_centre(KParams P, const int *__restrict__ blist, const double *__restrict__ p,
+                                                    double *__restrict__ Ec)
+{
+    const int n = blist[blockIdx.y];
+    const int e1 = P.bs[0] + 2, e2 = P.bs[1] + 2, e3 = P.bs[2] + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= e1 * e2 * e3) return;
+    int a, b, c;
+    region_index(t, e2, e3, a, b, c);
+    const int i = a + NG - 1, j = b + NG - 1, z = c + NG - 1;
+    const int64_t cell = ((int64_t)i * P.D[1] + j) * P.D[2] + z;
+    const double *pb = p + (int64_t)n * NPR * P.ksize + cell;
+    double pr[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) pr[k] = pb[(int64_t)k * P.ksize];
+    const Geom<double> G = load_geom(P, P.blk[n].geo, CENT, (int64_t)i * P.D[1] + j);
+    double ucon[4], ucov[4], bcon[4];
+    ucon_calc(pr, G, ucon);
+    lower10(ucon, G.gcov, ucov);
+    bcon_calc(pr, ucon, ucov, bcon);
+    double *E = Ec + (int64_t)n * 3 * P.ksize + cell;
+    E[0] = -G.g * (ucon[2] * bcon[3] - ucon[3] * bcon[2]);
+    E[P.ksize] = -G.g * (ucon[3] * bcon[1] - ucon[1] * bcon[3]);
+    E[2 * P.ksize] = -G.g * (ucon[1] * bcon[2] - ucon[2] * bcon[1]);
+}
+
+// Corner EMF upwinded on the mass-flux sign, consttransport2
+// (GPU_const_trans.h:121-339) with the polar rule (:233-236); corners [NG, NG+BS].
+__global__ void __launch_bounds__(256) k_emf_corner(KParams P, const int *__restrict__ blist, const double *__restrict__ Ec,
+                                                    const double *__restrict__ F1, const double *__restrict__ F2,
+                                                    const double *__restrict__ F3, double *__restrict__ E)
+{
+    const int n = blist[blockIdx.y];
+    const int e1 = P.bs[0] + 1, e2 = P.bs[1] + 1, e3 = P.bs[2] + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= e1 * e2 * e3) return;
+    int a, b, c;
+    region_index(t, e2, e3, a, b, c);
+    const int i = a + NG, j = b + NG, z = c + NG;
+    const int64_t g = ((int64_t)i * P.D[1] + j) * P.D[2] + z;
+    const int64_t K = P.ksize, si = (int64_t)P.D[1] * P.D[2], sj = P.D[2];
+    const double *E1c = Ec + (int64_t)n * 3 * K, *E2c = E1c + K, *E3c = E2c + K;
+    const double *f1 = F1 + (int64_t)n * NPR * K, *f2 = F2 + (int64_t)n * NPR * K, *f3 = F3 + (int64_t)n * NPR * K;
+    const double *F1R = f1 + RHO * K, *F2R = f2 + RHO * K, *F3R = f3 + RHO * K;
+    const double *F1B2 = f1 + B2 * K, *F1B3 = f1 + B3 * K;
+    const double *F2B1 = f2 + B1 * K, *F2B3 = f2 + B3 * K;
+    const double *F3B1 = f3 + B1 * K, *F3B2 = f3 + B2 * K;
+    const double dEL13_1 = E1c[g] - F3B2[g];
+    const double dEL13_2 = E1c[g - sj] - F3B2[g - sj];
+    const double dER13_1 = F3B2[g] - E1c[g - 1];
+    const double dER13_2 = F3B2[g - sj] - E1c[g - sj - 1];
+    const double dEL12_1 = E1c[g] + F2B3[g];
+    const double dEL12_2 = E1c[g - 1] + F2B3[g - 1];
+    const double dER12_1 = -F2B3[g] - E1c[g - sj];
+    const double dER12_2 = -F2B3[g - 1] - E1c[g - sj - 1];
+    const double dEL21_1 = E2c[g] - F1B3[g];
+    const double dEL21_2 = E2c[g - 1] - F1B3[g - 1];
+    const double dER21_1 = F1B3[g] - E2c[g - si];
+    const double dER21_2 = F1B3[g - 1] - E2c[g - si - 1];
+    const double dEL23_1 = E2c[g] + F3B1[g];
+    const double dEL23_2 = E2c[g - si] + F3B1[g - si];
+    const double dER23_1 = -F3B1[g] - E2c[g - 1];
+    const double dER23_2 = -F3B1[g - si] - E2c[g - si - 1];
+    const double dEL31_1 = E3c[g] + F1B2[g];
+    const double dEL31_2 = E3c[g - sj] + F1B2[g - sj];
+    const double dER31_1 = -F1B2[g] - E3c[g - si];
+    const double dER31_2 = -F1B2[g - sj] - E3c[g - si - sj];
+    const double dEL32_1 = E3c[g] - F2B1[g];
+    const double dEL32_2 = E3c[g - si] - F2B1[g - si];
+    const double dER32_1 = F2B1[g] - E3c[g - sj];
+    const double dER32_2 = F2B1[g - si] - E3c[g - si - sj];
+#define LE(x) ((double)((x) <= 0.0))
+#define GT(x) ((double)((x) > 0.0))
+    double e1v = 0.25 * ((-F2B3[g] - (dEL13_1 * LE(F2R[g]) + dEL13_2 * GT(F2R[g])))
+        + (-F2B3[g - 1] + (dER13_1 * LE(F2R[g - 1]) + dER13_2 * GT(F2R[g - 1])))
+        + (F3B2[g] - (dEL12_1 * LE(F3R[g]) + dEL12_2 * GT(F3R[g])))
+        + (F3B2[g - sj] + (dER12_1 * LE(F3R[g - sj]) + dER12_2 * GT(F3R[g - sj]))));
+    const double e2v = 0.25 * ((-F3B1[g] - (dEL21_1 * LE(F3R[g]) + dEL21_2 * GT(F3R[g])))
+        + (-F3B1[g - si] + (dER21_1 * LE(F3R[g - si]) + dER21_2 * GT(F3R[g - si])))
+        + (F1B3[g] - (dEL23_1 * LE(F1R[g]) + dEL23_2 * GT(F1R[g])))
+        + (F1B3[g - 1] + (dER23_1 * LE(F1R[g - 1]) + dER23_2 * GT(F1R[g - 1]))));
+    double e3v = 0.25 * ((F2B1[g] - (dEL31_1 * LE(F2R[g]) + dEL31_2 * GT(F2R[g])))
+        + (F2B1[g - si] + (dER31_1 * LE(F2R[g - si]) + dER31_2 * GT(F2R[g - si])))
+        + (-F1B2[g] - (dEL32_1 * LE(F1R[g]) + dEL32_2 * GT(F1R[g])))
+        + (-F1B2[g - sj] + (dER32_1 * LE(F1R[g - sj]) + dER32_2 * GT(F1R[g - sj]))));
+#undef LE
+#undef GT
+    const BlockDev &B = P.blk[n];
+    if (((B.pole & 1) && j == NG) || ((B.pole & 2) && j == NG + P.bs[1])) {
+        e3v = 0.;
+        e1v = -0.5 * (F2B3[g] + F2B3[g - 1]);
+    }
+    double *Eb = E + (int64_t)n * 3 * K + g;
+    Eb[0] = e1v;
+    Eb[K] = e2v;
+    Eb[2 * K] = e3v;
+}
+
+// Polar E1: mean over each pole block's interior corners in phi (weighted by
+// dt as packsend2E does), mean over the ring of phi-blocks, divided back by dt
+// (E_average / read_E_avg / write_E_avg, const_trans.c:268-543).  One thread per
+// (side, ring, radial corner); ring lists the NB3 blocks of each ring in c3 order.
+__global__ void k_pole_average(KParams P, const int *__restrict__ ring, int nring, double dt_val,
+                               const double *__restrict__ dt_ptr, double *__restrict__ E)
+{
+    const int ni = P.bs[0] + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * nring * ni) return;
+    const int il = t % ni, rg = (t / ni) % nring, side = t / (ni * nring);
+    const int i = il + NG, j = side ? NG + P.bs[1] : NG;
+    const double dt = dt_ptr ? dt_ptr[0] : dt_val;
+    const int64_t K = P.ksize;
+    const int *rb = ring + ((int64_t)side * nring + rg) * P.nb3;
+    double avg = 0.;
+    for (int b = 0; b < P.nb3; b++) {
+        const double *e1 = E + (int64_t)rb[b] * 3 * K + ((int64_t)i * P.D[1] + j) * P.D[2];
+        double s = 0.;
+        for (int zz = 0; zz < P.bs[2]; zz++) s += dt * e1[NG + zz];
+        s /= (double)(P.bs[2]);
+        if (b == 0) avg = s / ((double)P.nb3);
+        else avg += s / ((double)P.nb3);
+    }
+    const double v = avg / dt;
+    for (int b = 0; b < P.nb3; b++) {
+        double *e = E + (int64_t)rb[b] * 3 * K + ((int64_t)i * P.D[1] + j) * P.D[2];
+        for (int z = NG; z <= NG + P.bs[2]; z++) {
+            e[z] = v;
+            e[2 * K + z] = 0.;
+        }
+    }
+}
+
+// Face-B update from the curl of the corner EMF, consttransport3
+// (GPU_const_trans.h:426-530); faces [NG, NG+BS] along their normal.  psi and
+// psf may alias (each thread reads only the faces it writes).
+__global__ void __launch_bounds__(256) k_ct_update(KParams P, const int *__restrict__ blist, double Dt_val,
+                                                   const double *__restrict__ dt_ptr, double dt_mult, const double *psi,
+                                                   double *psf, const double *__restrict__ E)
+{
+    const int n = blist[blockIdx.y];
+    const int e1 = P.bs[0] + 1, e2 = P.bs[1] + 1, e3 = P.bs[2] + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= e1 * e2 * e3) return;
+    int a, b, c;
+    region_index(t, e2, e3, a, b, c);
+    const int i = a + NG, j = b + NG, z = c + NG;
+    const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
+    const int64_t K = P.ksize, si = (int64_t)P.D[1] * P.D[2], sj = P.D[2];
+    const int64_t g = ((int64_t)i * P.D[1] + j) * P.D[2] + z, col = (int64_t)i * P.D[1] + j;
+    const int geo = P.blk[n].geo;
+    const double *gd = P.gdet + (int64_t)geo * NPOS * P.kcol + col;
+    const double gF1 = gd[FACE1 * P.kcol], gF2 = gd[FACE2 * P.kcol], gF3 = gd[FACE3 * P.kcol];
+    const double *Eb = E + (int64_t)n * 3 * K + g;
+    const double *si_ = psi + (int64_t)n * 3 * K + g;
+    double *sf = psf + (int64_t)n * 3 * K + g;
+    const double dx1 = P.dx[0], dx2 = P.dx[1], dx3 = P.dx[2];
+    double temp;
+    if (j < NG + P.bs[1] && z < NG + P.bs[2]) {
+        temp = 0.;
+        temp += 1.0 / 1.0 * si_[0] * gF1;
+        temp += -Dt / (1.0 * dx2) * (Eb[2 * K + sj] - Eb[2 * K]);
+        temp += Dt / (1.0 * dx3) * (Eb[K + 1] - Eb[K]);
+        sf[0] = temp / gF1;
+    }
+    if (i < NG + P.bs[0] && j < NG + P.bs[1]) {
+        temp = si_[2 * K] * gF3;
+        temp += -Dt / dx1 * (Eb[K + si] - Eb[K]);
+        temp += Dt / dx2 * (Eb[sj] - Eb[0]);
+        sf[2 * K] = temp / gF3;
+    }
+    if (i < NG + P.bs[0] && z < NG + P.bs[2]) {
+        temp = 0.;
+        temp += 1.0 / 1.0 * si_[K] * gF2;
+        temp += Dt / (1.0 * dx1) * (Eb[2 * K + si] - Eb[2 * K]);
+        temp += -Dt / (1.0 * dx3) * (Eb[1] - Eb[0]);
+        sf[K] = temp / gF2;
+    }
+}
+
+// Conserved update, source, face-B average, Noble cascade, floors and flags:
+// the fixup kernel of GPU_fixup.h:5-425.  full = 0 (predictor): U^n from pi is
+// stored in U0, source and guess from pi.  full = 1 (corrector): U from U0,
+// source and guess from pb.
+__global__ void __launch_bounds__(256) k_update(KParams P, const int *__restrict__ blist, int full, double Dt_val,
+                                                const double *__restrict__ dt_ptr, double dt_mult,
+                                                const double *__restrict__ pi, const double *__restrict__ pb,
+                                                double *__restrict__ pf_out, double *__restrict__ U0,
+                                                const double *__restrict__ psf, const double *__restrict__ F1,
+                                                const double *__restrict__ F2, const double *__restrict__ F3,
+                                                int *__restrict__ pflag, int *__restrict__ failcnt)
+{
+    const int n = blist[blockIdx.y];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.bs[0] * P.bs[1] * P.bs[2]) return;
+    int a, b, c;
+    region_index(t, P.bs[1], P.bs[2], a, b, c);
+    const int i = a + NG, j = b + NG, z = c + NG;
+    const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
+    const int64_t K = P.ksize, si = (int64_t)P.D[1] * P.D[2], sj = P.D[2];
+    const int64_t g = ((int64_t)i * P.D[1] + j) * P.D[2] + z, col = (int64_t)i * P.D[1] + j;
+    const int64_t base = (int64_t)n * NPR * K + g;
+    const int geo = P.blk[n].geo;
+    const Geom<double> G = load_geom(P, geo, CENT, col);
+    double pf[NPR], U[NPR], dU[NPR];
+    State<double> q;
+    if (!full) {
+        double d1, d2;
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pf[k] = pi[base + k * K];
+        get_state(pf, G, q);
+        primtoflux(pf, q, 0, G, P.gam, U, &d1, &d2);
+#pragma unroll
+        for (int k = 0; k < NPR; k++) U0[base + k * K] = U[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) U[k] = U0[base + k * K];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pf[k] = pb[base + k * K];
+        get_state(pf, G, q);
+    }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        const double *f1 = F1 + base + k * K, *f2 = F2 + base + k * K, *f3 = F3 + base + k * K;
+        U[k] -= Dt * (f1[si] - f1[0]) / (P.dx[0] * 1.0);
+        U[k] -= Dt * (f2[sj] - f2[0]) / (P.dx[1] * 1.0);
+        U[k] -= Dt * (f3[1] - f3[0]) / (P.dx[2] * 1.0);
+    }
+    const double *cn = P.conn + (int64_t)geo * 40 * P.kcol + col;
+    const int64_t kc = P.kcol;
+    source(pf, q, [&](int lam, int s) { return cn[(lam * 10 + s) * kc]; }, G.g, P.gam, dU);
+#pragma unroll
+    for (int k = 0; k < NPR; k++) U[k] += Dt * (dU[k]);
+    {
+        const double *gd = P.gdet + (int64_t)geo * NPOS * P.kcol + col;
+        const double *s = psf + (int64_t)n * 3 * K + g;
+        U[B1] = 0.0;
+        U[B2] = 0.0;
+        U[B1] += (s[0] * gd[FACE1 * P.kcol] + s[si] * gd[FACE1 * P.kcol + P.D[1]]) / (2.0 * 1.0);
+        U[B2] += (s[K] * gd[FACE2 * P.kcol] + s[K + sj] * gd[FACE2 * P.kcol + 1]) / (2.0 * 1.0);
+        U[B3] = (s[2 * K] * gd[FACE3 * P.kcol] + s[2 * K + 1] * gd[FACE3 * P.kcol]) / 2.0;
+    }
+    int flag = utoprim(0, U, G.gcov, G.gcon, G.g, P.gam, pf);
+    if (flag) {
+        atomicAdd(&failcnt[n * 4 + 0], 1);
+        flag = utoprim(1, U, G.gcov, G.gcon, G.g, P.gam, pf);
+        if (flag) {
+            atomicAdd(&failcnt[n * 4 + 1], 1);
+            flag = utoprim(2, U, G.gcov, G.gcon, G.g, P.gam, pf);
+            if (flag) atomicAdd(&failcnt[n * 4 + 2], 1);
+        }
+    }
+    {
+        const double r = P.rcen[(int64_t)geo * P.kcol + col], sr = sqrt(r);
+        int hit = 0;
+        if (P.rho_floor > 0.) {
+            const double rhoflr = P.rho_floor / (r * sr);
+            if (pf[RHO] < rhoflr) { pf[RHO] = rhoflr; hit = 1; }
+        }
+        if (P.u_floor > 0.) {
+            const double uflr = P.u_floor / (r * r * sr);
+            if (pf[UU] < uflr) { pf[UU] = uflr; hit = 1; }
+        }
+        if (hit) {
+            flag = -333;
+            atomicAdd(&failcnt[n * 4 + 3], 1);
+        }
+    }
+    pflag[(int64_t)n * K + g] = flag;
+#pragma unroll
+    for (int k = 0; k < NPR; k++) pf_out[base + k * K] = pf[k];
+}
+
+// Ghost fill of one direction for the listed blocks: same-level copy, radial
+// outflow with inflow check, or the polar mirror from the pi-shifted partner.
+// Stage order x1, x2, x3, each over the full extent of the others.
+template <int DIR>
+__global__ void __launch_bounds__(256) k_bound(KParams P, const int *__restrict__ blist, double *p, double *ps,
+                                               int *pflag)
+{
+    const int n = blist[blockIdx.y];
+    const int d1 = P.D[0], d2 = P.D[1], d3 = P.D[2];
+    const int ta = (DIR == 1) ? d2 : d1, tb = (DIR == 3) ? d2 : d3;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * NG * ta * tb) return;
+    const int bb = t % tb, aa = (t / tb) % ta, gg = (t / (tb * ta)) % NG, side = t / (tb * ta * NG);
+    const BlockDev &B = P.blk[n];
+    const int face = 2 * (DIR - 1) + side;
+    const int kind = B.kind[face];
+    const int bsd = P.bs[DIR - 1];
+    // ghost and source index along DIR
+    const int gidx = side ? NG + bsd + gg : gg;
+    int sidx, src = B.nbr[face];
+    if (kind == FK_COPY) sidx = side ? NG + gg : bsd + gg;
+    else if (kind == FK_OUTFLOW) { sidx = side ? NG + bsd - 1 : NG; src = n; }
+    else sidx = side ? NG + bsd - 1 - gg : 2 * NG - 1 - gg;
+    int i, j, z, is, js, zs;
+    if (DIR == 1) { i = gidx; j = aa; z = bb; is = sidx; js = j; zs = z; }
+    else if (DIR == 2) { i = aa; j = gidx; z = bb; is = i; js = sidx; zs = z; }
+    else { i = aa; j = bb; z = gidx; is = i; js = j; zs = sidx; }
+    if (kind == FK_POLE && P.nb3 == 1) zs = ((z - NG + P.bs[2] / 2) % P.bs[2] + P.bs[2]) % P.bs[2] + NG;
+    const int64_t K = P.ksize;
+    const int64_t cg = ((int64_t)i * d2 + j) * d3 + z, cs = ((int64_t)is * d2 + js) * d3 + zs;
+    double *pd = p + (int64_t)n * NPR * K;
+    const double *psrc = p + (int64_t)src * NPR * K;
+    double *sd = ps + (int64_t)n * 3 * K;
+    const double *ssrc = ps + (int64_t)src * 3 * K;
+    double pr[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) pr[k] = psrc[k * K + cs];
+    if (kind == FK_COPY) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pd[k * K + cg] = pr[k];
+        sd[cg] = ssrc[cs];
+        sd[K + cg] = ssrc[K + cs];
+        sd[2 * K + cg] = ssrc[2 * K + cs];
+        if (pflag) pflag[(int64_t)n * K + cg] = pflag[(int64_t)src * K + cs];
+    } else if (kind == FK_OUTFLOW) {
+        const Geom<double> G = load_geom(P, B.geo, CENT, (int64_t)i * d2 + j);
+        inflow_check(pr, G, side ? 1 : 0);
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pd[k * K + cg] = pr[k];
+        sd[K + cg] = ssrc[K + cs];
+        sd[2 * K + cg] = ssrc[2 * K + cs];
+        if (pflag) pflag[(int64_t)n * K + cg] = pflag[(int64_t)n * K + cs];
+    } else {
+        pr[U2] *= -1.0;
+        pr[U3] *= -1.0;
+        pr[B2] *= -1.0;
+        pr[B3] *= -1.0;
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pd[k * K + cg] = pr[k];
+        sd[cg] = ssrc[cs];
+        sd[2 * K + cg] = -ssrc[2 * K + cs];
+        if (pflag) pflag[(int64_t)n * K + cg] = pflag[(int64_t)src * K + cs];
+    }
+}
+
+// Courant dt of the step: per-block harmonic of the direction minima, minimum
+// over blocks (step_ch.c:1313-1322), clamped to 1.3 dt (step_ch.c:974).  Resets
+// the minima for the next corrector.  dt[0] <- new dt, dt[1] <- previous dt.
+__global__ void k_dt_finalize(int nblk, unsigned long long *__restrict__ dtmin, double *__restrict__ dt)
+{
+    __shared__ double red[256];
+    double m = 1.e9;
+    for (int n = threadIdx.x; n < nblk; n += blockDim.x) {
+        const double d1 = __longlong_as_double((long long)dtmin[n * 3 + 0]);
+        const double d2 = __longlong_as_double((long long)dtmin[n * 3 + 1]);
+        const double d3 = __longlong_as_double((long long)dtmin[n * 3 + 2]);
+        const double b0 = 1. / (1. / d1 + 1. / d2 + 1. / d3);
+        m = mymin(m, b0);
+        const unsigned long long init = (unsigned long long)__double_as_longlong(1.e9);
+        dtmin[n * 3 + 0] = init;
+        dtmin[n * 3 + 1] = init;
+        dtmin[n * 3 + 2] = init;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = mymin(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double old = dt[0];
+        double ndt = red[0];
+        if (ndt > 1.3 * old) ndt = 1.3 * old;
+        dt[1] = old;
+        dt[0] = ndt;
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long *a, int n, unsigned long long v)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+static inline dim3 grid_for(int64_t work, int nb) { return dim3((unsigned)((work + 255) / 256), (unsigned)nb, 1); }
+
+void launch_flux(const KParams &P, int dir, const int *blist, int nb, const double *p, const double *ps, double *F,
+                 unsigned long long *dtmin, hipStream_t s)
+{
+    const int64_t e1 = P.bs[0] + (dir == 1 ? 1 : 2), e2 = P.bs[1] + (dir == 2 ? 1 : 2), e3 = P.bs[2] + (dir == 3 ? 1 : 2);
+    const dim3 g = grid_for(e1 * e2 * e3, nb);
+    if (dir == 1) hipLaunchKernelGGL(k_flux<1>, g, dim3(256), 0, s, P, blist, p, ps, F, dtmin);
+    else if (dir == 2) hipLaunchKernelGGL(k_flux<2>, g, dim3(256), 0, s, P, blist, p, ps, F, dtmin);
+    else hipLaunchKernelGGL(k_flux<3>, g, dim3(256), 0, s, P, blist, p, ps, F, dtmin);
+}
+
+void launch_emf(const KParams &P, const int *blist, int nb, const double *p, const double *F1, const double *F2,
+                const double *F3, double *Ec, double *E, hipStream_t s)
+{
+    const int64_t c = (int64_t)(P.bs[0] + 2) * (P.bs[1] + 2) * (P.bs[2] + 2);
+    hipLaunchKernelGGL(k_emf_centre, grid_for(c, nb), dim3(256), 0, s, P, blist, p, Ec);
+    const int64_t k = (int64_t)(P.bs[0] + 1) * (P.bs[1] + 1) * (P.bs[2] + 1);
+    hipLaunchKernelGGL(k_emf_corner, grid_for(k, nb), dim3(256), 0, s, P, blist, Ec, F1, F2, F3, E);
+}
+
+void launch_pole(const KParams &P, const int *ring, int nring, double dt_val, const double *dt_ptr, double *E,
+                 hipStream_t s)
+{
+    if (nring <= 0) return;
+    const int64_t w = (int64_t)2 * nring * (P.bs[0] + 1);
+    hipLaunchKernelGGL(k_pole_average, dim3((unsigned)((w + 63) / 64)), dim3(64), 0, s, P, ring, nring, dt_val,
+                       dt_ptr, E);
+}
+
+void launch_ct_update(const KParams &P, const int *blist, int nb, double Dt_val, const double *dt_ptr,
+                      double dt_mult, const double *psi, double *psf, const double *E, hipStream_t s)
+{
+    const int64_t k = (int64_t)(P.bs[0] + 1) * (P.bs[1] + 1) * (P.bs[2] + 1);
+    hipLaunchKernelGGL(k_ct_update, grid_for(k, nb), dim3(256), 0, s, P, blist, Dt_val, dt_ptr, dt_mult, psi, psf, E);
+}
+
+void launch_update(const KParams &P, const int *blist, int nb, int full, double Dt_val, const double *dt_ptr,
+                   double dt_mult, const double *pi, const double *pb, double *pf, double *U0, const double *psf,
+                   const double *F1, const double *F2, const double *F3, int *pflag, int *failcnt, hipStream_t s)
+{
+    const int64_t w = (int64_t)P.bs[0] * P.bs[1] * P.bs[2];
+    hipLaunchKernelGGL(k_update, grid_for(w, nb), dim3(256), 0, s, P, blist, full, Dt_val, dt_ptr, dt_mult, pi, pb, pf,
+                       U0, psf, F1, F2, F3, pflag, failcnt);
+}
+
+void launch_bound(const KParams &P, int dir, const int *blist, int nb, double *p, double *ps, int *pflag,
+                  hipStream_t s)
+{
+    const int d1 = P.D[0], d2 = P.D[1], d3 = P.D[2];
+    const int ta = (dir == 1) ? d2 : d1, tb = (dir == 3) ? d2 : d3;
+    const dim3 g = grid_for((int64_t)2 * NG * ta * tb, nb);
+    if (dir == 1) hipLaunchKernelGGL(k_bound<1>, g, dim3(256), 0, s, P, blist, p, ps, pflag);
+    else if (dir == 2) hipLaunchKernelGGL(k_bound<2>, g, dim3(256), 0, s, P, blist, p, ps, pflag);
+    else hipLaunchKernelGGL(k_bound<3>, g, dim3(256), 0, s, P, blist, p, ps, pflag);
+}
+
+void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s, nblk, dtmin, dt);
+}
+
+void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s)
+{
+    const int n = nblk * 3;
+    hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, s, dtmin, n,
+                       (unsigned long long)0x41CDCD6500000000ULL /* 1e9 */);
+}
+
+}  // namespace hamr

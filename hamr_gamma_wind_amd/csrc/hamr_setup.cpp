@@ -1,0 +1,632 @@
+// hamr_setup.cpp -- host set-up: block table, coordinates, Kerr-Schild metric in
+// code coordinates, and the Fishbone-Moncrief torus initial state.
+//
+// Coordinates follow coord.c (coord :26, bl_coord :111, vofx_matthewcoords :147
+// with the uniform-theta parameters BRAVO=0 TANGO=1 CHARLIE=0, set_points :327,
+// set_gridparam :357); the metric follows metric.c (gcov_func_spherical :13,
+// numerical dx/dx' Jacobian dxdxp_func :501, gdet_func :392, gcon_func :410,
+// conn_func :430 with EPS = 1e-5).  The torus follows init.c:808-1110
+// (init_torus with rin = 6, rmax = 12; lfish_calc of init_torus.c:501;
+// coord_transform of init.c:3218); the field is a poloidal loop from
+// A_phi ~ max(rho/rho_max - 0.2, 0) on cell corners, curled onto the faces.
+#include "hamr_grid.h"
+#include "hamr_phys.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace hamr {
+
+static int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+Layout make_layout(const GridDesc &d)
+{
+    Layout L;
+    for (int k = 0; k < 3; k++) L.D[k] = d.bs[k] + 2 * NG;
+    L.ncell = (int64_t)L.D[0] * L.D[1] * L.D[2];
+    L.ksize = round_up(L.ncell, 64);
+    L.ncol = (int64_t)L.D[0] * L.D[1];
+    L.kcol = round_up(L.ncol, 64);
+    L.nblk = d.nb[0] * d.nb[1] * d.nb[2];
+    L.ngeo = d.nb[0] * d.nb[1];
+    return L;
+}
+
+std::vector<BlockDev> build_block_table(const GridDesc &d, int *ngeo_out)
+{
+    const int nb1 = d.nb[0], nb2 = d.nb[1], nb3 = d.nb[2];
+    auto id = [&](int a, int b, int c) { return (a * nb2 + b) * nb3 + c; };
+    std::vector<BlockDev> t(nb1 * nb2 * nb3);
+    for (int c1 = 0; c1 < nb1; c1++)
+        for (int c2 = 0; c2 < nb2; c2++)
+            for (int c3 = 0; c3 < nb3; c3++) {
+                BlockDev &b = t[id(c1, c2, c3)];
+                std::memset(&b, 0, sizeof b);
+                b.geo = c1 * nb2 + c2;
+                b.c1 = c1;
+                b.c2 = c2;
+                b.c3 = c3;
+                const bool flat = d.minkowski != 0;
+                // x1
+                if (c1 > 0) { b.nbr[0] = id(c1 - 1, c2, c3); b.kind[0] = FK_COPY; }
+                else if (flat) { b.nbr[0] = id(nb1 - 1, c2, c3); b.kind[0] = FK_COPY; }
+                else { b.nbr[0] = -1; b.kind[0] = FK_OUTFLOW; }
+                if (c1 < nb1 - 1) { b.nbr[1] = id(c1 + 1, c2, c3); b.kind[1] = FK_COPY; }
+                else if (flat) { b.nbr[1] = id(0, c2, c3); b.kind[1] = FK_COPY; }
+                else { b.nbr[1] = -1; b.kind[1] = FK_OUTFLOW; }
+                // x2: polar axis uses the partner half a turn away in phi
+                const int cpi = (c3 + nb3 / 2) % nb3;
+                if (c2 > 0) { b.nbr[2] = id(c1, c2 - 1, c3); b.kind[2] = FK_COPY; }
+                else if (flat) { b.nbr[2] = id(c1, nb2 - 1, c3); b.kind[2] = FK_COPY; }
+                else { b.nbr[2] = id(c1, 0, cpi); b.kind[2] = FK_POLE; b.pole |= 1; }
+                if (c2 < nb2 - 1) { b.nbr[3] = id(c1, c2 + 1, c3); b.kind[3] = FK_COPY; }
+                else if (flat) { b.nbr[3] = id(c1, 0, c3); b.kind[3] = FK_COPY; }
+                else { b.nbr[3] = id(c1, nb2 - 1, cpi); b.kind[3] = FK_POLE; b.pole |= 2; }
+                // x3 periodic
+                b.nbr[4] = id(c1, c2, (c3 + nb3 - 1) % nb3);
+                b.kind[4] = FK_COPY;
+                b.nbr[5] = id(c1, c2, (c3 + 1) % nb3);
+                b.kind[5] = FK_COPY;
+            }
+    if (ngeo_out) *ngeo_out = nb1 * nb2;
+    return t;
+}
+
+static uint64_t spread3(uint64_t v)
+{
+    uint64_t r = 0;
+    for (int b = 0; b < 21; b++) r |= ((v >> b) & 1ULL) << (3 * b);
+    return r;
+}
+
+std::vector<int> zorder(const GridDesc &d)
+{
+    std::vector<std::pair<uint64_t, int>> keys;
+    for (int c1 = 0; c1 < d.nb[0]; c1++)
+        for (int c2 = 0; c2 < d.nb[1]; c2++)
+            for (int c3 = 0; c3 < d.nb[2]; c3++) {
+                uint64_t k = (spread3(c1) << 2) | (spread3(c2) << 1) | spread3(c3);
+                keys.push_back({k, (c1 * d.nb[1] + c2) * d.nb[2] + c3});
+            }
+    std::sort(keys.begin(), keys.end());
+    std::vector<int> out;
+    for (auto &k : keys) out.push_back(k.second);
+    return out;
+}
+
+uint64_t hash64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+// ---------------------------------------------------------------------------
+GridCoords grid_coords(const GridDesc &d)
+{
+    GridCoords g;
+    const int N1 = d.nb[0] * d.bs[0], N2 = d.nb[1] * d.bs[1], N3 = d.nb[2] * d.bs[2];
+    if (d.minkowski) {
+        for (int k = 0; k < 3; k++) g.startx[k] = 0.;
+        g.dx[0] = 1. / N1;
+        g.dx[1] = 1. / N2;
+        g.dx[2] = 1. / N3;
+        g.rin = g.rout = 0.;
+        g.fractheta = 1.;
+        return g;
+    }
+    g.rin = (d.rin > 0.) ? d.rin : 0.9 * (1. + std::sqrt(1. - d.a * d.a));
+    g.rout = d.rout;
+    g.fractheta = 1.0 - 2.0 / ((double)N2);
+    g.startx[0] = std::log(g.rin);
+    g.startx[1] = -1. + 1. * (1. - g.fractheta);
+    g.startx[2] = 0.;
+    g.dx[0] = (std::log(g.rout) - std::log(g.rin)) / (double)N1;
+    g.dx[1] = 2. * g.fractheta / (double)N2;
+    g.dx[2] = 2. * M_PI / (double)N3;
+    return g;
+}
+
+// vofx_matthewcoords (coord.c:147) with the uniform-theta parameters, plus the
+// axis excision of bl_coord (COORDSINGFIX, SINGSMALL = 1e-20).
+void bl_coord(const GridDesc &, const double *X, double *r, double *th)
+{
+    const double A1 = 1., A2 = 1., A3 = 1.;
+    double V1 = std::exp(X[1]);
+    double sign = 1., X_2 = (X[2] + 1.0) / 2.0, Xc = std::sqrt(X_2 * X_2), V2;
+    if (X_2 < 0.0) sign = -1.;
+    if (X_2 > 1.0) { sign = -1.; Xc = 2. - Xc; }
+    if (X_2 >= 0.5) {
+        Xc = 1. - Xc;
+        V2 = M_PI - sign * (A1 * M_PI * Xc + M_PI * (1. - A1) * (A3 * std::pow(Xc, A2) + 0.50 / M_PI * std::sin(M_PI + 2. * M_PI * (A3 * std::pow(Xc, A2)))));
+    } else {
+        V2 = sign * (A1 * M_PI * Xc + M_PI * (1. - A1) * (A3 * std::pow(Xc, A2) + 0.50 / M_PI * std::sin(M_PI + 2. * M_PI * (A3 * std::pow(Xc, A2)))));
+    }
+    if (std::fabs(V2) < 1e-20) V2 = (V2 >= 0.0) ? 1e-20 : -1e-20;
+    if (std::fabs(M_PI - V2) < 1e-20) V2 = (V2 >= M_PI) ? M_PI + 1e-20 : M_PI - 1e-20;
+    *r = V1;
+    *th = V2;
+}
+
+static void ks_gcov(double a, double r, double th, double g[4][4])
+{
+    double cth = std::cos(th), sth = std::sin(th), s2 = sth * sth, rho2 = r * r + a * a * cth * cth;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) g[i][j] = 0.;
+    g[0][0] = (-1. + 2. * r / rho2);
+    g[0][1] = (2. * r / rho2);
+    g[0][3] = (-2. * a * r * s2 / rho2);
+    g[1][0] = g[0][1];
+    g[1][1] = (1. + 2. * r / rho2);
+    g[1][3] = (-a * s2 * (1. + 2. * r / rho2));
+    g[2][2] = rho2;
+    g[3][0] = g[0][3];
+    g[3][1] = g[1][3];
+    g[3][3] = s2 * (rho2 + a * a * s2 * (1. + 2. * r / rho2));
+}
+
+static void dxdxp_fd(const GridDesc &d, const double *X, double J[4][4])
+{
+    for (int k = 0; k < 4; k++) {
+        double Xh[4], Xl[4], Vh[4], Vl[4];
+        for (int l = 0; l < 4; l++) Xh[l] = Xl[l] = X[l];
+        Xh[k] += 0.00001;
+        Xl[k] -= 0.00001;
+        Vh[0] = Xh[0];
+        Vl[0] = Xl[0];
+        bl_coord(d, Xh, &Vh[1], &Vh[2]);
+        bl_coord(d, Xl, &Vl[1], &Vl[2]);
+        Vh[3] = Xh[3];
+        Vl[3] = Xl[3];
+        for (int j = 0; j < 4; j++) J[j][k] = (Vh[j] - Vl[j]) / (Xh[k] - Xl[k]);
+    }
+}
+
+void metric_code(const GridDesc &d, const double *X, double gp[4][4])
+{
+    if (d.minkowski) {
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) gp[i][j] = (i == j) ? (i == 0 ? -1. : 1.) : 0.;
+        return;
+    }
+    double r, th, g[4][4], J[4][4];
+    bl_coord(d, X, &r, &th);
+    ks_gcov(d.a, r, th, g);
+    dxdxp_fd(d, X, J);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            gp[i][j] = 0.;
+            for (int k = 0; k < 4; k++)
+                for (int l = 0; l < 4; l++) gp[i][j] += g[k][l] * J[k][i] * J[l][j];
+        }
+}
+
+// LU decomposition with partial pivoting (Doolittle, in place); returns 1 if singular.
+static int lu4(double a[4][4], int perm[4], double *sign)
+{
+    *sign = 1.;
+    for (int i = 0; i < 4; i++) perm[i] = i;
+    for (int k = 0; k < 4; k++) {
+        int p = k;
+        double big = std::fabs(a[k][k]);
+        for (int i = k + 1; i < 4; i++)
+            if (std::fabs(a[i][k]) > big) { big = std::fabs(a[i][k]); p = i; }
+        if (big == 0.) return 1;
+        if (p != k) {
+            for (int j = 0; j < 4; j++) std::swap(a[k][j], a[p][j]);
+            std::swap(perm[k], perm[p]);
+            *sign = -*sign;
+        }
+        for (int i = k + 1; i < 4; i++) {
+            a[i][k] /= a[k][k];
+            for (int j = k + 1; j < 4; j++) a[i][j] -= a[i][k] * a[k][j];
+        }
+    }
+    return 0;
+}
+
+double gdet_of(const double g[4][4])
+{
+    double a[4][4], s;
+    int perm[4];
+    std::memcpy(a, g, sizeof a);
+    if (lu4(a, perm, &s)) return 0.;
+    double det = s;
+    for (int i = 0; i < 4; i++) det *= a[i][i];
+    return std::sqrt(std::fabs(det));
+}
+
+static void lu_solve4(const double a[4][4], const int perm[4], const double *b, double *x)
+{
+    double y[4];
+    for (int i = 0; i < 4; i++) {
+        double v = b[perm[i]];
+        for (int j = 0; j < i; j++) v -= a[i][j] * y[j];
+        y[i] = v;
+    }
+    for (int i = 3; i >= 0; i--) {
+        double v = y[i];
+        for (int j = i + 1; j < 4; j++) v -= a[i][j] * x[j];
+        x[i] = v / a[i][i];
+    }
+}
+
+// Inverse by LU with one step of iterative refinement (the metric's entries
+// span several decades at large r).
+int invert4(const double g[4][4], double inv[4][4])
+{
+    double a[4][4], s;
+    int perm[4];
+    std::memcpy(a, g, sizeof a);
+    if (lu4(a, perm, &s)) return 1;
+    for (int c = 0; c < 4; c++) {
+        double e[4] = {0., 0., 0., 0.}, x[4], r[4], dx[4];
+        e[c] = 1.;
+        lu_solve4(a, perm, e, x);
+        for (int i = 0; i < 4; i++) {
+            double v = e[i];
+            for (int j = 0; j < 4; j++) v -= g[i][j] * x[j];
+            r[i] = v;
+        }
+        lu_solve4(a, perm, r, dx);
+        for (int i = 0; i < 4; i++) inv[i][c] = x[i] + dx[i];
+    }
+    return 0;
+}
+
+void conn_code(const GridDesc &d, const double *X, const double gcon[4][4], double conn[4][4][4])
+{
+    double c[4][4][4], tmp[4][4][4];
+    for (int k = 0; k < 4; k++) {
+        double Xh[4], Xl[4], gh[4][4], gl[4][4];
+        for (int l = 0; l < 4; l++) Xh[l] = Xl[l] = X[l];
+        Xh[k] += 1.e-5;
+        Xl[k] -= 1.e-5;
+        metric_code(d, Xh, gh);
+        metric_code(d, Xl, gl);
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) c[i][j][k] = (gh[i][j] - gl[i][j]) / (Xh[k] - Xl[k]);
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 4; k++) tmp[i][j][k] = 0.5 * (c[j][i][k] + c[k][i][j] - c[k][j][i]);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 4; k++) {
+                conn[i][j][k] = 0.;
+                for (int l = 0; l < 4; l++) conn[i][j][k] += gcon[i][l] * tmp[l][j][k];
+            }
+}
+
+void coord_of(const GridDesc &d, const GridCoords &gc, int i, int j, int pos, double X[4])
+{
+    const int N2 = d.nb[1] * d.bs[1];
+    X[0] = 0.;
+    X[3] = gc.startx[2] + 0.5 * gc.dx[2];
+    if (d.minkowski) {
+        X[1] = gc.startx[0] + ((pos == FACE1 || pos == CORN) ? i : i + 0.5) * gc.dx[0];
+        X[2] = gc.startx[1] + ((pos == FACE2 || pos == CORN) ? j : j + 0.5) * gc.dx[1];
+        return;
+    }
+    int jl = j;
+    if (j < 0) jl = -j - 1;
+    if (j >= N2) jl = 2 * N2 - 1 - j;
+    if (j == N2 && pos == FACE2) jl = j;
+    if (pos == FACE1) { X[1] = gc.startx[0] + i * gc.dx[0]; X[2] = gc.startx[1] + (jl + 0.5) * gc.dx[1]; }
+    else if (pos == FACE2) { X[1] = gc.startx[0] + (i + 0.5) * gc.dx[0]; X[2] = gc.startx[1] + jl * gc.dx[1]; }
+    else if (pos == CORN) { X[1] = gc.startx[0] + i * gc.dx[0]; X[2] = gc.startx[1] + jl * gc.dx[1]; }
+    else { X[1] = gc.startx[0] + (i + 0.5) * gc.dx[0]; X[2] = gc.startx[1] + (jl + 0.5) * gc.dx[1]; }
+    if (j < 0) {
+        X[2] = X[2] + 1;
+        X[2] = -X[2];
+        X[2] = X[2] - 1;
+    } else if (j == N2 && pos == FACE2) {
+    } else if (j >= N2) {
+        X[2] = X[2] + 1;
+        X[2] = 4. - X[2];
+        X[2] = X[2] - 1;
+    }
+}
+
+static void pack10(const double g[4][4], double *o, int64_t stride)
+{
+    static const int pi[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, pj[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+    for (int s = 0; s < 10; s++) o[s * stride] = g[pi[s]][pj[s]];
+}
+
+void fill_geometry(const GridDesc &d, const Layout &L, int c1, int c2, double *gcov, double *gcon,
+                   double *gdet, double *conn, double *rcen)
+{
+    GridCoords gc = grid_coords(d);
+    static const int pi[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, pj[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+    for (int il = 0; il < L.D[0]; il++)
+        for (int jl = 0; jl < L.D[1]; jl++) {
+            const int i = c1 * d.bs[0] + il - NG, j = c2 * d.bs[1] + jl - NG;
+            const int64_t col = (int64_t)il * L.D[1] + jl;
+            for (int pos = 0; pos < NPOS; pos++) {
+                double X[4], g[4][4], gi[4][4];
+                coord_of(d, gc, i, j, pos, X);
+                metric_code(d, X, g);
+                invert4(g, gi);
+                pack10(g, gcov + ((int64_t)pos * 10) * L.kcol + col, L.kcol);
+                pack10(gi, gcon + ((int64_t)pos * 10) * L.kcol + col, L.kcol);
+                gdet[(int64_t)pos * L.kcol + col] = gdet_of(g);
+                if (pos == CENT) {
+                    double cn[4][4][4], r, th;
+                    conn_code(d, X, gi, cn);
+                    for (int lam = 0; lam < 4; lam++)
+                        for (int s = 0; s < 10; s++) conn[((int64_t)lam * 10 + s) * L.kcol + col] = cn[lam][pi[s]][pj[s]];
+                    if (d.minkowski) rcen[col] = 1.;
+                    else {
+                        bl_coord(d, X, &r, &th);
+                        rcen[col] = r;
+                    }
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Fishbone-Moncrief torus
+static double lfish_calc(double a, double r)
+{
+    return (((a * a - 2. * a * std::sqrt(r) + r * r) *
+             ((-2. * a * r * (a * a - 2. * a * std::sqrt(r) + r * r)) / std::sqrt(2. * a * std::sqrt(r) + (-3. + r) * r) +
+              ((a + (-2. + r) * std::sqrt(r)) * (r * r * r + a * a * (2. + r))) / std::sqrt(1 + (2. * a) / std::pow(r, 1.5) - 3. / r))) /
+            (r * r * r * std::sqrt(2. * a * std::sqrt(r) + (-3. + r) * r) * (a * a + (-2. + r) * r)));
+}
+
+struct TorusEval {
+    double a, gam, rin, kappa, l, lnh_in;
+};
+
+static double lnh_at(const TorusEval &T, double r, double th)
+{
+    const double a = T.a, l = T.l;
+    double sth = std::sin(th), cth = std::cos(th);
+    double DD = r * r - 2. * r + a * a;
+    double AA = (r * r + a * a) * (r * r + a * a) - DD * a * a * sth * sth;
+    double SS = r * r + a * a * cth * cth;
+    return 0.5 * std::log((1. + std::sqrt(1. + 4. * (l * l * SS * SS) * DD / (AA * sth * AA * sth))) / (SS * DD / AA)) -
+           0.5 * std::sqrt(1. + 4. * (l * l * SS * SS) * DD / (AA * AA * sth * sth)) - 2. * a * r * l / AA - T.lnh_in;
+}
+
+static TorusEval make_torus(const GridDesc &d, const TorusDesc &t)
+{
+    TorusEval T;
+    T.a = d.a;
+    T.gam = d.gam;
+    T.rin = t.rin;
+    T.kappa = t.kappa;
+    T.l = lfish_calc(d.a, t.rmax);
+    T.lnh_in = 0.;
+    T.lnh_in = lnh_at(T, t.rin, M_PI / 2.);
+    return T;
+}
+
+// torus density (unnormalised) at BL (r, th); negative outside the torus
+static double torus_rho(const TorusEval &T, double r, double th, double *u_out)
+{
+    double lnh = (r >= T.rin) ? lnh_at(T, r, th) : 1.;
+    if (lnh < 0. || r < T.rin) {
+        if (u_out) *u_out = 0.;
+        return -1.;
+    }
+    double hm1 = std::exp(lnh) - 1.;
+    double rho = std::pow(hm1 * (T.gam - 1.) / (T.kappa * T.gam), 1. / (T.gam - 1.));
+    if (u_out) *u_out = T.kappa * std::pow(rho, T.gam) / (T.gam - 1.);
+    return rho;
+}
+
+// BL u^phi of the torus -> relative 4-velocity in code coordinates (coord_transform)
+static void torus_velocity(const GridDesc &d, const TorusEval &T, const double *X, double r, double th,
+                           const double gcov[4][4], const double gcon[4][4], double *utcon)
+{
+    const double a = T.a, l = T.l;
+    double sth = std::sin(th), cth = std::cos(th);
+    double DD = r * r - 2. * r + a * a;
+    double AA = (r * r + a * a) * (r * r + a * a) - DD * a * a * sth * sth;
+    double SS = r * r + a * a * cth * cth;
+    double expm2chi = SS * SS * DD / (AA * AA * sth * sth);
+    double up1 = std::sqrt((-1. + std::sqrt(1. + 4. * l * l * expm2chi)) / 2.);
+    double up = 2. * a * r * std::sqrt(1. + up1 * up1) / std::sqrt(AA * SS * DD) + std::sqrt(SS / AA) * up1 / sth;
+    // Boyer-Lindquist metric
+    double rho2 = SS, s2 = sth * sth;
+    double bg00 = -(1. - 2. * r / rho2), bg03 = -2. * a * r * s2 / rho2;
+    double bg33 = s2 * (r * r + a * a + 2. * a * a * r * s2 / rho2);
+    double ucon[4] = {0., 0., 0., up};
+    double Aq = bg00, Bq = 2. * (bg03 * up), Cq = 1. + bg33 * up * up;
+    double discr = Bq * Bq - 4. * Aq * Cq;
+    ucon[0] = (-Bq - std::sqrt(discr)) / (2. * Aq);
+    // BL -> KS (u^r = 0, so only u^t and u^phi pick up terms that vanish here)
+    double tmp[4] = {ucon[0] + 2. * r / DD * ucon[1], ucon[1], ucon[2], ucon[3] + a / DD * ucon[1]};
+    // KS -> code coordinates
+    double J[4][4], Ji[4][4], uc[4];
+    dxdxp_fd(d, X, J);
+    invert4(J, Ji);
+    for (int i = 0; i < 4; i++) {
+        uc[i] = 0.;
+        for (int j = 0; j < 4; j++) uc[i] += Ji[i][j] * tmp[j];
+    }
+    double alpha = 1. / std::sqrt(-gcon[0][0]);
+    double gamma = alpha * uc[0];
+    for (int i = 1; i < 4; i++) {
+        double beta = gcon[0][i] * alpha * alpha;
+        utcon[i] = uc[i] + gamma * beta / alpha;
+    }
+    (void)gcov;
+}
+
+std::string init_torus(const GridDesc &d, const Layout &L, const std::vector<BlockDev> &blocks,
+                       const TorusDesc &t, const double *gcovA, const double *gconA, const double *gdetA,
+                       double *p, double *ps)
+{
+    if (d.minkowski) return "init_torus: grid is a Minkowski box";
+    const GridCoords gc = grid_coords(d);
+    const TorusEval T = make_torus(d, t);
+    const int N2 = d.nb[1] * d.bs[1], N3 = d.nb[2] * d.bs[2];
+    const int D1 = L.D[0], D2 = L.D[1], D3 = L.D[2];
+    auto cell = [&](int i, int j, int z) { return ((int64_t)i * D2 + j) * D3 + z; };
+    auto P = [&](int b, int k) { return p + ((int64_t)b * NPR + k) * L.ksize; };
+    auto PS = [&](int b, int k) { return ps + ((int64_t)b * 3 + k) * L.ksize; };
+    auto metric_at = [&](int geo, int pos, int64_t col, double g[4][4], double gi[4][4], double *gd) {
+        static const int pi[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, pj[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+        for (int s = 0; s < 10; s++) {
+            g[pi[s]][pj[s]] = g[pj[s]][pi[s]] = gcovA[(((int64_t)geo * NPOS + pos) * 10 + s) * L.kcol + col];
+            gi[pi[s]][pj[s]] = gi[pj[s]][pi[s]] = gconA[(((int64_t)geo * NPOS + pos) * 10 + s) * L.kcol + col];
+        }
+        if (gd) *gd = gdetA[((int64_t)geo * NPOS + pos) * L.kcol + col];
+    };
+    (void)N2;
+    // pass 1: densities and internal energies, rho_max
+    double rhomax = 0.;
+    const int nblk = L.nblk;
+    for (int b = 0; b < nblk; b++) {
+        const BlockDev &B = blocks[b];
+        for (int il = NG; il < NG + d.bs[0]; il++)
+            for (int jl = NG; jl < NG + d.bs[1]; jl++) {
+                const int i = B.c1 * d.bs[0] + il - NG, j = B.c2 * d.bs[1] + jl - NG;
+                double X[4], r, th, u;
+                coord_of(d, gc, i, j, CENT, X);
+                bl_coord(d, X, &r, &th);
+                double rho = torus_rho(T, r, th, &u);
+                for (int zl = NG; zl < NG + d.bs[2]; zl++) {
+                    const int64_t c = cell(il, jl, zl);
+                    P(b, RHO)[c] = rho;
+                    P(b, UU)[c] = u;
+                }
+                rhomax = std::max(rhomax, rho);
+            }
+    }
+    if (!(rhomax > 0.)) return "init_torus: the torus does not intersect the grid";
+    // pass 2: velocities, perturbation, normalisation, atmosphere
+    for (int b = 0; b < nblk; b++) {
+        const BlockDev &B = blocks[b];
+        for (int il = NG; il < NG + d.bs[0]; il++)
+            for (int jl = NG; jl < NG + d.bs[1]; jl++) {
+                const int i = B.c1 * d.bs[0] + il - NG, j = B.c2 * d.bs[1] + jl - NG;
+                const int64_t col = (int64_t)il * D2 + jl;
+                double X[4], r, th, g[4][4], gi[4][4], ut[4] = {0., 0., 0., 0.};
+                coord_of(d, gc, i, j, CENT, X);
+                bl_coord(d, X, &r, &th);
+                metric_at(B.geo, CENT, col, g, gi, nullptr);
+                const double rho_t = P(b, RHO)[cell(il, jl, NG)];
+                const bool in = rho_t > 0.;
+                if (in) torus_velocity(d, T, X, r, th, g, gi, ut);
+                const double sr = std::sqrt(r);
+                const double rhoflr = d.rho_floor / (r * sr), uflr = d.u_floor / (r * r * sr);
+                for (int zl = NG; zl < NG + d.bs[2]; zl++) {
+                    const int64_t c = cell(il, jl, zl);
+                    const int zg = B.c3 * d.bs[2] + zl - NG;
+                    double rho, u;
+                    if (in) {
+                        const uint64_t gid = ((uint64_t)(i + 8) * (uint64_t)(N2 + 16) + (uint64_t)(j + 8)) * (uint64_t)(N3 + 16) + (uint64_t)zg;
+                        const double U01 = (double)(hash64(t.seed ^ (gid * 0x9E3779B97F4A7C15ULL)) >> 11) * (1.0 / 9007199254740992.0);
+                        rho = P(b, RHO)[c] / rhomax;
+                        u = P(b, UU)[c] * (1. + t.perturb * (U01 - 0.5)) / rhomax;
+                    } else {
+                        rho = 0.;
+                        u = 0.;
+                    }
+                    if (!(rho >= rhoflr)) rho = rhoflr;
+                    if (!(u >= uflr)) u = uflr;
+                    P(b, RHO)[c] = rho;
+                    P(b, UU)[c] = u;
+                    P(b, U1)[c] = in ? ut[1] : 0.;
+                    P(b, U2)[c] = in ? ut[2] : 0.;
+                    P(b, U3)[c] = in ? ut[3] : 0.;
+                    P(b, KTOT)[c] = (d.gam - 1.) * u * std::pow(rho, -d.gam);
+                }
+            }
+    }
+    // pass 3: vector potential on corners and face fields (unnormalised)
+    auto aphi = [&](int i, int j) {
+        double X[4], r, th;
+        coord_of(d, gc, i, j, CORN, X);
+        bl_coord(d, X, &r, &th);
+        double rho = torus_rho(T, r, th, nullptr);
+        if (rho < 0.) return 0.;
+        return std::max(rho / rhomax - t.aphi_cut, 0.);
+    };
+    for (int b = 0; b < nblk; b++) {
+        const BlockDev &B = blocks[b];
+        for (int il = NG; il <= NG + d.bs[0]; il++)
+            for (int jl = NG; jl <= NG + d.bs[1]; jl++) {
+                const int i = B.c1 * d.bs[0] + il - NG, j = B.c2 * d.bs[1] + jl - NG;
+                const int64_t col = (int64_t)il * D2 + jl;
+                const double A00 = aphi(i, j), A01 = aphi(i, j + 1), A10 = aphi(i + 1, j);
+                const double gF1 = gdetA[((int64_t)B.geo * NPOS + FACE1) * L.kcol + col];
+                const double gF2 = gdetA[((int64_t)B.geo * NPOS + FACE2) * L.kcol + col];
+                const double b1 = (A01 - A00) / (gc.dx[1] * gF1);
+                const double b2 = -(A10 - A00) / (gc.dx[0] * gF2);
+                for (int zl = NG; zl <= NG + d.bs[2]; zl++) {
+                    const int64_t c = cell(il, jl, zl);
+                    if (jl < NG + d.bs[1] && zl < NG + d.bs[2]) PS(b, 0)[c] = b1;
+                    if (il < NG + d.bs[0] && zl < NG + d.bs[2]) PS(b, 1)[c] = b2;
+                    if (il < NG + d.bs[0] && jl < NG + d.bs[1]) PS(b, 2)[c] = 0.;
+                }
+            }
+    }
+    // cell-centred field from the faces, as the update kernel forms it
+    auto centre_field = [&](int b) {
+        const BlockDev &B = blocks[b];
+        for (int il = NG; il < NG + d.bs[0]; il++)
+            for (int jl = NG; jl < NG + d.bs[1]; jl++) {
+                const int64_t col = (int64_t)il * D2 + jl;
+                const double *gd = gdetA + (int64_t)B.geo * NPOS * L.kcol;
+                const double gF1a = gd[FACE1 * L.kcol + col], gF1b = gd[FACE1 * L.kcol + col + D2];
+                const double gF2a = gd[FACE2 * L.kcol + col], gF2b = gd[FACE2 * L.kcol + col + 1];
+                const double gF3 = gd[FACE3 * L.kcol + col], gC = gd[CENT * L.kcol + col];
+                for (int zl = NG; zl < NG + d.bs[2]; zl++) {
+                    const int64_t c = cell(il, jl, zl);
+                    const int64_t si = (int64_t)D2 * D3, sj = D3;
+                    double U1f = (PS(b, 0)[c] * gF1a + PS(b, 0)[c + si] * gF1b) / 2.0;
+                    double U2f = (PS(b, 1)[c] * gF2a + PS(b, 1)[c + sj] * gF2b) / 2.0;
+                    double U3f = (PS(b, 2)[c] * gF3 + PS(b, 2)[c + 1] * gF3) / 2.0;
+                    P(b, B1)[c] = U1f / gC;
+                    P(b, B2)[c] = U2f / gC;
+                    P(b, B3)[c] = U3f / gC;
+                }
+            }
+    };
+    for (int b = 0; b < nblk; b++) centre_field(b);
+    // pass 4: normalise min beta over field-carrying cells
+    double beta_min = 1e300;
+    for (int b = 0; b < nblk; b++) {
+        const BlockDev &B = blocks[b];
+        for (int il = NG; il < NG + d.bs[0]; il++)
+            for (int jl = NG; jl < NG + d.bs[1]; jl++) {
+                const int64_t col = (int64_t)il * D2 + jl;
+                double g[4][4], gi[4][4];
+                metric_at(B.geo, CENT, col, g, gi, nullptr);
+                Geom<double> G;
+                static const int pi[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, pj[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+                for (int s = 0; s < 10; s++) { G.gcov[s] = g[pi[s]][pj[s]]; G.gcon[s] = gi[pi[s]][pj[s]]; }
+                G.g = 1.;
+                for (int zl = NG; zl < NG + d.bs[2]; zl++) {
+                    const int64_t c = cell(il, jl, zl);
+                    double pr[NPR];
+                    for (int k = 0; k < NPR; k++) pr[k] = P(b, k)[c];
+                    State<double> q;
+                    get_state(pr, G, q);
+                    double bsq = dot4(q.bcon, q.bcov);
+                    if (bsq > 0.) beta_min = std::min(beta_min, (d.gam - 1.) * pr[UU] / (0.5 * bsq));
+                }
+            }
+    }
+    if (beta_min < 1e299 && t.beta_min > 0.) {
+        const double norm = std::sqrt(beta_min / t.beta_min);
+        for (int b = 0; b < nblk; b++)
+            for (int k = 0; k < 3; k++) {
+                double *f = PS(b, k);
+                for (int64_t c = 0; c < L.ncell; c++) f[c] *= norm;
+            }
+        for (int b = 0; b < nblk; b++) centre_field(b);
+    }
+    return "";
+}
+
+}  // namespace hamr

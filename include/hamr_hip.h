@@ -1,0 +1,171 @@
+/*
+ * hamr_hip.h -- C ABI of the MI355X GRMHD fluid step (libhamr_hip.so).
+ *
+ * A drop-in for the GPU launch layer that H-AMR's step_ch.c drives block by
+ * block (GPU_main.cu of tp2872/HAMR_gamma_wind), plus a batched step over the
+ * whole resident block set.  Plain C types only; every call returns 0 on
+ * success and a negative code on error, with the message in hamr_last_error().
+ * All device work is queued on the context's stream (asynchronous); calls that
+ * return host data synchronise.  One host thread per context; not re-entrant.
+ *
+ * Conventions (as the reference's): NPR = 9 primitives RHO, UU, U1, U2, U3, B1,
+ * B2, B3, KTOT (config.h:819-827); NG = 3 ghost zones; cell index
+ * (i*(BS2+6) + j)*(BS3+6) + z with ghosts, z fastest (index_3D, main.c:355);
+ * face fields ps[0..2] = B^1 at FACE1, B^2 at FACE2, B^3 at FACE3 (not
+ * gdet-weighted); flag 0 = half (predictor) step reading p/ps and writing
+ * ph/psh, flag 1 = full (corrector) step reading ph/psh and writing p/ps.
+ */
+#ifndef HAMR_HIP_H
+#define HAMR_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hamr_ctx hamr_ctx;
+
+/* The run description.  Replaces the compile-time block geometry and physics
+ * constants of config.h (BS_1..3, NB_1..3, GAMMA, BH_SPIN, COUR, ROUT;
+ * config.h:102-105, :355-362, :532) with a runtime descriptor. */
+typedef struct hamr_desc {
+    int bs[3];          /* block interior cells per direction */
+    int nb[3];          /* blocks per direction on the base level */
+    double gam;         /* adiabatic index */
+    double spin;        /* black-hole spin a */
+    double cour;        /* Courant factor */
+    double rin;         /* inner radius (<= 0: 0.9 (1 + sqrt(1 - a^2)), coord.c:364) */
+    double rout;        /* outer radius */
+    double rho_floor;   /* rho >= rho_floor r^-1.5 (0: off) */
+    double u_floor;     /* u   >= u_floor   r^-2.5 (0: off) */
+    int minkowski;      /* 1: flat, periodic unit box instead of the Kerr-Schild sphere */
+    int device;         /* HIP device ordinal */
+} hamr_desc;
+
+/* Fishbone-Moncrief torus with a poloidal loop (init.c:808-1110, init_mag.c:6). */
+typedef struct hamr_torus_desc {
+    double rin, rmax, kappa;  /* inner edge, pressure maximum, polytropic constant */
+    double beta_min;          /* field normalisation: min over cells of p_gas/p_mag */
+    double aphi_cut;          /* A_phi ~ max(rho/rho_max - aphi_cut, 0) */
+    double perturb;           /* u *= 1 + perturb (U[0,1) - 1/2), seeded */
+    unsigned long long seed;
+} hamr_torus_desc;
+
+/* Array selectors of hamr_get_array / hamr_set_array (all blocks at once,
+ * layout [block][component][cell] with ncell = (BS1+6)(BS2+6)(BS3+6) per slab). */
+enum {
+    HAMR_P = 0,     /* primitives, 9 components */
+    HAMR_PS = 1,    /* face fields, 3 */
+    HAMR_PH = 2,    /* half-step primitives, 9 */
+    HAMR_PSH = 3,   /* half-step face fields, 3 */
+    HAMR_F1 = 4,    /* fluxes through the lower x1 faces, 9 */
+    HAMR_F2 = 5,
+    HAMR_F3 = 6,
+    HAMR_EMF = 7,   /* corner EMF E_1, E_2, E_3 */
+    HAMR_ECENT = 8, /* cell-centre EMF, 3 */
+    HAMR_U0 = 9     /* conserved variables at the start of the step, 9 */
+};
+
+/* ---- host-only set-up (no device needed; for drivers and tests) ---- */
+/* Layout of a descriptor: D[3], cells per slab, padded slab length, metric
+ * columns per slab and padded length, block and geometry counts. */
+int hamr_desc_layout(const hamr_desc *desc, int *D, long long *ncell, long long *ksize, long long *ncol,
+                     long long *kcol, int *nblk, int *ngeo);
+/* Block table rows (17 ints per block, as hamr_block_row) and the Z-order. */
+int hamr_desc_block_table(const hamr_desc *desc, int *rows, int *order);
+/* Grid coordinates startx[3], dx[3] (set_points / set_gridparam, coord.c:327-460). */
+int hamr_desc_coords(const hamr_desc *desc, double *startx, double *dx);
+/* Metric at code coordinates X[4] (gcov_func, gdet_func, gcon_func, conn_func,
+ * metric.c:13-500): packed gcov[10], gcon[10], gdet, conn[40] = Gamma^lam of
+ * pair s at lam*10+s, and Boyer-Lindquist r, theta (bl_coord, coord.c:111). */
+int hamr_desc_metric_at(const hamr_desc *desc, const double *X, double *gcov, double *gcon, double *gdet,
+                        double *conn, double *r_th);
+/* One geometry slab in the device layout (as hamr_upload_metric takes it). */
+int hamr_desc_geometry(const hamr_desc *desc, int geo, double *gcov, double *gcon, double *gdet, double *conn,
+                       double *rcen);
+/* Initial torus state, interior cells only, [block][component][ncell]. */
+int hamr_desc_torus(const hamr_desc *desc, const hamr_torus_desc *t, double *p, double *ps);
+
+/* GPU_init (GPU_main.cu:10) + set_AMR (AMR.c:473): allocate the resident block
+ * set and the block table. */
+int hamr_create(const hamr_desc *desc, hamr_ctx **out);
+int hamr_destroy(hamr_ctx *ctx);
+const char *hamr_last_error(const hamr_ctx *ctx);
+
+/* Layout: D[3] = BS+6 extents, ncell, padded slab length, block and geometry counts. */
+int hamr_layout(const hamr_ctx *ctx, int *D, long long *ncell, long long *ksize, int *nblk, int *ngeo);
+/* Block table row (AMR_coord / neighbour / pole entries of block[n][*], AMR.c:168-757):
+ * row[0] geometry slab, [1..3] block coordinates c1 c2 c3, [4] pole bits
+ * (1: lower x2 face on the axis, 2: upper), [5..10] neighbour (or axis partner) of
+ * faces -x1 +x1 -x2 +x2 -x3 +x3 (-1: physical outflow edge), [11..16] face kinds
+ * (0 copy, 1 outflow, 2 polar mirror). */
+int hamr_block_row(const hamr_ctx *ctx, int n, int *row17);
+/* Base-level Z-order of the blocks (n_ord). */
+int hamr_block_order(const hamr_ctx *ctx, int *order);
+/* set_grid (main.c:380) + GPU_write_metric (GPU_main.cu:139): compute the
+ * Kerr-Schild metric in code coordinates on the host and upload it. */
+int hamr_set_grid(hamr_ctx *ctx);
+/* Metric slab of geometry `geo` in the device layout: gcov/gcon[(pos*10+s)*kcol+col],
+ * gdet[pos*kcol+col], conn[(lam*10+s)*kcol+col], rcen[col]; kcol = ncol padded to 64. */
+int hamr_metric_sizes(const hamr_ctx *ctx, long long *ncol, long long *kcol);
+int hamr_upload_metric(hamr_ctx *ctx, int geo, const double *gcov, const double *gcon, const double *gdet,
+                       const double *conn, const double *rcen);
+int hamr_download_metric(hamr_ctx *ctx, int geo, double *gcov, double *gcon, double *gdet, double *conn,
+                         double *rcen);
+/* init_torus (init.c:808) + set_mag (init_mag.c:6): initial state, uploaded
+ * to p/ps (interior only; call hamr_boundprim(ctx, 1) to fill ghosts). */
+int hamr_init_torus(hamr_ctx *ctx, const hamr_torus_desc *t);
+
+/* GPU_write / GPU_read (GPU_main.cu:309, :2646): one block's state in the
+ * reference's host layout p[cell][NPR], ps[cell][3] (ghost-inclusive index_3D). */
+int hamr_upload(hamr_ctx *ctx, int n, const double *p_aos, const double *ps_aos);
+int hamr_download(hamr_ctx *ctx, int n, double *p_aos, double *ps_aos);
+/* Whole arrays in [block][component][cell] order (see HAMR_* selectors). */
+int hamr_set_array(hamr_ctx *ctx, int which, const double *src);
+int hamr_get_array(hamr_ctx *ctx, int which, double *dst);
+
+/* ---- per-block stages, the call sequence of advance_GPU (step_ch.c:997-1336) ---- */
+/* GPU_fluxcalc2D (GPU_main.cu:560): PPM + HLL flux through the lower dir-faces
+ * of block n; records the block's Courant dt for hamr_block_dt. */
+int hamr_fluxcalc(hamr_ctx *ctx, int dir, int flag, int n);
+/* fluxcalc_GPU (GPU_main.cu:1091): the block's minimum cour*dx/ctop of the last
+ * fluxcalc in direction dir (1e9 if none); blocking; resets the minimum. */
+int hamr_block_dt(hamr_ctx *ctx, int n, int dir, double *out);
+/* GPU_consttransport1 + GPU_consttransport2 (GPU_main.cu:1365, :1393). */
+int hamr_ct_emf(hamr_ctx *ctx, int flag, int n);
+/* GPU_consttransport_bound (GPU_main.cu:1144): polar E1 average over each ring
+ * of phi-blocks; dt is the step's dt (the reference's scale factor). */
+int hamr_ct_bound(hamr_ctx *ctx, double dt);
+/* GPU_consttransport3 (GPU_main.cu:1535): face-B update with step Dt. */
+int hamr_ct_update(hamr_ctx *ctx, int flag, double Dt, int n);
+/* GPU_fixup (GPU_main.cu:1740): conserved update + Noble cascade + floors. */
+int hamr_update(hamr_ctx *ctx, int flag, int n, double Dt);
+/* GPU_boundprim (GPU_main.cu:2113): ghost fill of all blocks; flag 0 fills
+ * ph/psh, flag 1 fills p/ps. */
+int hamr_boundprim(hamr_ctx *ctx, int flag);
+/* pflag (ghost-inclusive, per cell) and failimage counts of block n:
+ * [0] 2D failures, [1] 1D-v^2 failures, [2] 1D failures, [3] floor hits. */
+int hamr_flags(hamr_ctx *ctx, int n, int *pflag, int *failimage);
+/* Zero the failimage counters of all blocks. */
+int hamr_clear_flags(hamr_ctx *ctx);
+
+/* ---- batched step (GPU_step_ch, step_ch.c:916, in single-timelevel form) ---- */
+/* Courant dt of the current state p/ps (per-block harmonic, min over blocks). */
+int hamr_estimate_dt(hamr_ctx *ctx, double *dt);
+/* nsteps predictor+corrector steps over all blocks with the dt kept on the
+ * device (next dt = min over blocks, clamped to 1.3 dt).  *dt_inout: the first
+ * step's dt on input (<= 0: keep the device value), the next dt on output. */
+int hamr_step(hamr_ctx *ctx, int nsteps, double *dt_inout);
+/* Capture one step into a HIP graph and replay it nsteps times (same result as
+ * hamr_step). */
+int hamr_step_graph(hamr_ctx *ctx, int nsteps, double *dt_inout);
+int hamr_sync(hamr_ctx *ctx);
+/* Average device time (ms, HIP events on the context stream) of one launch of
+ * a stage over all blocks, with predictor-step operands (p and ps are not
+ * modified): 0-2 flux x1..x3, 3 EMF, 4 polar average, 5 face-B update,
+ * 6 conserved update + inversion, 7 ghost fill (3 directions). */
+int hamr_time_stage(hamr_ctx *ctx, int stage, int nrep, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,97 @@
+/*
+ * hamr_oracle.h -- FP64 CPU restatement of H-AMR's GRMHD fluid step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP step in
+ * hamr_gamma_wind_amd/: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * It works on ONE global grid with NG=3 ghost zones (whereas the reference and
+ * the HIP path work block by block); a correct multi-block step reproduces it,
+ * because every block-boundary face, corner and ghost value is a function of the
+ * global state only.  The block sizes enter only where the reference's own
+ * semantics depend on them: the per-block harmonic Courant dt and the polar E1
+ * average (an average of per-block averages).
+ *
+ * Array layout (all double, structure of arrays):
+ *   cell(i,j,z) = (i*D2 + j)*D3 + z  with D_d = n_d + 6 and i,j,z in [0,D_d)
+ *   (ghost-inclusive; interior starts at 3; z fastest, as the reference's index_3D)
+ *   prims   p [k*ncell + cell]      k = RHO,UU,U1,U2,U3,B1,B2,B3,KTOT
+ *   face B  ps[d*ncell + cell]      d = 0,1,2 -> B^1 at FACE1, B^2 at FACE2, B^3 at FACE3
+ *   fluxes  F [k*ncell + cell]      flux through the lower face of the cell
+ *   EMF     E [d*ncell + cell]      d = 0,1,2 -> corner E_1, E_2, E_3
+ *   metric  col(i,j) = i*D2 + j, positions FACE1=0 FACE2=1 CORN=2 CENT=3 FACE3=4
+ *           gcov[(pos*ncol + col)*10 + s], gcon likewise, s over the symmetric
+ *           pairs 00,01,02,03,11,12,13,22,23,33;  gdet[pos*ncol + col];
+ *           conn[col*40 + lam*10 + s] = Gamma^lam_{s} at CENT;  rcen[col] = r at CENT.
+ */
+#ifndef HAMR_ORACLE_H
+#define HAMR_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NPR 9
+#define ORC_NG 3
+
+enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_POLE = 2 };
+
+typedef struct oracle_params {
+    int n1, n2, n3;        /* interior cells of the global grid */
+    int bs1, bs2, bs3;     /* block interior size (per-block dt, polar average) */
+    int bc[3];             /* boundary type per direction (ORC_BC_*) */
+    double gam;            /* adiabatic index */
+    double cour;           /* Courant factor */
+    double dx[3];          /* code-coordinate spacings dx1, dx2, dx3 */
+    double rho_floor;      /* rho >= rho_floor * r^-1.5  (0 disables) */
+    double u_floor;        /* u   >= u_floor   * r^-2.5  (0 disables) */
+} oracle_params;
+
+typedef struct oracle_metric {
+    const double *gcov, *gcon, *gdet, *conn, *rcen;
+} oracle_metric;
+
+/* ---- pointwise physics (for golden-vector tests) ---- */
+void oracle_get_state(const double *pr, const double *gcov, const double *gcon,
+                      double *ucon, double *ucov, double *bcon, double *bcov);
+void oracle_primtoflux(const double *pr, const double *gcov, const double *gcon, double g,
+                       double gam, int dir, double *flux, double *vmax, double *vmin);
+void oracle_source(const double *pr, const double *gcov, const double *gcon, double g,
+                   const double *conn40, double gam, double *dU);
+void oracle_para(const double *x5, double *l, double *r);
+/* which: 0 = Noble 2D, 1 = 1D v^2 entropy fix, 2 = 1D fix.  prim is the guess
+ * on input and the solution on success; returns the reference's return code. */
+int oracle_utoprim(int which, const double *U, const double *gcov, const double *gcon,
+                   double g, double gam, double *prim);
+
+/* ---- grid stages ---- */
+/* HLL flux through the lower dir-faces; dtface (may be NULL) gets cour*dx/ctop
+ * per face (0 where no face is computed). */
+void oracle_flux(const oracle_params *par, int dir, const double *p, const double *ps,
+                 const oracle_metric *m, double *F, double *dtface);
+/* E_cent + upwinded corner EMF + polar E1 average (factor = dt of the step). */
+void oracle_emf(const oracle_params *par, const double *pb, const double *F1,
+                const double *F2, const double *F3, const oracle_metric *m,
+                double dt_factor, double *E, double *Ecent_scratch);
+void oracle_ct_update(const oracle_params *par, double Dt, const double *psi, double *psf,
+                      const double *E, const oracle_metric *m);
+/* full=0: U0 <- U(pi), source from pi, guess pi;  full=1: U from U0, source and guess from pb. */
+void oracle_update(const oracle_params *par, int full, double Dt, const double *pi,
+                   const double *pb, double *pf, double *U0, const double *psf,
+                   const double *F1, const double *F2, const double *F3,
+                   const oracle_metric *m, int *pflag, long long *failcount);
+void oracle_bound(const oracle_params *par, double *p, double *ps, const oracle_metric *m);
+/* Per-block harmonic dt from the three dtface arrays, minimum over blocks. */
+double oracle_block_dt(const oracle_params *par, const double *dt1, const double *dt2,
+                       const double *dt3);
+
+/* One predictor + corrector step.  p/ps are updated in place (ghosts filled),
+ * pflag gets the corrector's flags; returns dt for the next step
+ * (min over blocks, clamped to 1.3*dt), or a negative value on allocation failure. */
+double oracle_step(const oracle_params *par, double *p, double *ps, const oracle_metric *m,
+                   double dt, int *pflag, long long *failcount);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

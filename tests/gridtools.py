@@ -1,0 +1,94 @@
+"""Bridges between the product's block layout and the oracle's global grid.
+
+Test infrastructure.  The product stores each block with its own ghost zones
+([block][component][i][j][z], extents BS+6); the oracle works on one global
+grid with ghosts.  Global cell index g (interior-relative) of block coordinate c
+and local index l is g = c*BS + l - 3, so a block is the global slice
+[c*BS, c*BS + BS + 6) in array coordinates.
+"""
+import numpy as np
+
+import hamr_gamma_wind_amd as H
+from tests import oracle_ctypes as O
+
+NG = 3
+
+
+def single_block_desc(desc):
+    """The same grid as one block (the oracle's global metric comes from it)."""
+    n = [desc.bs[k] * desc.nb[k] for k in range(3)]
+    return H.make_desc(n, (1, 1, 1), gam=desc.gam, spin=desc.spin, cour=desc.cour, rin=desc.rin,
+                       rout=desc.rout, rho_floor=desc.rho_floor, u_floor=desc.u_floor, minkowski=desc.minkowski)
+
+
+def oracle_metric(desc):
+    """Global metric in the oracle's layout (hamr_oracle.h)."""
+    sd = single_block_desc(desc)
+    lay = H.desc_layout(sd)
+    ncol = lay["ncol"]
+    m = H.desc_geometry(sd, 0)
+    gcov = np.ascontiguousarray(m["gcov"][:, :, :ncol].transpose(0, 2, 1))
+    gcon = np.ascontiguousarray(m["gcon"][:, :, :ncol].transpose(0, 2, 1))
+    gdet = np.ascontiguousarray(m["gdet"][:, :ncol])
+    conn = np.ascontiguousarray(m["conn"][:, :ncol].T)
+    rcen = np.ascontiguousarray(m["rcen"][:ncol])
+    return dict(gcov=gcov, gcon=gcon, gdet=gdet, conn=conn, rcen=rcen)
+
+
+def oracle_grid(oracle, desc, metric=None):
+    n = tuple(desc.bs[k] * desc.nb[k] for k in range(3))
+    bs = tuple(desc.bs[k] for k in range(3))
+    sx, dx = H.desc_coords(desc)
+    if desc.minkowski:
+        bc = (O.BC_PERIODIC, O.BC_PERIODIC, O.BC_PERIODIC)
+    else:
+        bc = (O.BC_OUTFLOW, O.BC_POLE, O.BC_PERIODIC)
+    if metric is None:
+        metric = oracle_metric(desc)
+    return O.Grid(oracle, n, bs, bc, desc.gam, desc.cour, dx, metric, desc.rho_floor, desc.u_floor)
+
+
+def block_coords(desc):
+    nb = [desc.nb[k] for k in range(3)]
+    out = []
+    for c1 in range(nb[0]):
+        for c2 in range(nb[1]):
+            for c3 in range(nb[2]):
+                out.append((c1, c2, c3))
+    return out
+
+
+def blocks_to_global(desc, arr):
+    """Interior of every block -> a global array with zero ghosts."""
+    bs = [desc.bs[k] for k in range(3)]
+    n = [bs[k] * desc.nb[k] for k in range(3)]
+    ncomp = arr.shape[1]
+    G = np.zeros((ncomp, n[0] + 2 * NG, n[1] + 2 * NG, n[2] + 2 * NG), dtype=arr.dtype)
+    for b, (c1, c2, c3) in enumerate(block_coords(desc)):
+        G[:, NG + c1 * bs[0]: NG + (c1 + 1) * bs[0], NG + c2 * bs[1]: NG + (c2 + 1) * bs[1],
+          NG + c3 * bs[2]: NG + (c3 + 1) * bs[2]] = arr[b][:, NG: NG + bs[0], NG: NG + bs[1], NG: NG + bs[2]]
+    return np.ascontiguousarray(G)
+
+
+def global_to_blocks(desc, G):
+    """Every block's ghost-inclusive slice of a global array."""
+    bs = [desc.bs[k] for k in range(3)]
+    D = [b + 2 * NG for b in bs]
+    out = np.zeros((len(block_coords(desc)), G.shape[0]) + tuple(D), dtype=G.dtype)
+    for b, (c1, c2, c3) in enumerate(block_coords(desc)):
+        out[b] = G[:, c1 * bs[0]: c1 * bs[0] + D[0], c2 * bs[1]: c2 * bs[1] + D[1], c3 * bs[2]: c3 * bs[2] + D[2]]
+    return out
+
+
+def block_view(desc, G):
+    """Same as global_to_blocks for a global array without a component axis."""
+    return global_to_blocks(desc, G[None])[:, 0]
+
+
+def relerr_by_var(a, b, axis_comp=1):
+    """max |a - b| / max |b| per component (component axis `axis_comp`)."""
+    a = np.moveaxis(a, axis_comp, 0).reshape(a.shape[axis_comp], -1)
+    b = np.moveaxis(b, axis_comp, 0).reshape(b.shape[axis_comp], -1)
+    scale = np.max(np.abs(b), axis=1)
+    scale[scale == 0] = 1.0
+    return np.max(np.abs(a - b), axis=1) / scale

@@ -1,0 +1,169 @@
+"""ctypes bindings of the FP64 CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: the oracle is the checker of the HIP step, never the thing
+measured or shipped.  Arrays follow the layout documented in oracle/hamr_oracle.h
+(structure of arrays over the ghost-inclusive global grid, z fastest).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+
+NPR = 9
+NG = 3
+BC_PERIODIC, BC_OUTFLOW, BC_POLE = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int),
+        ("bs1", C.c_int), ("bs2", C.c_int), ("bs3", C.c_int),
+        ("bc", C.c_int * 3),
+        ("gam", C.c_double), ("cour", C.c_double),
+        ("dx", C.c_double * 3),
+        ("rho_floor", C.c_double), ("u_floor", C.c_double),
+    ]
+
+
+class Metric(C.Structure):
+    _fields_ = [("gcov", _dp), ("gcon", _dp), ("gdet", _dp), ("conn", _dp), ("rcen", _dp)]
+
+
+def _ptr(a, t=_dp):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    def __init__(self, lib):
+        self.lib = lib
+        L = lib
+        L.oracle_get_state.argtypes = [_dp] * 7
+        L.oracle_primtoflux.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.oracle_source.argtypes = [_dp, _dp, _dp, C.c_double, _dp, C.c_double, _dp]
+        L.oracle_para.argtypes = [_dp, _dp, _dp]
+        L.oracle_utoprim.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, _dp]
+        L.oracle_utoprim.restype = C.c_int
+        P = C.POINTER(Params)
+        M = C.POINTER(Metric)
+        L.oracle_flux.argtypes = [P, C.c_int, _dp, _dp, M, _dp, _dp]
+        L.oracle_emf.argtypes = [P, _dp, _dp, _dp, _dp, M, C.c_double, _dp, _dp]
+        L.oracle_ct_update.argtypes = [P, C.c_double, _dp, _dp, _dp, M]
+        L.oracle_update.argtypes = [P, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, M,
+                                    _ip, C.POINTER(C.c_longlong)]
+        L.oracle_bound.argtypes = [P, _dp, _dp, M]
+        L.oracle_block_dt.argtypes = [P, _dp, _dp, _dp]
+        L.oracle_block_dt.restype = C.c_double
+        L.oracle_step.argtypes = [P, _dp, _dp, M, C.c_double, _ip, C.POINTER(C.c_longlong)]
+        L.oracle_step.restype = C.c_double
+
+    # ---- pointwise ----
+    def get_state(self, pr, gcov, gcon):
+        out = [np.zeros(4) for _ in range(4)]
+        self.lib.oracle_get_state(_ptr(np.ascontiguousarray(pr, float)), _ptr(np.ascontiguousarray(gcov, float)),
+                                  _ptr(np.ascontiguousarray(gcon, float)), *[_ptr(o) for o in out])
+        return out
+
+    def primtoflux(self, pr, gcov, gcon, g, gam, d):
+        f = np.zeros(NPR)
+        vx = np.zeros(1)
+        vn = np.zeros(1)
+        self.lib.oracle_primtoflux(_ptr(np.ascontiguousarray(pr, float)), _ptr(np.ascontiguousarray(gcov, float)),
+                                   _ptr(np.ascontiguousarray(gcon, float)), g, gam, d, _ptr(f), _ptr(vx), _ptr(vn))
+        return f, vx[0], vn[0]
+
+    def source(self, pr, gcov, gcon, g, conn40, gam):
+        dU = np.zeros(NPR)
+        self.lib.oracle_source(_ptr(np.ascontiguousarray(pr, float)), _ptr(np.ascontiguousarray(gcov, float)),
+                               _ptr(np.ascontiguousarray(gcon, float)), g,
+                               _ptr(np.ascontiguousarray(conn40, float)), gam, _ptr(dU))
+        return dU
+
+    def para(self, x5):
+        l = np.zeros(1)
+        r = np.zeros(1)
+        self.lib.oracle_para(_ptr(np.ascontiguousarray(x5, float)), _ptr(l), _ptr(r))
+        return l[0], r[0]
+
+    def utoprim(self, which, U, gcov, gcon, g, gam, guess):
+        prim = np.array(guess, dtype=float, copy=True)
+        ret = self.lib.oracle_utoprim(which, _ptr(np.ascontiguousarray(U, float)),
+                                      _ptr(np.ascontiguousarray(gcov, float)),
+                                      _ptr(np.ascontiguousarray(gcon, float)), g, gam, _ptr(prim))
+        return ret, prim
+
+
+class Grid:
+    """A global ghost-inclusive grid stepped by the oracle."""
+
+    def __init__(self, oracle, n, bs, bc, gam, cour, dx, metric, rho_floor=0.0, u_floor=0.0):
+        self.o = oracle
+        self.par = Params()
+        self.par.n1, self.par.n2, self.par.n3 = n
+        self.par.bs1, self.par.bs2, self.par.bs3 = bs
+        for d in range(3):
+            self.par.bc[d] = bc[d]
+            self.par.dx[d] = dx[d]
+        self.par.gam = gam
+        self.par.cour = cour
+        self.par.rho_floor = rho_floor
+        self.par.u_floor = u_floor
+        self.D = tuple(x + 2 * NG for x in n)
+        self.ncell = self.D[0] * self.D[1] * self.D[2]
+        # metric arrays kept alive here
+        self._m = {k: np.ascontiguousarray(v, dtype=float) for k, v in metric.items()}
+        self.metric = Metric(_ptr(self._m["gcov"]), _ptr(self._m["gcon"]), _ptr(self._m["gdet"]),
+                             _ptr(self._m["conn"]), _ptr(self._m["rcen"]))
+        self.failcount = (C.c_longlong * 4)()
+
+    def zeros(self, ncomp):
+        return np.zeros((ncomp,) + self.D)
+
+    def step(self, p, ps, dt):
+        pflag = np.zeros(self.D, dtype=np.int32)
+        ndt = self.o.lib.oracle_step(C.byref(self.par), _ptr(p), _ptr(ps), C.byref(self.metric), dt,
+                                     _ptr(pflag, _ip), self.failcount)
+        return ndt, pflag
+
+    def bound(self, p, ps):
+        self.o.lib.oracle_bound(C.byref(self.par), _ptr(p), _ptr(ps), C.byref(self.metric))
+
+    def flux(self, d, p, ps):
+        F = self.zeros(NPR)
+        dtf = np.zeros(self.D)
+        self.o.lib.oracle_flux(C.byref(self.par), d, _ptr(p), _ptr(ps), C.byref(self.metric), _ptr(F), _ptr(dtf))
+        return F, dtf
+
+    def emf(self, pb, F1, F2, F3, dt):
+        E = self.zeros(3)
+        Ec = self.zeros(3)
+        self.o.lib.oracle_emf(C.byref(self.par), _ptr(pb), _ptr(F1), _ptr(F2), _ptr(F3), C.byref(self.metric),
+                              dt, _ptr(E), _ptr(Ec))
+        return E
+
+    def ct_update(self, Dt, psi, psf, E):
+        self.o.lib.oracle_ct_update(C.byref(self.par), Dt, _ptr(psi), _ptr(psf), _ptr(E), C.byref(self.metric))
+
+    def update(self, full, Dt, pi, pb, pf, U0, psf, F1, F2, F3):
+        pflag = np.zeros(self.D, dtype=np.int32)
+        self.o.lib.oracle_update(C.byref(self.par), full, Dt, _ptr(pi), _ptr(pb), _ptr(pf), _ptr(U0), _ptr(psf),
+                                 _ptr(F1), _ptr(F2), _ptr(F3), C.byref(self.metric), _ptr(pflag, _ip),
+                                 self.failcount)
+        return pflag
+
+    def block_dt(self, d1, d2, d3):
+        return self.o.lib.oracle_block_dt(C.byref(self.par), _ptr(d1), _ptr(d2), _ptr(d3))
+
+
+def load():
+    if not os.path.exists(LIB):
+        raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+    return Oracle(C.CDLL(LIB))
