@@ -43,6 +43,15 @@ def setup(oracle):
     ctx.close()
 
 
+def _check_faces(name, desc, got, want):
+    """Face fields on the faces a block updates: [NG, NG+BS] along the normal,
+    interior across."""
+    for d in range(3):
+        sl = [slice(NG, NG + desc.bs[k]) for k in range(3)]
+        sl[d] = slice(NG, NG + desc.bs[d] + 1)
+        _check("%s B%d" % (name, d + 1), got[:, d:d + 1, sl[0], sl[1], sl[2]], want[:, d:d + 1, sl[0], sl[1], sl[2]])
+
+
 def _global_state(desc, grid, p0, ps0):
     P = GT.blocks_to_global(desc, p0)
     PS = GT.blocks_to_global(desc, ps0)
@@ -123,16 +132,13 @@ def test_fluxes_emf_and_ct(setup):
     ctx.ct_bound(dt)
     got = ctx.get(H.HAMR_EMF)
     want = GT.global_to_blocks(desc, Eo)
-    sl = slice(NG, -NG + 1) if False else [slice(NG, NG + desc.bs[k] + 1) for k in range(3)]
+    sl = [slice(NG, NG + desc.bs[k] + 1) for k in range(3)]
     _check("EMF", got[:, :, sl[0], sl[1], sl[2]], want[:, :, sl[0], sl[1], sl[2]])
     PSH = PS.copy()
     grid.ct_update(0.5 * dt, PS, PSH, Eo)
     for n in range(nb):
         ctx.ct_update(0, 0.5 * dt, n)
-    got = ctx.get(H.HAMR_PSH)
-    want = GT.global_to_blocks(desc, PSH)
-    I = [slice(NG, NG + desc.bs[k] + 1) for k in range(3)]
-    _check("face B", got[:, :, I[0], I[1], I[2]], want[:, :, I[0], I[1], I[2]])
+    _check_faces("face", desc, ctx.get(H.HAMR_PSH), GT.global_to_blocks(desc, PSH))
 
 
 def test_one_step_fp64(setup):
@@ -157,8 +163,7 @@ def test_one_step_fp64(setup):
     want_ps = GT.global_to_blocks(desc, PS)
     I = slice(NG, -NG)
     _check("P after 1 step", got_p[:, :, I, I, I], want_p[:, :, I, I, I])
-    J = [slice(NG, NG + desc.bs[k] + 1) for k in range(3)]
-    _check("face B after 1 step", got_ps[:, :, J[0], J[1], J[2]], want_ps[:, :, J[0], J[1], J[2]])
+    _check_faces("face after 1 step", desc, got_ps, want_ps)
     pf, fi = ctx.all_flags()
     want_flag = GT.block_view(desc, pflag_o)
     assert np.array_equal(pf[:, I, I, I], want_flag[:, I, I, I])
