@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-graph", action="store_true", help="enqueue each step instead of replaying a HIP graph")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle baseline (0: skip)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one streaming copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (each GPU box holds one MI355X); "
@@ -131,6 +133,11 @@ def main():
     bpz = stage_bytes(dom, bs)
     achieved = nz * bpz / (stage_ms[dom] * 1e-3) / 1e9 if bpz > 0 else 0.0
     traffic = read_traffic(dom, args.config)
+    copy = None
+    if args.copy_probe_mib > 0:
+        nbytes = args.copy_probe_mib << 20
+        ms = ctx.copy_probe(nbytes, 1)
+        copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
     ctx.close()
 
     cpu = None
@@ -163,6 +170,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": nz * bpz, "avg_launch_ms": stage_ms[dom]},
         "stages_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
+        "copy_probe": copy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

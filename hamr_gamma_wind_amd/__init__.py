@@ -116,6 +116,7 @@ def load_library(path=None):
         "hamr_step_graph": [ctxp, C.c_int, _dp],
         "hamr_sync": [ctxp],
         "hamr_time_stage": [ctxp, C.c_int, C.c_int, _dp],
+        "hamr_copy_probe": [ctxp, C.c_longlong, C.c_int, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -322,6 +323,11 @@ class Context:
     def time_stage(self, stage, nrep=10):
         v = C.c_double()
         self._chk(self.L.hamr_time_stage(self.h, stage, nrep, C.byref(v)), "hamr_time_stage")
+        return v.value
+
+    def copy_probe(self, nbytes, nrep=1):
+        v = C.c_double()
+        self._chk(self.L.hamr_copy_probe(self.h, nbytes, nrep, C.byref(v)), "hamr_copy_probe")
         return v.value
 
     # batched

@@ -735,6 +735,31 @@ int hamr_time_stage(hamr_ctx *c, int stage, int nrep, double *avg_ms)
     return 0;
 }
 
+int hamr_copy_probe(hamr_ctx *c, long long bytes, int nrep, double *avg_ms)
+{
+    if (!c || bytes < 16 || nrep < 1) return fail(c, "hamr_copy_probe: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc(&a, (size_t)bytes));
+    HIPCHK(c, hipMalloc(&b, (size_t)bytes));
+    HIPCHK(c, hipMemsetAsync(a, 0, (size_t)bytes, c->s));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->s));
+    for (int r = 0; r < nrep; r++) launch_copy(a, b, bytes, c->s);
+    HIPCHK(c, hipEventRecord(e1, c->s));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (avg_ms) *avg_ms = (double)ms / nrep;
+    return check_launch(c, "hamr_copy_probe");
+}
+
 int hamr_sync(hamr_ctx *c)
 {
     if (!c) return -1;

@@ -37,5 +37,6 @@ void launch_bound(const KParams &P, int dir, const int *blist, int nb, double *p
                   hipStream_t s);
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);
 void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s);
+void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
 
 }  // namespace hamr

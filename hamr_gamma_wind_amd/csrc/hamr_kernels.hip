@@ -481,6 +481,20 @@ __global__ void k_dt_finalize(int nblk, unsigned long long *__restrict__ dtmin, 
     }
 }
 
+// Plain streaming copy (16 B per thread per iteration): the known-size kernel
+// that calibrates the HBM byte counters.
+__global__ void __launch_bounds__(256) k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+
+void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s)
+{
+    const int64_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, (const double2 *)src, (double2 *)dst, n);
+}
+
 __global__ void k_fill_u64(unsigned long long *a, int n, unsigned long long v)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;

@@ -164,6 +164,9 @@ int hamr_sync(hamr_ctx *ctx);
  * modified): 0-2 flux x1..x3, 3 EMF, 4 polar average, 5 face-B update,
  * 6 conserved update + inversion, 7 ghost fill (3 directions). */
 int hamr_time_stage(hamr_ctx *ctx, int stage, int nrep, double *avg_ms);
+/* Streaming copy of `bytes` bytes (read once, written once), nrep launches:
+ * the known-size kernel that calibrates HBM byte counters; average ms. */
+int hamr_copy_probe(hamr_ctx *ctx, long long bytes, int nrep, double *avg_ms);
 
 #ifdef __cplusplus
 }
