@@ -142,12 +142,12 @@ class Grid:
         self.o.lib.oracle_flux(C.byref(self.par), d, _ptr(p), _ptr(ps), C.byref(self.metric), _ptr(F), _ptr(dtf))
         return F, dtf
 
-    def emf(self, pb, F1, F2, F3, dt):
+    def emf(self, pb, F1, F2, F3, dt, with_centre=False):
         E = self.zeros(3)
         Ec = self.zeros(3)
         self.o.lib.oracle_emf(C.byref(self.par), _ptr(pb), _ptr(F1), _ptr(F2), _ptr(F3), C.byref(self.metric),
                               dt, _ptr(E), _ptr(Ec))
-        return E
+        return (E, Ec) if with_centre else E
 
     def ct_update(self, Dt, psi, psf, E):
         self.o.lib.oracle_ct_update(C.byref(self.par), Dt, _ptr(psi), _ptr(psf), _ptr(E), C.byref(self.metric))

@@ -1,0 +1,481 @@
+"""Pins the oracle's grid-level stages (flux, CT, update, ghost fill, Courant
+dt, step) by what the mathematics fixes, independently of how the oracle
+writes each formula:
+
+  - HLL upwinding: in supersonic flow the face flux is the flux of the upwind
+    state (reconstructed exactly at a step, PPM's monotone limit);
+  - a uniform state is a steady solution of the discrete step (flat metric);
+  - a small entropy wave advects at its velocity (sign and rate of the update
+    in each direction);
+  - conservation: in a periodic flat box the total of every conserved
+    variable, field included, is constant (telescoping fluxes and EMFs);
+  - constrained transport keeps the face-field divergence at roundoff on the
+    Kerr torus grid;
+  - the corner EMF reduces to the 1D face EMF for plane-parallel data, and
+    equals the upwind corner EMF of Gardiner & Stone (2005, JCP 205, 509,
+    eqs. 41-51) on arbitrary face and centre EMFs;
+  - the Courant dt of a gas at rest is cour / (c_s (n1 + n2 + n3)) with the
+    relativistic sound speed c_s^2 = Gamma p / (rho + Gamma u), and the step's
+    next dt is clamped to 1.3 dt;
+  - ghost fill: periodicity in x3, the polar mirror through the axis (theta ->
+    -theta, phi -> phi + pi: theta and phi vector components change sign), and
+    the radial outflow condition with no flow into the grid from the ghosts.
+
+Test infrastructure: runs the oracle only (CPU).
+"""
+import numpy as np
+import pytest
+
+import hamr_gamma_wind_amd as H
+from tests import gridtools as GT
+
+NG = 3
+RHO, UU, U1, U2, U3, B1, B2, B3, KTOT = range(9)
+GAM = 13.0 / 9.0
+FLAT_GCOV = np.array([-1.0, 0, 0, 0, 1.0, 0, 0, 1.0, 0, 1.0])
+
+
+def _flat(oracle, n, bs=None, cour=0.8):
+    bs = n if bs is None else bs
+    nb = tuple(n[k] // bs[k] for k in range(3))
+    desc = H.make_desc(bs, nb, gam=GAM, cour=cour, minkowski=1, rho_floor=0.0, u_floor=0.0)
+    return desc, GT.oracle_grid(oracle, desc)
+
+
+def _interior(n):
+    return tuple(slice(NG, NG + n[k]) for k in range(3))
+
+
+def _fill(grid, n, prims, faces):
+    """Global arrays with the given interior prims (9, n) and lower-face fields
+    (3, n); ghosts from the oracle's periodic fill."""
+    P = grid.zeros(9)
+    PS = grid.zeros(3)
+    I = _interior(n)
+    P[(slice(None),) + I] = prims
+    PS[(slice(None),) + I] = faces
+    grid.bound(P, PS)
+    return P, PS
+
+
+def _four_velocity(v):
+    """Relative 4-velocity U^i = gamma v^i in flat space."""
+    v = np.asarray(v, float)
+    return v / np.sqrt(1.0 - np.sum(v * v, axis=0))
+
+
+def _smooth(rng, n, nmodes=3, amp=1.0):
+    x = [(np.arange(n[k]) + 0.5) / n[k] for k in range(3)]
+    X = np.meshgrid(*x, indexing="ij")
+    f = np.zeros(n)
+    for _ in range(nmodes):
+        k = rng.integers(-2, 3, size=3)
+        f += rng.normal() * np.sin(2 * np.pi * (k[0] * X[0] + k[1] * X[1] + k[2] * X[2]) + rng.uniform(0, 2 * np.pi))
+    return amp * f / max(1.0, np.abs(f).max())
+
+
+def _curl_faces(A, dx):
+    """Face fields B = curl A on the periodic grid (edge-centred A): the
+    discrete divergence of the result vanishes identically."""
+    A1, A2, A3 = A
+    r = np.roll
+    b1 = (r(A3, -1, 1) - A3) / dx[1] - (r(A2, -1, 2) - A2) / dx[2]
+    b2 = (r(A1, -1, 2) - A1) / dx[2] - (r(A3, -1, 0) - A3) / dx[0]
+    b3 = (r(A2, -1, 0) - A2) / dx[0] - (r(A1, -1, 1) - A1) / dx[1]
+    return np.array([b1, b2, b3])
+
+
+def _centre_from_faces(faces):
+    return np.array([0.5 * (faces[d] + np.roll(faces[d], -1, d)) for d in range(3)])
+
+
+def _random_flat_state(rng, n, dx, vamp=0.3, bamp=0.05):
+    rho = 1.0 + 0.3 * _smooth(rng, n)
+    u = 0.5 * (1.0 + 0.3 * _smooth(rng, n))
+    v = np.array([vamp * _smooth(rng, n) for _ in range(3)])
+    U = _four_velocity(v)
+    A = np.array([bamp * _smooth(rng, n) * min(dx) * max(n) / (2 * np.pi) for _ in range(3)])
+    faces = _curl_faces(A, dx)
+    Bc = _centre_from_faces(faces)
+    prims = np.zeros((9,) + tuple(n))
+    prims[RHO], prims[UU], prims[U1:U3 + 1], prims[B1:B3 + 1] = rho, u, U, Bc
+    prims[KTOT] = (GAM - 1.0) * u * rho ** (-GAM)
+    return prims, faces
+
+
+def _conserved_totals(oracle, P, n):
+    I = _interior(n)
+    cells = P[(slice(None),) + I].reshape(9, -1).T
+    tot = np.zeros(9)
+    mag = np.zeros(9)
+    for pr in cells:
+        U, _, _ = oracle.primtoflux(pr, FLAT_GCOV, FLAT_GCOV, 1.0, GAM, 0)
+        tot += U
+        mag += np.abs(U)
+    return tot, mag
+
+
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_hll_flux_is_upwind_in_supersonic_flow(oracle, d, sign):
+    """|v| = 0.9 >> fast speed: F through the face at a density step equals the
+    flux of the upwind cell (left for v > 0, right for v < 0)."""
+    n = [6, 6, 6]
+    n[d - 1] = 16
+    _, grid = _flat(oracle, tuple(n))
+    prims = np.zeros((9,) + tuple(n))
+    idx = np.arange(n[d - 1])
+    shape = [1, 1, 1]
+    shape[d - 1] = n[d - 1]
+    rho = np.where(idx < n[d - 1] // 2, 1.0, 2.0).reshape(shape) * np.ones(n)
+    v = np.zeros((3,) + tuple(n))
+    v[d - 1] = 0.9 * sign
+    v[d % 3] = 0.05
+    prims[RHO] = rho
+    prims[UU] = 1e-3
+    prims[U1:U3 + 1] = _four_velocity(v)
+    faces = np.zeros((3,) + tuple(n))
+    faces[d - 1] = 0.05          # uniform normal field
+    faces[(d + 1) % 3] = 0.1     # uniform transverse field
+    prims[B1:B3 + 1] = _centre_from_faces(faces)
+    prims[KTOT] = 1.0
+    P, PS = _fill(grid, n, prims, faces)
+    F, dtf = grid.flux(d, P, PS)
+    face = [NG + 1, NG + 1, NG + 1]
+    face[d - 1] = NG + n[d - 1] // 2
+    up = list(face)
+    if sign > 0:
+        up[d - 1] -= 1
+    want, vmax, vmin = oracle.primtoflux(P[(slice(None),) + tuple(up)], FLAT_GCOV, FLAT_GCOV, 1.0, GAM, d)
+    assert (vmin > 0) if sign > 0 else (vmax < 0)
+    got = F[(slice(None),) + tuple(face)]
+    assert np.allclose(got, want, rtol=1e-14, atol=1e-14 * np.abs(want).max())
+    # and the downwind state would give a different flux
+    dn = list(face)
+    if sign < 0:
+        dn[d - 1] -= 1
+    other, _, _ = oracle.primtoflux(P[(slice(None),) + tuple(dn)], FLAT_GCOV, FLAT_GCOV, 1.0, GAM, d)
+    assert np.abs(other[RHO] - got[RHO]) > 0.1 * np.abs(got[RHO])
+
+
+def test_uniform_state_is_steady(oracle):
+    rng = np.random.default_rng(7)
+    n = (8, 7, 6)
+    _, grid = _flat(oracle, n)
+    prims = np.ones((9,) + n)
+    vals = np.array([1.3, 0.7, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.2])
+    vals[U1:U3 + 1] = _four_velocity(rng.uniform(-0.4, 0.4, 3))
+    faces = np.ones((3,) + n) * rng.uniform(-0.3, 0.3, 3)[:, None, None, None]
+    vals[B1:B3 + 1] = faces[:, 0, 0, 0]
+    prims *= vals[:, None, None, None]
+    P, PS = _fill(grid, n, prims, faces)
+    P0, PS0 = P.copy(), PS.copy()
+    dt = 0.3 / 8
+    grid.step(P, PS, dt)
+    I = (slice(None),) + _interior(n)
+    err = GT.relerr_by_var(P[I][None], P0[I][None])
+    assert np.all(err <= 1e-13), err
+    assert np.abs(PS[I] - PS0[I]).max() <= 1e-14
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_entropy_wave_advects_at_its_velocity(oracle, d):
+    """rho = 1 + eps sin(2 pi x_d), uniform p and v_d = 0.3: after one step
+    rho(x) -> rho(x - v dt) (cell averages: the same shift times sinc(pi dx))."""
+    n = [6, 6, 6]
+    n[d - 1] = 64
+    n = tuple(n)
+    _, grid = _flat(oracle, n)
+    eps, vel = 1e-3, 0.3
+    x = (np.arange(n[d - 1]) + 0.5) / n[d - 1]
+    shape = [1, 1, 1]
+    shape[d - 1] = n[d - 1]
+    prims = np.zeros((9,) + n)
+    prims[RHO] = (1.0 + eps * np.sin(2 * np.pi * x)).reshape(shape)
+    prims[UU] = 0.1
+    v = np.zeros((3,) + n)
+    v[d - 1] = vel
+    prims[U1:U3 + 1] = _four_velocity(v)
+    prims[KTOT] = 1.0
+    P, PS = _fill(grid, n, prims, np.zeros((3,) + n))
+    _, d1 = grid.flux(1, P, PS)
+    _, d2 = grid.flux(2, P, PS)
+    _, d3 = grid.flux(3, P, PS)
+    dt = grid.block_dt(d1, d2, d3)
+    rho0 = P[RHO][_interior(n)].copy()
+    grid.step(P, PS, dt)
+    got = (P[RHO][_interior(n)] - rho0).reshape(-1)
+    h = 1.0 / n[d - 1]
+    sinc = np.sin(np.pi * h) / (np.pi * h)
+    want = (eps * sinc * (np.sin(2 * np.pi * (x - vel * dt)) - np.sin(2 * np.pi * x))).reshape(shape) * np.ones(n)
+    want = want.reshape(-1)
+    # L1 over the wave (PPM clips the two extrema to first order, a few cells)
+    l1 = np.abs(got - want).sum() / np.abs(want).sum()
+    print("entropy wave dir %d: relative L1 error %.3g" % (d, l1))
+    assert l1 <= 0.01
+
+
+def test_periodic_box_conserves_every_total(oracle):
+    rng = np.random.default_rng(3)
+    n = (12, 10, 8)
+    dx = tuple(1.0 / k for k in n)
+    _, grid = _flat(oracle, n)
+    prims, faces = _random_flat_state(rng, n, dx)
+    P, PS = _fill(grid, n, prims, faces)
+    _, d1 = grid.flux(1, P, PS)
+    _, d2 = grid.flux(2, P, PS)
+    _, d3 = grid.flux(3, P, PS)
+    dt = grid.block_dt(d1, d2, d3)
+    t0, mag = _conserved_totals(oracle, P, n)
+    grid.failcount[:] = [0, 0, 0, 0]
+    for _ in range(2):
+        dt, pflag = grid.step(P, PS, dt)
+        assert np.all(pflag[_interior(n)] == 0)
+    t1, _ = _conserved_totals(oracle, P, n)
+    # RHO, UU, U1-3 (to the inversion tolerance) and B1-3; KTOT is advected, not conserved
+    assert np.all(np.abs(t1 - t0)[:8] <= 1e-9 * mag[:8]), (t1 - t0) / mag
+    # and the state did evolve
+    assert np.abs(P[RHO][_interior(n)] - prims[RHO]).max() > 1e-4
+
+
+def _div_face(grid_metric, PS, dx, n):
+    """gdet-weighted divergence of the face field over interior cells."""
+    D = PS.shape[1:]
+    g = grid_metric["gdet"]
+    gF = [g[p].reshape(D[0], D[1])[:, :, None] for p in (0, 1, 4)]
+    B = [PS[d] * gF[d] for d in range(3)]
+    I = _interior(n)
+    Ip = [slice(NG + 1, NG + 1 + n[k]) for k in range(3)]
+    div = ((B[0][Ip[0], I[1], I[2]] - B[0][I]) / dx[0] + (B[1][I[0], Ip[1], I[2]] - B[1][I]) / dx[1]
+           + (B[2][I[0], I[1], Ip[2]] - B[2][I]) / dx[2])
+    scale = max(np.abs(B[d][I]).max() / dx[d] for d in range(3))
+    return np.abs(div).max(), scale
+
+
+def test_ct_keeps_divergence_at_roundoff_on_the_torus(oracle):
+    desc = H.make_desc((16, 8, 8), (2, 2, 2), rout=60.0)
+    n = tuple(desc.bs[k] * desc.nb[k] for k in range(3))
+    m = GT.oracle_metric(desc)
+    grid = GT.oracle_grid(oracle, desc, m)
+    p, ps = H.desc_torus(desc, H.torus_desc())
+    P = GT.blocks_to_global(desc, p)
+    PS = GT.blocks_to_global(desc, ps)
+    grid.bound(P, PS)
+    _, dx = H.desc_coords(desc)
+    div0, scale = _div_face(m, PS, dx, n)
+    assert div0 <= 1e-12 * scale
+    _, d1 = grid.flux(1, P, PS)
+    _, d2 = grid.flux(2, P, PS)
+    _, d3 = grid.flux(3, P, PS)
+    dt = grid.block_dt(d1, d2, d3)
+    PS_init = PS.copy()
+    for _ in range(3):
+        dt, _ = grid.step(P, PS, dt)
+    div1, scale1 = _div_face(m, PS, dx, n)
+    assert div1 <= 1e-12 * scale1, (div1, scale1)
+    assert np.abs(PS - PS_init)[(slice(None),) + _interior(n)].max() > 1e-8 * np.abs(PS_init).max()
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_corner_emf_reduces_to_face_emf_for_plane_parallel_data(oracle, d):
+    """Data varying along x_d only: the corner EMFs equal the 1D face EMFs of the
+    x_d faces (E = -F_d[B_b] or +F_d[B_c] for the two transverse components)
+    and the x_d-directed EMF equals its cell-centre value."""
+    rng = np.random.default_rng(11 + d)
+    n = [6, 6, 6]
+    n[d - 1] = 16
+    n = tuple(n)
+    _, grid = _flat(oracle, n)
+    x = (np.arange(n[d - 1]) + 0.5) / n[d - 1]
+    shape = [1, 1, 1]
+    shape[d - 1] = n[d - 1]
+
+    def prof(a0, a1):
+        ph = rng.uniform(0, 2 * np.pi)
+        return (a0 + a1 * np.sin(2 * np.pi * x + ph)).reshape(shape) * np.ones(n)
+
+    prims = np.zeros((9,) + n)
+    prims[RHO], prims[UU] = prof(1.0, 0.3), prof(0.5, 0.2)
+    prims[U1:U3 + 1] = _four_velocity(np.array([prof(0.0, 0.3) + 0.1 * k for k in range(3)]))
+    faces = np.zeros((3,) + n)
+    faces[d - 1] = 0.2                      # normal field uniform (div B = 0)
+    for c in range(3):
+        if c != d - 1:
+            faces[c] = prof(0.0, 0.3)
+    prims[B1:B3 + 1] = _centre_from_faces(faces)
+    prims[KTOT] = 1.0
+    P, PS = _fill(grid, n, prims, faces)
+    F = {k: grid.flux(k, P, PS)[0] for k in (1, 2, 3)}
+    E, Ec = grid.emf(P, F[1], F[2], F[3], 1.0, with_centre=True)
+    C = tuple(slice(NG, NG + n[k] + 1) for k in range(3))
+    a = d - 1                 # the varying direction
+    b, c = (a + 1) % 3, (a + 2) % 3
+    Fd = F[d]
+    # E_c (c = a + 2) on the a-faces is -F_a[B_b]; E_b on the a-faces is +F_a[B_c]
+    scale = np.abs(Fd[[B1 + b, B1 + c]]).max()
+    assert np.abs(E[c][C] - (-Fd[B1 + b][C])).max() <= 1e-14 * scale
+    assert np.abs(E[b][C] - Fd[B1 + c][C]).max() <= 1e-14 * scale
+    # E_a lives at the a-cell centre: the cell value of the same a-column
+    Ccell = list(C)
+    Ccell[a] = slice(NG, NG + n[a])
+    Ccell = tuple(Ccell)
+    assert np.abs(E[a][Ccell] - Ec[a][Ccell]).max() <= 1e-14 * max(np.abs(Ec[a]).max(), 1e-300)
+
+
+def _gs05_corner(Ec, F, comp):
+    """Upwind corner EMF of Gardiner & Stone (2005), eqs. 41-51, for the
+    component `comp` (0, 1, 2 = E_1, E_2, E_3) at the lower corner of every
+    cell, with a = comp + 1, b = comp + 2 (cyclic) spanning the corner's plane.
+    Face EMFs: on a-faces E = -F_a[B_b], on b-faces E = +F_b[B_a] (E = -v x B).
+    Mass-flux sign <= 0 takes the upper cell (the reference's tie rule)."""
+    a, b = (comp + 1) % 3, (comp + 2) % 3
+    Fa, Fb = F[a + 1], F[b + 1]
+    Ea_face = -Fa[B1 + b]       # at (a - 1/2, b)
+    Eb_face = Fb[B1 + a]        # at (a, b - 1/2)
+    Ecc = Ec[comp]              # at (a, b)
+    ma, mb = Fa[RHO], Fb[RHO]
+
+    def sh(x, da, db):          # x at (a - da, b - db)
+        out = np.roll(x, da, axis=a)
+        return np.roll(out, db, axis=b)
+
+    # (d E / d b) * db/2 at (a - 1/2, b - 1/4), upwinded on the a-face mass flux at (a - 1/2, b)
+    dEdb_q = np.where(ma > 0, sh(Ecc, 1, 0) - sh(Eb_face, 1, 0), Ecc - Eb_face)
+    # ... at (a - 1/2, b - 3/4), upwinded on the a-face flux at (a - 1/2, b - 1)
+    dEdb_3q = np.where(sh(ma, 0, 1) > 0, sh(Eb_face, 1, 0) - sh(Ecc, 1, 1), Eb_face - sh(Ecc, 0, 1))
+    # (d E / d a) * da/2 at (a - 1/4, b - 1/2), upwinded on the b-face flux at (a, b - 1/2)
+    dEda_q = np.where(mb > 0, sh(Ecc, 0, 1) - sh(Ea_face, 0, 1), Ecc - Ea_face)
+    # ... at (a - 3/4, b - 1/2), upwinded on the b-face flux at (a - 1, b - 1/2)
+    dEda_3q = np.where(sh(mb, 1, 0) > 0, sh(Ea_face, 0, 1) - sh(Ecc, 1, 1), Ea_face - sh(Ecc, 1, 0))
+    avg = 0.25 * (Ea_face + sh(Ea_face, 0, 1) + Eb_face + sh(Eb_face, 1, 0))
+    return avg + 0.25 * (dEdb_3q - dEdb_q) + 0.25 * (dEda_3q - dEda_q)
+
+
+def test_corner_emf_is_gardiner_stone_upwind(oracle):
+    rng = np.random.default_rng(5)
+    n = (8, 7, 6)
+    dx = tuple(1.0 / k for k in n)
+    _, grid = _flat(oracle, n)
+    prims, faces = _random_flat_state(rng, n, dx, vamp=0.4, bamp=0.3)
+    P, PS = _fill(grid, n, prims, faces)
+    F = {k: rng.normal(size=(9,) + P.shape[1:]) for k in (1, 2, 3)}
+    E, Ec = grid.emf(P, F[1], F[2], F[3], 1.0, with_centre=True)
+    C = (slice(NG, NG + n[0] + 1), slice(NG, NG + n[1] + 1), slice(NG, NG + n[2] + 1))
+    # cell-centre EMF: E_i = -gdet eps_ijk u^j b^k (gdet = 1 here)
+    Cc = tuple(slice(NG - 1, NG + n[k] + 1) for k in range(3))
+    for idx in [(NG, NG, NG), (NG + 2, NG + 1, NG + 3), (NG - 1, NG + n[1], NG)]:
+        ucon, _, bcon, _ = oracle.get_state(P[(slice(None),) + idx], FLAT_GCOV, FLAT_GCOV)
+        want = -np.cross(ucon[1:], bcon[1:])
+        assert np.allclose(Ec[(slice(None),) + idx], want, rtol=1e-14, atol=1e-15)
+    assert np.isfinite(Ec[(slice(None),) + Cc]).all()
+    for comp in range(3):
+        want = _gs05_corner(Ec, F, comp)
+        err = np.abs(E[comp][C] - want[C]).max()
+        assert err <= 1e-14 * np.abs(want[C]).max(), (comp, err)
+
+
+def test_courant_dt_closed_form_and_clamp(oracle):
+    n, bs = (12, 24, 6), (6, 12, 6)
+    _, grid = _flat(oracle, n, bs)
+    rho, u = 1.0, 0.1
+    prims = np.zeros((9,) + n)
+    prims[RHO], prims[UU], prims[KTOT] = rho, u, 1.0
+    P, PS = _fill(grid, n, prims, np.zeros((3,) + n))
+    _, d1 = grid.flux(1, P, PS)
+    _, d2 = grid.flux(2, P, PS)
+    _, d3 = grid.flux(3, P, PS)
+    cs = np.sqrt(GAM * (GAM - 1.0) * u / (rho + GAM * u))
+    want = 0.8 / (cs * (n[0] + n[1] + n[2]))
+    assert abs(grid.block_dt(d1, d2, d3) - want) <= 1e-14 * want
+    got, _ = grid.step(P.copy(), PS.copy(), 1e-6)
+    assert got == 1.3 * 1e-6
+    got, _ = grid.step(P.copy(), PS.copy(), 10.0 * want)
+    assert abs(got - want) <= 1e-14 * want
+
+
+@pytest.fixture(scope="module")
+def torus_bound(oracle):
+    desc = H.make_desc((8, 8, 8), (2, 2, 2), rout=40.0)
+    m = GT.oracle_metric(desc)
+    grid = GT.oracle_grid(oracle, desc, m)
+    p, ps = H.desc_torus(desc, H.torus_desc())
+    P0 = GT.blocks_to_global(desc, p)
+    PS0 = GT.blocks_to_global(desc, ps)
+    # a seeded random velocity field (physical speeds of order 1 / r) so that
+    # both signs of u^r reach the radial ghosts
+    rng = np.random.default_rng(9)
+    n = tuple(desc.bs[k] * desc.nb[k] for k in range(3))
+    I = _interior(n)
+    rc = m["rcen"].reshape(n[0] + 2 * NG, n[1] + 2 * NG)[I[0], I[1]][:, :, None]
+    P0[U1][I] += rng.normal(size=n) * 1.5 / rc
+    P0[U2][I] += rng.normal(size=n) * 0.1 / rc
+    P0[U3][I] += rng.normal(size=n) * 0.1 / rc
+    # and a toroidal field (the torus has none) so that every sign rule is visible
+    PS0[(slice(None),) + I] += rng.normal(size=(3,) + n) * 1e-3
+    P0[B3][I] += rng.normal(size=n) * 1e-3
+    # mark face-field ghosts so that untouched ones are visible
+    PS0[:, :NG] = PS0[:, -NG:] = 7.0
+    PS0[:, :, :NG] = PS0[:, :, -NG:] = 7.0
+    P, PS = P0.copy(), PS0.copy()
+    grid.bound(P, PS)
+    return desc, m, n, P0, PS0, P, PS
+
+
+def test_ghosts_periodic_in_phi(torus_bound):
+    desc, m, n, P0, PS0, P, PS = torus_bound
+    for A in (P, PS):
+        assert np.array_equal(A[..., :NG], A[..., n[2]:n[2] + NG])
+        assert np.array_equal(A[..., NG + n[2]:], A[..., NG:2 * NG])
+
+
+def test_ghosts_polar_mirror(torus_bound):
+    """Ghost (theta < 0 or > pi) = the cell mirrored through the axis at phi + pi;
+    theta and phi components of u and B change sign; face B^1 copied, face B^3
+    negated, face B^2 ghosts untouched."""
+    desc, m, n, P0, PS0, P, PS = torus_bound
+    zi = np.arange(NG, NG + n[2])
+    zpart = (zi - NG + n[2] // 2) % n[2] + NG
+    sign = np.ones(9)
+    sign[[U2, U3, B2, B3]] = -1.0
+    for g in range(NG):
+        for jg, js in ((NG - 1 - g, NG + g), (NG + n[1] + g, NG + n[1] - 1 - g)):
+            ghost = P[:, :, jg, NG:NG + n[2]]
+            src = P[:, :, js][:, :, zpart] * sign[:, None, None]
+            assert np.array_equal(ghost, src)
+            assert np.array_equal(PS[0, :, jg, NG:NG + n[2]], PS[0, :, js][:, zpart])
+            assert np.array_equal(PS[2, :, jg, NG:NG + n[2]], -PS[2, :, js][:, zpart])
+            assert np.all(PS[1, :, jg, NG:NG + n[2]] == 7.0)
+
+
+def test_ghosts_radial_outflow_without_inflow(oracle, torus_bound):
+    """Radial ghosts copy the edge cell; the velocity is kept when it carries
+    no flow from the ghost into the grid and is otherwise replaced by one that
+    does not (u^r <= 0 at the inner edge, u^r >= 0 at the outer)."""
+    desc, m, n, P0, PS0, P, PS = torus_bound
+    ncol = m["gdet"].shape[1]
+    D2 = n[1] + 2 * NG
+    kept = changed = 0
+    for side, (ig, isrc) in enumerate([(g, NG) for g in range(NG)] + [(NG + n[0] + g, NG + n[0] - 1) for g in range(NG)]):
+        outer = ig >= NG
+        for j in range(NG, NG + n[1]):
+            col = ig * D2 + j
+            gcov, gcon = m["gcov"][3, col], m["gcon"][3, col]
+            for z in range(NG, NG + n[2], 3):
+                pg = P[:, ig, j, z]
+                ps_ = P[:, isrc, j, z]
+                assert np.array_equal(pg[[RHO, UU, B1, B2, B3, KTOT]], ps_[[RHO, UU, B1, B2, B3, KTOT]])
+                u_copy = oracle.get_state(ps_, gcov, gcon)[0]
+                u_g = oracle.get_state(pg, gcov, gcon)[0]
+                bad = (u_copy[1] < 0) if outer else (u_copy[1] > 0)
+                if bad:
+                    changed += 1
+                    tiny = 1e-12 * np.abs(u_g).max()
+                    assert (u_g[1] >= -tiny) if outer else (u_g[1] <= tiny)
+                    assert not np.array_equal(pg, ps_)
+                else:
+                    kept += 1
+                    assert np.array_equal(pg, ps_)
+            assert np.array_equal(PS[1:, ig, j, NG:NG + n[2]], PS[1:, isrc, j, NG:NG + n[2]])
+            assert np.all(PS[0, ig, j, NG:NG + n[2]] == 7.0)
+    assert kept > 0 and changed > 0
