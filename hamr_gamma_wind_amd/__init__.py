@@ -273,7 +273,7 @@ class Context:
 
     def download(self, n):
         p = np.zeros(self.D + (NPR,))
-        ps = np.zeros(self.D + (3,))
+        ps = np.zeros(self.D + (4,))
         self._chk(self.L.hamr_download(self.h, n, _ptr(p), _ptr(ps)), "hamr_download")
         return p, ps
 

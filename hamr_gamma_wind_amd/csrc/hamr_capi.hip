@@ -499,7 +499,7 @@ int hamr_upload(hamr_ctx *c, int n, const double *p_aos, const double *ps_aos)
     for (size_t cell = 0; cell < N; cell++) {
         for (int k = 0; k < NPR; k++) tp[k * K + cell] = p_aos[cell * NPR + k];
         if (ps_aos)
-            for (int k = 0; k < 3; k++) ts[k * K + cell] = ps_aos[cell * 3 + k];
+            for (int k = 0; k < 3; k++) ts[k * K + cell] = ps_aos[cell * 4 + 1 + k];
     }
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (h2d(c, c->p + (size_t)n * NPR * K, tp.data(), NPR * K * sizeof(double))) return -1;
@@ -519,7 +519,7 @@ int hamr_download(hamr_ctx *c, int n, double *p_aos, double *ps_aos)
         if (p_aos)
             for (int k = 0; k < NPR; k++) p_aos[cell * NPR + k] = tp[k * K + cell];
         if (ps_aos)
-            for (int k = 0; k < 3; k++) ps_aos[cell * 3 + k] = ts[k * K + cell];
+            for (int k = 0; k < 3; k++) ps_aos[cell * 4 + 1 + k] = ts[k * K + cell];
     }
     return 0;
 }

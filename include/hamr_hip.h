@@ -116,7 +116,9 @@ int hamr_download_metric(hamr_ctx *ctx, int geo, double *gcov, double *gcon, dou
 int hamr_init_torus(hamr_ctx *ctx, const hamr_torus_desc *t);
 
 /* GPU_write / GPU_read (GPU_main.cu:309, :2646): one block's state in the
- * reference's host layout p[cell][NPR], ps[cell][3] (ghost-inclusive index_3D). */
+ * reference's host layout (memory.c:27): p[cell][NPR] and ps[cell][NDIM] with
+ * B^1..B^3 in components 1..3 (component 0 is not read, and not written back),
+ * cells in ghost-inclusive index_3D order.  ps may be NULL. */
 int hamr_upload(hamr_ctx *ctx, int n, const double *p_aos, const double *ps_aos);
 int hamr_download(hamr_ctx *ctx, int n, double *p_aos, double *ps_aos);
 /* Whole arrays in [block][component][cell] order (see HAMR_* selectors). */

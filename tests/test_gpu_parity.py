@@ -219,3 +219,26 @@ def test_ten_steps_track_oracle(setup):
     print("10 steps: max rel err per var", err, "dt", dt, dto, flush=True)
     assert np.all(err < 1e-9)
     assert abs(dt - dto) <= 1e-9 * dto
+
+
+def test_aos_staging_round_trip(setup):
+    """GPU_write / GPU_read layout: p[cell][NPR], ps[cell][NDIM] (B^1..B^3 in
+    components 1..3) in index_3D order, ghosts included <-> the device's
+    per-block component slabs, bit-exact."""
+    desc, ctx, grid, p0, ps0 = setup
+    ctx.set(H.HAMR_P, p0)
+    ctx.set(H.HAMR_PS, ps0)
+    for n in (0, ctx.nblk // 2, ctx.nblk - 1):
+        pa, psa = ctx.download(n)
+        assert np.array_equal(pa, np.moveaxis(p0[n], 0, -1))
+        assert np.array_equal(psa[..., 1:], np.moveaxis(ps0[n], 0, -1)) and np.all(psa[..., 0] == 0)
+        rng = np.random.default_rng(n)
+        qa = rng.normal(size=pa.shape)
+        qsa = rng.normal(size=psa.shape)
+        ctx.upload(n, qa, qsa)
+        assert np.array_equal(ctx.get(H.HAMR_P)[n], np.moveaxis(qa, -1, 0))
+        assert np.array_equal(ctx.get(H.HAMR_PS)[n], np.moveaxis(qsa[..., 1:], -1, 0))
+        back = ctx.download(n)
+        assert np.array_equal(back[0], qa) and np.array_equal(back[1][..., 1:], qsa[..., 1:])
+    ctx.set(H.HAMR_P, p0)
+    ctx.set(H.HAMR_PS, ps0)

@@ -479,3 +479,34 @@ def test_ghosts_radial_outflow_without_inflow(oracle, torus_bound):
             assert np.array_equal(PS[1:, ig, j, NG:NG + n[2]], PS[1:, isrc, j, NG:NG + n[2]])
             assert np.all(PS[0, ig, j, NG:NG + n[2]] == 7.0)
     assert kept > 0 and changed > 0
+
+
+def test_floors_after_a_torus_step(oracle):
+    """rho >= rho_floor r^-1.5 and u >= u_floor r^-2.5 in every interior cell
+    after a step; cells raised to a floor carry flag -333, and the floor hits
+    are counted."""
+    desc = H.make_desc((16, 8, 8), (2, 2, 2), rout=60.0)
+    n = tuple(desc.bs[k] * desc.nb[k] for k in range(3))
+    m = GT.oracle_metric(desc)
+    grid = GT.oracle_grid(oracle, desc, m)
+    p, ps = H.desc_torus(desc, H.torus_desc())
+    P = GT.blocks_to_global(desc, p)
+    PS = GT.blocks_to_global(desc, ps)
+    grid.bound(P, PS)
+    _, d1 = grid.flux(1, P, PS)
+    _, d2 = grid.flux(2, P, PS)
+    _, d3 = grid.flux(3, P, PS)
+    dt = grid.block_dt(d1, d2, d3)
+    grid.failcount[:] = [0, 0, 0, 0]
+    _, pflag = grid.step(P, PS, dt)
+    I = _interior(n)
+    r = m["rcen"].reshape(n[0] + 2 * NG, n[1] + 2 * NG)[I[0], I[1]][:, :, None]
+    rho_f = desc.rho_floor * r ** -1.5
+    u_f = desc.u_floor * r ** -2.5
+    rho, u, fl = P[RHO][I], P[UU][I], pflag[I]
+    assert np.all(rho >= rho_f * (1 - 1e-14)) and np.all(u >= u_f * (1 - 1e-14))
+    at_floor = (np.abs(rho / rho_f - 1) <= 1e-14) | (np.abs(u / u_f - 1) <= 1e-14)
+    assert np.all(fl[at_floor] == -333)
+    assert np.all(fl[~at_floor] != -333)
+    # failcount[3] sums the floor hits of the predictor's and the corrector's update
+    assert at_floor.sum() > 0 and grid.failcount[3] >= at_floor.sum()
