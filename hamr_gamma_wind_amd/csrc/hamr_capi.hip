@@ -25,7 +25,7 @@ struct hamr_ctx {
     // device buffers
     double *p = nullptr, *ph = nullptr, *ps = nullptr, *psh = nullptr;
     double *F[3] = {nullptr, nullptr, nullptr};
-    double *E = nullptr, *Ec = nullptr, *U0 = nullptr;
+    double *E = nullptr, *Ec = nullptr, *U0 = nullptr, *Un = nullptr;  // Un: updated conserved variables (scratch)
     int *pflag = nullptr, *failcnt = nullptr;
     unsigned long long *dtmin = nullptr;
     double *dt = nullptr;          // [0] dt of the next step, [1] previous
@@ -128,7 +128,7 @@ int hamr_create(const hamr_desc *desc, hamr_ctx **out)
     if (dalloc(c, &c->p, nb * NPR * K) || dalloc(c, &c->ph, nb * NPR * K) || dalloc(c, &c->ps, nb * 3 * K) ||
         dalloc(c, &c->psh, nb * 3 * K) || dalloc(c, &c->F[0], nb * NPR * K) || dalloc(c, &c->F[1], nb * NPR * K) ||
         dalloc(c, &c->F[2], nb * NPR * K) || dalloc(c, &c->E, nb * 3 * K) || dalloc(c, &c->Ec, nb * 3 * K) ||
-        dalloc(c, &c->U0, nb * NPR * K) || dalloc(c, &c->pflag, nb * K) || dalloc(c, &c->failcnt, nb * 4) ||
+        dalloc(c, &c->U0, nb * NPR * K) || dalloc(c, &c->Un, nb * NPR * K) || dalloc(c, &c->pflag, nb * K) || dalloc(c, &c->failcnt, nb * 4) ||
         dalloc(c, &c->dtmin, nb * 3) || dalloc(c, &c->dt, 4))
         return -1;
     const size_t KC = (size_t)L.kcol, ng = (size_t)L.ngeo;
@@ -571,10 +571,10 @@ int hamr_update(hamr_ctx *c, int flag, int n, double Dt)
 {
     if (!c || !valid_block(c, n)) return fail(c, "hamr_update: bad block");
     if (flag)
-        launch_update(c->P, c->ident + n, 1, 1, Dt, nullptr, 1.0, c->ph, c->ph, c->p, c->U0, c->ps, c->F[0], c->F[1],
+        launch_update(c->P, c->ident + n, 1, 1, Dt, nullptr, 1.0, c->ph, c->ph, c->p, c->U0, c->Un, c->ps, c->F[0], c->F[1],
                       c->F[2], c->pflag, c->failcnt, c->s);
     else
-        launch_update(c->P, c->ident + n, 1, 0, Dt, nullptr, 1.0, c->p, c->p, c->ph, c->U0, c->psh, c->F[0], c->F[1],
+        launch_update(c->P, c->ident + n, 1, 0, Dt, nullptr, 1.0, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1],
                       c->F[2], c->pflag, c->failcnt, c->s);
     return check_launch(c, "hamr_update");
 }
@@ -621,7 +621,7 @@ static void enqueue_step(hamr_ctx *c)
     launch_emf(P, bl, nb, c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, s);
     launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, s);
     launch_ct_update(P, bl, nb, 0., c->dt, 0.5, c->ps, c->psh, c->E, s);
-    launch_update(P, bl, nb, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->psh, c->F[0], c->F[1], c->F[2], c->pflag,
+    launch_update(P, bl, nb, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2], c->pflag,
                   c->failcnt, s);
     bound_all(c, c->ph, c->psh);
     // corrector: U0 -> p/ps with Dt = dt, fluxes from ph/psh, Courant minima recorded
@@ -629,7 +629,7 @@ static void enqueue_step(hamr_ctx *c)
     launch_emf(P, bl, nb, c->ph, c->F[0], c->F[1], c->F[2], c->Ec, c->E, s);
     launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, s);
     launch_ct_update(P, bl, nb, 0., c->dt, 1.0, c->ps, c->ps, c->E, s);
-    launch_update(P, bl, nb, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->U0, c->ps, c->F[0], c->F[1], c->F[2], c->pflag,
+    launch_update(P, bl, nb, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->U0, c->Un, c->ps, c->F[0], c->F[1], c->F[2], c->pflag,
                   c->failcnt, s);
     bound_all(c, c->p, c->ps);
     launch_dt_finalize(nb, c->dtmin, c->dt, s);
@@ -718,7 +718,7 @@ int hamr_time_stage(hamr_ctx *c, int stage, int nrep, double *avg_ms)
         case 4: launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, c->s); break;
         case 5: launch_ct_update(P, bl, nb, 0., c->dt, 0.5, c->ps, c->psh, c->E, c->s); break;
         case 6:
-            launch_update(P, bl, nb, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->psh, c->F[0], c->F[1], c->F[2],
+            launch_update(P, bl, nb, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2],
                           c->pflag, c->failcnt, c->s);
             break;
         default: bound_all(c, c->ph, c->psh); break;

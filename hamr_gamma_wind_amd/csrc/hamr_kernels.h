@@ -31,8 +31,9 @@ void launch_pole(const KParams &P, const int *ring, int nring, double dt_val, co
 void launch_ct_update(const KParams &P, const int *blist, int nb, double Dt_val, const double *dt_ptr,
                       double dt_mult, const double *psi, double *psf, const double *E, hipStream_t s);
 void launch_update(const KParams &P, const int *blist, int nb, int full, double Dt_val, const double *dt_ptr,
-                   double dt_mult, const double *pi, const double *pb, double *pf, double *U0, const double *psf,
-                   const double *F1, const double *F2, const double *F3, int *pflag, int *failcnt, hipStream_t s);
+                   double dt_mult, const double *pi, const double *pb, double *pf, double *U0, double *Unew,
+                   const double *psf, const double *F1, const double *F2, const double *F3, int *pflag, int *failcnt,
+                   hipStream_t s);
 void launch_bound(const KParams &P, int dir, const int *blist, int nb, double *p, double *ps, int *pflag,
                   hipStream_t s);
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);

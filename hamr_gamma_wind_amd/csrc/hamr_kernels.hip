@@ -291,29 +291,78 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, const int *__restr
 }
 
 // Conserved update, source, face-B average, Noble cascade, floors and flags:
-// the fixup kernel of GPU_fixup.h:5-425.  full = 0 (predictor): U^n from pi is
-// stored in U0, source and guess from pi.  full = 1 (corrector): U from U0,
-// source and guess from pb.
-__global__ void __launch_bounds__(256) k_update(KParams P, const int *__restrict__ blist, int full, double Dt_val,
-                                                const double *__restrict__ dt_ptr, double dt_mult,
-                                                const double *__restrict__ pi, const double *__restrict__ pb,
-                                                double *__restrict__ pf_out, double *__restrict__ U0,
-                                                const double *__restrict__ psf, const double *__restrict__ F1,
-                                                const double *__restrict__ F2, const double *__restrict__ F3,
-                                                int *__restrict__ pflag, int *__restrict__ failcnt)
+// the fixup kernel of GPU_fixup.h:5-425, as three kernels.  k_cons forms the
+// updated conserved variables (full = 0, predictor: U^n from pi is stored in
+// U0, source from pi; full = 1, corrector: U from U0, source from pb) and
+// writes them to Unew; k_invert runs the 2D inversion on Unew with the same
+// initial guess (pi or pb), applies the floors and sets the flags; k_fallback
+// finishes the cascade for the cells whose 2D inversion failed.  The split
+// keeps each kernel's registers low enough for several waves per SIMD, which
+// hides the latency of its loads; the arithmetic is that of one fused kernel.
+
+// Per-column metric values k_cons stages in LDS (every z-row of a block shares
+// one column): gcov, gcon, gdet at CENT, the connection, and the face gdets the
+// face-field average uses.
+enum { MV_GCOV = 0, MV_GCON = 10, MV_G = 20, MV_CONN = 21, MV_GF1 = 61, MV_GF1P = 62, MV_GF2 = 63, MV_GF2P = 64,
+       MV_GF3 = 65, MV_N = 66 };
+
+__device__ __forceinline__ double metric_value(const KParams &P, int geo, int v, int64_t col)
 {
+    const int64_t kc = P.kcol, gb = (int64_t)geo * NPOS;
+    if (v < MV_GCON) return P.gcov[((gb + CENT) * 10 + v) * kc + col];
+    if (v < MV_G) return P.gcon[((gb + CENT) * 10 + (v - MV_GCON)) * kc + col];
+    if (v == MV_G) return P.gdet[(gb + CENT) * kc + col];
+    if (v < MV_GF1) return P.conn[((int64_t)geo * 40 + (v - MV_CONN)) * kc + col];
+    if (v == MV_GF1) return P.gdet[(gb + FACE1) * kc + col];
+    if (v == MV_GF1P) return P.gdet[(gb + FACE1) * kc + col + P.D[1]];
+    if (v == MV_GF2) return P.gdet[(gb + FACE2) * kc + col];
+    if (v == MV_GF2P) return P.gdet[(gb + FACE2) * kc + col + 1];
+    return P.gdet[(gb + FACE3) * kc + col];
+}
+
+// Columns a 256-thread group of interior cells can touch.
+__host__ __device__ inline int cons_max_cols(int bs3) { return 255 / bs3 + 2; }
+
+__global__ void __launch_bounds__(256) k_cons(KParams P, const int *__restrict__ blist, int full, double Dt_val,
+                                              const double *__restrict__ dt_ptr, double dt_mult,
+                                              const double *__restrict__ pi, const double *__restrict__ pb,
+                                              double *__restrict__ U0, double *__restrict__ Unew,
+                                              const double *__restrict__ psf, const double *__restrict__ F1,
+                                              const double *__restrict__ F2, const double *__restrict__ F3)
+{
+    extern __shared__ double smem[];
     const int n = blist[blockIdx.y];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P.bs[0] * P.bs[1] * P.bs[2]) return;
+    const int bs2 = P.bs[1], bs3 = P.bs[2];
+    const int nint = P.bs[0] * bs2 * bs3;
+    const int t0 = blockIdx.x * blockDim.x;
+    const int t = t0 + threadIdx.x;
+    const int geo = P.blk[n].geo;
+    // stage the group's columns
+    const int q0 = t0 / bs3, q1 = mymin(t0 + (int)blockDim.x - 1, nint - 1) / bs3;
+    const int ncw = q1 - q0 + 1;
+    for (int e = threadIdx.x; e < MV_N * ncw; e += blockDim.x) {
+        const int v = e / ncw, qq = e - v * ncw, q = q0 + qq;
+        const int64_t col = (int64_t)(q / bs2 + NG) * P.D[1] + (q % bs2 + NG);
+        smem[e] = metric_value(P, geo, v, col);
+    }
+    __syncthreads();
+    if (t >= nint) return;
     int a, b, c;
-    region_index(t, P.bs[1], P.bs[2], a, b, c);
+    region_index(t, bs2, bs3, a, b, c);
     const int i = a + NG, j = b + NG, z = c + NG;
+    const int qq = t / bs3 - q0;
+    const double *mv = smem + qq;
     const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
     const int64_t K = P.ksize, si = (int64_t)P.D[1] * P.D[2], sj = P.D[2];
-    const int64_t g = ((int64_t)i * P.D[1] + j) * P.D[2] + z, col = (int64_t)i * P.D[1] + j;
+    const int64_t g = ((int64_t)i * P.D[1] + j) * P.D[2] + z;
     const int64_t base = (int64_t)n * NPR * K + g;
-    const int geo = P.blk[n].geo;
-    const Geom<double> G = load_geom(P, geo, CENT, col);
+    Geom<double> G;
+#pragma unroll
+    for (int s = 0; s < 10; s++) {
+        G.gcov[s] = mv[(MV_GCOV + s) * ncw];
+        G.gcon[s] = mv[(MV_GCON + s) * ncw];
+    }
+    G.g = mv[MV_G * ncw];
     double pf[NPR], U[NPR], dU[NPR];
     State<double> q;
     if (!full) {
@@ -331,71 +380,173 @@ __global__ void __launch_bounds__(256) k_update(KParams P, const int *__restrict
         for (int k = 0; k < NPR; k++) pf[k] = pb[base + k * K];
         get_state(pf, G, q);
     }
+    double f1l[NPR], f1h[NPR], f2l[NPR], f2h[NPR], f3l[NPR], f3h[NPR];
 #pragma unroll
     for (int k = 0; k < NPR; k++) {
         const double *f1 = F1 + base + k * K, *f2 = F2 + base + k * K, *f3 = F3 + base + k * K;
-        U[k] -= Dt * (f1[si] - f1[0]) / (P.dx[0] * 1.0);
-        U[k] -= Dt * (f2[sj] - f2[0]) / (P.dx[1] * 1.0);
-        U[k] -= Dt * (f3[1] - f3[0]) / (P.dx[2] * 1.0);
+        f1l[k] = f1[0];
+        f1h[k] = f1[si];
+        f2l[k] = f2[0];
+        f2h[k] = f2[sj];
+        f3l[k] = f3[0];
+        f3h[k] = f3[1];
     }
-    const double *cn = P.conn + (int64_t)geo * 40 * P.kcol + col;
-    const int64_t kc = P.kcol;
-    source(pf, q, [&](int lam, int s) { return cn[(lam * 10 + s) * kc]; }, G.g, P.gam, dU);
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        U[k] -= Dt * (f1h[k] - f1l[k]) / (P.dx[0] * 1.0);
+        U[k] -= Dt * (f2h[k] - f2l[k]) / (P.dx[1] * 1.0);
+        U[k] -= Dt * (f3h[k] - f3l[k]) / (P.dx[2] * 1.0);
+    }
+    source(pf, q, [&](int lam, int s) { return mv[(MV_CONN + lam * 10 + s) * ncw]; }, G.g, P.gam, dU);
 #pragma unroll
     for (int k = 0; k < NPR; k++) U[k] += Dt * (dU[k]);
     {
-        const double *gd = P.gdet + (int64_t)geo * NPOS * P.kcol + col;
-        const double *s = psf + (int64_t)n * 3 * K + g;
+        const double *sp = psf + (int64_t)n * 3 * K + g;
         U[B1] = 0.0;
         U[B2] = 0.0;
-        U[B1] += (s[0] * gd[FACE1 * P.kcol] + s[si] * gd[FACE1 * P.kcol + P.D[1]]) / (2.0 * 1.0);
-        U[B2] += (s[K] * gd[FACE2 * P.kcol] + s[K + sj] * gd[FACE2 * P.kcol + 1]) / (2.0 * 1.0);
-        U[B3] = (s[2 * K] * gd[FACE3 * P.kcol] + s[2 * K + 1] * gd[FACE3 * P.kcol]) / 2.0;
+        U[B1] += (sp[0] * mv[MV_GF1 * ncw] + sp[si] * mv[MV_GF1P * ncw]) / (2.0 * 1.0);
+        U[B2] += (sp[K] * mv[MV_GF2 * ncw] + sp[K + sj] * mv[MV_GF2P * ncw]) / (2.0 * 1.0);
+        U[B3] = (sp[2 * K] * mv[MV_GF3 * ncw] + sp[2 * K + 1] * mv[MV_GF3 * ncw]) / 2.0;
     }
-    int flag = utoprim(0, U, G.gcov, G.gcon, G.g, P.gam, pf);
-    if (flag) {
-        atomicAdd(&failcnt[n * 4 + 0], 1);
-        flag = utoprim(1, U, G.gcov, G.gcon, G.g, P.gam, pf);
-        if (flag) {
-            atomicAdd(&failcnt[n * 4 + 1], 1);
-            flag = utoprim(2, U, G.gcov, G.gcon, G.g, P.gam, pf);
-            if (flag) atomicAdd(&failcnt[n * 4 + 2], 1);
-        }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) Unew[base + k * K] = U[k];
+}
+
+// Floors (the build's specification: rho >= rho_floor r^-1.5, u >= u_floor
+// r^-2.5, flag -333 as GPU_fixup.h:399-415) and the result of one cell.
+__device__ __forceinline__ void finish_cell(const KParams &P, int n, int geo, int64_t col, int64_t g, int64_t base,
+                                            int flag, double *pf, double *__restrict__ pf_out,
+                                            int *__restrict__ pflag, int *__restrict__ failcnt)
+{
+    const int64_t K = P.ksize;
+    const double r = P.rcen[(int64_t)geo * P.kcol + col], sr = sqrt(r);
+    int hit = 0;
+    if (P.rho_floor > 0.) {
+        const double rhoflr = P.rho_floor / (r * sr);
+        if (pf[RHO] < rhoflr) { pf[RHO] = rhoflr; hit = 1; }
     }
-    {
-        const double r = P.rcen[(int64_t)geo * P.kcol + col], sr = sqrt(r);
-        int hit = 0;
-        if (P.rho_floor > 0.) {
-            const double rhoflr = P.rho_floor / (r * sr);
-            if (pf[RHO] < rhoflr) { pf[RHO] = rhoflr; hit = 1; }
-        }
-        if (P.u_floor > 0.) {
-            const double uflr = P.u_floor / (r * r * sr);
-            if (pf[UU] < uflr) { pf[UU] = uflr; hit = 1; }
-        }
-        if (hit) {
-            flag = -333;
-            atomicAdd(&failcnt[n * 4 + 3], 1);
-        }
+    if (P.u_floor > 0.) {
+        const double uflr = P.u_floor / (r * r * sr);
+        if (pf[UU] < uflr) { pf[UU] = uflr; hit = 1; }
+    }
+    if (hit) {
+        flag = -333;
+        atomicAdd(&failcnt[n * 4 + 3], 1);
     }
     pflag[(int64_t)n * K + g] = flag;
 #pragma unroll
     for (int k = 0; k < NPR; k++) pf_out[base + k * K] = pf[k];
 }
 
+// Marks a cell whose 2D inversion failed until k_fallback has run the 1D
+// inversions on it (never a return code of the inversions).
+constexpr int kFallbackPending = 0x40000000;
+
+// 2D inversion of every interior cell.  A failed cell keeps the primitives the
+// 2D solver left in pf_out and is marked for k_fallback, so that the rare
+// 1D inversions do not set this kernel's register budget.
+__global__ void __launch_bounds__(256) k_invert(KParams P, const int *__restrict__ blist, int full,
+                                                const double *__restrict__ pi, const double *__restrict__ pb,
+                                                const double *__restrict__ Unew, double *__restrict__ pf_out,
+                                                int *__restrict__ pflag, int *__restrict__ failcnt)
+{
+    const int n = blist[blockIdx.y];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.bs[0] * P.bs[1] * P.bs[2]) return;
+    int a, b, c;
+    region_index(t, P.bs[1], P.bs[2], a, b, c);
+    const int i = a + NG, j = b + NG, z = c + NG;
+    const int64_t K = P.ksize;
+    const int64_t g = ((int64_t)i * P.D[1] + j) * P.D[2] + z, col = (int64_t)i * P.D[1] + j;
+    const int64_t base = (int64_t)n * NPR * K + g;
+    const int geo = P.blk[n].geo;
+    const Geom<double> G = load_geom(P, geo, CENT, col);
+    const double *guess = full ? pb : pi;
+    double pf[NPR], U[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        U[k] = Unew[base + k * K];
+        pf[k] = guess[base + k * K];
+    }
+    const int flag = utoprim(0, U, G.gcov, G.gcon, G.g, P.gam, pf);
+    if (flag == 0) {
+        finish_cell(P, n, geo, col, g, base, 0, pf, pf_out, pflag, failcnt);
+        return;
+    }
+    atomicAdd(&failcnt[n * 4 + 0], 1);
+    pflag[(int64_t)n * K + g] = kFallbackPending;
+#pragma unroll
+    for (int k = 0; k < NPR; k++) pf_out[base + k * K] = pf[k];
+}
+
+// The rest of the cascade for the cells k_invert marked: entropy-based 1D
+// (GPU_utoprim_1dvsq2fix1.h), then 1D (GPU_utoprim_1dfix1.h), then floors.
+__global__ void __launch_bounds__(256) k_fallback(KParams P, const int *__restrict__ blist,
+                                                  const double *__restrict__ Unew, double *__restrict__ pf_out,
+                                                  int *__restrict__ pflag, int *__restrict__ failcnt)
+{
+    const int n = blist[blockIdx.y];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.bs[0] * P.bs[1] * P.bs[2]) return;
+    int a, b, c;
+    region_index(t, P.bs[1], P.bs[2], a, b, c);
+    const int i = a + NG, j = b + NG, z = c + NG;
+    const int64_t K = P.ksize;
+    const int64_t g = ((int64_t)i * P.D[1] + j) * P.D[2] + z, col = (int64_t)i * P.D[1] + j;
+    if (pflag[(int64_t)n * K + g] != kFallbackPending) return;
+    const int64_t base = (int64_t)n * NPR * K + g;
+    const int geo = P.blk[n].geo;
+    const Geom<double> G = load_geom(P, geo, CENT, col);
+    double pf[NPR], U[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        U[k] = Unew[base + k * K];
+        pf[k] = pf_out[base + k * K];
+    }
+    int flag = utoprim(1, U, G.gcov, G.gcon, G.g, P.gam, pf);
+    if (flag) {
+        atomicAdd(&failcnt[n * 4 + 1], 1);
+        flag = utoprim(2, U, G.gcov, G.gcon, G.g, P.gam, pf);
+        if (flag) atomicAdd(&failcnt[n * 4 + 2], 1);
+    }
+    finish_cell(P, n, geo, col, g, base, flag, pf, pf_out, pflag, failcnt);
+}
+
 // Ghost fill of one direction for the listed blocks: same-level copy, radial
 // outflow with inflow check, or the polar mirror from the pi-shifted partner.
 // Stage order x1, x2, x3, each over the full extent of the others.
+//
+// Thread mapping: for x1 and x2 the fastest thread index runs along z, so
+// every access of a wavefront is contiguous.  For x3 the ghosts are the first
+// and last NG cells of each z-row; a group of 2*NG consecutive threads writes
+// the upper ghosts of row r-1 and the lower ghosts of row r, which are adjacent
+// in memory, so each group stores one contiguous 48-byte segment.
 template <int DIR>
 __global__ void __launch_bounds__(256) k_bound(KParams P, const int *__restrict__ blist, double *p, double *ps,
                                                int *pflag)
 {
     const int n = blist[blockIdx.y];
     const int d1 = P.D[0], d2 = P.D[1], d3 = P.D[2];
-    const int ta = (DIR == 1) ? d2 : d1, tb = (DIR == 3) ? d2 : d3;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * NG * ta * tb) return;
-    const int bb = t % tb, aa = (t / tb) % ta, gg = (t / (tb * ta)) % NG, side = t / (tb * ta * NG);
+    int aa, bb, gg, side;
+    if (DIR == 3) {
+        const int nrow = d1 * d2;
+        if (t >= 2 * NG * (nrow + 1)) return;
+        const int g6 = t % (2 * NG), r = t / (2 * NG);
+        side = (g6 < NG) ? 1 : 0;
+        gg = side ? g6 : g6 - NG;
+        const int row = side ? r - 1 : r;
+        if (row < 0 || row >= nrow) return;
+        aa = row / d2;
+        bb = row % d2;
+    } else {
+        const int ta = (DIR == 1) ? d2 : d1, tb = d3;
+        if (t >= 2 * NG * ta * tb) return;
+        bb = t % tb;
+        aa = (t / tb) % ta;
+        gg = (t / (tb * ta)) % NG;
+        side = t / (tb * ta * NG);
+    }
     const BlockDev &B = P.blk[n];
     const int face = 2 * (DIR - 1) + side;
     const int kind = B.kind[face];
@@ -541,20 +692,24 @@ void launch_ct_update(const KParams &P, const int *blist, int nb, double Dt_val,
 }
 
 void launch_update(const KParams &P, const int *blist, int nb, int full, double Dt_val, const double *dt_ptr,
-                   double dt_mult, const double *pi, const double *pb, double *pf, double *U0, const double *psf,
-                   const double *F1, const double *F2, const double *F3, int *pflag, int *failcnt, hipStream_t s)
+                   double dt_mult, const double *pi, const double *pb, double *pf, double *U0, double *Unew,
+                   const double *psf, const double *F1, const double *F2, const double *F3, int *pflag, int *failcnt,
+                   hipStream_t s)
 {
     const int64_t w = (int64_t)P.bs[0] * P.bs[1] * P.bs[2];
-    hipLaunchKernelGGL(k_update, grid_for(w, nb), dim3(256), 0, s, P, blist, full, Dt_val, dt_ptr, dt_mult, pi, pb, pf,
-                       U0, psf, F1, F2, F3, pflag, failcnt);
+    const size_t lds = sizeof(double) * MV_N * cons_max_cols(P.bs[2]);
+    hipLaunchKernelGGL(k_cons, grid_for(w, nb), dim3(256), lds, s, P, blist, full, Dt_val, dt_ptr, dt_mult, pi, pb, U0,
+                       Unew, psf, F1, F2, F3);
+    hipLaunchKernelGGL(k_invert, grid_for(w, nb), dim3(256), 0, s, P, blist, full, pi, pb, Unew, pf, pflag, failcnt);
+    hipLaunchKernelGGL(k_fallback, grid_for(w, nb), dim3(256), 0, s, P, blist, Unew, pf, pflag, failcnt);
 }
 
 void launch_bound(const KParams &P, int dir, const int *blist, int nb, double *p, double *ps, int *pflag,
                   hipStream_t s)
 {
     const int d1 = P.D[0], d2 = P.D[1], d3 = P.D[2];
-    const int ta = (dir == 1) ? d2 : d1, tb = (dir == 3) ? d2 : d3;
-    const dim3 g = grid_for((int64_t)2 * NG * ta * tb, nb);
+    const int64_t work = (dir == 3) ? (int64_t)2 * NG * (d1 * d2 + 1) : (int64_t)2 * NG * ((dir == 1) ? d2 : d1) * d3;
+    const dim3 g = grid_for(work, nb);
     if (dir == 1) hipLaunchKernelGGL(k_bound<1>, g, dim3(256), 0, s, P, blist, p, ps, pflag);
     else if (dir == 2) hipLaunchKernelGGL(k_bound<2>, g, dim3(256), 0, s, P, blist, p, ps, pflag);
     else hipLaunchKernelGGL(k_bound<3>, g, dim3(256), 0, s, P, blist, p, ps, pflag);
