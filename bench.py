@@ -1,16 +1,25 @@
 """Throughput of the MI355X GRMHD fluid step (zone-updates/s) on one GPU.
 
-Workload (default): the FP64 torus step on a single-level 256x128x128 grid
-(r 1.2-1000 M log-spaced, Kerr a = 0.9375, Gamma = 13/9, FM torus r_in = 6,
-r_max = 12, poloidal loop normalised to min beta = 100, seeded +-4% perturbation
-of u, floors 1e-5 r^-1.5 and 1e-7 r^-2.5) as 8x4x4 blocks of 32^3 interior
-zones plus 3 ghost zones, all resident in HBM.  A step is one predictor +
-corrector over every block (PPM + HLL + CT + Noble 2D + ghost fill, Courant dt
-kept on the device).  A zone-update is one interior zone advanced one full step.
+Workload (default): the torus on a single-level 256x128x128 grid (r 1.2-1000 M
+log-spaced, Kerr a = 0.9375, Gamma = 13/9, FM torus r_in = 6, r_max = 12,
+poloidal loop normalised to min beta = 100, seeded +-4% perturbation of u,
+floors 1e-5 r^-1.5 and 1e-7 r^-2.5) as 8x4x4 blocks of 32^3 interior zones
+plus 3 ghost zones, all resident in HBM.  A step is one predictor + corrector
+over every block (PPM + HLL + CT + Noble 2D + ghost fill, Courant dt kept on the
+device), replayed from a HIP graph.  A zone-update is one interior zone
+advanced one full step.
 
-Prints ONE JSON line with the contract's fields plus `roofline` (the dominant
-kernel, timed with HIP events on the step's stream) and `cpu_baseline` (the
-FP64 CPU oracle on the host cores, on a bounded sample of the same workload).
+Both arithmetics run from the same initial state for warmup + steps steps:
+the FP64 step and the mixed step (FP32 reconstruction, fluxes and EMFs; FP64
+metric, face-field update, compensated conserved update and inversions).
+`value` is the mixed step's throughput; the FP64 throughput, the speed-up and
+the error of the mixed state against the FP64 state after the same number of
+steps (max and L1, relative, for rho and the field) are reported beside it.
+
+Prints ONE JSON line with the contract's fields plus `roofline` (the mixed
+step's dominant stage, timed with HIP events on the step's stream) and
+`cpu_baseline` (the FP64 CPU oracle on the host cores, on a bounded sample of
+the same workload).
 """
 import argparse
 import json
@@ -31,21 +40,25 @@ CONFIGS = {
     "c1": ((16, 8, 8), (4, 4, 4)),
 }
 
-# Algorithmic HBM bytes per interior zone per launch (FP64, each array element
-# once per stage; halo re-reads and intermediates excluded):
-#   flux (one direction): 9 prims + 1 face B read, 9 fluxes written = 19 values
-#   EMF: U1-3, B1-3, 6 flux B-components, 3 mass fluxes read, 3 EMFs written = 18
-#   face-B update: 3 EMFs + 3 face B read, 3 written = 9
-#   update + inversion: P 9 + U^n 9 + fluxes 27 + face B 3 read, P 9 written = 57 (+ 4 B flag)
+# Algorithmic HBM bytes per interior zone per launch (each array element once
+# per stage; halo re-reads and intermediates excluded).  Values are FP64 (8 B)
+# except the fluxes and EMFs of the mixed step (FP32, 4 B):
+#   flux (one direction): 9 prims + 1 face B read, 9 fluxes written
+#   EMF: U1-3, B1-3, 6 flux B-components, 3 mass fluxes read, 3 EMFs written
+#   face-B update: 3 EMFs + 3 face B read, 3 written
+#   update + inversion (predictor operands): prims 9, fluxes 27, face B 3 read;
+#     U^n 9, prims 9 written; 4 B flag
 #   ghost fill: (NPR+3) values per ghost cell, both ways, 24*g with g the ghost/interior ratio
 STAGES = ["flux_x1", "flux_x2", "flux_x3", "emf", "pole", "ct_update", "update", "bound"]
 
 
-def stage_bytes(stage, bs):
+def stage_bytes(stage, bs, mixed=False):
     b1, b2, b3 = bs
     g = ((b1 + 6) * (b2 + 6) * (b3 + 6) - b1 * b2 * b3) / (b1 * b2 * b3)
-    per_zone = {"flux_x1": 19 * 8, "flux_x2": 19 * 8, "flux_x3": 19 * 8, "emf": 18 * 8, "pole": 0.0,
-                "ct_update": 9 * 8, "update": 57 * 8 + 4, "bound": 24 * g * 8}
+    w = 4 if mixed else 8
+    per_zone = {"flux_x1": 10 * 8 + 9 * w, "flux_x2": 10 * 8 + 9 * w, "flux_x3": 10 * 8 + 9 * w,
+                "emf": 6 * 8 + 9 * w + 3 * w, "pole": 0.0, "ct_update": 3 * w + 6 * 8,
+                "update": 12 * 8 + 27 * w + 18 * 8 + 4, "bound": 24 * g * 8}
     return per_zone[stage]
 
 
@@ -89,21 +102,45 @@ def cpu_baseline(desc, nsteps):
     return nz * nsteps / el, cores, el
 
 
+def run_mode(H, ctx, p0, ps0, mode, args):
+    """Time `steps` graph-replayed steps after `warmup` steps from (p0, ps0);
+    return (zone-updates/s, ms/step, final state, per-stage ms)."""
+    ctx.set_precision(mode)
+    ctx.set(H.HAMR_P, p0)
+    ctx.set(H.HAMR_PS, ps0)
+    ctx.clear_flags()
+    graph = not args.no_graph
+    dt0 = ctx.estimate_dt()
+    dt = ctx.step(args.warmup, dt0, graph=graph) if args.warmup > 0 else dt0
+    ctx.sync()
+    t0 = time.perf_counter()
+    dt = ctx.step(args.steps, None, graph=graph)  # synchronises at the end (reads dt back)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    p = ctx.get(H.HAMR_P)
+    _, fi = ctx.all_flags()
+    stage_ms = {s: ctx.time_stage(i, 5) for i, s in enumerate(STAGES)}
+    return el, p, stage_ms, dt, fi
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=97)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-graph", action="store_true", help="enqueue each step instead of replaying a HIP graph")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU oracle baseline (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one streaming copy of this many MiB (counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None, help="run one arithmetic only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (each GPU box holds one MI355X); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    import numpy as np
 
     import hamr_gamma_wind_amd as H
 
@@ -114,23 +151,22 @@ def main():
     ctx.set_grid()
     ctx.init_torus(H.torus_desc())
     ctx.boundprim(1)
-    dt0 = ctx.estimate_dt()
-    graph = not args.no_graph
-    dt = ctx.step(args.warmup, dt0, graph=graph) if args.warmup > 0 else dt0
-    ctx.sync()
-    t0 = time.perf_counter()
-    dt = ctx.step(args.steps, None, graph=graph)  # synchronises at the end (reads dt back)
-    ctx.sync()
-    el = time.perf_counter() - t0
-    value = nz * args.steps / el
-    _, fi = ctx.all_flags()
+    p0 = ctx.get(H.HAMR_P)
+    ps0 = ctx.get(H.HAMR_PS)
 
-    # per-stage device time (HIP events on the step's stream)
-    stage_ms = {s: ctx.time_stage(i, 5) for i, s in enumerate(STAGES)}
+    modes = {"fp64": H.HAMR_PREC_FP64, "mixed": H.HAMR_PREC_MIXED}
+    if args.only:
+        modes = {args.only: modes[args.only]}
+    res = {name: run_mode(H, ctx, p0, ps0, m, args) for name, m in modes.items()}
+    head = "mixed" if "mixed" in res else "fp64"
+    el, p_head, stage_ms, dt, fi = res[head]
+    value = nz * args.steps / el
+    mixed = head == "mixed"
+
     # one step runs every stage twice (predictor, corrector)
     per_step = {s: 2 * ms for s, ms in stage_ms.items()}
     dom = max(per_step, key=per_step.get)
-    bpz = stage_bytes(dom, bs)
+    bpz = stage_bytes(dom, bs, mixed)
     achieved = nz * bpz / (stage_ms[dom] * 1e-3) / 1e9 if bpz > 0 else 0.0
     traffic = read_traffic(dom, args.config)
     copy = None
@@ -139,6 +175,21 @@ def main():
         ms = ctx.copy_probe(nbytes, 1)
         copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
     ctx.close()
+
+    err = None
+    fp64 = None
+    if "fp64" in res and "mixed" in res:
+        I = slice(3, -3)
+        a = res["mixed"][1][:, :, I, I, I]
+        b = res["fp64"][1][:, :, I, I, I]
+        err = {"steps": args.warmup + args.steps}
+        for name, k in (("rho", 0), ("B1", 5), ("B2", 6), ("B3", 7), ("u", 1)):
+            d = np.abs(a[:, k] - b[:, k])
+            err[name + "_max"] = float(d.max() / np.abs(b[:, k]).max())
+            err[name + "_l1"] = float(d.sum() / np.abs(b[:, k]).sum())
+        el64 = res["fp64"][0]
+        fp64 = {"value": nz * args.steps / el64, "ms_per_step": el64 / args.steps * 1e3,
+                "stages_ms_per_step": {k: round(2 * v, 4) for k, v in res["fp64"][2].items()}}
 
     cpu = None
     if args.cpu_steps > 0:
@@ -158,18 +209,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64 (mixed)" if mixed else "f64",
         "data": "synthetic (seeded Fishbone-Moncrief torus initial state)",
-        "config": {"workload": "FP64 GRMHD step, torus %dx%dx%d as %dx%dx%d blocks of %dx%dx%d + 3 ghosts, "
+        "config": {"workload": "%s GRMHD step, torus %dx%dx%d as %dx%dx%d blocks of %dx%dx%d + 3 ghosts, "
                                "PPM+HLL+CT+Noble2D, %s"
-                               % (bs[0] * nb[0], bs[1] * nb[1], bs[2] * nb[2], nb[0], nb[1], nb[2], bs[0], bs[1],
-                                  bs[2], "HIP graph replay" if graph else "stream"),
+                               % ("mixed-precision" if mixed else "FP64", bs[0] * nb[0], bs[1] * nb[1],
+                                  bs[2] * nb[2], nb[0], nb[1], nb[2], bs[0], bs[1], bs[2],
+                                  "HIP graph replay" if not args.no_graph else "stream"),
                    "zones": nz, "dt_final": dt, "inversion_failures_2d": int(fi[:, 0].sum()),
                    "floor_hits": int(fi[:, 3].sum())},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": nz * bpz, "avg_launch_ms": stage_ms[dom]},
         "stages_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
+        "fp64": fp64,
+        "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
+        "error_vs_fp64": err,
         "copy_probe": copy,
         "cpu_baseline": cpu,
     }

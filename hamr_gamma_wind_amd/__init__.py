@@ -21,6 +21,7 @@ FACE1, FACE2, CORN, CENT, FACE3 = range(5)
 (HAMR_P, HAMR_PS, HAMR_PH, HAMR_PSH, HAMR_F1, HAMR_F2, HAMR_F3, HAMR_EMF, HAMR_ECENT, HAMR_U0) = range(10)
 NCOMP = {HAMR_P: 9, HAMR_PS: 3, HAMR_PH: 9, HAMR_PSH: 3, HAMR_F1: 9, HAMR_F2: 9, HAMR_F3: 9, HAMR_EMF: 3,
          HAMR_ECENT: 3, HAMR_U0: 9}
+HAMR_PREC_FP64, HAMR_PREC_MIXED = 0, 1
 
 
 class HamrError(RuntimeError):
@@ -117,6 +118,8 @@ def load_library(path=None):
         "hamr_sync": [ctxp],
         "hamr_time_stage": [ctxp, C.c_int, C.c_int, _dp],
         "hamr_copy_probe": [ctxp, C.c_longlong, C.c_int, _dp],
+        "hamr_set_precision": [ctxp, C.c_int],
+        "hamr_get_precision": [ctxp, _ip],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -319,6 +322,15 @@ class Context:
 
     def sync(self):
         self._chk(self.L.hamr_sync(self.h), "hamr_sync")
+
+    def set_precision(self, mode):
+        """HAMR_PREC_FP64 or HAMR_PREC_MIXED (FP32 reconstruction, fluxes, EMFs)."""
+        self._chk(self.L.hamr_set_precision(self.h, mode), "hamr_set_precision")
+
+    def precision(self):
+        v = C.c_int()
+        self._chk(self.L.hamr_get_precision(self.h, C.byref(v)), "hamr_get_precision")
+        return v.value
 
     def time_stage(self, stage, nrep=10):
         v = C.c_double()

@@ -36,6 +36,7 @@ struct hamr_ctx {
     int *ring = nullptr;           // [side][ring][c3] block ids of the pole rings
     int nring = 0;
     hipGraphExec_t graph = nullptr;
+    int prec = HAMR_PREC_FP64;     // arithmetic of reconstruction, fluxes and EMFs
     std::vector<void *> allocs;
 };
 
@@ -463,16 +464,31 @@ static int array_info(hamr_ctx *c, int which, double **ptr, int *ncomp)
     }
 }
 
+// Fluxes and EMFs are float arrays in the mixed step.
+static bool is_float_array(const hamr_ctx *c, int which)
+{
+    return c->prec == HAMR_PREC_MIXED &&
+           (which == HAMR_F1 || which == HAMR_F2 || which == HAMR_F3 || which == HAMR_EMF || which == HAMR_ECENT);
+}
+
 int hamr_set_array(hamr_ctx *c, int which, const double *src)
 {
     if (!c || !src) return -1;
     double *d;
     int nc;
     if (array_info(c, which, &d, &nc)) return -1;
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell;
+    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell, rows = (size_t)c->L.nblk * nc;
     HIPCHK(c, hipStreamSynchronize(c->s));
-    HIPCHK(c, hipMemcpy2DAsync(d, K * sizeof(double), src, N * sizeof(double), N * sizeof(double),
-                               (size_t)c->L.nblk * nc, hipMemcpyHostToDevice, c->s));
+    if (is_float_array(c, which)) {
+        std::vector<float> t(rows * N);
+        for (size_t e = 0; e < t.size(); e++) t[e] = (float)src[e];
+        HIPCHK(c, hipMemcpy2DAsync(d, K * sizeof(float), t.data(), N * sizeof(float), N * sizeof(float), rows,
+                                   hipMemcpyHostToDevice, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        return 0;
+    }
+    HIPCHK(c, hipMemcpy2DAsync(d, K * sizeof(double), src, N * sizeof(double), N * sizeof(double), rows,
+                               hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return 0;
 }
@@ -483,11 +499,39 @@ int hamr_get_array(hamr_ctx *c, int which, double *dst)
     double *d;
     int nc;
     if (array_info(c, which, &d, &nc)) return -1;
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell;
+    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell, rows = (size_t)c->L.nblk * nc;
     HIPCHK(c, hipStreamSynchronize(c->s));
-    HIPCHK(c, hipMemcpy2DAsync(dst, N * sizeof(double), d, K * sizeof(double), N * sizeof(double),
-                               (size_t)c->L.nblk * nc, hipMemcpyDeviceToHost, c->s));
+    if (is_float_array(c, which)) {
+        std::vector<float> t(rows * N);
+        HIPCHK(c, hipMemcpy2DAsync(t.data(), N * sizeof(float), d, K * sizeof(float), N * sizeof(float), rows,
+                                   hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        for (size_t e = 0; e < t.size(); e++) dst[e] = (double)t[e];
+        return 0;
+    }
+    HIPCHK(c, hipMemcpy2DAsync(dst, N * sizeof(double), d, K * sizeof(double), N * sizeof(double), rows,
+                               hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int hamr_set_precision(hamr_ctx *c, int mode)
+{
+    if (!c) return -1;
+    if (mode != HAMR_PREC_FP64 && mode != HAMR_PREC_MIXED) return fail(c, "hamr_set_precision: unknown mode");
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (mode != c->prec && c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    c->prec = mode;
+    return 0;
+}
+
+int hamr_get_precision(const hamr_ctx *c, int *mode)
+{
+    if (!c || !mode) return -1;
+    *mode = c->prec;
     return 0;
 }
 
@@ -528,7 +572,7 @@ int hamr_download(hamr_ctx *c, int n, double *p_aos, double *ps_aos)
 int hamr_fluxcalc(hamr_ctx *c, int dir, int flag, int n)
 {
     if (!c || !valid_block(c, n) || dir < 1 || dir > 3) return fail(c, "hamr_fluxcalc: bad arguments");
-    launch_flux(c->P, dir, c->ident + n, 1, flag ? c->ph : c->p, flag ? c->psh : c->ps, c->F[dir - 1], c->dtmin, c->s);
+    launch_flux(c->P, dir, c->ident + n, 1, flag ? c->ph : c->p, flag ? c->psh : c->ps, c->F[dir - 1], c->dtmin, c->prec, c->s);
     return check_launch(c, "hamr_fluxcalc");
 }
 
@@ -549,21 +593,21 @@ int hamr_block_dt(hamr_ctx *c, int n, int dir, double *out)
 int hamr_ct_emf(hamr_ctx *c, int flag, int n)
 {
     if (!c || !valid_block(c, n)) return fail(c, "hamr_ct_emf: bad block");
-    launch_emf(c->P, c->ident + n, 1, flag ? c->ph : c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->s);
+    launch_emf(c->P, c->ident + n, 1, flag ? c->ph : c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, c->s);
     return check_launch(c, "hamr_ct_emf");
 }
 
 int hamr_ct_bound(hamr_ctx *c, double dt)
 {
     if (!c) return -1;
-    launch_pole(c->P, c->ring, c->nring, dt, nullptr, c->E, c->s);
+    launch_pole(c->P, c->ring, c->nring, dt, nullptr, c->E, c->prec, c->s);
     return check_launch(c, "hamr_ct_bound");
 }
 
 int hamr_ct_update(hamr_ctx *c, int flag, double Dt, int n)
 {
     if (!c || !valid_block(c, n)) return fail(c, "hamr_ct_update: bad block");
-    launch_ct_update(c->P, c->ident + n, 1, Dt, nullptr, 1.0, c->ps, flag ? c->ps : c->psh, c->E, c->s);
+    launch_ct_update(c->P, c->ident + n, 1, Dt, nullptr, 1.0, c->ps, flag ? c->ps : c->psh, c->E, c->prec, c->s);
     return check_launch(c, "hamr_ct_update");
 }
 
@@ -572,10 +616,10 @@ int hamr_update(hamr_ctx *c, int flag, int n, double Dt)
     if (!c || !valid_block(c, n)) return fail(c, "hamr_update: bad block");
     if (flag)
         launch_update(c->P, c->ident + n, 1, 1, Dt, nullptr, 1.0, c->ph, c->ph, c->p, c->U0, c->Un, c->ps, c->F[0], c->F[1],
-                      c->F[2], c->pflag, c->failcnt, c->s);
+                      c->F[2], c->pflag, c->failcnt, c->prec, c->s);
     else
         launch_update(c->P, c->ident + n, 1, 0, Dt, nullptr, 1.0, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1],
-                      c->F[2], c->pflag, c->failcnt, c->s);
+                      c->F[2], c->pflag, c->failcnt, c->prec, c->s);
     return check_launch(c, "hamr_update");
 }
 
@@ -617,20 +661,20 @@ static void enqueue_step(hamr_ctx *c)
     const int *bl = c->zlist;
     hipStream_t s = c->s;
     // predictor: p/ps -> ph/psh with Dt = dt/2
-    for (int dir = 1; dir <= 3; dir++) launch_flux(P, dir, bl, nb, c->p, c->ps, c->F[dir - 1], nullptr, s);
-    launch_emf(P, bl, nb, c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, s);
-    launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, s);
-    launch_ct_update(P, bl, nb, 0., c->dt, 0.5, c->ps, c->psh, c->E, s);
+    for (int dir = 1; dir <= 3; dir++) launch_flux(P, dir, bl, nb, c->p, c->ps, c->F[dir - 1], nullptr, c->prec, s);
+    launch_emf(P, bl, nb, c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, s);
+    launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, c->prec, s);
+    launch_ct_update(P, bl, nb, 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, s);
     launch_update(P, bl, nb, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2], c->pflag,
-                  c->failcnt, s);
+                  c->failcnt, c->prec, s);
     bound_all(c, c->ph, c->psh);
     // corrector: U0 -> p/ps with Dt = dt, fluxes from ph/psh, Courant minima recorded
-    for (int dir = 1; dir <= 3; dir++) launch_flux(P, dir, bl, nb, c->ph, c->psh, c->F[dir - 1], c->dtmin, s);
-    launch_emf(P, bl, nb, c->ph, c->F[0], c->F[1], c->F[2], c->Ec, c->E, s);
-    launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, s);
-    launch_ct_update(P, bl, nb, 0., c->dt, 1.0, c->ps, c->ps, c->E, s);
+    for (int dir = 1; dir <= 3; dir++) launch_flux(P, dir, bl, nb, c->ph, c->psh, c->F[dir - 1], c->dtmin, c->prec, s);
+    launch_emf(P, bl, nb, c->ph, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, s);
+    launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, c->prec, s);
+    launch_ct_update(P, bl, nb, 0., c->dt, 1.0, c->ps, c->ps, c->E, c->prec, s);
     launch_update(P, bl, nb, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->U0, c->Un, c->ps, c->F[0], c->F[1], c->F[2], c->pflag,
-                  c->failcnt, s);
+                  c->failcnt, c->prec, s);
     bound_all(c, c->p, c->ps);
     launch_dt_finalize(nb, c->dtmin, c->dt, s);
 }
@@ -643,7 +687,7 @@ int hamr_estimate_dt(hamr_ctx *c, double *dt)
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (h2d(c, c->dt, big, sizeof big)) return -1;
     launch_dt_reset(c->L.nblk, c->dtmin, c->s);
-    for (int dir = 1; dir <= 3; dir++) launch_flux(c->P, dir, c->zlist, c->L.nblk, c->p, c->ps, c->F[dir - 1], c->dtmin, c->s);
+    for (int dir = 1; dir <= 3; dir++) launch_flux(c->P, dir, c->zlist, c->L.nblk, c->p, c->ps, c->F[dir - 1], c->dtmin, c->prec, c->s);
     launch_dt_finalize(c->L.nblk, c->dtmin, c->dt, c->s);
     if (check_launch(c, "hamr_estimate_dt")) return -1;
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -713,13 +757,13 @@ int hamr_time_stage(hamr_ctx *c, int stage, int nrep, double *avg_ms)
         switch (stage) {
         case 0:
         case 1:
-        case 2: launch_flux(P, stage + 1, bl, nb, c->p, c->ps, c->F[stage], nullptr, c->s); break;
-        case 3: launch_emf(P, bl, nb, c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->s); break;
-        case 4: launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, c->s); break;
-        case 5: launch_ct_update(P, bl, nb, 0., c->dt, 0.5, c->ps, c->psh, c->E, c->s); break;
+        case 2: launch_flux(P, stage + 1, bl, nb, c->p, c->ps, c->F[stage], nullptr, c->prec, c->s); break;
+        case 3: launch_emf(P, bl, nb, c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, c->s); break;
+        case 4: launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, c->prec, c->s); break;
+        case 5: launch_ct_update(P, bl, nb, 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, c->s); break;
         case 6:
             launch_update(P, bl, nb, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2],
-                          c->pflag, c->failcnt, c->s);
+                          c->pflag, c->failcnt, c->prec, c->s);
             break;
         default: bound_all(c, c->ph, c->psh); break;
         }

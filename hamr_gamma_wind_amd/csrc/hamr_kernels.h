@@ -22,18 +22,20 @@ struct KParams {
     const double *gcov, *gcon, *gdet, *conn, *rcen;
 };
 
-void launch_flux(const KParams &P, int dir, const int *blist, int nb, const double *p, const double *ps, double *F,
-                 unsigned long long *dtmin, hipStream_t s);
-void launch_emf(const KParams &P, const int *blist, int nb, const double *p, const double *F1, const double *F2,
-                const double *F3, double *Ec, double *E, hipStream_t s);
-void launch_pole(const KParams &P, const int *ring, int nring, double dt_val, const double *dt_ptr, double *E,
+// `mixed` selects the arithmetic of reconstruction, fluxes and EMFs: 0 = FP64
+// (F, Ec, E are double arrays), 1 = FP32 (float arrays with the same slab layout).
+void launch_flux(const KParams &P, int dir, const int *blist, int nb, const double *p, const double *ps, void *F,
+                 unsigned long long *dtmin, int mixed, hipStream_t s);
+void launch_emf(const KParams &P, const int *blist, int nb, const double *p, const void *F1, const void *F2,
+                const void *F3, void *Ec, void *E, int mixed, hipStream_t s);
+void launch_pole(const KParams &P, const int *ring, int nring, double dt_val, const double *dt_ptr, void *E, int mixed,
                  hipStream_t s);
 void launch_ct_update(const KParams &P, const int *blist, int nb, double Dt_val, const double *dt_ptr,
-                      double dt_mult, const double *psi, double *psf, const double *E, hipStream_t s);
+                      double dt_mult, const double *psi, double *psf, const void *E, int mixed, hipStream_t s);
 void launch_update(const KParams &P, const int *blist, int nb, int full, double Dt_val, const double *dt_ptr,
                    double dt_mult, const double *pi, const double *pb, double *pf, double *U0, double *Unew,
-                   const double *psf, const double *F1, const double *F2, const double *F3, int *pflag, int *failcnt,
-                   hipStream_t s);
+                   const double *psf, const void *F1, const void *F2, const void *F3, int *pflag, int *failcnt,
+                   int mixed, hipStream_t s);
 void launch_bound(const KParams &P, int dir, const int *blist, int nb, double *p, double *ps, int *pflag,
                   hipStream_t s);
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);

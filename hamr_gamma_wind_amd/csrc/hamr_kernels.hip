@@ -20,17 +20,20 @@ __device__ __forceinline__ void region_index(int t, int n2, int n3, int &a, int 
     a = q / n2;
 }
 
-__device__ __forceinline__ Geom<double> load_geom(const KParams &P, int geo, int pos, int64_t col)
+// Metric at one position of one column, rounded to R (the metric is stored in FP64).
+template <class R = double>
+__device__ __forceinline__ Geom<R> load_geom(const KParams &P, int geo, int pos, int64_t col)
 {
-    Geom<double> G;
+    Geom<R> G;
     const double *gc = P.gcov + ((int64_t)geo * NPOS + pos) * 10 * P.kcol + col;
     const double *gi = P.gcon + ((int64_t)geo * NPOS + pos) * 10 * P.kcol + col;
 #pragma unroll
     for (int s = 0; s < 10; s++) {
-        G.gcov[s] = gc[s * P.kcol];
-        G.gcon[s] = gi[s * P.kcol];
+        G.gcov[s] = (R)gc[s * P.kcol];
+        G.gcon[s] = (R)gi[s * P.kcol];
     }
-    G.g = P.gdet[((int64_t)geo * NPOS + pos) * P.kcol + col];
+    G.g = (R)P.gdet[((int64_t)geo * NPOS + pos) * P.kcol + col];
+    if constexpr (sizeof(R) == 4) G.oma = (R)(1. - 1. / sqrt(-gi[0]));
     return G;
 }
 
@@ -46,9 +49,13 @@ __device__ __forceinline__ double wave_min(double v)
 // fluxcalcprep (GPU_program1.cu:78-190) and fluxcalc2D2 (:791-1068) in one pass,
 // without the dq1/dq2 round trip through HBM.  Face range: [NG, NG+BS] along
 // dir, [NG-1, NG+BS] across (the transverse ghost faces feed the corner EMFs).
-template <int DIR>
+//
+// R is the arithmetic of reconstruction, state, fluxes and speeds: double for
+// the FP64 step, float for the mixed step (primitives, face fields and metric
+// are read in FP64 and rounded once; fluxes are stored as R).
+template <int DIR, class R>
 __global__ void __launch_bounds__(256) k_flux(KParams P, const int *__restrict__ blist, const double *__restrict__ p,
-                                              const double *__restrict__ ps, double *__restrict__ F,
+                                              const double *__restrict__ ps, R *__restrict__ F,
                                               unsigned long long *__restrict__ dtmin)
 {
     const int n = blist[blockIdx.y];
@@ -64,43 +71,45 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, const int *__restrict__
         const int64_t cell = ((int64_t)i * P.D[1] + j) * P.D[2] + z;
         const int64_t sd = (DIR == 1) ? (int64_t)P.D[1] * P.D[2] : (DIR == 2) ? (int64_t)P.D[2] : 1;
         const double *pb = p + (int64_t)n * NPR * P.ksize;
-        double pl[NPR], pr[NPR];
+        R pl[NPR], pr[NPR];
 #pragma unroll
         for (int k = 0; k < NPR; k++) {
             const double *q = pb + (int64_t)k * P.ksize + cell;
-            const double ym3 = q[-3 * sd], ym2 = q[-2 * sd], ym1 = q[-sd], y0 = q[0], yp1 = q[sd], yp2 = q[2 * sd];
-            double l, r;
+            const R ym3 = (R)q[-3 * sd], ym2 = (R)q[-2 * sd], ym1 = (R)q[-sd], y0 = (R)q[0], yp1 = (R)q[sd],
+                    yp2 = (R)q[2 * sd];
+            R l, r;
             para(ym3, ym2, ym1, y0, yp1, l, r);
             pl[k] = r;
             para(ym2, ym1, y0, yp1, yp2, l, r);
             pr[k] = l;
         }
-        const double bn = ps[((int64_t)n * 3 + (DIR - 1)) * P.ksize + cell];
+        const R bn = (R)ps[((int64_t)n * 3 + (DIR - 1)) * P.ksize + cell];
         pl[B1 + DIR - 1] = bn;
         pr[B1 + DIR - 1] = bn;
         const BlockDev &B = P.blk[n];
         if (DIR == 2 && (((B.pole & 1) && j == NG) || ((B.pole & 2) && j == NG + P.bs[1]))) {
-            pl[B1] = 0.;
-            pr[B1] = 0.;
+            pl[B1] = R(0.);
+            pr[B1] = R(0.);
         }
         const int64_t col = (int64_t)i * P.D[1] + j;
-        const Geom<double> G = load_geom(P, B.geo, DIR == 1 ? FACE1 : DIR == 2 ? FACE2 : FACE3, col);
-        double t1[NPR], t2[NPR], t3[NPR], t4[NPR], cmax_l = 0., cmin_l = 0., cmax_r = 0., cmin_r = 0., dum;
-        State<double> q;
+        const Geom<R> G = load_geom<R>(P, B.geo, DIR == 1 ? FACE1 : DIR == 2 ? FACE2 : FACE3, col);
+        const R gam = (R)P.gam;
+        R t1[NPR], t2[NPR], t3[NPR], t4[NPR], cmax_l = R(0.), cmin_l = R(0.), cmax_r = R(0.), cmin_r = R(0.), dum;
+        State<R> q;
         get_state(pl, G, q);
-        primtoflux(pl, q, DIR, G, P.gam, t1, &cmax_l, &cmin_l);
-        primtoflux(pl, q, 0, G, P.gam, t2, &dum, &dum);
+        primtoflux(pl, q, DIR, G, gam, t1, &cmax_l, &cmin_l);
+        primtoflux(pl, q, 0, G, gam, t2, &dum, &dum);
         get_state(pr, G, q);
-        primtoflux(pr, q, DIR, G, P.gam, t3, &cmax_r, &cmin_r);
-        primtoflux(pr, q, 0, G, P.gam, t4, &dum, &dum);
-        const double cmax = fabs(mymax(mymax(0., cmax_l), cmax_r));
-        const double cmin = fabs(mymax(mymax(0., -cmin_l), -cmin_r));
-        const double ctop = mymax(cmax, cmin);
-        double *Fb = F + (int64_t)n * NPR * P.ksize + cell;
+        primtoflux(pr, q, DIR, G, gam, t3, &cmax_r, &cmin_r);
+        primtoflux(pr, q, 0, G, gam, t4, &dum, &dum);
+        const R cmax = fabs(mymax(mymax(R(0.), cmax_l), cmax_r));
+        const R cmin = fabs(mymax(mymax(R(0.), -cmin_l), -cmin_r));
+        const R ctop = mymax(cmax, cmin);
+        R *Fb = F + (int64_t)n * NPR * P.ksize + cell;
 #pragma unroll
         for (int k = 0; k < NPR; k++)
-            Fb[(int64_t)k * P.ksize] = (cmax * t1[k] + cmin * t3[k] - cmax * cmin * (t4[k] - t2[k])) / (cmax + cmin + kSMALL);
-        const double v = P.cour * P.dx[DIR - 1] / ctop;
+            Fb[(int64_t)k * P.ksize] = (cmax * t1[k] + cmin * t3[k] - cmax * cmin * (t4[k] - t2[k])) / (cmax + cmin + R(kSMALL));
+        const double v = P.cour * P.dx[DIR - 1] / (double)ctop;
         if (v < 1.e9 && v > 1.e-6) vdt = v;
     }
     if (dtmin) {
@@ -111,8 +120,9 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, const int *__restrict__
 
 // ---------------------------------------------------------------------------
 // Cell-centre EMF, consttransport1 (GPU_const_trans.h:63-119), cells [NG-1, NG+BS].
+template <class R>
 __global__ void __launch_bounds__(256) k_emf_centre(KParams P, const int *__restrict__ blist, const double *__restrict__ p,
-                                                    double *__restrict__ Ec)
+                                                    R *__restrict__ Ec)
 {
     const int n = blist[blockIdx.y];
     const int e1 = P.bs[0] + 2, e2 = P.bs[1] + 2, e3 = P.bs[2] + 2;
@@ -123,15 +133,15 @@ __global__ void __launch_bounds__(256) k_emf_centre(KParams P, const int *__rest
     const int i = a + NG - 1, j = b + NG - 1, z = c + NG - 1;
     const int64_t cell = ((int64_t)i * P.D[1] + j) * P.D[2] + z;
     const double *pb = p + (int64_t)n * NPR * P.ksize + cell;
-    double pr[NPR];
+    R pr[NPR];
 #pragma unroll
-    for (int k = 0; k < NPR; k++) pr[k] = pb[(int64_t)k * P.ksize];
-    const Geom<double> G = load_geom(P, P.blk[n].geo, CENT, (int64_t)i * P.D[1] + j);
-    double ucon[4], ucov[4], bcon[4];
+    for (int k = 0; k < NPR; k++) pr[k] = (R)pb[(int64_t)k * P.ksize];
+    const Geom<R> G = load_geom<R>(P, P.blk[n].geo, CENT, (int64_t)i * P.D[1] + j);
+    R ucon[4], ucov[4], bcon[4];
     ucon_calc(pr, G, ucon);
     lower10(ucon, G.gcov, ucov);
     bcon_calc(pr, ucon, ucov, bcon);
-    double *E = Ec + (int64_t)n * 3 * P.ksize + cell;
+    R *E = Ec + (int64_t)n * 3 * P.ksize + cell;
     E[0] = -G.g * (ucon[2] * bcon[3] - ucon[3] * bcon[2]);
     E[P.ksize] = -G.g * (ucon[3] * bcon[1] - ucon[1] * bcon[3]);
     E[2 * P.ksize] = -G.g * (ucon[1] * bcon[2] - ucon[2] * bcon[1]);
@@ -139,9 +149,10 @@ __global__ void __launch_bounds__(256) k_emf_centre(KParams P, const int *__rest
 
 // Corner EMF upwinded on the mass-flux sign, consttransport2
 // (GPU_const_trans.h:121-339) with the polar rule (:233-236); corners [NG, NG+BS].
-__global__ void __launch_bounds__(256) k_emf_corner(KParams P, const int *__restrict__ blist, const double *__restrict__ Ec,
-                                                    const double *__restrict__ F1, const double *__restrict__ F2,
-                                                    const double *__restrict__ F3, double *__restrict__ E)
+template <class R>
+__global__ void __launch_bounds__(256) k_emf_corner(KParams P, const int *__restrict__ blist, const R *__restrict__ Ec,
+                                                    const R *__restrict__ F1, const R *__restrict__ F2,
+                                                    const R *__restrict__ F3, R *__restrict__ E)
 {
     const int n = blist[blockIdx.y];
     const int e1 = P.bs[0] + 1, e2 = P.bs[1] + 1, e3 = P.bs[2] + 1;
@@ -152,47 +163,47 @@ __global__ void __launch_bounds__(256) k_emf_corner(KParams P, const int *__rest
     const int i = a + NG, j = b + NG, z = c + NG;
     const int64_t g = ((int64_t)i * P.D[1] + j) * P.D[2] + z;
     const int64_t K = P.ksize, si = (int64_t)P.D[1] * P.D[2], sj = P.D[2];
-    const double *E1c = Ec + (int64_t)n * 3 * K, *E2c = E1c + K, *E3c = E2c + K;
-    const double *f1 = F1 + (int64_t)n * NPR * K, *f2 = F2 + (int64_t)n * NPR * K, *f3 = F3 + (int64_t)n * NPR * K;
-    const double *F1R = f1 + RHO * K, *F2R = f2 + RHO * K, *F3R = f3 + RHO * K;
-    const double *F1B2 = f1 + B2 * K, *F1B3 = f1 + B3 * K;
-    const double *F2B1 = f2 + B1 * K, *F2B3 = f2 + B3 * K;
-    const double *F3B1 = f3 + B1 * K, *F3B2 = f3 + B2 * K;
-    const double dEL13_1 = E1c[g] - F3B2[g];
-    const double dEL13_2 = E1c[g - sj] - F3B2[g - sj];
-    const double dER13_1 = F3B2[g] - E1c[g - 1];
-    const double dER13_2 = F3B2[g - sj] - E1c[g - sj - 1];
-    const double dEL12_1 = E1c[g] + F2B3[g];
-    const double dEL12_2 = E1c[g - 1] + F2B3[g - 1];
-    const double dER12_1 = -F2B3[g] - E1c[g - sj];
-    const double dER12_2 = -F2B3[g - 1] - E1c[g - sj - 1];
-    const double dEL21_1 = E2c[g] - F1B3[g];
-    const double dEL21_2 = E2c[g - 1] - F1B3[g - 1];
-    const double dER21_1 = F1B3[g] - E2c[g - si];
-    const double dER21_2 = F1B3[g - 1] - E2c[g - si - 1];
-    const double dEL23_1 = E2c[g] + F3B1[g];
-    const double dEL23_2 = E2c[g - si] + F3B1[g - si];
-    const double dER23_1 = -F3B1[g] - E2c[g - 1];
-    const double dER23_2 = -F3B1[g - si] - E2c[g - si - 1];
-    const double dEL31_1 = E3c[g] + F1B2[g];
-    const double dEL31_2 = E3c[g - sj] + F1B2[g - sj];
-    const double dER31_1 = -F1B2[g] - E3c[g - si];
-    const double dER31_2 = -F1B2[g - sj] - E3c[g - si - sj];
-    const double dEL32_1 = E3c[g] - F2B1[g];
-    const double dEL32_2 = E3c[g - si] - F2B1[g - si];
-    const double dER32_1 = F2B1[g] - E3c[g - sj];
-    const double dER32_2 = F2B1[g - si] - E3c[g - si - sj];
-#define LE(x) ((double)((x) <= 0.0))
-#define GT(x) ((double)((x) > 0.0))
-    double e1v = 0.25 * ((-F2B3[g] - (dEL13_1 * LE(F2R[g]) + dEL13_2 * GT(F2R[g])))
+    const R *E1c = Ec + (int64_t)n * 3 * K, *E2c = E1c + K, *E3c = E2c + K;
+    const R *f1 = F1 + (int64_t)n * NPR * K, *f2 = F2 + (int64_t)n * NPR * K, *f3 = F3 + (int64_t)n * NPR * K;
+    const R *F1R = f1 + RHO * K, *F2R = f2 + RHO * K, *F3R = f3 + RHO * K;
+    const R *F1B2 = f1 + B2 * K, *F1B3 = f1 + B3 * K;
+    const R *F2B1 = f2 + B1 * K, *F2B3 = f2 + B3 * K;
+    const R *F3B1 = f3 + B1 * K, *F3B2 = f3 + B2 * K;
+    const R dEL13_1 = E1c[g] - F3B2[g];
+    const R dEL13_2 = E1c[g - sj] - F3B2[g - sj];
+    const R dER13_1 = F3B2[g] - E1c[g - 1];
+    const R dER13_2 = F3B2[g - sj] - E1c[g - sj - 1];
+    const R dEL12_1 = E1c[g] + F2B3[g];
+    const R dEL12_2 = E1c[g - 1] + F2B3[g - 1];
+    const R dER12_1 = -F2B3[g] - E1c[g - sj];
+    const R dER12_2 = -F2B3[g - 1] - E1c[g - sj - 1];
+    const R dEL21_1 = E2c[g] - F1B3[g];
+    const R dEL21_2 = E2c[g - 1] - F1B3[g - 1];
+    const R dER21_1 = F1B3[g] - E2c[g - si];
+    const R dER21_2 = F1B3[g - 1] - E2c[g - si - 1];
+    const R dEL23_1 = E2c[g] + F3B1[g];
+    const R dEL23_2 = E2c[g - si] + F3B1[g - si];
+    const R dER23_1 = -F3B1[g] - E2c[g - 1];
+    const R dER23_2 = -F3B1[g - si] - E2c[g - si - 1];
+    const R dEL31_1 = E3c[g] + F1B2[g];
+    const R dEL31_2 = E3c[g - sj] + F1B2[g - sj];
+    const R dER31_1 = -F1B2[g] - E3c[g - si];
+    const R dER31_2 = -F1B2[g - sj] - E3c[g - si - sj];
+    const R dEL32_1 = E3c[g] - F2B1[g];
+    const R dEL32_2 = E3c[g - si] - F2B1[g - si];
+    const R dER32_1 = F2B1[g] - E3c[g - sj];
+    const R dER32_2 = F2B1[g - si] - E3c[g - si - sj];
+#define LE(x) ((R)((x) <= R(0.)))
+#define GT(x) ((R)((x) > R(0.)))
+    R e1v = R(0.25) * ((-F2B3[g] - (dEL13_1 * LE(F2R[g]) + dEL13_2 * GT(F2R[g])))
         + (-F2B3[g - 1] + (dER13_1 * LE(F2R[g - 1]) + dER13_2 * GT(F2R[g - 1])))
         + (F3B2[g] - (dEL12_1 * LE(F3R[g]) + dEL12_2 * GT(F3R[g])))
         + (F3B2[g - sj] + (dER12_1 * LE(F3R[g - sj]) + dER12_2 * GT(F3R[g - sj]))));
-    const double e2v = 0.25 * ((-F3B1[g] - (dEL21_1 * LE(F3R[g]) + dEL21_2 * GT(F3R[g])))
+    const R e2v = R(0.25) * ((-F3B1[g] - (dEL21_1 * LE(F3R[g]) + dEL21_2 * GT(F3R[g])))
         + (-F3B1[g - si] + (dER21_1 * LE(F3R[g - si]) + dER21_2 * GT(F3R[g - si])))
         + (F1B3[g] - (dEL23_1 * LE(F1R[g]) + dEL23_2 * GT(F1R[g])))
         + (F1B3[g - 1] + (dER23_1 * LE(F1R[g - 1]) + dER23_2 * GT(F1R[g - 1]))));
-    double e3v = 0.25 * ((F2B1[g] - (dEL31_1 * LE(F2R[g]) + dEL31_2 * GT(F2R[g])))
+    R e3v = R(0.25) * ((F2B1[g] - (dEL31_1 * LE(F2R[g]) + dEL31_2 * GT(F2R[g])))
         + (F2B1[g - si] + (dER31_1 * LE(F2R[g - si]) + dER31_2 * GT(F2R[g - si])))
         + (-F1B2[g] - (dEL32_1 * LE(F1R[g]) + dEL32_2 * GT(F1R[g])))
         + (-F1B2[g - sj] + (dER32_1 * LE(F1R[g - sj]) + dER32_2 * GT(F1R[g - sj]))));
@@ -200,10 +211,10 @@ __global__ void __launch_bounds__(256) k_emf_corner(KParams P, const int *__rest
 #undef GT
     const BlockDev &B = P.blk[n];
     if (((B.pole & 1) && j == NG) || ((B.pole & 2) && j == NG + P.bs[1])) {
-        e3v = 0.;
-        e1v = -0.5 * (F2B3[g] + F2B3[g - 1]);
+        e3v = R(0.);
+        e1v = R(-0.5) * (F2B3[g] + F2B3[g - 1]);
     }
-    double *Eb = E + (int64_t)n * 3 * K + g;
+    R *Eb = E + (int64_t)n * 3 * K + g;
     Eb[0] = e1v;
     Eb[K] = e2v;
     Eb[2 * K] = e3v;
@@ -213,8 +224,9 @@ __global__ void __launch_bounds__(256) k_emf_corner(KParams P, const int *__rest
 // dt as packsend2E does), mean over the ring of phi-blocks, divided back by dt
 // (E_average / read_E_avg / write_E_avg, const_trans.c:268-543).  One thread per
 // (side, ring, radial corner); ring lists the NB3 blocks of each ring in c3 order.
+template <class R>
 __global__ void k_pole_average(KParams P, const int *__restrict__ ring, int nring, double dt_val,
-                               const double *__restrict__ dt_ptr, double *__restrict__ E)
+                               const double *__restrict__ dt_ptr, R *__restrict__ E)
 {
     const int ni = P.bs[0] + 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -226,19 +238,19 @@ __global__ void k_pole_average(KParams P, const int *__restrict__ ring, int nrin
     const int *rb = ring + ((int64_t)side * nring + rg) * P.nb3;
     double avg = 0.;
     for (int b = 0; b < P.nb3; b++) {
-        const double *e1 = E + (int64_t)rb[b] * 3 * K + ((int64_t)i * P.D[1] + j) * P.D[2];
+        const R *e1 = E + (int64_t)rb[b] * 3 * K + ((int64_t)i * P.D[1] + j) * P.D[2];
         double s = 0.;
-        for (int zz = 0; zz < P.bs[2]; zz++) s += dt * e1[NG + zz];
+        for (int zz = 0; zz < P.bs[2]; zz++) s += dt * (double)e1[NG + zz];
         s /= (double)(P.bs[2]);
         if (b == 0) avg = s / ((double)P.nb3);
         else avg += s / ((double)P.nb3);
     }
     const double v = avg / dt;
     for (int b = 0; b < P.nb3; b++) {
-        double *e = E + (int64_t)rb[b] * 3 * K + ((int64_t)i * P.D[1] + j) * P.D[2];
+        R *e = E + (int64_t)rb[b] * 3 * K + ((int64_t)i * P.D[1] + j) * P.D[2];
         for (int z = NG; z <= NG + P.bs[2]; z++) {
-            e[z] = v;
-            e[2 * K + z] = 0.;
+            e[z] = (R)v;
+            e[2 * K + z] = R(0.);
         }
     }
 }
@@ -246,9 +258,10 @@ __global__ void k_pole_average(KParams P, const int *__restrict__ ring, int nrin
 // Face-B update from the curl of the corner EMF, consttransport3
 // (GPU_const_trans.h:426-530); faces [NG, NG+BS] along their normal.  psi and
 // psf may alias (each thread reads only the faces it writes).
+template <class R>
 __global__ void __launch_bounds__(256) k_ct_update(KParams P, const int *__restrict__ blist, double Dt_val,
                                                    const double *__restrict__ dt_ptr, double dt_mult, const double *psi,
-                                                   double *psf, const double *__restrict__ E)
+                                                   double *psf, const R *__restrict__ E)
 {
     const int n = blist[blockIdx.y];
     const int e1 = P.bs[0] + 1, e2 = P.bs[1] + 1, e3 = P.bs[2] + 1;
@@ -263,7 +276,8 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, const int *__restr
     const int geo = P.blk[n].geo;
     const double *gd = P.gdet + (int64_t)geo * NPOS * P.kcol + col;
     const double gF1 = gd[FACE1 * P.kcol], gF2 = gd[FACE2 * P.kcol], gF3 = gd[FACE3 * P.kcol];
-    const double *Eb = E + (int64_t)n * 3 * K + g;
+    const R *Ebr = E + (int64_t)n * 3 * K + g;
+    auto Eb = [&](int64_t o) { return (double)Ebr[o]; };
     const double *si_ = psi + (int64_t)n * 3 * K + g;
     double *sf = psf + (int64_t)n * 3 * K + g;
     const double dx1 = P.dx[0], dx2 = P.dx[1], dx3 = P.dx[2];
@@ -271,21 +285,21 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, const int *__restr
     if (j < NG + P.bs[1] && z < NG + P.bs[2]) {
         temp = 0.;
         temp += 1.0 / 1.0 * si_[0] * gF1;
-        temp += -Dt / (1.0 * dx2) * (Eb[2 * K + sj] - Eb[2 * K]);
-        temp += Dt / (1.0 * dx3) * (Eb[K + 1] - Eb[K]);
+        temp += -Dt / (1.0 * dx2) * (Eb(2 * K + sj) - Eb(2 * K));
+        temp += Dt / (1.0 * dx3) * (Eb(K + 1) - Eb(K));
         sf[0] = temp / gF1;
     }
     if (i < NG + P.bs[0] && j < NG + P.bs[1]) {
         temp = si_[2 * K] * gF3;
-        temp += -Dt / dx1 * (Eb[K + si] - Eb[K]);
-        temp += Dt / dx2 * (Eb[sj] - Eb[0]);
+        temp += -Dt / dx1 * (Eb(K + si) - Eb(K));
+        temp += Dt / dx2 * (Eb(sj) - Eb(0));
         sf[2 * K] = temp / gF3;
     }
     if (i < NG + P.bs[0] && z < NG + P.bs[2]) {
         temp = 0.;
         temp += 1.0 / 1.0 * si_[K] * gF2;
-        temp += Dt / (1.0 * dx1) * (Eb[2 * K + si] - Eb[2 * K]);
-        temp += -Dt / (1.0 * dx3) * (Eb[1] - Eb[0]);
+        temp += Dt / (1.0 * dx1) * (Eb(2 * K + si) - Eb(2 * K));
+        temp += -Dt / (1.0 * dx3) * (Eb(1) - Eb(0));
         sf[K] = temp / gF2;
     }
 }
@@ -323,12 +337,26 @@ __device__ __forceinline__ double metric_value(const KParams &P, int geo, int v,
 // Columns a 256-thread group of interior cells can touch.
 __host__ __device__ inline int cons_max_cols(int bs3) { return 255 / bs3 + 2; }
 
+// Neumaier's compensated sum: s + c carries the running total without the
+// rounding of each addition (used by the mixed step, whose FP32 flux
+// differences are exact in FP64 and are added to U^n without loss).
+__device__ __forceinline__ void comp_add(double &s, double &c, double t)
+{
+    const double u = s + t;
+    if (fabs(s) >= fabs(t)) c += (s - u) + t;
+    else c += (t - u) + s;
+    s = u;
+}
+
+// R is the storage type of the fluxes: double (FP64 step, the reference's
+// arithmetic) or float (mixed step, compensated FP64 accumulation).
+template <class R>
 __global__ void __launch_bounds__(256) k_cons(KParams P, const int *__restrict__ blist, int full, double Dt_val,
                                               const double *__restrict__ dt_ptr, double dt_mult,
                                               const double *__restrict__ pi, const double *__restrict__ pb,
                                               double *__restrict__ U0, double *__restrict__ Unew,
-                                              const double *__restrict__ psf, const double *__restrict__ F1,
-                                              const double *__restrict__ F2, const double *__restrict__ F3)
+                                              const double *__restrict__ psf, const R *__restrict__ F1,
+                                              const R *__restrict__ F2, const R *__restrict__ F3)
 {
     extern __shared__ double smem[];
     const int n = blist[blockIdx.y];
@@ -383,23 +411,35 @@ __global__ void __launch_bounds__(256) k_cons(KParams P, const int *__restrict__
     double f1l[NPR], f1h[NPR], f2l[NPR], f2h[NPR], f3l[NPR], f3h[NPR];
 #pragma unroll
     for (int k = 0; k < NPR; k++) {
-        const double *f1 = F1 + base + k * K, *f2 = F2 + base + k * K, *f3 = F3 + base + k * K;
-        f1l[k] = f1[0];
-        f1h[k] = f1[si];
-        f2l[k] = f2[0];
-        f2h[k] = f2[sj];
-        f3l[k] = f3[0];
-        f3h[k] = f3[1];
-    }
-#pragma unroll
-    for (int k = 0; k < NPR; k++) {
-        U[k] -= Dt * (f1h[k] - f1l[k]) / (P.dx[0] * 1.0);
-        U[k] -= Dt * (f2h[k] - f2l[k]) / (P.dx[1] * 1.0);
-        U[k] -= Dt * (f3h[k] - f3l[k]) / (P.dx[2] * 1.0);
+        const R *f1 = F1 + base + k * K, *f2 = F2 + base + k * K, *f3 = F3 + base + k * K;
+        f1l[k] = (double)f1[0];
+        f1h[k] = (double)f1[si];
+        f2l[k] = (double)f2[0];
+        f2h[k] = (double)f2[sj];
+        f3l[k] = (double)f3[0];
+        f3h[k] = (double)f3[1];
     }
     source(pf, q, [&](int lam, int s) { return mv[(MV_CONN + lam * 10 + s) * ncw]; }, G.g, P.gam, dU);
+    if (sizeof(R) == sizeof(double)) {
 #pragma unroll
-    for (int k = 0; k < NPR; k++) U[k] += Dt * (dU[k]);
+        for (int k = 0; k < NPR; k++) {
+            U[k] -= Dt * (f1h[k] - f1l[k]) / (P.dx[0] * 1.0);
+            U[k] -= Dt * (f2h[k] - f2l[k]) / (P.dx[1] * 1.0);
+            U[k] -= Dt * (f3h[k] - f3l[k]) / (P.dx[2] * 1.0);
+        }
+#pragma unroll
+        for (int k = 0; k < NPR; k++) U[k] += Dt * (dU[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            double sum = U[k], cmp = 0.;
+            comp_add(sum, cmp, -(Dt * (f1h[k] - f1l[k]) / P.dx[0]));
+            comp_add(sum, cmp, -(Dt * (f2h[k] - f2l[k]) / P.dx[1]));
+            comp_add(sum, cmp, -(Dt * (f3h[k] - f3l[k]) / P.dx[2]));
+            comp_add(sum, cmp, Dt * dU[k]);
+            U[k] = sum + cmp;
+        }
+    }
     {
         const double *sp = psf + (int64_t)n * 3 * K + g;
         U[B1] = 0.0;
@@ -656,50 +696,78 @@ __global__ void k_fill_u64(unsigned long long *a, int n, unsigned long long v)
 // launchers
 static inline dim3 grid_for(int64_t work, int nb) { return dim3((unsigned)((work + 255) / 256), (unsigned)nb, 1); }
 
-void launch_flux(const KParams &P, int dir, const int *blist, int nb, const double *p, const double *ps, double *F,
-                 unsigned long long *dtmin, hipStream_t s)
+template <class R>
+static void flux_t(const KParams &P, int dir, const int *blist, int nb, const double *p, const double *ps, R *F,
+                   unsigned long long *dtmin, hipStream_t s)
 {
     const int64_t e1 = P.bs[0] + (dir == 1 ? 1 : 2), e2 = P.bs[1] + (dir == 2 ? 1 : 2), e3 = P.bs[2] + (dir == 3 ? 1 : 2);
     const dim3 g = grid_for(e1 * e2 * e3, nb);
-    if (dir == 1) hipLaunchKernelGGL(k_flux<1>, g, dim3(256), 0, s, P, blist, p, ps, F, dtmin);
-    else if (dir == 2) hipLaunchKernelGGL(k_flux<2>, g, dim3(256), 0, s, P, blist, p, ps, F, dtmin);
-    else hipLaunchKernelGGL(k_flux<3>, g, dim3(256), 0, s, P, blist, p, ps, F, dtmin);
+    if (dir == 1) hipLaunchKernelGGL((k_flux<1, R>), g, dim3(256), 0, s, P, blist, p, ps, F, dtmin);
+    else if (dir == 2) hipLaunchKernelGGL((k_flux<2, R>), g, dim3(256), 0, s, P, blist, p, ps, F, dtmin);
+    else hipLaunchKernelGGL((k_flux<3, R>), g, dim3(256), 0, s, P, blist, p, ps, F, dtmin);
 }
 
-void launch_emf(const KParams &P, const int *blist, int nb, const double *p, const double *F1, const double *F2,
-                const double *F3, double *Ec, double *E, hipStream_t s)
+void launch_flux(const KParams &P, int dir, const int *blist, int nb, const double *p, const double *ps, void *F,
+                 unsigned long long *dtmin, int mixed, hipStream_t s)
+{
+    if (mixed) flux_t(P, dir, blist, nb, p, ps, (float *)F, dtmin, s);
+    else flux_t(P, dir, blist, nb, p, ps, (double *)F, dtmin, s);
+}
+
+template <class R>
+static void emf_t(const KParams &P, const int *blist, int nb, const double *p, const R *F1, const R *F2, const R *F3,
+                  R *Ec, R *E, hipStream_t s)
 {
     const int64_t c = (int64_t)(P.bs[0] + 2) * (P.bs[1] + 2) * (P.bs[2] + 2);
-    hipLaunchKernelGGL(k_emf_centre, grid_for(c, nb), dim3(256), 0, s, P, blist, p, Ec);
+    hipLaunchKernelGGL((k_emf_centre<R>), grid_for(c, nb), dim3(256), 0, s, P, blist, p, Ec);
     const int64_t k = (int64_t)(P.bs[0] + 1) * (P.bs[1] + 1) * (P.bs[2] + 1);
-    hipLaunchKernelGGL(k_emf_corner, grid_for(k, nb), dim3(256), 0, s, P, blist, Ec, F1, F2, F3, E);
+    hipLaunchKernelGGL((k_emf_corner<R>), grid_for(k, nb), dim3(256), 0, s, P, blist, Ec, F1, F2, F3, E);
 }
 
-void launch_pole(const KParams &P, const int *ring, int nring, double dt_val, const double *dt_ptr, double *E,
+void launch_emf(const KParams &P, const int *blist, int nb, const double *p, const void *F1, const void *F2,
+                const void *F3, void *Ec, void *E, int mixed, hipStream_t s)
+{
+    if (mixed)
+        emf_t(P, blist, nb, p, (const float *)F1, (const float *)F2, (const float *)F3, (float *)Ec, (float *)E, s);
+    else
+        emf_t(P, blist, nb, p, (const double *)F1, (const double *)F2, (const double *)F3, (double *)Ec, (double *)E, s);
+}
+
+void launch_pole(const KParams &P, const int *ring, int nring, double dt_val, const double *dt_ptr, void *E, int mixed,
                  hipStream_t s)
 {
     if (nring <= 0) return;
     const int64_t w = (int64_t)2 * nring * (P.bs[0] + 1);
-    hipLaunchKernelGGL(k_pole_average, dim3((unsigned)((w + 63) / 64)), dim3(64), 0, s, P, ring, nring, dt_val,
-                       dt_ptr, E);
+    const dim3 g((unsigned)((w + 63) / 64));
+    if (mixed) hipLaunchKernelGGL((k_pole_average<float>), g, dim3(64), 0, s, P, ring, nring, dt_val, dt_ptr, (float *)E);
+    else hipLaunchKernelGGL((k_pole_average<double>), g, dim3(64), 0, s, P, ring, nring, dt_val, dt_ptr, (double *)E);
 }
 
 void launch_ct_update(const KParams &P, const int *blist, int nb, double Dt_val, const double *dt_ptr,
-                      double dt_mult, const double *psi, double *psf, const double *E, hipStream_t s)
+                      double dt_mult, const double *psi, double *psf, const void *E, int mixed, hipStream_t s)
 {
     const int64_t k = (int64_t)(P.bs[0] + 1) * (P.bs[1] + 1) * (P.bs[2] + 1);
-    hipLaunchKernelGGL(k_ct_update, grid_for(k, nb), dim3(256), 0, s, P, blist, Dt_val, dt_ptr, dt_mult, psi, psf, E);
+    if (mixed)
+        hipLaunchKernelGGL((k_ct_update<float>), grid_for(k, nb), dim3(256), 0, s, P, blist, Dt_val, dt_ptr, dt_mult, psi,
+                           psf, (const float *)E);
+    else
+        hipLaunchKernelGGL((k_ct_update<double>), grid_for(k, nb), dim3(256), 0, s, P, blist, Dt_val, dt_ptr, dt_mult,
+                           psi, psf, (const double *)E);
 }
 
 void launch_update(const KParams &P, const int *blist, int nb, int full, double Dt_val, const double *dt_ptr,
                    double dt_mult, const double *pi, const double *pb, double *pf, double *U0, double *Unew,
-                   const double *psf, const double *F1, const double *F2, const double *F3, int *pflag, int *failcnt,
-                   hipStream_t s)
+                   const double *psf, const void *F1, const void *F2, const void *F3, int *pflag, int *failcnt,
+                   int mixed, hipStream_t s)
 {
     const int64_t w = (int64_t)P.bs[0] * P.bs[1] * P.bs[2];
     const size_t lds = sizeof(double) * MV_N * cons_max_cols(P.bs[2]);
-    hipLaunchKernelGGL(k_cons, grid_for(w, nb), dim3(256), lds, s, P, blist, full, Dt_val, dt_ptr, dt_mult, pi, pb, U0,
-                       Unew, psf, F1, F2, F3);
+    if (mixed)
+        hipLaunchKernelGGL((k_cons<float>), grid_for(w, nb), dim3(256), lds, s, P, blist, full, Dt_val, dt_ptr, dt_mult,
+                           pi, pb, U0, Unew, psf, (const float *)F1, (const float *)F2, (const float *)F3);
+    else
+        hipLaunchKernelGGL((k_cons<double>), grid_for(w, nb), dim3(256), lds, s, P, blist, full, Dt_val, dt_ptr,
+                           dt_mult, pi, pb, U0, Unew, psf, (const double *)F1, (const double *)F2, (const double *)F3);
     hipLaunchKernelGGL(k_invert, grid_for(w, nb), dim3(256), 0, s, P, blist, full, pi, pb, Unew, pf, pflag, failcnt);
     hipLaunchKernelGGL(k_fallback, grid_for(w, nb), dim3(256), 0, s, P, blist, Unew, pf, pflag, failcnt);
 }

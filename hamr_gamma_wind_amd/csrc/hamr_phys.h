@@ -35,8 +35,25 @@ template <class T> HD T mymax(T a, T b) { return (a > b) ? a : b; }
 template <class T> HD T mymin(T a, T b) { return (a < b) ? a : b; }
 
 // Local geometry: packed symmetric gcov/gcon (00,01,02,03,11,12,13,22,23,33) and sqrt(-g).
-template <class R> struct Geom { R gcov[10], gcon[10], g; };
-template <class R> struct State { R ucon[4], ucov[4], bcon[4], bcov[4]; };
+// oma = 1 - alpha (lapse alpha = 1/sqrt(-g^00)), formed in FP64 by the FP32 path only.
+template <class R> struct Geom { R gcov[10], gcon[10], g, oma; };
+// opu = 1 + u_t, formed without cancellation by the FP32 path only.
+template <class R> struct State { R ucon[4], ucov[4], bcon[4], bcov[4], opu; };
+
+// 1 + u_t for the FP32 path.  With u_t = -alpha gamma + beta_i U^i (beta_i = g_0i,
+// U^i the relative 4-velocity), 1 + u_t = (1 - alpha) - alpha (gamma - 1)
+// + beta_i U^i and gamma - 1 = q^2 / (gamma + 1): every term is small where
+// 1 + u_t is, so FP32 keeps its relative precision (the direct sum of
+// T^i_t and rho u^i in the energy flux cancels the rest-mass part).
+template <class R> HD R one_plus_ut(const R *pr, const Geom<R> &G)
+{
+    R qsq = G.gcov[4] * pr[U1] * pr[U1] + G.gcov[7] * pr[U2] * pr[U2] + G.gcov[9] * pr[U3] * pr[U3]
+        + R(2.) * (G.gcov[5] * pr[U1] * pr[U2] + G.gcov[6] * pr[U1] * pr[U3] + G.gcov[8] * pr[U2] * pr[U3]);
+    if (qsq < R(0.)) qsq = (fabs(qsq) > R(1.E-10)) ? R(0.) : R(1.E-10);
+    const R gamma = sqrt(R(1.) + qsq);
+    const R alpha = R(1.) - G.oma;
+    return G.oma - alpha * (qsq / (gamma + R(1.))) + (G.gcov[1] * pr[U1] + G.gcov[2] * pr[U2] + G.gcov[3] * pr[U3]);
+}
 
 // GPU_metric.h:3-34
 template <class R> HD void lower10(const R *u, const R *g, R *o)
@@ -85,6 +102,7 @@ template <class R> HD void get_state(const R *pr, const Geom<R> &G, State<R> &q)
     lower10(q.ucon, G.gcov, q.ucov);
     bcon_calc(pr, q.ucon, q.ucov, q.bcon);
     lower10(q.bcon, G.gcov, q.bcov);
+    if constexpr (sizeof(R) == 4) q.opu = one_plus_ut(pr, G);
 }
 
 template <class R> HD R dot4(const R *a, const R *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; }
@@ -103,7 +121,11 @@ template <class R> HD void primtoflux(const R *pr, const State<R> &q, int dir, c
     flux[RHO] = pr[RHO] * q.ucon[dir];
     for (int j = 0; j < 4; j++)
         mhd[j] = eta * q.ucon[dir] * q.ucov[j] + ptot * (dir == j ? R(1.) : R(0.)) - q.bcon[dir] * q.bcov[j];
-    flux[UU] = mhd[0] + flux[RHO];
+    if constexpr (sizeof(R) == 4)  // same quantity, without the rest-mass cancellation
+        flux[UU] = flux[RHO] * q.opu + (u + P + bsq) * q.ucon[dir] * q.ucov[0] + ptot * (dir == 0 ? R(1.) : R(0.))
+            - q.bcon[dir] * q.bcov[0];
+    else
+        flux[UU] = mhd[0] + flux[RHO];
     flux[U1] = mhd[1];
     flux[U2] = mhd[2];
     flux[U3] = mhd[3];

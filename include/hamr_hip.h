@@ -65,6 +65,14 @@ enum {
     HAMR_U0 = 9     /* conserved variables at the start of the step, 9 */
 };
 
+/* Arithmetic of the step (hamr_set_precision).  FP64: every stage in double,
+ * the reference's arithmetic.  MIXED: reconstruction, fluxes, speeds and EMFs
+ * in FP32 (F1-F3, EMF and ECENT are then float arrays on the device; the array
+ * calls below convert them), while the metric and connection, the face-field
+ * update, the conserved update (a compensated FP64 sum) and the inversions stay
+ * FP64. */
+enum { HAMR_PREC_FP64 = 0, HAMR_PREC_MIXED = 1 };
+
 /* ---- host-only set-up (no device needed; for drivers and tests) ---- */
 /* Layout of a descriptor: D[3], cells per slab, padded slab length, metric
  * columns per slab and padded length, block and geometry counts. */
@@ -90,6 +98,11 @@ int hamr_desc_torus(const hamr_desc *desc, const hamr_torus_desc *t, double *p, 
 int hamr_create(const hamr_desc *desc, hamr_ctx **out);
 int hamr_destroy(hamr_ctx *ctx);
 const char *hamr_last_error(const hamr_ctx *ctx);
+
+/* Select the arithmetic of every later stage call and step (no reference
+ * counterpart: the reference's precision is fixed at compile time). */
+int hamr_set_precision(hamr_ctx *ctx, int mode);
+int hamr_get_precision(const hamr_ctx *ctx, int *mode);
 
 /* Layout: D[3] = BS+6 extents, ncell, padded slab length, block and geometry counts. */
 int hamr_layout(const hamr_ctx *ctx, int *D, long long *ncell, long long *ksize, int *nblk, int *ngeo);
