@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(256) k_flux(KParams P, const int *__restrict__
         R *Fb = F + (int64_t)n * NPR * P.ksize + cell;
 #pragma unroll
         for (int k = 0; k < NPR; k++)
-            Fb[(int64_t)k * P.ksize] = (cmax * t1[k] + cmin * t3[k] - cmax * cmin * (t4[k] - t2[k])) / (cmax + cmin + R(kSMALL));
+            Fb[(int64_t)k * P.ksize] = dv(cmax * t1[k] + cmin * t3[k] - cmax * cmin * (t4[k] - t2[k]), cmax + cmin + R(kSMALL));
         const double v = P.cour * P.dx[DIR - 1] / (double)ctop;
         if (v < 1.e9 && v > 1.e-6) vdt = v;
     }

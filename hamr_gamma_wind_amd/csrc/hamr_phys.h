@@ -34,6 +34,19 @@ constexpr double kGAMMAMAX = 80.;
 template <class T> HD T mymax(T a, T b) { return (a > b) ? a : b; }
 template <class T> HD T mymin(T a, T b) { return (a < b) ? a : b; }
 
+// Division, reciprocal square root and power in the arithmetic of R: IEEE
+// double operations for the FP64 step (the reference's), the hardware's
+// single-precision reciprocal, reciprocal square root and exp2/log2 (about
+// one ulp each) for the FP32 path.
+template <class R> HD R dv(R a, R b) { return a / b; }
+template <class R> HD R rsq(R a) { return R(1.) / sqrt(a); }
+template <class R> HD R rpow(R a, R e) { return pow(a, e); }
+#if defined(__HIP_DEVICE_COMPILE__)
+template <> HD float dv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <> HD float rsq<float>(float a) { return __builtin_amdgcn_rsqf(a); }
+template <> HD float rpow<float>(float a, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(a)); }
+#endif
+
 // Local geometry: packed symmetric gcov/gcon (00,01,02,03,11,12,13,22,23,33) and sqrt(-g).
 // oma = 1 - alpha (lapse alpha = 1/sqrt(-g^00)), formed in FP64 by the FP32 path only.
 template <class R> struct Geom { R gcov[10], gcon[10], g, oma; };
@@ -52,7 +65,7 @@ template <class R> HD R one_plus_ut(const R *pr, const Geom<R> &G)
     if (qsq < R(0.)) qsq = (fabs(qsq) > R(1.E-10)) ? R(0.) : R(1.E-10);
     const R gamma = sqrt(R(1.) + qsq);
     const R alpha = R(1.) - G.oma;
-    return G.oma - alpha * (qsq / (gamma + R(1.))) + (G.gcov[1] * pr[U1] + G.gcov[2] * pr[U2] + G.gcov[3] * pr[U3]);
+    return G.oma - alpha * dv(qsq, gamma + R(1.)) + (G.gcov[1] * pr[U1] + G.gcov[2] * pr[U2] + G.gcov[3] * pr[U3]);
 }
 
 // GPU_metric.h:3-34
@@ -81,18 +94,18 @@ template <class R> HD int gamma_calc(const R *pr, const R *gcov, R *gamma)
 template <class R> HD void ucon_calc(const R *pr, const Geom<R> &G, R *ucon)
 {
     R alpha, gamma, beta[4];
-    alpha = R(1.) / sqrt(-G.gcon[0]);
+    alpha = rsq(-G.gcon[0]);
     for (int j = 1; j < 4; j++) beta[j] = G.gcon[j] * alpha * alpha;
     gamma_calc(pr, G.gcov, &gamma);
-    ucon[0] = gamma / alpha;
-    for (int j = 1; j < 4; j++) ucon[j] = pr[U1 + j - 1] - gamma * beta[j] / alpha;
+    ucon[0] = dv(gamma, alpha);
+    for (int j = 1; j < 4; j++) ucon[j] = pr[U1 + j - 1] - dv(gamma * beta[j], alpha);
 }
 
 // GPU_phys_mhd.h:738
 template <class R> HD void bcon_calc(const R *pr, const R *ucon, const R *ucov, R *bcon)
 {
     bcon[0] = pr[B1] * ucov[1] + pr[B2] * ucov[2] + pr[B3] * ucov[3];
-    for (int j = 1; j < 4; j++) bcon[j] = (pr[B1 - 1 + j] + bcon[0] * ucon[j]) / ucon[0];
+    for (int j = 1; j < 4; j++) bcon[j] = dv(pr[B1 - 1 + j] + bcon[0] * ucon[j], ucon[0]);
 }
 
 // GPU_phys_mhd.h:692
@@ -134,7 +147,7 @@ template <class R> HD void primtoflux(const R *pr, const State<R> &q, int dir, c
     flux[B3] = q.bcon[3] * q.ucon[dir] - q.bcon[dir] * q.ucon[3];
     {
         R Pe = (gam - R(1.0)) * pr[UU];
-        R entr = Pe * pow(pr[RHO], -gam);
+        R entr = Pe * rpow(pr[RHO], -gam);
         flux[KTOT] = flux[RHO] * entr;
     }
     for (int k = 0; k < NPR; k++) flux[k] *= G.g;
@@ -143,8 +156,8 @@ template <class R> HD void primtoflux(const R *pr, const State<R> &q, int dir, c
         if (dir == 1) { Acon_0 = G.gcon[1]; Acon_js = G.gcon[4]; }
         else if (dir == 2) { Acon_0 = G.gcon[2]; Acon_js = G.gcon[7]; }
         else { Acon_0 = G.gcon[3]; Acon_js = G.gcon[9]; }
-        cs2 = gam * (gam - R(1.)) * pr[UU] / w;
-        va2 = bsq / eta;
+        cs2 = dv(gam * (gam - R(1.)) * pr[UU], w);
+        va2 = dv(bsq, eta);
         cms2 = cs2 + va2 - cs2 * va2;
         if (cms2 < R(0.)) cms2 = R(kSMALL);
         if (cms2 > R(1.)) cms2 = R(1.);
@@ -163,8 +176,8 @@ template <class R> HD void primtoflux(const R *pr, const State<R> &q, int dir, c
         if ((discr < R(0.0)) && (discr > R(-1.e-10))) discr = R(0.0);
         else if (discr < R(-1.e-10)) discr = R(0.);
         discr = sqrt(discr);
-        vp = -(-B + discr) / (R(2.) * A);
-        vm = -(-B - discr) / (R(2.) * A);
+        vp = dv(-(-B + discr), R(2.) * A);
+        vm = dv(-(-B - discr), R(2.) * A);
         *vmax = mymax(vp, vm);
         *vmin = mymin(vp, vm);
     }
@@ -204,29 +217,29 @@ template <class R, class ConnF> HD void source(const R *ph, const State<R> &q, C
     dU[KTOT] *= g;
 }
 
+// MC slope with factor 2 (GPU_interp.h:2-25): the minmod of 2 dq_-, 2 dq_+ and
+// the centred difference, written with min and copysign (the same value as
+// the reference's selects, up to the sign of a zero slope).
+template <class R> HD R mc_slope(R ym, R y0, R yp)
+{
+    const R Dqm = R(2.) * (y0 - ym), Dqp = R(2.) * (yp - y0), Dqc = R(0.5) * (yp - ym);
+    const R m = fmin(fabs(Dqm), fabs(Dqp));
+    const R v = copysign(fmin(fabs(Dqc), m), Dqc);
+    return (Dqm * Dqp > R(0.)) ? v : R(0.);
+}
+
 // GPU_interp.h:60-111: CW84 PPM with MC (factor 2) slopes; l = a_L, r = a_R.
 template <class R> HD void para(R x1, R x2, R x3, R x4, R x5, R &lout, R &rout)
 {
-    R y[5] = {x1, x2, x3, x4, x5}, dq[5];
-    for (int i = 1; i < 4; i++) {
-        R Dqm = R(2.) * (y[i] - y[i - 1]);
-        R Dqp = R(2.) * (y[i + 1] - y[i]);
-        R Dqc = R(0.5) * (y[i + 1] - y[i - 1]);
-        R aDqm = fabs(Dqm), aDqp = fabs(Dqp), aDqc = fabs(Dqc);
-        R s = Dqm * Dqp;
-        Dqm = mymin(aDqm, aDqp);
-        R sel = (s > R(0.)) ? R(1.) : R(0.);
-        if (aDqc < Dqm) dq[i] = (Dqc > R(0.)) ? (aDqc) * sel : (-aDqc) * sel;
-        else dq[i] = (Dqc > R(0.)) ? (Dqm) * sel : (-Dqm) * sel;
-    }
-    R l = R(0.5) * (y[2] + y[1]) - (dq[2] - dq[1]) / R(6.0);
-    R r = R(0.5) * (y[3] + y[2]) - (dq[3] - dq[2]) / R(6.0);
-    R qa = (r - y[2]) * (y[2] - l);
-    R qd = (r - l);
-    R qe = R(6.0) * (y[2] - R(0.5) * (l + r));
-    if (qa <= R(0.)) { l = y[2]; r = y[2]; }
-    if (qd * (qd - qe) < R(0.0)) l = R(3.0) * y[2] - R(2.0) * r;
-    else if (qd * (qd + qe) < R(0.0)) r = R(3.0) * y[2] - R(2.0) * l;
+    const R dq1 = mc_slope(x1, x2, x3), dq2 = mc_slope(x2, x3, x4), dq3 = mc_slope(x3, x4, x5);
+    R l = R(0.5) * (x3 + x2) - dv(dq2 - dq1, R(6.0));
+    R r = R(0.5) * (x4 + x3) - dv(dq3 - dq2, R(6.0));
+    const R qa = (r - x3) * (x3 - l);
+    const R qd = (r - l);
+    const R qe = R(6.0) * (x3 - R(0.5) * (l + r));
+    if (qa <= R(0.)) { l = x3; r = x3; }
+    if (qd * (qd - qe) < R(0.0)) l = R(3.0) * x3 - R(2.0) * r;
+    else if (qd * (qd + qe) < R(0.0)) r = R(3.0) * x3 - R(2.0) * l;
     lout = l;
     rout = r;
 }

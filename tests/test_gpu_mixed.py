@@ -82,7 +82,7 @@ def test_mixed_one_step_close_to_fp64(setup):
         out[mode] = (dt, ndt, ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS))
     d64, n64, p64, s64 = out[H.HAMR_PREC_FP64]
     d32, n32, p32, s32 = out[H.HAMR_PREC_MIXED]
-    assert abs(d32 - d64) <= 1e-6 * d64 and abs(n32 - n64) <= 1e-6 * n64
+    assert abs(d32 - d64) <= 1e-5 * d64 and abs(n32 - n64) <= 1e-5 * n64
     err = GT.relerr_by_var(p32[:, :, I, I, I], p64[:, :, I, I, I])
     print("one step, mixed vs FP64, max rel err per var:", np.array2string(err, precision=2), flush=True)
     assert np.all(err <= 1e-5), err
