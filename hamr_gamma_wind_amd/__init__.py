@@ -18,9 +18,10 @@ NPR = 9
 NG = 3
 RHO, UU, U1, U2, U3, B1, B2, B3, KTOT = range(9)
 FACE1, FACE2, CORN, CENT, FACE3 = range(5)
-(HAMR_P, HAMR_PS, HAMR_PH, HAMR_PSH, HAMR_F1, HAMR_F2, HAMR_F3, HAMR_EMF, HAMR_ECENT, HAMR_U0) = range(10)
+(HAMR_P, HAMR_PS, HAMR_PH, HAMR_PSH, HAMR_F1, HAMR_F2, HAMR_F3, HAMR_EMF, HAMR_ECENT, HAMR_U0,
+ HAMR_UNEW) = range(11)
 NCOMP = {HAMR_P: 9, HAMR_PS: 3, HAMR_PH: 9, HAMR_PSH: 3, HAMR_F1: 9, HAMR_F2: 9, HAMR_F3: 9, HAMR_EMF: 3,
-         HAMR_ECENT: 3, HAMR_U0: 9}
+         HAMR_ECENT: 3, HAMR_U0: 9, HAMR_UNEW: 9}
 HAMR_PREC_FP64, HAMR_PREC_MIXED = 0, 1
 
 
@@ -120,6 +121,11 @@ def load_library(path=None):
         "hamr_copy_probe": [ctxp, C.c_longlong, C.c_int, _dp],
         "hamr_set_precision": [ctxp, C.c_int],
         "hamr_get_precision": [ctxp, _ip],
+        "hamr_set_array_dev": [ctxp, C.c_int, C.c_void_p],
+        "hamr_get_array_dev": [ctxp, C.c_int, C.c_void_p],
+        "hamr_stage_forward": [ctxp, C.c_double],
+        "hamr_stage_adjoint": [ctxp, C.c_double, _dp, _dp, _dp],
+        "hamr_stage_adjoint_dev": [ctxp, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -322,6 +328,30 @@ class Context:
 
     def sync(self):
         self._chk(self.L.hamr_sync(self.h), "hamr_sync")
+
+    # device-pointer access (torch tensors or any device buffer of this layout)
+    def set_dev(self, which, ptr):
+        self._chk(self.L.hamr_set_array_dev(self.h, which, C.c_void_p(ptr)), "hamr_set_array_dev")
+
+    def get_dev(self, which, ptr):
+        self._chk(self.L.hamr_get_array_dev(self.h, which, C.c_void_p(ptr)), "hamr_get_array_dev")
+
+    # one forward stage and its adjoint
+    def stage_forward(self, Dt):
+        self._chk(self.L.hamr_stage_forward(self.h, Dt), "hamr_stage_forward")
+
+    def stage_adjoint(self, Dt, w):
+        """dL/dP and dL/d(face field) of L = sum w . U' (host arrays)."""
+        w = np.ascontiguousarray(w, dtype=float)
+        assert w.shape == (self.nblk, 9) + self.D
+        gp = np.zeros((self.nblk, 9) + self.D)
+        gps = np.zeros((self.nblk, 3) + self.D)
+        self._chk(self.L.hamr_stage_adjoint(self.h, Dt, _ptr(w), _ptr(gp), _ptr(gps)), "hamr_stage_adjoint")
+        return gp, gps
+
+    def stage_adjoint_dev(self, Dt, w_ptr, gp_ptr, gps_ptr):
+        self._chk(self.L.hamr_stage_adjoint_dev(self.h, Dt, C.c_void_p(w_ptr), C.c_void_p(gp_ptr),
+                                                C.c_void_p(gps_ptr)), "hamr_stage_adjoint_dev")
 
     def set_precision(self, mode):
         """HAMR_PREC_FP64 or HAMR_PREC_MIXED (FP32 reconstruction, fluxes, EMFs)."""

@@ -22,6 +22,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 
 SOURCES = [
     ("hamr_kernels.hip", True),
+    ("hamr_adjoint.hip", True),
     ("hamr_capi.hip", True),
     ("hamr_setup.cpp", False),
 ]

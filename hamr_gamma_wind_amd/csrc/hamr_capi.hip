@@ -37,6 +37,9 @@ struct hamr_ctx {
     int nring = 0;
     hipGraphExec_t graph = nullptr;
     int prec = HAMR_PREC_FP64;     // arithmetic of reconstruction, fluxes and EMFs
+    // adjoint scratch (allocated by the first adjoint call)
+    double *aw = nullptr, *alP = nullptr, *alps = nullptr, *alpsf = nullptr, *alE = nullptr, *alEc = nullptr;
+    double *alF[3] = {nullptr, nullptr, nullptr};
     std::vector<void *> allocs;
 };
 
@@ -460,6 +463,7 @@ static int array_info(hamr_ctx *c, int which, double **ptr, int *ncomp)
     case HAMR_EMF: *ptr = c->E; *ncomp = 3; return 0;
     case HAMR_ECENT: *ptr = c->Ec; *ncomp = 3; return 0;
     case HAMR_U0: *ptr = c->U0; *ncomp = NPR; return 0;
+    case HAMR_UNEW: *ptr = c->Un; *ncomp = NPR; return 0;
     default: return fail(c, "unknown array selector");
     }
 }
@@ -511,6 +515,34 @@ int hamr_get_array(hamr_ctx *c, int which, double *dst)
     }
     HIPCHK(c, hipMemcpy2DAsync(dst, N * sizeof(double), d, K * sizeof(double), N * sizeof(double), rows,
                                hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int hamr_set_array_dev(hamr_ctx *c, int which, const double *dsrc)
+{
+    if (!c || !dsrc) return -1;
+    double *d;
+    int nc;
+    if (array_info(c, which, &d, &nc)) return -1;
+    if (is_float_array(c, which)) return fail(c, "hamr_set_array_dev: float array in the mixed step");
+    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell, rows = (size_t)c->L.nblk * nc;
+    HIPCHK(c, hipMemcpy2DAsync(d, K * sizeof(double), dsrc, N * sizeof(double), N * sizeof(double), rows,
+                               hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int hamr_get_array_dev(hamr_ctx *c, int which, double *ddst)
+{
+    if (!c || !ddst) return -1;
+    double *d;
+    int nc;
+    if (array_info(c, which, &d, &nc)) return -1;
+    if (is_float_array(c, which)) return fail(c, "hamr_get_array_dev: float array in the mixed step");
+    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell, rows = (size_t)c->L.nblk * nc;
+    HIPCHK(c, hipMemcpy2DAsync(ddst, N * sizeof(double), d, K * sizeof(double), N * sizeof(double), rows,
+                               hipMemcpyDeviceToDevice, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return 0;
 }
@@ -777,6 +809,70 @@ int hamr_time_stage(hamr_ctx *c, int stage, int nrep, double *avg_ms)
     (void)hipEventDestroy(e1);
     *avg_ms = (double)ms / nrep;
     return 0;
+}
+
+// ---- one forward stage and its adjoint ----
+static void enqueue_stage_forward(hamr_ctx *c, double Dt)
+{
+    const KParams &P = c->P;
+    const int nb = c->L.nblk;
+    const int *bl = c->zlist;
+    bound_all(c, c->p, c->ps);
+    for (int dir = 1; dir <= 3; dir++) launch_flux(P, dir, bl, nb, c->p, c->ps, c->F[dir - 1], nullptr, c->prec, c->s);
+    launch_emf(P, bl, nb, c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, c->s);
+    launch_pole(P, c->ring, c->nring, Dt, nullptr, c->E, c->prec, c->s);
+    launch_ct_update(P, bl, nb, Dt, nullptr, 1.0, c->ps, c->psh, c->E, c->prec, c->s);
+    launch_cons(P, bl, nb, Dt, c->p, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2], c->prec, c->s);
+}
+
+int hamr_stage_forward(hamr_ctx *c, double Dt)
+{
+    if (!c) return -1;
+    HIPCHK(c, hipSetDevice(c->device));
+    enqueue_stage_forward(c, Dt);
+    return check_launch(c, "hamr_stage_forward");
+}
+
+// w, grad_p, grad_ps: pointers of `kind` (host or device), unpadded layout.
+static int stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, double *gps, hipMemcpyKind in,
+                         hipMemcpyKind out)
+{
+    if (!c || !w) return -1;
+    if (!c->d.minkowski || c->L.nblk != 1 || c->prec != HAMR_PREC_FP64)
+        return fail(c, "hamr_stage_adjoint: needs a flat periodic box of one block in FP64");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell;
+    if (!c->aw) {
+        if (dalloc(c, &c->aw, NPR * K) || dalloc(c, &c->alP, NPR * K) || dalloc(c, &c->alps, 3 * K) ||
+            dalloc(c, &c->alpsf, 3 * K) || dalloc(c, &c->alE, 3 * K) || dalloc(c, &c->alEc, 3 * K) ||
+            dalloc(c, &c->alF[0], NPR * K) || dalloc(c, &c->alF[1], NPR * K) || dalloc(c, &c->alF[2], NPR * K))
+            return -1;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    HIPCHK(c, hipMemcpy2DAsync(c->aw, K * sizeof(double), w, N * sizeof(double), N * sizeof(double), NPR, in, c->s));
+    // the fluxes of the stage (their mass-flux signs select the EMF upwinding)
+    bound_all(c, c->p, c->ps);
+    for (int dir = 1; dir <= 3; dir++)
+        launch_flux(c->P, dir, c->zlist, 1, c->p, c->ps, c->F[dir - 1], nullptr, HAMR_PREC_FP64, c->s);
+    launch_adjoint(c->P, c->blocks[0], Dt, c->p, c->ps, c->F[0], c->F[1], c->F[2], c->aw, c->alP, c->alps, c->alpsf,
+                   c->alE, c->alEc, c->alF[0], c->alF[1], c->alF[2], c->s);
+    if (check_launch(c, "hamr_stage_adjoint")) return -1;
+    if (gp)
+        HIPCHK(c, hipMemcpy2DAsync(gp, N * sizeof(double), c->alP, K * sizeof(double), N * sizeof(double), NPR, out, c->s));
+    if (gps)
+        HIPCHK(c, hipMemcpy2DAsync(gps, N * sizeof(double), c->alps, K * sizeof(double), N * sizeof(double), 3, out, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int hamr_stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, double *gps)
+{
+    return stage_adjoint(c, Dt, w, gp, gps, hipMemcpyHostToDevice, hipMemcpyDeviceToHost);
+}
+
+int hamr_stage_adjoint_dev(hamr_ctx *c, double Dt, const double *w, double *gp, double *gps)
+{
+    return stage_adjoint(c, Dt, w, gp, gps, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice);
 }
 
 int hamr_copy_probe(hamr_ctx *c, long long bytes, int nrep, double *avg_ms)

@@ -36,6 +36,17 @@ void launch_update(const KParams &P, const int *blist, int nb, int full, double 
                    double dt_mult, const double *pi, const double *pb, double *pf, double *U0, double *Unew,
                    const double *psf, const void *F1, const void *F2, const void *F3, int *pflag, int *failcnt,
                    int mixed, hipStream_t s);
+// Conserved update only (k_cons, predictor form with step Dt): U(p) -> U0,
+// U(p) - Dt div F + Dt S(p) with the face average of psf -> Unew.
+void launch_cons(const KParams &P, const int *blist, int nb, double Dt, const double *p, double *U0, double *Unew,
+                 const double *psf, const void *F1, const void *F2, const void *F3, int mixed, hipStream_t s);
+// Adjoint of one forward stage of a single periodic flat block (FP64): from the
+// weights w of U' (9 slabs) to lP (9 slabs, dL/dP) and lps (3 slabs, dL/d face
+// field); F1-F3 are the stage's fluxes; lpsf, lE, lEc, lF1-lF3 are scratch.
+void launch_adjoint(const KParams &P, const BlockDev &B, double Dt, const double *p, const double *ps,
+                    const double *F1, const double *F2, const double *F3, const double *w, double *lP, double *lps,
+                    double *lpsf, double *lE, double *lEc, double *lF1, double *lF2, double *lF3, hipStream_t s);
+
 void launch_bound(const KParams &P, int dir, const int *blist, int nb, double *p, double *ps, int *pflag,
                   hipStream_t s);
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);

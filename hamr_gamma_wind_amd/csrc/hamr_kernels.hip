@@ -772,6 +772,19 @@ void launch_update(const KParams &P, const int *blist, int nb, int full, double 
     hipLaunchKernelGGL(k_fallback, grid_for(w, nb), dim3(256), 0, s, P, blist, Unew, pf, pflag, failcnt);
 }
 
+void launch_cons(const KParams &P, const int *blist, int nb, double Dt, const double *p, double *U0, double *Unew,
+                 const double *psf, const void *F1, const void *F2, const void *F3, int mixed, hipStream_t s)
+{
+    const int64_t w = (int64_t)P.bs[0] * P.bs[1] * P.bs[2];
+    const size_t lds = sizeof(double) * MV_N * cons_max_cols(P.bs[2]);
+    if (mixed)
+        hipLaunchKernelGGL((k_cons<float>), grid_for(w, nb), dim3(256), lds, s, P, blist, 0, Dt, nullptr, 1.0, p, p, U0,
+                           Unew, psf, (const float *)F1, (const float *)F2, (const float *)F3);
+    else
+        hipLaunchKernelGGL((k_cons<double>), grid_for(w, nb), dim3(256), lds, s, P, blist, 0, Dt, nullptr, 1.0, p, p,
+                           U0, Unew, psf, (const double *)F1, (const double *)F2, (const double *)F3);
+}
+
 void launch_bound(const KParams &P, int dir, const int *blist, int nb, double *p, double *ps, int *pflag,
                   hipStream_t s)
 {

@@ -62,7 +62,8 @@ enum {
     HAMR_F3 = 6,
     HAMR_EMF = 7,   /* corner EMF E_1, E_2, E_3 */
     HAMR_ECENT = 8, /* cell-centre EMF, 3 */
-    HAMR_U0 = 9     /* conserved variables at the start of the step, 9 */
+    HAMR_U0 = 9,    /* conserved variables at the start of the step, 9 */
+    HAMR_UNEW = 10  /* conserved variables after the last forward stage / update, 9 */
 };
 
 /* Arithmetic of the step (hamr_set_precision).  FP64: every stage in double,
@@ -137,6 +138,10 @@ int hamr_download(hamr_ctx *ctx, int n, double *p_aos, double *ps_aos);
 /* Whole arrays in [block][component][cell] order (see HAMR_* selectors). */
 int hamr_set_array(hamr_ctx *ctx, int which, const double *src);
 int hamr_get_array(hamr_ctx *ctx, int which, double *dst);
+/* The same with device pointers (same layout, device memory of the context's
+ * device; for callers that keep the state in their own device buffers). */
+int hamr_set_array_dev(hamr_ctx *ctx, int which, const double *dsrc);
+int hamr_get_array_dev(hamr_ctx *ctx, int which, double *ddst);
 
 /* ---- per-block stages, the call sequence of advance_GPU (step_ch.c:997-1336) ---- */
 /* GPU_fluxcalc2D (GPU_main.cu:560): PPM + HLL flux through the lower dir-faces
@@ -179,6 +184,20 @@ int hamr_sync(hamr_ctx *ctx);
  * modified): 0-2 flux x1..x3, 3 EMF, 4 polar average, 5 face-B update,
  * 6 conserved update + inversion, 7 ghost fill (3 directions). */
 int hamr_time_stage(hamr_ctx *ctx, int stage, int nrep, double *avg_ms);
+/* ---- one forward stage and its adjoint (no reference counterpart) ----
+ * Forward stage: the predictor's flux, EMF, face-field and conserved updates
+ * with the full step Dt, from p/ps (ghosts filled first):
+ *   U' = U(P) - Dt sum_d dF_d/dx_d + Dt S(P) (HAMR_UNEW), U'[B] = the face
+ *   average of the updated face field (HAMR_PSH).
+ * Adjoint: for L = sum over interior cells of w . U', the gradients dL/dP
+ * (HAMR_P layout) and dL/d(face field) (HAMR_PS layout), FP64, recomputing the
+ * reconstructed states; the limiter and upwind selections are those of the
+ * forward stage.  Supported for a flat periodic box of one block in FP64
+ * (HAMR_PREC_FP64, minkowski = 1, nb = 1, 1, 1). */
+int hamr_stage_forward(hamr_ctx *ctx, double Dt);
+int hamr_stage_adjoint(hamr_ctx *ctx, double Dt, const double *w, double *grad_p, double *grad_ps);
+int hamr_stage_adjoint_dev(hamr_ctx *ctx, double Dt, const double *w_dev, double *grad_p_dev, double *grad_ps_dev);
+
 /* Streaming copy of `bytes` bytes (read once, written once), nrep launches:
  * the known-size kernel that calibrates HBM byte counters; average ms. */
 int hamr_copy_probe(hamr_ctx *ctx, long long bytes, int nrep, double *avg_ms);

@@ -942,6 +942,72 @@ void oracle_ct_update(const oracle_params *par, double Dt, const double *psi, do
             }
 }
 
+/* The conserved update of one cell (GPU_fixup.h:75-217): U^n (from pi on the
+ * predictor, stored in U0; from U0 on the corrector), flux divergence, source,
+ * face-field average.  pf returns the state the source used (the inversion's
+ * initial guess). */
+static void cons_cell(const oracle_params *par, const ogrid *Gp, int full, double Dt, long g, long col,
+                      const double *pi, const double *pb, double *U0, const double *psf, const double *F1,
+                      const double *F2, const double *F3, const oracle_metric *m, double *pf, double *U)
+{
+    const ogrid G = *Gp;
+    const long N = G.ncell, si = G.s1, sj = G.s2;
+    const double dx1 = par->dx[0], dx2 = par->dx[1], dx3 = par->dx[2];
+    const double gam = par->gam;
+    const double *gcov, *gcon;
+    double gd, dU[ORC_NPR];
+    ostate q;
+    int k;
+    geom_at(m, &G, CENT, col, &gcov, &gcon, &gd);
+    if (!full) {
+        double dv1, dv2;
+        for (k = 0; k < ORC_NPR; k++) pf[k] = pi[(long)k * N + g];
+        get_state(pf, gcov, gcon, &q);
+        primtoflux(pf, &q, 0, gcon, gd, gam, U, &dv1, &dv2);
+        for (k = 0; k < ORC_NPR; k++) U0[(long)k * N + g] = U[k];
+    } else {
+        for (k = 0; k < ORC_NPR; k++) U[k] = U0[(long)k * N + g];
+        for (k = 0; k < ORC_NPR; k++) pf[k] = pb[(long)k * N + g];
+        get_state(pf, gcov, gcon, &q);
+    }
+    for (k = 0; k < ORC_NPR; k++) {
+        const double *f1 = F1 + (long)k * N, *f2 = F2 + (long)k * N, *f3 = F3 + (long)k * N;
+        U[k] -= Dt * (f1[g + si] - f1[g]) / (dx1 * 1.0);
+        U[k] -= Dt * (f2[g + sj] - f2[g]) / (dx2 * 1.0);
+        U[k] -= Dt * (f3[g + 1] - f3[g]) / (dx3 * 1.0);
+    }
+    source(pf, &q, m->conn + col * 40, gd, gam, dU);
+    for (k = 0; k < ORC_NPR; k++) U[k] += Dt * (dU[k]);
+    {
+        const double *gdF1 = m->gdet + (long)FACE1 * G.ncol, *gdF2 = m->gdet + (long)FACE2 * G.ncol,
+                     *gdF3 = m->gdet + (long)FACE3 * G.ncol;
+        U[B1] = 0.0;
+        U[B2] = 0.0;
+        U[B1] += (psf[0 * N + g] * gdF1[col] + psf[0 * N + g + si] * gdF1[col + G.D2]) / (2.0 * 1.0);
+        U[B2] += (psf[1 * N + g] * gdF2[col] + psf[1 * N + g + sj] * gdF2[col + 1]) / (2.0 * 1.0);
+        U[B3] = (psf[2 * N + g] * gdF3[col] + psf[2 * N + g + 1] * gdF3[col]) / 2.0;
+    }
+}
+
+/* The conserved update alone over the interior (predictor form, step Dt): U^n
+ * from pi into U0 and the updated conserved variables into Unew. */
+void oracle_cons(const oracle_params *par, double Dt, const double *pi, double *U0, double *Unew,
+                 const double *psf, const double *F1, const double *F2, const double *F3, const oracle_metric *m)
+{
+    ogrid G = mkgrid(par);
+    const int NG = ORC_NG;
+    const long N = G.ncell;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = NG; i < NG + par->n1; i++)
+        for (int j = NG; j < NG + par->n2; j++)
+            for (int z = NG; z < NG + par->n3; z++) {
+                long g = CELL(G, i, j, z), col = COL(G, i, j);
+                double pf[ORC_NPR], U[ORC_NPR];
+                cons_cell(par, &G, 0, Dt, g, col, pi, pi, U0, psf, F1, F2, F3, m, pf, U);
+                for (int k = 0; k < ORC_NPR; k++) Unew[(long)k * N + g] = U[k];
+            }
+}
+
 /* fixup kernel, GPU_fixup.h:5-425: conserved update, source, face-B average,
  * Noble cascade, floors and flags. */
 void oracle_update(const oracle_params *par, int full, double Dt, const double *pi, const double *pb,
@@ -950,8 +1016,7 @@ void oracle_update(const oracle_params *par, int full, double Dt, const double *
 {
     ogrid G = mkgrid(par);
     const int NG = ORC_NG;
-    const long N = G.ncell, si = G.s1, sj = G.s2;
-    const double dx1 = par->dx[0], dx2 = par->dx[1], dx3 = par->dx[2];
+    const long N = G.ncell;
     const double gam = par->gam;
     long long fc0 = 0, fc1 = 0, fc2 = 0, fc3 = 0;
 #pragma omp parallel for collapse(2) schedule(dynamic, 4) reduction(+ : fc0, fc1, fc2, fc3)
@@ -960,38 +1025,10 @@ void oracle_update(const oracle_params *par, int full, double Dt, const double *
             for (int z = NG; z < NG + par->n3; z++) {
                 long g = CELL(G, i, j, z), col = COL(G, i, j);
                 const double *gcov, *gcon;
-                double gd, pf[ORC_NPR], U[ORC_NPR], dU[ORC_NPR];
-                ostate q;
+                double gd, pf[ORC_NPR], U[ORC_NPR];
                 int k, flag;
                 geom_at(m, &G, CENT, col, &gcov, &gcon, &gd);
-                if (!full) {
-                    double dv1, dv2;
-                    for (k = 0; k < ORC_NPR; k++) pf[k] = pi[(long)k * N + g];
-                    get_state(pf, gcov, gcon, &q);
-                    primtoflux(pf, &q, 0, gcon, gd, gam, U, &dv1, &dv2);
-                    for (k = 0; k < ORC_NPR; k++) U0[(long)k * N + g] = U[k];
-                } else {
-                    for (k = 0; k < ORC_NPR; k++) U[k] = U0[(long)k * N + g];
-                    for (k = 0; k < ORC_NPR; k++) pf[k] = pb[(long)k * N + g];
-                    get_state(pf, gcov, gcon, &q);
-                }
-                for (k = 0; k < ORC_NPR; k++) {
-                    const double *f1 = F1 + (long)k * N, *f2 = F2 + (long)k * N, *f3 = F3 + (long)k * N;
-                    U[k] -= Dt * (f1[g + si] - f1[g]) / (dx1 * 1.0);
-                    U[k] -= Dt * (f2[g + sj] - f2[g]) / (dx2 * 1.0);
-                    U[k] -= Dt * (f3[g + 1] - f3[g]) / (dx3 * 1.0);
-                }
-                source(pf, &q, m->conn + col * 40, gd, gam, dU);
-                for (k = 0; k < ORC_NPR; k++) U[k] += Dt * (dU[k]);
-                {
-                    const double *gdF1 = m->gdet + (long)FACE1 * G.ncol, *gdF2 = m->gdet + (long)FACE2 * G.ncol,
-                                 *gdF3 = m->gdet + (long)FACE3 * G.ncol;
-                    U[B1] = 0.0;
-                    U[B2] = 0.0;
-                    U[B1] += (psf[0 * N + g] * gdF1[col] + psf[0 * N + g + si] * gdF1[col + G.D2]) / (2.0 * 1.0);
-                    U[B2] += (psf[1 * N + g] * gdF2[col] + psf[1 * N + g + sj] * gdF2[col + 1]) / (2.0 * 1.0);
-                    U[B3] = (psf[2 * N + g] * gdF3[col] + psf[2 * N + g + 1] * gdF3[col]) / 2.0;
-                }
+                cons_cell(par, &G, full, Dt, g, col, pi, pb, U0, psf, F1, F2, F3, m, pf, U);
                 flag = oracle_utoprim(0, U, gcov, gcon, gd, gam, pf);
                 if (flag) {
                     fc0++;

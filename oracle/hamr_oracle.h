@@ -76,6 +76,10 @@ void oracle_emf(const oracle_params *par, const double *pb, const double *F1,
 void oracle_ct_update(const oracle_params *par, double Dt, const double *psi, double *psf,
                       const double *E, const oracle_metric *m);
 /* full=0: U0 <- U(pi), source from pi, guess pi;  full=1: U from U0, source and guess from pb. */
+/* The conserved update alone over the interior (predictor form, step Dt):
+ * U(pi) into U0, U(pi) - Dt div F + Dt S(pi) with the face average of psf into Unew. */
+void oracle_cons(const oracle_params *par, double Dt, const double *pi, double *U0, double *Unew,
+                 const double *psf, const double *F1, const double *F2, const double *F3, const oracle_metric *m);
 void oracle_update(const oracle_params *par, int full, double Dt, const double *pi,
                    const double *pb, double *pf, double *U0, const double *psf,
                    const double *F1, const double *F2, const double *F3,

@@ -59,6 +59,7 @@ class Oracle:
         L.oracle_ct_update.argtypes = [P, C.c_double, _dp, _dp, _dp, M]
         L.oracle_update.argtypes = [P, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, M,
                                     _ip, C.POINTER(C.c_longlong)]
+        L.oracle_cons.argtypes = [P, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, M]
         L.oracle_bound.argtypes = [P, _dp, _dp, M]
         L.oracle_block_dt.argtypes = [P, _dp, _dp, _dp]
         L.oracle_block_dt.restype = C.c_double
@@ -158,6 +159,22 @@ class Grid:
                                  _ptr(F1), _ptr(F2), _ptr(F3), C.byref(self.metric), _ptr(pflag, _ip),
                                  self.failcount)
         return pflag
+
+    def cons(self, Dt, pi, psf, F1, F2, F3):
+        U0 = self.zeros(NPR)
+        Un = self.zeros(NPR)
+        self.o.lib.oracle_cons(C.byref(self.par), Dt, _ptr(pi), _ptr(U0), _ptr(Un), _ptr(psf), _ptr(F1), _ptr(F2),
+                               _ptr(F3), C.byref(self.metric))
+        return Un
+
+    def stage(self, Dt, P, PS):
+        """One forward stage (the predictor with step Dt) from a state with
+        filled ghosts: U' over the interior and the updated face fields."""
+        F = [self.flux(d, P, PS)[0] for d in (1, 2, 3)]
+        E = self.emf(P, F[0], F[1], F[2], Dt)
+        PSF = PS.copy()
+        self.ct_update(Dt, PS, PSF, E)
+        return self.cons(Dt, P, PSF, F[0], F[1], F[2]), PSF
 
     def block_dt(self, d1, d2, d3):
         return self.o.lib.oracle_block_dt(C.byref(self.par), _ptr(d1), _ptr(d2), _ptr(d3))
