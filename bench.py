@@ -27,6 +27,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -123,6 +125,50 @@ def run_mode(H, ctx, p0, ps0, mode, args):
     return el, p, stage_ms, dt, fi
 
 
+def adjoint_bench(H, n=128, reps=5):
+    """Forward stage and its adjoint on the flat periodic n^3 box of one block
+    (the adjoint workload), device-resident; ms per call (wall time between
+    device synchronisations)."""
+    import torch
+
+    from hamr_gamma_wind_amd import workloads as W
+
+    gam = 13.0 / 9.0
+    desc = H.make_desc((n, n, n), (1, 1, 1), gam=gam, minkowski=1, rho_floor=0.0, u_floor=0.0)
+    ctx = H.Context(desc)
+    try:
+        ctx.set_grid()
+        prims, faces = W.minkowski_box((n, n, n), gam=gam, seed=1)
+        p, ps = W.to_block(prims, faces)
+        ctx.set(H.HAMR_P, p)
+        ctx.set(H.HAMR_PS, ps)
+        _, dx = H.desc_coords(desc)
+        Dt = 0.3 * min(dx)
+        dev = torch.device("cuda:0")
+        D = tuple(k + 6 for k in (n, n, n))
+        w = torch.tensor(np.random.default_rng(2).normal(size=(1, 9) + D), device=dev)
+        gp = torch.empty((1, 9) + D, dtype=torch.float64, device=dev)
+        gps = torch.empty((1, 3) + D, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        out = {}
+        for name, fn in (("forward_ms", lambda: ctx.stage_forward(Dt)),
+                         ("adjoint_ms", lambda: ctx.stage_adjoint_dev(Dt, w.data_ptr(), gp.data_ptr(), gps.data_ptr()))):
+            fn()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            ctx.sync()
+            out[name] = (time.perf_counter() - t0) / reps * 1e3
+        out["zones"] = n ** 3
+        out["adjoint_zones_per_s"] = n ** 3 / (out["adjoint_ms"] * 1e-3)
+        out["workload"] = ("flat periodic %d^3 box (one block), random state: rho lognormal, u = rho lognormal, "
+                           "Lorentz factor <= 5, max b^2/rho = 10; L = sum w . U', FP64" % n)
+        return out
+    finally:
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +180,7 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one streaming copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None, help="run one arithmetic only")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement (128^3 flat box)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (each GPU box holds one MI355X); "
@@ -191,6 +238,10 @@ def main():
         fp64 = {"value": nz * args.steps / el64, "ms_per_step": el64 / args.steps * 1e3,
                 "stages_ms_per_step": {k: round(2 * v, 4) for k, v in res["fp64"][2].items()}}
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_bench(H)
+
     cpu = None
     if args.cpu_steps > 0:
         v, cores, el_cpu = cpu_baseline(desc, args.cpu_steps)
@@ -225,6 +276,7 @@ def main():
         "fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "error_vs_fp64": err,
+        "adjoint": adj,
         "copy_probe": copy,
         "cpu_baseline": cpu,
     }

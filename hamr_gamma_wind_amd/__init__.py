@@ -80,6 +80,13 @@ def load_library(path=None):
     if not os.path.exists(p):
         raise HamrError("libhamr_hip.so is not built: run __graft_entry__.build() "
                         "(hamr_gamma_wind_amd/build.py)")
+    # torch ships its own HIP runtime.  Loading it before the library makes the
+    # library bind to that same runtime, so that torch device memory can be
+    # handed to the library (two runtimes in one process do not share a device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(p)
     D = C.POINTER(Desc)
     ctxp = C.c_void_p

@@ -15,7 +15,10 @@ Bars:
     on a 12x10x8 box;
   - on the 128^3 box, 64 random probes of dL/dP agree with central
     differences (h = 1e-6 of the larger of the probed value and the rms of its
-    variable) to 1e-5 relative;
+    variable) to 1e-5 relative, plus the rounding bound of the difference
+    quotient itself, 2^-52 sum |w| (|U'(+h)| + |U'(-h)|) / 2h over the cells
+    the probe changes (a probe whose derivative is small next to the U' it
+    moves is limited by that rounding, not by the adjoint);
   - torch.autograd through hamr_gamma_wind_amd.autograd gives the gradient of
     the direct C-ABI adjoint call (to 1e-13: the adjoint's accumulation order
     is that of its atomic adds).
@@ -156,13 +159,16 @@ def test_adjoint_central_differences_128():
             # U' changes only near the probe: difference first, so that the sum
             # over the box adds exact zeros elsewhere
             fd = float((w * (Us[0] - Us[1])).sum()) / (2 * h)
+            moved = Us[0] != Us[1]
+            noise = 2.0**-52 * float((w.abs() * (Us[0].abs() + Us[1].abs()))[moved].sum()) / (2 * h)
             ad = float(gP[(k,) + c])
             scale = max(abs(ad), abs(fd), 1e-8 * float(np.abs(gP[k]).max()))
-            rel = abs(fd - ad) / scale
-            worst = max(worst, rel)
-            if rel > 1e-5:
-                bad.append((k, c, ad, fd, rel))
-        print("128^3 central differences: 64 probes, worst rel err %.3g" % worst, bad, flush=True)
+            ratio = abs(fd - ad) / (1e-5 * scale + noise)
+            worst = max(worst, ratio)
+            if ratio > 1.0:
+                bad.append((k, c, ad, fd, abs(fd - ad) / scale, noise))
+        print("128^3 central differences: 64 probes, worst |fd - ad| / (1e-5 scale + rounding) = %.3g" % worst, bad,
+              flush=True)
         assert not bad, bad
     finally:
         ctx.close()
