@@ -40,6 +40,26 @@ class TorusDesc(C.Structure):
                 ("aphi_cut", C.c_double), ("perturb", C.c_double), ("seed", C.c_ulonglong)]
 
 
+class AmrDescC(C.Structure):
+    _fields_ = [("nlev", C.c_int), ("box", ((C.c_int * 2) * 3) * 3), ("seed", C.c_ulonglong), ("p_big", C.c_double),
+                ("p_mid", C.c_double)]
+
+
+def make_amr_desc(boxes, seed=1, p_big=0.35, p_mid=0.35):
+    """boxes[l-1] = ((lo1, hi1), (lo2, hi2), (lo3, hi3)) in cells of level l, for l = 1 .. nlev-1."""
+    a = AmrDescC()
+    a.nlev = len(boxes) + 1
+    for l, b in enumerate(boxes, start=1):
+        for k in range(3):
+            a.box[l][k][0], a.box[l][k][1] = int(b[k][0]), int(b[k][1])
+    a.seed, a.p_big, a.p_mid = seed, p_big, p_mid
+    return a
+
+
+HAMR_TUNE_NAMES = ("flux1", "flux2", "flux3", "emf_centre", "emf_corner", "ct_update", "cons", "invert",
+                   "fallback", "bound1", "bound2", "bound3", "exchange")
+
+
 def make_desc(bs, nb, gam=13.0 / 9.0, spin=0.9375, cour=0.8, rin=0.0, rout=1000.0, rho_floor=1e-5,
               u_floor=1e-7, minkowski=0, device=0):
     d = Desc()
@@ -133,6 +153,16 @@ def load_library(path=None):
         "hamr_stage_forward": [ctxp, C.c_double],
         "hamr_stage_adjoint": [ctxp, C.c_double, _dp, _dp, _dp],
         "hamr_stage_adjoint_dev": [ctxp, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p],
+        "hamr_create_amr": [D, C.POINTER(AmrDescC), C.POINTER(ctxp)],
+        "hamr_desc_amr": [D, C.POINTER(AmrDescC), _llp, _ip, _ip, _ip],
+        "hamr_desc_amr_geometry": [D, C.POINTER(AmrDescC), C.c_int, _dp, _dp, _dp, _dp, _dp],
+        "hamr_desc_amr_torus": [D, C.POINTER(AmrDescC), C.POINTER(TorusDesc), _dp, _dp],
+        "hamr_amr_info": [ctxp, _llp],
+        "hamr_block_layout": [ctxp, C.c_int, _ip],
+        "hamr_geometry_info": [ctxp, C.c_int, _llp],
+        "hamr_autotune": [ctxp, C.c_int],
+        "hamr_tune_get": [ctxp, C.c_int, _ip, _dp],
+        "hamr_tune_set": [ctxp, C.c_int, _ip],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -208,6 +238,68 @@ def desc_torus(desc, tdesc):
     if r:
         raise HamrError("hamr_desc_torus failed (%d)" % r)
     return p, ps
+
+
+AMR_INFO_KEYS = ("blocks", "geometries", "cells", "zones", "ex1", "ex2", "ex3", "pro1", "pro2", "pro3", "rings", "amr")
+
+
+def desc_amr(desc, amr, tables=False):
+    """The multi-level block set (host only): info, block rows (level, o, bs,
+    geo, pole, face kinds) and, with tables=True, the exchange entries per stage."""
+    L = load_library()
+    info = np.zeros(12, dtype=np.int64)
+    r = L.hamr_desc_amr(C.byref(desc), C.byref(amr), _ptr(info, _llp), None, None, None)
+    if r:
+        raise HamrError("hamr_desc_amr: description rejected (%d)" % r)
+    nb = int(info[0])
+    rows = np.zeros((nb, 15), dtype=np.int32)
+    nex = [int(info[4 + s]) for s in range(3)]
+    npro = [int(info[7 + s]) for s in range(3)]
+    ex = np.zeros((sum(nex), 4), dtype=np.int32) if tables else None
+    pro = np.zeros((sum(npro), 16), dtype=np.int32) if tables else None
+    L.hamr_desc_amr(C.byref(desc), C.byref(amr), _ptr(info, _llp), _ptr(rows, _ip),
+                    _ptr(ex, _ip) if tables else None, _ptr(pro, _ip) if tables else None)
+    out = dict(info={k: int(v) for k, v in zip(AMR_INFO_KEYS, info)}, rows=rows)
+    if tables:
+        out["ex"] = np.split(ex, np.cumsum(nex)[:-1])
+        out["pro"] = np.split(pro, np.cumsum(npro)[:-1])
+    return out
+
+
+def amr_block_dims(rows):
+    return [tuple(int(b) + 2 * NG for b in r[4:7]) for r in rows]
+
+
+def desc_amr_geometry(desc, amr, geo, D1, D2):
+    """Geometry slab geo of the block set (device layout; D1 x D2 columns)."""
+    L = load_library()
+    kc = (D1 * D2 + 63) // 64 * 64
+    gcov, gcon = np.zeros(5 * 10 * kc), np.zeros(5 * 10 * kc)
+    gdet, conn, rcen = np.zeros(5 * kc), np.zeros(40 * kc), np.zeros(kc)
+    if L.hamr_desc_amr_geometry(C.byref(desc), C.byref(amr), geo, _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn),
+                                _ptr(rcen)):
+        raise HamrError("hamr_desc_amr_geometry failed")
+    return dict(gcov=gcov.reshape(5, 10, kc), gcon=gcon.reshape(5, 10, kc), gdet=gdet.reshape(5, kc),
+                conn=conn.reshape(40, kc), rcen=rcen)
+
+
+def desc_amr_torus(desc, amr, tdesc):
+    """Torus initial state of the block set: lists of per-block p (9,)+D and ps (3,)+D."""
+    L = load_library()
+    dims = amr_block_dims(desc_amr(desc, amr)["rows"])
+    n = sum(int(np.prod(D)) for D in dims)
+    p, ps = np.zeros(NPR * n), np.zeros(3 * n)
+    r = L.hamr_desc_amr_torus(C.byref(desc), C.byref(amr), C.byref(tdesc), _ptr(p), _ptr(ps))
+    if r:
+        raise HamrError("hamr_desc_amr_torus failed (%d)" % r)
+    P, S, a, b = [], [], 0, 0
+    for D in dims:
+        m = int(np.prod(D))
+        P.append(p[a:a + NPR * m].reshape((NPR,) + D))
+        S.append(ps[b:b + 3 * m].reshape((3,) + D))
+        a += NPR * m
+        b += 3 * m
+    return P, S
 
 
 # ---------------------------------------------------------------------------
@@ -390,6 +482,95 @@ class Context:
         f = self.L.hamr_step_graph if graph else self.L.hamr_step
         self._chk(f(self.h, nsteps, C.byref(v)), "hamr_step")
         return v.value
+
+    # launch autotuner
+    def autotune(self, nrep=3):
+        self._chk(self.L.hamr_autotune(self.h, nrep), "hamr_autotune")
+        return self.tune_table()
+
+    def tune_table(self):
+        out = {}
+        for k, name in enumerate(HAMR_TUNE_NAMES):
+            cfg = np.zeros(3, dtype=np.int32)
+            ms = C.c_double()
+            self._chk(self.L.hamr_tune_get(self.h, k, _ptr(cfg, _ip), C.byref(ms)), "hamr_tune_get")
+            out[name] = dict(threads=int(cfg[0]), grid=int(cfg[1]), xcd_order=int(cfg[2]), ms=ms.value)
+        return out
+
+    def tune_set(self, slot, threads, grid, xcd_order):
+        cfg = np.array([threads, grid, xcd_order], dtype=np.int32)
+        self._chk(self.L.hamr_tune_set(self.h, slot, _ptr(cfg, _ip)), "hamr_tune_set")
+
+
+class AmrContext(Context):
+    """A multi-level block set of uneven block sizes (hamr_create_amr).  Whole
+    arrays are lists of per-block arrays, each (ncomp,) + that block's D."""
+
+    def __init__(self, desc, amr):
+        self.L = load_library()
+        self.desc = desc
+        self.amr = amr
+        self.h = C.c_void_p()
+        r = self.L.hamr_create_amr(C.byref(desc), C.byref(amr), C.byref(self.h))
+        if r:
+            msg = self.L.hamr_last_error(self.h).decode() if self.h.value else "hamr_create_amr failed"
+            if self.h.value:
+                self.L.hamr_destroy(self.h)
+            self.h = C.c_void_p()
+            raise HamrError(msg)
+        info = self.info()
+        self.nblk = info["blocks"]
+        self.ngeo = info["geometries"]
+        self.layout = np.zeros((self.nblk, 15), dtype=np.int32)
+        for n in range(self.nblk):
+            self._chk(self.L.hamr_block_layout(self.h, n, _ptr(self.layout[n], _ip)), "hamr_block_layout")
+        self.Ds = [tuple(int(b) + 2 * NG for b in self.layout[n, 4:7]) for n in range(self.nblk)]
+
+    def info(self):
+        v = np.zeros(12, dtype=np.int64)
+        self._chk(self.L.hamr_amr_info(self.h, _ptr(v, _llp)), "hamr_amr_info")
+        return {k: int(x) for k, x in zip(AMR_INFO_KEYS, v)}
+
+    def geometry_info(self, geo):
+        v = np.zeros(6, dtype=np.int64)
+        self._chk(self.L.hamr_geometry_info(self.h, geo, _ptr(v, _llp)), "hamr_geometry_info")
+        return dict(level=int(v[0]), o1=int(v[1]), o2=int(v[2]), D1=int(v[3]), D2=int(v[4]), kcol=int(v[5]))
+
+    def download_metric(self, geo):
+        kc = self.geometry_info(geo)["kcol"]
+        gcov, gcon = np.zeros(5 * 10 * kc), np.zeros(5 * 10 * kc)
+        gdet, conn, rcen = np.zeros(5 * kc), np.zeros(40 * kc), np.zeros(kc)
+        self._chk(self.L.hamr_download_metric(self.h, geo, _ptr(gcov), _ptr(gcon), _ptr(gdet), _ptr(conn),
+                                              _ptr(rcen)), "hamr_download_metric")
+        return dict(gcov=gcov.reshape(5, 10, kc), gcon=gcon.reshape(5, 10, kc), gdet=gdet.reshape(5, kc),
+                    conn=conn.reshape(40, kc), rcen=rcen)
+
+    def get(self, which):
+        nc = NCOMP[which]
+        flat = np.zeros(sum(nc * int(np.prod(D)) for D in self.Ds))
+        self._chk(self.L.hamr_get_array(self.h, which, _ptr(flat)), "hamr_get_array")
+        out, pos = [], 0
+        for D in self.Ds:
+            m = nc * int(np.prod(D))
+            out.append(flat[pos:pos + m].reshape((nc,) + D))
+            pos += m
+        return out
+
+    def set(self, which, arrays):
+        nc = NCOMP[which]
+        assert len(arrays) == self.nblk
+        flat = np.concatenate([np.ascontiguousarray(a, dtype=float).reshape(-1) for a in arrays])
+        assert flat.size == sum(nc * int(np.prod(D)) for D in self.Ds)
+        self._chk(self.L.hamr_set_array(self.h, which, _ptr(flat)), "hamr_set_array")
+
+    def flags(self, n):
+        pf = np.zeros(self.Ds[n], dtype=np.int32)
+        fi = np.zeros(4, dtype=np.int32)
+        self._chk(self.L.hamr_flags(self.h, n, _ptr(pf, _ip), _ptr(fi, _ip)), "hamr_flags")
+        return pf, fi
+
+    def all_flags(self):
+        return [self.flags(n) for n in range(self.nblk)]
 
 
 def advance_step(ctx, dt):

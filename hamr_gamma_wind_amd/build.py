@@ -25,6 +25,7 @@ SOURCES = [
     ("hamr_adjoint.hip", True),
     ("hamr_capi.hip", True),
     ("hamr_setup.cpp", False),
+    ("hamr_amr.cpp", False),
 ]
 
 

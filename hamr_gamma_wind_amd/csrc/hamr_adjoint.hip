@@ -373,17 +373,17 @@ void launch_adjoint(const KParams &P, const BlockDev &B, double Dt, const double
 {
     AdjArgs A;
     for (int k = 0; k < 3; k++) {
-        A.D[k] = P.D[k];
-        A.bs[k] = P.bs[k];
-        A.dx[k] = P.dx[k];
+        A.D[k] = B.D[k];
+        A.bs[k] = B.bs[k];
+        A.dx[k] = B.dx[k];
     }
-    A.K = P.ksize;
+    A.K = B.ks;
     A.Dt = Dt;
     A.gam = P.gam;
-    A.kcol = P.kcol;
-    A.gcov = P.gcov + (int64_t)B.geo * NPOS * 10 * P.kcol;
-    A.gcon = P.gcon + (int64_t)B.geo * NPOS * 10 * P.kcol;
-    A.gdet = P.gdet + (int64_t)B.geo * NPOS * P.kcol;
+    A.kcol = B.kcol;
+    A.gcov = P.gcov + B.moff * NPOS * 10;
+    A.gcon = P.gcon + B.moff * NPOS * 10;
+    A.gdet = P.gdet + B.moff * NPOS;
     const int64_t N = (int64_t)A.D[0] * A.D[1] * A.D[2];
     const size_t bytes3 = sizeof(double) * 3 * (size_t)A.K;
     (void)hipMemsetAsync(lps, 0, bytes3, s);

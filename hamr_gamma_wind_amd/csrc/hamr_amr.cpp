@@ -1,0 +1,497 @@
+// hamr_amr.cpp -- multi-level block sets of uneven block sizes: the block
+// table, slab and metric layout, pole rings and the ghost-exchange tables.
+//
+// Levels have ratio 2 in every direction.  Level l covers the box box[l]
+// (level 0: the whole grid); the leaves of level l are box[l] minus
+// box[l+1].  Each level is tiled by 64x32x32 superblocks; a superblock (or the
+// part of it that is a leaf of its level) is split into blocks of 64x32x32,
+// 32x16x16 or 16^3 cells by a seeded choice.  The blocks of one polar ring (same
+// level, radial range and size, all phi) get the same choice, so that the
+// ring's polar E1 average and the pi-shifted partner exist as in the uniform
+// grid.
+//
+// Ghost exchange (this build's specification of the multi-level ghost fill;
+// the reference exchanges through boundcomP.c / GPU_program2.cu packs):
+//   stage d = 1, 2, 3 fills the ghosts along d over the extent already filled
+//   along the earlier directions and the interior along the later ones (the
+//   order of bound_prim, bounds.c:11-105).  A ghost cell at global position Y
+//   of its block's level l:
+//   - along d beyond the radial boundary or the polar axis: the outflow and
+//     polar-mirror kernels (not in the tables);
+//   - in a polar ghost row (x2 beyond the axis, stage 3): a copy of the
+//     same-level leaf that holds the row's clamped position, at Y itself (that
+//     leaf's mirror ghost);
+//   - beyond the radial boundary along an earlier direction: left to the final
+//     outflow pass, which applies the outflow rule to the block's own edge cell;
+//   - otherwise (x3 periodic wrap): the leaf L containing Y gives a copy
+//     (level l), the mean of the 2x2x2 children (level l+1), or the
+//     MC-limited linear prolongation of its cell (level l-1), whose neighbour
+//     values are the cells of that coarse level, the means of finer children,
+//     or the cell of a coarser leaf; a neighbour beyond the radial boundary or
+//     the axis gives no slope along that direction.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "hamr_amr.h"
+#include "hamr_phys.h"
+
+namespace hamr {
+
+static int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+static uint64_t spread3(uint64_t v)
+{
+    uint64_t r = 0;
+    for (int b = 0; b < 21; b++) r |= ((v >> b) & 1ULL) << (3 * b);
+    return r;
+}
+
+namespace {
+
+constexpr int UNIT = 16;                   // smallest block edge
+constexpr int SB[3] = {64, 32, 32};        // superblock
+
+struct Level {
+    int N[3];                              // cells of the level
+    int U[3];                              // units (16^3) of the level
+    std::vector<int> own;                  // unit -> leaf block id, or kCoarse / kRefined
+    int uid(int a, int b, int c) const { return (a * U[1] + b) * U[2] + c; }
+};
+constexpr int kCoarse = -1, kRefined = -2;
+
+struct Builder {
+    const GridDesc &d;
+    const AmrDesc &a;
+    std::vector<Level> lev;
+    std::vector<BlockDev> blk;
+    std::string err;
+
+    Builder(const GridDesc &d_, const AmrDesc &a_) : d(d_), a(a_) {}
+
+    bool in_box(int l, int u1, int u2, int u3) const
+    {
+        // unit (u1, u2, u3) of level l inside box[l] (l = 0: the whole grid)
+        if (l == 0) return true;
+        const int u[3] = {u1, u2, u3};
+        for (int k = 0; k < 3; k++)
+            if (u[k] * UNIT < a.box[l][k][0] || u[k] * UNIT >= a.box[l][k][1]) return false;
+        return true;
+    }
+    bool leaf_unit(int l, int u1, int u2, int u3) const
+    {
+        return in_box(l, u1, u2, u3) && !(l + 1 < a.nlev && in_box(l + 1, 2 * u1, 2 * u2, 2 * u3));
+    }
+    uint64_t choice(int l, int s1, int s2, int s3, int extra) const
+    {
+        return hash64(a.seed ^ hash64(((((uint64_t)l * 1024 + s1) * 1024 + s2) * 1024 + s3) * 64 + extra));
+    }
+    int pick(uint64_t h) const
+    {
+        const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+        return u < a.p_big ? 0 : (u < a.p_big + a.p_mid ? 1 : 2);
+    }
+    void add(int l, int o1, int o2, int o3, int b1, int b2, int b3)
+    {
+        BlockDev B;
+        std::memset(&B, 0, sizeof B);
+        B.level = l;
+        B.o[0] = o1; B.o[1] = o2; B.o[2] = o3;
+        B.bs[0] = b1; B.bs[1] = b2; B.bs[2] = b3;
+        B.c1 = o1 / UNIT; B.c2 = o2 / UNIT; B.c3 = o3 / UNIT;
+        blk.push_back(B);
+    }
+
+    bool build_levels()
+    {
+        const int N0[3] = {d.nb[0] * d.bs[0], d.nb[1] * d.bs[1], d.nb[2] * d.bs[2]};
+        if (a.nlev < 1 || a.nlev > kAmrMaxLevels) { err = "amr: 1 to 3 levels"; return false; }
+        for (int k = 0; k < 3; k++)
+            if (N0[k] % SB[k]) { err = "amr: base grid must be a whole number of 64x32x32 superblocks"; return false; }
+        lev.resize(a.nlev);
+        for (int l = 0; l < a.nlev; l++) {
+            Level &L = lev[l];
+            for (int k = 0; k < 3; k++) {
+                L.N[k] = N0[k] << l;
+                L.U[k] = L.N[k] / UNIT;
+            }
+            L.own.assign((size_t)L.U[0] * L.U[1] * L.U[2], kCoarse);
+            if (l >= 1)
+                for (int k = 0; k < 3; k++) {
+                    const int lo = a.box[l][k][0], hi = a.box[l][k][1];
+                    // a refined box is a whole number of units of its own level and of the level above
+                    if (lo < 0 || hi > L.N[k] || lo >= hi || lo % (2 * UNIT) || hi % (2 * UNIT)) {
+                        err = "amr: refined boxes must be aligned to 32 cells of their level and inside the grid";
+                        return false;
+                    }
+                }
+            if (l >= 2)
+                for (int k = 0; k < 3; k++)
+                    if (a.box[l][k][0] < 2 * a.box[l - 1][k][0] || a.box[l][k][1] > 2 * a.box[l - 1][k][1]) {
+                        err = "amr: refined boxes must nest";
+                        return false;
+                    }
+        }
+        return true;
+    }
+
+    void tile()
+    {
+        for (int l = 0; l < a.nlev; l++) {
+            const Level &L = lev[l];
+            const int ns[3] = {L.N[0] / SB[0], L.N[1] / SB[1], L.N[2] / SB[2]};
+            for (int s1 = 0; s1 < ns[0]; s1++)
+                for (int s2 = 0; s2 < ns[1]; s2++)
+                    for (int s3 = 0; s3 < ns[2]; s3++) {
+                        const int u0[3] = {s1 * 4, s2 * 2, s3 * 2};
+                        int cnt = 0;
+                        for (int x = 0; x < 4; x++)
+                            for (int y = 0; y < 2; y++)
+                                for (int z = 0; z < 2; z++) cnt += leaf_unit(l, u0[0] + x, u0[1] + y, u0[2] + z);
+                        if (!cnt) continue;
+                        // superblocks on the polar axis choose per ring (no phi index)
+                        const bool polar = (s2 == 0 || s2 == ns[1] - 1) && !d.minkowski;
+                        const int c3 = polar ? 0 : s3;
+                        if (cnt == 16) {
+                            const int k = pick(choice(l, s1, s2, c3, 0));
+                            if (k == 0 && !(polar && ns[2] == 1)) {
+                                add(l, s1 * 64, s2 * 32, s3 * 32, 64, 32, 32);
+                                continue;
+                            }
+                            if (k <= 1) {
+                                for (int x = 0; x < 2; x++)
+                                    for (int y = 0; y < 2; y++)
+                                        for (int z = 0; z < 2; z++)
+                                            add(l, s1 * 64 + 32 * x, s2 * 32 + 16 * y, s3 * 32 + 16 * z, 32, 16, 16);
+                                continue;
+                            }
+                        }
+                        // partial superblock (or split to 16^3): pairs of units along x1 where both are leaves
+                        for (int x = 0; x < 4; x += 2)
+                            for (int y = 0; y < 2; y++)
+                                for (int z = 0; z < 2; z++) {
+                                    const int a1 = u0[0] + x, a2 = u0[1] + y, a3 = u0[2] + z;
+                                    const bool e0 = leaf_unit(l, a1, a2, a3), e1 = leaf_unit(l, a1 + 1, a2, a3);
+                                    const bool pair = e0 && e1 && cnt < 16 &&
+                                                      pick(choice(l, s1, s2, c3, 1 + x * 4 + y * 2 + (polar ? 0 : z))) <= 1;
+                                    if (pair) add(l, a1 * UNIT, a2 * UNIT, a3 * UNIT, 32, 16, 16);
+                                    else {
+                                        if (e0) add(l, a1 * UNIT, a2 * UNIT, a3 * UNIT, 16, 16, 16);
+                                        if (e1) add(l, (a1 + 1) * UNIT, a2 * UNIT, a3 * UNIT, 16, 16, 16);
+                                    }
+                                }
+                    }
+        }
+    }
+
+    void order_and_own()
+    {
+        // launch order: level, then Morton order of the origin's units
+        std::stable_sort(blk.begin(), blk.end(), [](const BlockDev &x, const BlockDev &y) {
+            if (x.level != y.level) return x.level < y.level;
+            const uint64_t kx = (spread3(x.o[0] / UNIT) << 2) | (spread3(x.o[1] / UNIT) << 1) | spread3(x.o[2] / UNIT);
+            const uint64_t ky = (spread3(y.o[0] / UNIT) << 2) | (spread3(y.o[1] / UNIT) << 1) | spread3(y.o[2] / UNIT);
+            return kx < ky;
+        });
+        for (int l = 0; l < a.nlev; l++) {
+            Level &L = lev[l];
+            for (int u1 = 0; u1 < L.U[0]; u1++)
+                for (int u2 = 0; u2 < L.U[1]; u2++)
+                    for (int u3 = 0; u3 < L.U[2]; u3++)
+                        if (!leaf_unit(l, u1, u2, u3) && in_box(l, u1, u2, u3)) L.own[L.uid(u1, u2, u3)] = kRefined;
+        }
+        for (int n = 0; n < (int)blk.size(); n++) {
+            const BlockDev &B = blk[n];
+            Level &L = lev[B.level];
+            for (int u1 = B.o[0] / UNIT; u1 < (B.o[0] + B.bs[0]) / UNIT; u1++)
+                for (int u2 = B.o[1] / UNIT; u2 < (B.o[1] + B.bs[1]) / UNIT; u2++)
+                    for (int u3 = B.o[2] / UNIT; u3 < (B.o[2] + B.bs[2]) / UNIT; u3++) L.own[L.uid(u1, u2, u3)] = n;
+        }
+    }
+
+    // leaf holding cell Y of level l (Y inside the grid); returns block, sets its level
+    int find(int l, const int Y[3], int *lc)
+    {
+        const Level &L = lev[l];
+        const int o = L.own[L.uid(Y[0] / UNIT, Y[1] / UNIT, Y[2] / UNIT)];
+        if (o >= 0) { *lc = l; return o; }
+        if (o == kRefined) {
+            if (l + 1 >= a.nlev) return -1;
+            const Level &F = lev[l + 1];
+            const int f = F.own[F.uid(2 * Y[0] / UNIT, 2 * Y[1] / UNIT, 2 * Y[2] / UNIT)];
+            *lc = l + 1;
+            return f;  // -1/-2: more than one level apart (checked by the caller)
+        }
+        if (l == 0) return -1;
+        const Level &C = lev[l - 1];
+        const int c = C.own[C.uid((Y[0] >> 1) / UNIT, (Y[1] >> 1) / UNIT, (Y[2] >> 1) / UNIT)];
+        *lc = l - 1;
+        if (c < 0 && l >= 2) {
+            const Level &C2 = lev[l - 2];
+            *lc = l - 2;
+            return C2.own[C2.uid((Y[0] >> 2) / UNIT, (Y[1] >> 2) / UNIT, (Y[2] >> 2) / UNIT)];
+        }
+        return c;
+    }
+};
+
+int cell_in(const BlockDev &B, const int Y[3])
+{
+    const int i = Y[0] - B.o[0] + NG, j = Y[1] - B.o[1] + NG, z = Y[2] - B.o[2] + NG;
+    return (i * B.D[1] + j) * B.D[2] + z;
+}
+
+bool interior_of(const BlockDev &B, const int Y[3])
+{
+    for (int k = 0; k < 3; k++)
+        if (Y[k] < B.o[k] || Y[k] >= B.o[k] + B.bs[k]) return false;
+    return true;
+}
+
+bool storage_of(const BlockDev &B, const int Y[3])
+{
+    for (int k = 0; k < 3; k++)
+        if (Y[k] < B.o[k] - NG || Y[k] >= B.o[k] + B.bs[k] + NG) return false;
+    return true;
+}
+
+}  // namespace
+
+std::string build_amr(const GridDesc &d, const AmrDesc &a, AmrGrid &g)
+{
+    if (d.minkowski) return "amr: the multi-level grid is the torus grid";
+    Builder b(d, a);
+    if (!b.build_levels()) return b.err;
+    b.tile();
+    if (b.blk.empty()) return "amr: no blocks";
+    b.order_and_own();
+    const GridCoords gc = grid_coords(d);
+    const int nblk = (int)b.blk.size();
+    // layout: slabs in launch order, one geometry slab per distinct (level, o1, o2, bs1, bs2)
+    std::map<std::tuple<int, int, int, int, int>, int> geo;
+    g.geos.clear();
+    int64_t off = 0, moff = 0;
+    for (int n = 0; n < nblk; n++) {
+        BlockDev &B = b.blk[n];
+        for (int k = 0; k < 3; k++) {
+            B.D[k] = B.bs[k] + 2 * NG;
+            B.dx[k] = gc.dx[k] * std::ldexp(1.0, -B.level);
+        }
+        const int64_t ncell = (int64_t)B.D[0] * B.D[1] * B.D[2];
+        B.ks = (int)round_up(ncell, 64);
+        B.kcol = (int)round_up((int64_t)B.D[0] * B.D[1], 64);
+        B.off = off;
+        off += B.ks;
+        auto key = std::make_tuple(B.level, B.o[0], B.o[1], B.bs[0], B.bs[1]);
+        auto it = geo.find(key);
+        if (it == geo.end()) {
+            AmrGeo G{B.level, B.o[0], B.o[1], B.D[0], B.D[1], B.kcol, moff};
+            it = geo.emplace(key, (int)g.geos.size()).first;
+            g.geos.push_back(G);
+            moff += B.kcol;
+        }
+        B.geo = it->second;
+        B.moff = g.geos[B.geo].moff;
+    }
+    g.cells = off;
+    g.cols = moff;
+    // faces: physical boundaries and pole partners; every other face is exchanged
+    std::map<std::tuple<int, int, int, int, int, int, int>, int> where;  // (level, o, bs) -> block
+    for (int n = 0; n < nblk; n++) {
+        const BlockDev &B = b.blk[n];
+        where[std::make_tuple(B.level, B.o[0], B.o[1], B.o[2], B.bs[0], B.bs[1], B.bs[2])] = n;
+    }
+    for (int n = 0; n < nblk; n++) {
+        BlockDev &B = b.blk[n];
+        const Level &L = b.lev[B.level];
+        for (int f = 0; f < 6; f++) {
+            B.nbr[f] = -1;
+            B.kind[f] = FK_AMR;
+        }
+        if (!d.minkowski) {
+            if (B.o[0] == 0) B.kind[0] = FK_OUTFLOW;
+            if (B.o[0] + B.bs[0] == L.N[0]) B.kind[1] = FK_OUTFLOW;
+            for (int side = 0; side < 2; side++) {
+                const bool axis = side ? (B.o[1] + B.bs[1] == L.N[1]) : (B.o[1] == 0);
+                if (!axis) continue;
+                const int o3 = (B.o[2] + L.N[2] / 2) % L.N[2];
+                auto it = where.find(std::make_tuple(B.level, B.o[0], B.o[1], o3, B.bs[0], B.bs[1], B.bs[2]));
+                if (it == where.end()) return "amr: a polar block has no pi-shifted partner of its shape";
+                B.kind[2 + side] = FK_POLE;
+                B.nbr[2 + side] = it->second;
+                B.pole |= 1 << side;
+            }
+        }
+    }
+    // pole rings: same side, level, radial range and size, in phi order
+    g.ring.clear();
+    g.rside.clear();
+    g.ring_len = 0;
+    g.ni_max = 0;
+    {
+        std::map<std::tuple<int, int, int, int, int, int>, std::vector<std::pair<int, int>>> rings;
+        for (int n = 0; n < nblk; n++) {
+            const BlockDev &B = b.blk[n];
+            for (int side = 0; side < 2; side++)
+                if (B.pole & (1 << side))
+                    rings[std::make_tuple(side, B.level, B.o[0], B.bs[0], B.bs[1], B.bs[2])].push_back({B.o[2], n});
+        }
+        for (auto &r : rings) {
+            auto v = r.second;
+            std::sort(v.begin(), v.end());
+            const int side = std::get<0>(r.first);
+            if (g.ring_len == 0) g.ring_len = (int)v.size();
+            if ((int)v.size() != g.ring_len) return "amr: polar rings of different lengths";
+            int tot = 0;
+            for (auto &e : v) tot += b.blk[e.second].bs[2];
+            if (tot != b.lev[b.blk[v[0].second].level].N[2]) return "amr: a polar ring does not close";
+            for (auto &e : v) g.ring.push_back(e.second);
+            g.rside.push_back(side);
+            g.ni_max = std::max(g.ni_max, b.blk[v[0].second].bs[0] + 1);
+        }
+    }
+    // exchange tables
+    for (int s = 0; s < 3; s++) {
+        g.ex[s].clear();
+        g.pro[s].clear();
+    }
+    for (int n = 0; n < nblk; n++) {
+        const BlockDev &B = b.blk[n];
+        const int l = B.level;
+        const Level &L = b.lev[l];
+        for (int d0 = 0; d0 < 3; d0++) {
+            int lo[3], hi[3];
+            for (int k = 0; k < 3; k++) {
+                if (k < d0) { lo[k] = 0; hi[k] = B.D[k]; }
+                else { lo[k] = NG; hi[k] = NG + B.bs[k]; }
+            }
+            for (int side = 0; side < 2; side++) {
+                if (B.kind[2 * d0 + side] != FK_AMR) continue;
+                lo[d0] = side ? NG + B.bs[d0] : 0;
+                hi[d0] = side ? B.D[d0] : NG;
+                for (int i = lo[0]; i < hi[0]; i++)
+                    for (int j = lo[1]; j < hi[1]; j++)
+                        for (int z = lo[2]; z < hi[2]; z++) {
+                            const int dcell = (i * B.D[1] + j) * B.D[2] + z;
+                            int Y[3] = {B.o[0] + i - NG, B.o[1] + j - NG, B.o[2] + z - NG};
+                            Y[2] = ((Y[2] % L.N[2]) + L.N[2]) % L.N[2];
+                            const bool out1 = Y[0] < 0 || Y[0] >= L.N[0];
+                            const bool out2 = Y[1] < 0 || Y[1] >= L.N[1];
+                            if (d.minkowski && (out1 || out2)) return "amr: flat boxes are periodic in x3 only";
+                            if (out2) {
+                                // polar ghost row: the same-level leaf at the clamped position holds it
+                                int Yc[3] = {std::min(std::max(Y[0], 0), L.N[0] - 1), Y[1] < 0 ? 0 : L.N[1] - 1, Y[2]};
+                                int lc, c = b.find(l, Yc, &lc);
+                                if (c < 0 || lc != l || !storage_of(b.blk[c], Y))
+                                    return "amr: a polar ghost row borders another level";
+                                g.ex[d0].push_back({n, dcell, c, (AS_CELL << 28) | cell_in(b.blk[c], Y)});
+                                continue;
+                            }
+                            if (out1) continue;  // final outflow pass
+                            int lc, c = b.find(l, Y, &lc);
+                            if (c < 0) return "amr: neighbouring blocks more than one level apart";
+                            const BlockDev &C = b.blk[c];
+                            if (lc == l) {
+                                g.ex[d0].push_back({n, dcell, c, (AS_CELL << 28) | cell_in(C, Y)});
+                            } else if (lc == l + 1) {
+                                const int Yf[3] = {2 * Y[0], 2 * Y[1], 2 * Y[2]};
+                                const int Yf1[3] = {2 * Y[0] + 1, 2 * Y[1] + 1, 2 * Y[2] + 1};
+                                if (!interior_of(C, Yf) || !interior_of(C, Yf1)) return "amr: children split across blocks";
+                                g.ex[d0].push_back({n, dcell, c, (AS_AVG8 << 28) | cell_in(C, Yf)});
+                            } else if (lc == l - 1) {
+                                const Level &Lc = b.lev[l - 1];
+                                const int Pc[3] = {Y[0] >> 1, Y[1] >> 1, Y[2] >> 1};
+                                if (!interior_of(C, Pc)) return "amr: coarse cell outside its leaf";
+                                AmrProlong E;
+                                E.dblk = n;
+                                E.dcell = dcell;
+                                E.cblk = c;
+                                E.ccell = cell_in(C, Pc);
+                                for (int dd = 0; dd < 3; dd++)
+                                    for (int sg = 0; sg < 2; sg++) {
+                                        int Q[3] = {Pc[0], Pc[1], Pc[2]};
+                                        Q[dd] += sg ? 1 : -1;
+                                        int *slot = E.nb[2 * dd + sg];
+                                        if (dd == 2) Q[2] = ((Q[2] % Lc.N[2]) + Lc.N[2]) % Lc.N[2];
+                                        if (Q[dd] < 0 || Q[dd] >= Lc.N[dd]) { slot[0] = 0; slot[1] = AS_NONE << 28; continue; }
+                                        if (interior_of(C, Q)) { slot[0] = c; slot[1] = (AS_CELL << 28) | cell_in(C, Q); continue; }
+                                        int lq, q = b.find(l - 1, Q, &lq);
+                                        if (q < 0) return "amr: slope neighbour not found";
+                                        const BlockDev &S = b.blk[q];
+                                        if (lq == l - 1) { slot[0] = q; slot[1] = (AS_CELL << 28) | cell_in(S, Q); }
+                                        else if (lq == l) {
+                                            const int Qf[3] = {2 * Q[0], 2 * Q[1], 2 * Q[2]};
+                                            const int Qf1[3] = {2 * Q[0] + 1, 2 * Q[1] + 1, 2 * Q[2] + 1};
+                                            if (!interior_of(S, Qf) || !interior_of(S, Qf1)) return "amr: children split across blocks";
+                                            slot[0] = q;
+                                            slot[1] = (AS_AVG8 << 28) | cell_in(S, Qf);
+                                        } else {
+                                            const int sh = (l - 1) - lq;
+                                            const int Qc[3] = {Q[0] >> sh, Q[1] >> sh, Q[2] >> sh};
+                                            if (!interior_of(S, Qc)) return "amr: coarse neighbour outside its leaf";
+                                            slot[0] = q;
+                                            slot[1] = (AS_CELL << 28) | cell_in(S, Qc);
+                                        }
+                                    }
+                                g.pro[d0].push_back(E);
+                            } else {
+                                return "amr: neighbouring blocks more than one level apart";
+                            }
+                        }
+            }
+        }
+    }
+    g.blocks = b.blk;
+    g.nlev = a.nlev;
+    for (int l = 0; l < a.nlev; l++)
+        for (int k = 0; k < 3; k++) g.N[l][k] = b.lev[l].N[k];
+    return validate_amr(g);
+}
+
+// Every index a table entry names lies inside its block's slab: checked on the
+// host before the tables reach the device.
+static bool src_ok(const std::vector<BlockDev> &blk, int b, int code)
+{
+    if (b < 0 || b >= (int)blk.size()) return false;
+    const BlockDev &S = blk[b];
+    const int kind = code >> 28, c = code & 0x0FFFFFFF;
+    const int64_t ncell = (int64_t)S.D[0] * S.D[1] * S.D[2];
+    if (kind == AS_NONE) return true;
+    if (kind == AS_CELL) return c >= 0 && c < ncell;
+    if (kind != AS_AVG8) return false;
+    const int64_t last = (int64_t)c + (int64_t)S.D[1] * S.D[2] + S.D[2] + 1;
+    return c >= 0 && last < ncell;
+}
+
+std::string validate_amr(const AmrGrid &g)
+{
+    const auto &blk = g.blocks;
+    auto dst_ok = [&](int b, int c) {
+        if (b < 0 || b >= (int)blk.size()) return false;
+        const BlockDev &B = blk[b];
+        return c >= 0 && (int64_t)c < (int64_t)B.D[0] * B.D[1] * B.D[2];
+    };
+    for (int s = 0; s < 3; s++) {
+        for (const AmrEntry &e : g.ex[s])
+            if (!dst_ok(e.dblk, e.dcell) || !src_ok(blk, e.sblk, e.scode) || (e.scode >> 28) == AS_NONE)
+                return "amr: exchange entry out of range";
+        for (const AmrProlong &e : g.pro[s]) {
+            if (!dst_ok(e.dblk, e.dcell) || !src_ok(blk, e.cblk, (AS_CELL << 28) | e.ccell))
+                return "amr: prolongation entry out of range";
+            for (int k = 0; k < 6; k++)
+                if (!src_ok(blk, e.nb[k][0], e.nb[k][1])) return "amr: prolongation neighbour out of range";
+        }
+    }
+    for (size_t n = 0; n < blk.size(); n++) {
+        const BlockDev &B = blk[n];
+        for (int f = 0; f < 6; f++)
+            if (B.kind[f] == FK_POLE && (B.nbr[f] < 0 || B.nbr[f] >= (int)blk.size())) return "amr: bad pole partner";
+    }
+    for (int r : g.ring)
+        if (r < 0 || r >= (int)blk.size()) return "amr: bad ring entry";
+    return "";
+}
+
+}  // namespace hamr

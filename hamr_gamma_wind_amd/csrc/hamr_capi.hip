@@ -1,23 +1,59 @@
 // hamr_capi.hip -- the C ABI of include/hamr_hip.h: context, resident block
-// set, metric and state staging, per-block stages and the batched step.
+// set, metric and state staging, per-block stages, the batched step, the
+// multi-level block set and the launch autotuner.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/hamr_hip.h"
+#include "hamr_amr.h"
 #include "hamr_grid.h"
 #include "hamr_kernels.h"
 
 using namespace hamr;
 
+namespace {
+
+constexpr int kTiles[3] = {64, 128, 256};   // tile sizes (threads per workgroup) the work lists exist for
+constexpr int kDefaultTile = 2;
+
+// Kernels the autotuner configures (each walks the work list of one class).
+enum TuneSlot : int {
+    TS_FLUX1 = 0, TS_FLUX2, TS_FLUX3, TS_ECENT, TS_ECORNER, TS_CT, TS_CONS, TS_INVERT, TS_FALLBACK,
+    TS_BOUND1, TS_BOUND2, TS_BOUND3, TS_EXCHANGE, TS_N
+};
+constexpr int kSlotClass[TS_N] = {KC_FLUX1, KC_FLUX2, KC_FLUX3, KC_ECENT, KC_CORNER, KC_CORNER, KC_INTERIOR,
+                                   KC_INTERIOR, KC_INTERIOR, KC_BOUND1, KC_BOUND2, KC_BOUND3, -1};
+
+struct TuneCfg {
+    int tile = kDefaultTile;  // index into kTiles
+    int grid = 0;             // workgroups; 0: one per item
+    int swz = 0;              // XCD-contiguous item order
+    float ms = -1.f;          // measured time of the chosen configuration
+};
+
+// Work lists of one kernel class over its block list, for each tile size.
+struct ClassList {
+    int2 *items[3] = {nullptr, nullptr, nullptr};
+    int n[3] = {0, 0, 0};
+    int lds_cols[3] = {0, 0, 0};
+    std::vector<int> start[3], count[3];  // per block id (count 0: not in the list)
+};
+
+}  // namespace
+
 struct hamr_ctx {
     GridDesc d;
-    Layout L;
+    Layout L;                      // uniform grid: the common block layout
+    bool amr = false;
     std::vector<BlockDev> blocks;
-    std::vector<int> order;
+    std::vector<AmrGeo> geos;      // geometry slabs (level, first column, extents, offset)
+    std::vector<int> order;        // launch order
+    int64_t cells = 0, cols = 0;   // sum of slab lengths / metric slab widths
     std::string err;
     hipStream_t s = nullptr;
     int device = 0;
@@ -31,10 +67,14 @@ struct hamr_ctx {
     double *dt = nullptr;          // [0] dt of the next step, [1] previous
     double *gcov = nullptr, *gcon = nullptr, *gdet = nullptr, *conn = nullptr, *rcen = nullptr;
     BlockDev *blk = nullptr;
-    int *ident = nullptr;          // 0..nblk-1 (single-block lists are offsets into it)
-    int *zlist = nullptr;          // all blocks in Z-order
-    int *ring = nullptr;           // [side][ring][c3] block ids of the pole rings
-    int nring = 0;
+    int *ring = nullptr, *rside = nullptr;  // polar rings
+    int nring = 0, ring_len = 0, ni_max = 0;
+    ClassList cl[KC_N];
+    TuneCfg tune[TS_N];
+    // multi-level exchange tables per stage
+    AmrEntry *ex[3] = {nullptr, nullptr, nullptr};
+    AmrProlong *pro[3] = {nullptr, nullptr, nullptr};
+    int64_t nex[3] = {0, 0, 0}, npro[3] = {0, 0, 0};
     hipGraphExec_t graph = nullptr;
     int prec = HAMR_PREC_FP64;     // arithmetic of reconstruction, fluxes and EMFs
     // adjoint scratch (allocated by the first adjoint call)
@@ -74,7 +114,9 @@ template <class T> static int dalloc(hamr_ctx *c, T **ptr, size_t n)
     return 0;
 }
 
-static bool valid_block(const hamr_ctx *c, int n) { return n >= 0 && n < c->L.nblk; }
+static int nblk_of(const hamr_ctx *c) { return (int)c->blocks.size(); }
+static bool valid_block(const hamr_ctx *c, int n) { return n >= 0 && n < nblk_of(c); }
+static int64_t ncell_of(const BlockDev &B) { return (int64_t)B.D[0] * B.D[1] * B.D[2]; }
 
 // Host<->device copies ordered on the context stream and completed before return.
 static int h2d(hamr_ctx *c, void *dst, const void *src, size_t bytes)
@@ -90,19 +132,175 @@ static int d2h(hamr_ctx *c, void *dst, const void *src, size_t bytes)
     return 0;
 }
 
+// ---- work lists and launches ----
+static int build_class_lists(hamr_ctx *c)
+{
+    const int nb = nblk_of(c);
+    for (int cls = 0; cls < KC_N; cls++) {
+        // block list of the class: every block, except the ghost fill of a
+        // multi-level set, whose kernels serve only the outflow and polar faces
+        std::vector<int> bl;
+        for (int n : c->order) {
+            const BlockDev &B = c->blocks[n];
+            bool use = true;
+            if (c->amr) {
+                if (cls == KC_BOUND1) use = B.kind[0] == FK_OUTFLOW || B.kind[1] == FK_OUTFLOW;
+                else if (cls == KC_BOUND2) use = B.kind[2] == FK_POLE || B.kind[3] == FK_POLE;
+                else if (cls == KC_BOUND3) use = false;
+            }
+            if (use) bl.push_back(n);
+        }
+        ClassList &L = c->cl[cls];
+        for (int ti = 0; ti < 3; ti++) {
+            const int T = kTiles[ti];
+            std::vector<int2> items;
+            L.start[ti].assign(nb, 0);
+            L.count[ti].assign(nb, 0);
+            int cols = 1;
+            for (int n : bl) {
+                const BlockDev &B = c->blocks[n];
+                const int64_t w = class_work(B, cls);
+                const int nt = (int)((w + T - 1) / T);
+                L.start[ti][n] = (int)items.size();
+                L.count[ti][n] = nt;
+                for (int t = 0; t < nt; t++) items.push_back(make_int2(n, t));
+                cols = std::max(cols, cons_max_cols(B.bs[2], T));
+            }
+            L.n[ti] = (int)items.size();
+            L.lds_cols[ti] = cols;
+            if (!items.empty()) {
+                if (dalloc(c, &L.items[ti], items.size())) return -1;
+                if (h2d(c, L.items[ti], items.data(), items.size() * sizeof(int2))) return -1;
+            }
+        }
+    }
+    return 0;
+}
+
+// Launch of a tune slot over its whole block list with the tuned configuration.
+static Launch all_launch(const hamr_ctx *c, int slot)
+{
+    const TuneCfg &t = c->tune[slot];
+    const ClassList &L = c->cl[kSlotClass[slot]];
+    Launch x;
+    x.threads = kTiles[t.tile];
+    x.wl.items = L.items[t.tile];
+    x.wl.n = L.n[t.tile];
+    x.wl.swz = t.swz;
+    x.grid = (t.grid > 0 && t.grid < x.wl.n) ? t.grid : x.wl.n;
+    x.lds_cols = L.lds_cols[t.tile];
+    return x;
+}
+
+// Launch of a class over one block (the per-block entry points).
+static Launch block_launch(const hamr_ctx *c, int cls, int n)
+{
+    const ClassList &L = c->cl[cls];
+    Launch x;
+    x.threads = kTiles[kDefaultTile];
+    x.wl.items = L.items[kDefaultTile] ? L.items[kDefaultTile] + L.start[kDefaultTile][n] : nullptr;
+    x.wl.n = L.count[kDefaultTile][n];
+    x.wl.swz = 0;
+    x.grid = x.wl.n;
+    x.lds_cols = L.lds_cols[kDefaultTile];
+    return x;
+}
+
+// Device geometry slabs and state arrays for the block table already in c.
+static int init_device(hamr_ctx *c)
+{
+    const size_t nb = (size_t)nblk_of(c), K = (size_t)c->cells;
+    if (dalloc(c, &c->p, NPR * K) || dalloc(c, &c->ph, NPR * K) || dalloc(c, &c->ps, 3 * K) ||
+        dalloc(c, &c->psh, 3 * K) || dalloc(c, &c->F[0], NPR * K) || dalloc(c, &c->F[1], NPR * K) ||
+        dalloc(c, &c->F[2], NPR * K) || dalloc(c, &c->E, 3 * K) || dalloc(c, &c->Ec, 3 * K) ||
+        dalloc(c, &c->U0, NPR * K) || dalloc(c, &c->Un, NPR * K) || dalloc(c, &c->pflag, K) ||
+        dalloc(c, &c->failcnt, nb * 4) || dalloc(c, &c->dtmin, nb * 3) || dalloc(c, &c->dt, 4))
+        return -1;
+    const size_t KC = (size_t)c->cols;
+    if (dalloc(c, &c->gcov, NPOS * 10 * KC) || dalloc(c, &c->gcon, NPOS * 10 * KC) || dalloc(c, &c->gdet, NPOS * KC) ||
+        dalloc(c, &c->conn, 40 * KC) || dalloc(c, &c->rcen, KC) || dalloc(c, &c->blk, nb))
+        return -1;
+    if (h2d(c, c->blk, c->blocks.data(), nb * sizeof(BlockDev))) return -1;
+    if (build_class_lists(c)) return -1;
+    KParams &P = c->P;
+    P.gam = c->d.gam;
+    P.cour = c->d.cour;
+    P.rho_floor = c->d.rho_floor;
+    P.u_floor = c->d.u_floor;
+    P.blk = c->blk;
+    P.gcov = c->gcov;
+    P.gcon = c->gcon;
+    P.gdet = c->gdet;
+    P.conn = c->conn;
+    P.rcen = c->rcen;
+    launch_dt_reset((int)nb, c->dtmin, c->s);
+    if (check_launch(c, "dt reset")) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+static int upload_rings(hamr_ctx *c, const std::vector<int> &ring, const std::vector<int> &rside, int len, int ni_max)
+{
+    c->nring = (int)rside.size();
+    c->ring_len = len;
+    c->ni_max = ni_max;
+    if (!c->nring) return 0;
+    if (dalloc(c, &c->ring, ring.size()) || dalloc(c, &c->rside, rside.size())) return -1;
+    if (h2d(c, c->ring, ring.data(), ring.size() * sizeof(int))) return -1;
+    if (h2d(c, c->rside, rside.data(), rside.size() * sizeof(int))) return -1;
+    return 0;
+}
+
+static void pole(hamr_ctx *c, double dt_val, const double *dt_ptr)
+{
+    launch_pole(c->P, c->ring, c->rside, c->nring, c->ring_len, c->ni_max, dt_val, dt_ptr, c->E, c->prec, c->s);
+}
+
+// Ghost fill of all blocks.  Uniform grid: x1, x2, x3 stages of k_bound.
+// Multi-level set: outflow faces, exchange stage 1, polar faces, exchange
+// stages 2 and 3, and the final outflow pass over the radial ghosts that the
+// later stages left (see hamr_amr.cpp).
+static void bound_all(hamr_ctx *c, double *p, double *ps)
+{
+    const int T = kTiles[c->tune[TS_EXCHANGE].tile], G = c->tune[TS_EXCHANGE].grid;
+    if (!c->amr) {
+        for (int dir = 1; dir <= 3; dir++) launch_bound(c->P, dir, all_launch(c, TS_BOUND1 + dir - 1), p, ps, c->pflag, 0, c->s);
+        return;
+    }
+    launch_bound(c->P, 1, all_launch(c, TS_BOUND1), p, ps, c->pflag, 0, c->s);
+    launch_amr_exchange(c->P, c->ex[0], c->nex[0], c->pro[0], c->npro[0], p, ps, c->pflag, T, G, c->s);
+    launch_bound(c->P, 2, all_launch(c, TS_BOUND2), p, ps, c->pflag, 0, c->s);
+    launch_amr_exchange(c->P, c->ex[1], c->nex[1], c->pro[1], c->npro[1], p, ps, c->pflag, T, G, c->s);
+    launch_amr_exchange(c->P, c->ex[2], c->nex[2], c->pro[2], c->npro[2], p, ps, c->pflag, T, G, c->s);
+    launch_bound(c->P, 1, all_launch(c, TS_BOUND1), p, ps, c->pflag, 1, c->s);
+}
+
+static void emf_all(hamr_ctx *c, const double *p)
+{
+    launch_emf(c->P, all_launch(c, TS_ECENT), all_launch(c, TS_ECORNER), p, c->F[0], c->F[1], c->F[2], c->Ec, c->E,
+               c->prec, c->s);
+}
+
+static void update_all(hamr_ctx *c, int full, double Dt_val, const double *dt_ptr, double dt_mult, const double *pi,
+                       const double *pb, double *pf, const double *psf)
+{
+    const int slots[3] = {TS_CONS, TS_INVERT, TS_FALLBACK};
+    for (int part = 0; part < 3; part++)
+        launch_update_part(c->P, part, all_launch(c, slots[part]), full, Dt_val, dt_ptr, dt_mult, pi, pb, pf, c->U0,
+                           c->Un, psf, c->F[0], c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, c->s);
+}
+
 extern "C" {
 
 const char *hamr_last_error(const hamr_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int hamr_create(const hamr_desc *desc, hamr_ctx **out)
+static int desc_to_grid(const hamr_desc *desc, GridDesc &d)
 {
-    if (!desc || !out) return -1;
-    *out = nullptr;
-    hamr_ctx *c = new hamr_ctx();
-    GridDesc &d = c->d;
+    if (!desc) return -1;
     for (int k = 0; k < 3; k++) {
         d.bs[k] = desc->bs[k];
         d.nb[k] = desc->nb[k];
+        if (d.bs[k] < 2 * NG || d.nb[k] < 1) return -1;
     }
     d.gam = desc->gam;
     d.a = desc->spin;
@@ -112,10 +310,19 @@ int hamr_create(const hamr_desc *desc, hamr_ctx **out)
     d.rho_floor = desc->rho_floor;
     d.u_floor = desc->u_floor;
     d.minkowski = desc->minkowski;
+    return 0;
+}
+
+int hamr_create(const hamr_desc *desc, hamr_ctx **out)
+{
+    if (!desc || !out) return -1;
+    *out = nullptr;
+    hamr_ctx *c = new hamr_ctx();
+    GridDesc &d = c->d;
     c->device = desc->device;
     *out = c;
-    for (int k = 0; k < 3; k++)
-        if (d.bs[k] < 2 * NG || d.nb[k] < 1) return fail(c, "hamr_create: each block needs >= 6 cells and >= 1 block per direction");
+    if (desc_to_grid(desc, d))
+        return fail(c, "hamr_create: each block needs >= 6 cells and >= 1 block per direction");
     if (!d.minkowski) {
         if (d.nb[2] > 1 && (d.nb[2] % 2)) return fail(c, "hamr_create: the polar axis needs an even number of phi-blocks (or one)");
         if (d.nb[2] == 1 && (d.bs[2] % 2)) return fail(c, "hamr_create: the polar axis needs an even number of phi cells");
@@ -128,60 +335,107 @@ int hamr_create(const hamr_desc *desc, hamr_ctx **out)
     c->blocks = build_block_table(d, &ngeo);
     c->order = zorder(d);
     const Layout &L = c->L;
-    const size_t K = (size_t)L.ksize, nb = (size_t)L.nblk;
-    if (dalloc(c, &c->p, nb * NPR * K) || dalloc(c, &c->ph, nb * NPR * K) || dalloc(c, &c->ps, nb * 3 * K) ||
-        dalloc(c, &c->psh, nb * 3 * K) || dalloc(c, &c->F[0], nb * NPR * K) || dalloc(c, &c->F[1], nb * NPR * K) ||
-        dalloc(c, &c->F[2], nb * NPR * K) || dalloc(c, &c->E, nb * 3 * K) || dalloc(c, &c->Ec, nb * 3 * K) ||
-        dalloc(c, &c->U0, nb * NPR * K) || dalloc(c, &c->Un, nb * NPR * K) || dalloc(c, &c->pflag, nb * K) || dalloc(c, &c->failcnt, nb * 4) ||
-        dalloc(c, &c->dtmin, nb * 3) || dalloc(c, &c->dt, 4))
-        return -1;
-    const size_t KC = (size_t)L.kcol, ng = (size_t)L.ngeo;
-    if (dalloc(c, &c->gcov, ng * NPOS * 10 * KC) || dalloc(c, &c->gcon, ng * NPOS * 10 * KC) ||
-        dalloc(c, &c->gdet, ng * NPOS * KC) || dalloc(c, &c->conn, ng * 40 * KC) || dalloc(c, &c->rcen, ng * KC) ||
-        dalloc(c, &c->blk, nb) || dalloc(c, &c->ident, nb) || dalloc(c, &c->zlist, nb))
-        return -1;
-    if (h2d(c, c->blk, c->blocks.data(), nb * sizeof(BlockDev))) return -1;
-    std::vector<int> id(nb);
-    for (size_t n = 0; n < nb; n++) id[n] = (int)n;
-    if (h2d(c, c->ident, id.data(), nb * sizeof(int))) return -1;
-    if (h2d(c, c->zlist, c->order.data(), nb * sizeof(int))) return -1;
+    c->cells = (int64_t)L.nblk * L.ksize;
+    c->cols = (int64_t)L.ngeo * L.kcol;
+    for (int g = 0; g < L.ngeo; g++) {
+        const int c1 = g / d.nb[1], c2 = g % d.nb[1];
+        c->geos.push_back(AmrGeo{0, c1 * d.bs[0], c2 * d.bs[1], L.D[0], L.D[1], (int)L.kcol, (int64_t)g * L.kcol});
+    }
+    if (init_device(c)) return -1;
     // pole rings: for each c1, the blocks of c2 = 0 (side 0) and c2 = NB2-1 (side 1), in c3 order
     if (!d.minkowski) {
-        c->nring = d.nb[0];
-        std::vector<int> ring((size_t)2 * c->nring * d.nb[2]);
+        std::vector<int> ring, rside;
         for (int side = 0; side < 2; side++)
-            for (int c1 = 0; c1 < d.nb[0]; c1++)
-                for (int c3 = 0; c3 < d.nb[2]; c3++) {
-                    int c2 = side ? d.nb[1] - 1 : 0;
-                    ring[((size_t)side * c->nring + c1) * d.nb[2] + c3] = (c1 * d.nb[1] + c2) * d.nb[2] + c3;
-                }
-        if (dalloc(c, &c->ring, ring.size())) return -1;
-        if (h2d(c, c->ring, ring.data(), ring.size() * sizeof(int))) return -1;
+            for (int c1 = 0; c1 < d.nb[0]; c1++) {
+                const int c2 = side ? d.nb[1] - 1 : 0;
+                for (int c3 = 0; c3 < d.nb[2]; c3++) ring.push_back((c1 * d.nb[1] + c2) * d.nb[2] + c3);
+                rside.push_back(side);
+            }
+        if (upload_rings(c, ring, rside, d.nb[2], d.bs[0] + 1)) return -1;
     }
-    const GridCoords gc = grid_coords(d);
-    KParams &P = c->P;
-    for (int k = 0; k < 3; k++) {
-        P.D[k] = L.D[k];
-        P.bs[k] = d.bs[k];
-        P.dx[k] = gc.dx[k];
-    }
-    P.ksize = L.ksize;
-    P.kcol = L.kcol;
-    P.nb3 = d.nb[2];
-    P.gam = d.gam;
-    P.cour = d.cour;
-    P.rho_floor = d.rho_floor;
-    P.u_floor = d.u_floor;
-    P.blk = c->blk;
-    P.gcov = c->gcov;
-    P.gcon = c->gcon;
-    P.gdet = c->gdet;
-    P.conn = c->conn;
-    P.rcen = c->rcen;
-    launch_dt_reset(L.nblk, c->dtmin, c->s);
-    if (check_launch(c, "dt reset")) return -1;
-    HIPCHK(c, hipStreamSynchronize(c->s));
     c->err.clear();
+    return 0;
+}
+
+int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *a, hamr_ctx **out)
+{
+    if (!desc || !a || !out) return -1;
+    *out = nullptr;
+    hamr_ctx *c = new hamr_ctx();
+    c->device = desc->device;
+    *out = c;
+    if (desc_to_grid(desc, c->d)) return fail(c, "hamr_create_amr: bad base description");
+    if (c->d.minkowski || !(c->d.rout > 0.) || !(c->d.gam > 1.))
+        return fail(c, "hamr_create_amr: the multi-level grid is the torus grid (minkowski = 0, rout > 0, gam > 1)");
+    AmrDesc ad;
+    ad.nlev = a->nlev;
+    for (int l = 0; l < kAmrMaxLevels; l++)
+        for (int k = 0; k < 3; k++)
+            for (int e = 0; e < 2; e++) ad.box[l][k][e] = a->box[l][k][e];
+    ad.seed = a->seed;
+    ad.p_big = a->p_big;
+    ad.p_mid = a->p_mid;
+    AmrGrid g;
+    std::string e = build_amr(c->d, ad, g);
+    if (!e.empty()) return fail(c, "hamr_create_amr: " + e);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    c->amr = true;
+    c->L = make_layout(c->d);
+    c->blocks = g.blocks;
+    c->geos = g.geos;
+    c->cells = g.cells;
+    c->cols = g.cols;
+    c->order.resize(c->blocks.size());
+    for (size_t n = 0; n < c->blocks.size(); n++) c->order[n] = (int)n;
+    if (init_device(c)) return -1;
+    if (upload_rings(c, g.ring, g.rside, g.ring_len, g.ni_max)) return -1;
+    for (int s = 0; s < 3; s++) {
+        c->nex[s] = (int64_t)g.ex[s].size();
+        c->npro[s] = (int64_t)g.pro[s].size();
+        if (c->nex[s]) {
+            if (dalloc(c, &c->ex[s], g.ex[s].size())) return -1;
+            if (h2d(c, c->ex[s], g.ex[s].data(), g.ex[s].size() * sizeof(AmrEntry))) return -1;
+        }
+        if (c->npro[s]) {
+            if (dalloc(c, &c->pro[s], g.pro[s].size())) return -1;
+            if (h2d(c, c->pro[s], g.pro[s].data(), g.pro[s].size() * sizeof(AmrProlong))) return -1;
+        }
+    }
+    c->err.clear();
+    return 0;
+}
+
+int hamr_amr_info(const hamr_ctx *c, long long *info)
+{
+    if (!c || !info) return -1;
+    info[0] = nblk_of(c);
+    info[1] = (long long)c->geos.size();
+    info[2] = c->cells;
+    long long zones = 0;
+    for (const BlockDev &B : c->blocks) zones += (long long)B.bs[0] * B.bs[1] * B.bs[2];
+    info[3] = zones;
+    for (int s = 0; s < 3; s++) {
+        info[4 + s] = c->nex[s];
+        info[7 + s] = c->npro[s];
+    }
+    info[10] = c->nring;
+    info[11] = c->amr ? 1 : 0;
+    return 0;
+}
+
+int hamr_block_layout(const hamr_ctx *c, int n, int *row)
+{
+    if (!c || !row || !valid_block(c, n)) return -1;
+    const BlockDev &B = c->blocks[n];
+    row[0] = B.level;
+    for (int k = 0; k < 3; k++) {
+        row[1 + k] = B.o[k];
+        row[4 + k] = B.bs[k];
+    }
+    row[7] = B.geo;
+    row[8] = B.pole;
+    for (int k = 0; k < 6; k++) row[9 + k] = B.kind[k];
     return 0;
 }
 
@@ -200,12 +454,13 @@ int hamr_destroy(hamr_ctx *c)
 int hamr_layout(const hamr_ctx *c, int *D, long long *ncell, long long *ksize, int *nblk, int *ngeo)
 {
     if (!c) return -1;
+    const BlockDev &B = c->blocks[0];
     if (D)
-        for (int k = 0; k < 3; k++) D[k] = c->L.D[k];
-    if (ncell) *ncell = c->L.ncell;
-    if (ksize) *ksize = c->L.ksize;
-    if (nblk) *nblk = c->L.nblk;
-    if (ngeo) *ngeo = c->L.ngeo;
+        for (int k = 0; k < 3; k++) D[k] = B.D[k];
+    if (ncell) *ncell = ncell_of(B);
+    if (ksize) *ksize = B.ks;
+    if (nblk) *nblk = nblk_of(c);
+    if (ngeo) *ngeo = (int)c->geos.size();
     return 0;
 }
 
@@ -229,27 +484,6 @@ int hamr_block_order(const hamr_ctx *c, int *order)
 {
     if (!c || !order) return -1;
     std::memcpy(order, c->order.data(), c->order.size() * sizeof(int));
-    return 0;
-}
-
-
-// ---- host-only set-up (no device needed) ----
-static int desc_to_grid(const hamr_desc *desc, GridDesc &d)
-{
-    if (!desc) return -1;
-    for (int k = 0; k < 3; k++) {
-        d.bs[k] = desc->bs[k];
-        d.nb[k] = desc->nb[k];
-        if (d.bs[k] < 2 * NG || d.nb[k] < 1) return -1;
-    }
-    d.gam = desc->gam;
-    d.a = desc->spin;
-    d.cour = desc->cour;
-    d.rin = desc->rin;
-    d.rout = desc->rout;
-    d.rho_floor = desc->rho_floor;
-    d.u_floor = desc->u_floor;
-    d.minkowski = desc->minkowski;
     return 0;
 }
 
@@ -364,18 +598,175 @@ int hamr_desc_torus(const hamr_desc *desc, const hamr_torus_desc *t, double *p, 
     std::vector<BlockDev> blocks = build_block_table(d, nullptr);
     std::vector<double> tp((size_t)L.nblk * NPR * K, 0.), ts((size_t)L.nblk * 3 * K, 0.);
     TorusDesc td{t->rin, t->rmax, t->kappa, t->beta_min, t->aphi_cut, t->perturb, t->seed};
-    std::string e = init_torus(d, L, blocks, td, gcov.data(), gcon.data(), gdet.data(), tp.data(), ts.data());
+    std::string e = init_torus(d, blocks, td, gcov.data(), gcon.data(), gdet.data(), tp.data(), ts.data());
     if (!e.empty()) return -2;
     for (size_t s = 0; s < (size_t)L.nblk * NPR; s++) std::memcpy(p + s * N, tp.data() + s * K, N * sizeof(double));
     for (size_t s = 0; s < (size_t)L.nblk * 3; s++) std::memcpy(ps + s * N, ts.data() + s * K, N * sizeof(double));
     return 0;
 }
 
+int hamr_desc_amr(const hamr_desc *desc, const hamr_amr_desc *a, long long *info, int *rows, int *ex, int *pro)
+{
+    GridDesc d;
+    if (desc_to_grid(desc, d) || !a || !info) return -1;
+    AmrDesc ad;
+    ad.nlev = a->nlev;
+    for (int l = 0; l < kAmrMaxLevels; l++)
+        for (int k = 0; k < 3; k++)
+            for (int e = 0; e < 2; e++) ad.box[l][k][e] = a->box[l][k][e];
+    ad.seed = a->seed;
+    ad.p_big = a->p_big;
+    ad.p_mid = a->p_mid;
+    AmrGrid g;
+    if (!build_amr(d, ad, g).empty()) return -2;
+    long long zones = 0;
+    for (const BlockDev &B : g.blocks) zones += (long long)B.bs[0] * B.bs[1] * B.bs[2];
+    info[0] = (long long)g.blocks.size();
+    info[1] = (long long)g.geos.size();
+    info[2] = g.cells;
+    info[3] = zones;
+    for (int s = 0; s < 3; s++) {
+        info[4 + s] = (long long)g.ex[s].size();
+        info[7 + s] = (long long)g.pro[s].size();
+    }
+    info[10] = (long long)g.rside.size();
+    info[11] = 1;
+    if (rows)
+        for (size_t n = 0; n < g.blocks.size(); n++) {
+            const BlockDev &B = g.blocks[n];
+            int *r = rows + n * 15;
+            r[0] = B.level;
+            for (int k = 0; k < 3; k++) {
+                r[1 + k] = B.o[k];
+                r[4 + k] = B.bs[k];
+            }
+            r[7] = B.geo;
+            r[8] = B.pole;
+            for (int k = 0; k < 6; k++) r[9 + k] = B.kind[k];
+        }
+    if (ex)
+        for (int s = 0; s < 3; s++)
+            for (const AmrEntry &e : g.ex[s]) {
+                std::memcpy(ex, &e, sizeof e);
+                ex += 4;
+            }
+    if (pro)
+        for (int s = 0; s < 3; s++)
+            for (const AmrProlong &e : g.pro[s]) {
+                std::memcpy(pro, &e, sizeof e);
+                pro += 16;
+            }
+    return 0;
+}
+
+static int amr_from_desc(const hamr_desc *desc, const hamr_amr_desc *a, GridDesc &d, AmrGrid &g)
+{
+    if (desc_to_grid(desc, d) || !a) return -1;
+    AmrDesc ad;
+    ad.nlev = a->nlev;
+    for (int l = 0; l < kAmrMaxLevels; l++)
+        for (int k = 0; k < 3; k++)
+            for (int e = 0; e < 2; e++) ad.box[l][k][e] = a->box[l][k][e];
+    ad.seed = a->seed;
+    ad.p_big = a->p_big;
+    ad.p_mid = a->p_mid;
+    if (!build_amr(d, ad, g).empty()) return -2;
+    return 0;
+}
+
+int hamr_desc_amr_geometry(const hamr_desc *desc, const hamr_amr_desc *a, int geo, double *gcov, double *gcon,
+                           double *gdet, double *conn, double *rcen)
+{
+    GridDesc d;
+    AmrGrid g;
+    int r = amr_from_desc(desc, a, d, g);
+    if (r) return r;
+    if (geo < 0 || geo >= (int)g.geos.size()) return -1;
+    const AmrGeo &G = g.geos[geo];
+    const size_t KC = (size_t)G.kcol;
+    std::memset(gcov, 0, NPOS * 10 * KC * sizeof(double));
+    std::memset(gcon, 0, NPOS * 10 * KC * sizeof(double));
+    std::memset(gdet, 0, NPOS * KC * sizeof(double));
+    std::memset(conn, 0, 40 * KC * sizeof(double));
+    std::memset(rcen, 0, KC * sizeof(double));
+    fill_geometry_level(d, G.level, G.o1, G.o2, G.D1, G.D2, G.kcol, gcov, gcon, gdet, conn, rcen);
+    return 0;
+}
+
+int hamr_desc_amr_torus(const hamr_desc *desc, const hamr_amr_desc *a, const hamr_torus_desc *t, double *p,
+                        double *ps)
+{
+    GridDesc d;
+    AmrGrid g;
+    int r = amr_from_desc(desc, a, d, g);
+    if (r || !t) return r ? r : -1;
+    const size_t KC = (size_t)g.cols, K = (size_t)g.cells;
+    std::vector<double> gcov(NPOS * 10 * KC, 0.), gcon(NPOS * 10 * KC, 0.), gdet(NPOS * KC, 0.), conn(40 * KC, 0.),
+        rcen(KC, 0.);
+    for (const AmrGeo &G : g.geos)
+        fill_geometry_level(d, G.level, G.o1, G.o2, G.D1, G.D2, G.kcol, gcov.data() + G.moff * NPOS * 10,
+                            gcon.data() + G.moff * NPOS * 10, gdet.data() + G.moff * NPOS, conn.data() + G.moff * 40,
+                            rcen.data() + G.moff);
+    std::vector<double> tp(NPR * K, 0.), ts(3 * K, 0.);
+    TorusDesc td{t->rin, t->rmax, t->kappa, t->beta_min, t->aphi_cut, t->perturb, t->seed};
+    if (!init_torus(d, g.blocks, td, gcov.data(), gcon.data(), gdet.data(), tp.data(), ts.data()).empty()) return -3;
+    // unpadded [block][component][cell]
+    size_t pp = 0, sp = 0;
+    for (const BlockDev &B : g.blocks) {
+        const size_t N = (size_t)B.D[0] * B.D[1] * B.D[2];
+        for (int k = 0; k < NPR; k++, pp += N) std::memcpy(p + pp, tp.data() + NPR * B.off + (size_t)k * B.ks, N * sizeof(double));
+        for (int k = 0; k < 3; k++, sp += N) std::memcpy(ps + sp, ts.data() + 3 * B.off + (size_t)k * B.ks, N * sizeof(double));
+    }
+    return 0;
+}
+
 int hamr_metric_sizes(const hamr_ctx *c, long long *ncol, long long *kcol)
 {
     if (!c) return -1;
-    if (ncol) *ncol = c->L.ncol;
-    if (kcol) *kcol = c->L.kcol;
+    const AmrGeo &G = c->geos[0];
+    if (ncol) *ncol = (long long)G.D1 * G.D2;
+    if (kcol) *kcol = G.kcol;
+    return 0;
+}
+
+int hamr_geometry_info(const hamr_ctx *c, int geo, long long *info)
+{
+    if (!c || !info || geo < 0 || geo >= (int)c->geos.size()) return -1;
+    const AmrGeo &G = c->geos[geo];
+    info[0] = G.level;
+    info[1] = G.o1;
+    info[2] = G.o2;
+    info[3] = G.D1;
+    info[4] = G.D2;
+    info[5] = G.kcol;
+    return 0;
+}
+
+int hamr_upload_metric(hamr_ctx *c, int geo, const double *gcov, const double *gcon, const double *gdet,
+                       const double *conn, const double *rcen)
+{
+    if (!c || geo < 0 || geo >= (int)c->geos.size()) return fail(c, "hamr_upload_metric: bad geometry index");
+    const AmrGeo &G = c->geos[geo];
+    const size_t KC = (size_t)G.kcol, mo = (size_t)G.moff;
+    if (h2d(c, c->gcov + mo * NPOS * 10, gcov, NPOS * 10 * KC * sizeof(double))) return -1;
+    if (h2d(c, c->gcon + mo * NPOS * 10, gcon, NPOS * 10 * KC * sizeof(double))) return -1;
+    if (h2d(c, c->gdet + mo * NPOS, gdet, NPOS * KC * sizeof(double))) return -1;
+    if (h2d(c, c->conn + mo * 40, conn, 40 * KC * sizeof(double))) return -1;
+    if (h2d(c, c->rcen + mo, rcen, KC * sizeof(double))) return -1;
+    return 0;
+}
+
+int hamr_download_metric(hamr_ctx *c, int geo, double *gcov, double *gcon, double *gdet, double *conn, double *rcen)
+{
+    if (!c || geo < 0 || geo >= (int)c->geos.size()) return fail(c, "hamr_download_metric: bad geometry index");
+    const AmrGeo &G = c->geos[geo];
+    const size_t KC = (size_t)G.kcol, mo = (size_t)G.moff;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (d2h(c, gcov, c->gcov + mo * NPOS * 10, NPOS * 10 * KC * sizeof(double))) return -1;
+    if (d2h(c, gcon, c->gcon + mo * NPOS * 10, NPOS * 10 * KC * sizeof(double))) return -1;
+    if (d2h(c, gdet, c->gdet + mo * NPOS, NPOS * KC * sizeof(double))) return -1;
+    if (d2h(c, conn, c->conn + mo * 40, 40 * KC * sizeof(double))) return -1;
+    if (d2h(c, rcen, c->rcen + mo, KC * sizeof(double))) return -1;
     return 0;
 }
 
@@ -383,48 +774,17 @@ int hamr_set_grid(hamr_ctx *c)
 {
     if (!c) return -1;
     HIPCHK(c, hipSetDevice(c->device));
-    const Layout &L = c->L;
-    const size_t KC = (size_t)L.kcol;
-    std::vector<double> gcov(NPOS * 10 * KC), gcon(NPOS * 10 * KC), gdet(NPOS * KC), conn(40 * KC), rcen(KC);
-    for (int c1 = 0; c1 < c->d.nb[0]; c1++)
-        for (int c2 = 0; c2 < c->d.nb[1]; c2++) {
-            std::fill(gcov.begin(), gcov.end(), 0.);
-            std::fill(gcon.begin(), gcon.end(), 0.);
-            std::fill(gdet.begin(), gdet.end(), 0.);
-            std::fill(conn.begin(), conn.end(), 0.);
-            std::fill(rcen.begin(), rcen.end(), 0.);
-            fill_geometry(c->d, L, c1, c2, gcov.data(), gcon.data(), gdet.data(), conn.data(), rcen.data());
-            int r = hamr_upload_metric(c, c1 * c->d.nb[1] + c2, gcov.data(), gcon.data(), gdet.data(), conn.data(),
-                                       rcen.data());
-            if (r) return r;
-        }
+    for (int geo = 0; geo < (int)c->geos.size(); geo++) {
+        const AmrGeo &G = c->geos[geo];
+        const size_t KC = (size_t)G.kcol;
+        std::vector<double> gcov(NPOS * 10 * KC, 0.), gcon(NPOS * 10 * KC, 0.), gdet(NPOS * KC, 0.), conn(40 * KC, 0.),
+            rcen(KC, 0.);
+        fill_geometry_level(c->d, G.level, G.o1, G.o2, G.D1, G.D2, G.kcol, gcov.data(), gcon.data(), gdet.data(),
+                            conn.data(), rcen.data());
+        int r = hamr_upload_metric(c, geo, gcov.data(), gcon.data(), gdet.data(), conn.data(), rcen.data());
+        if (r) return r;
+    }
     HIPCHK(c, hipStreamSynchronize(c->s));
-    return 0;
-}
-
-int hamr_upload_metric(hamr_ctx *c, int geo, const double *gcov, const double *gcon, const double *gdet,
-                       const double *conn, const double *rcen)
-{
-    if (!c || geo < 0 || geo >= c->L.ngeo) return fail(c, "hamr_upload_metric: bad geometry index");
-    const size_t KC = (size_t)c->L.kcol;
-    if (h2d(c, c->gcov + (size_t)geo * NPOS * 10 * KC, gcov, NPOS * 10 * KC * sizeof(double))) return -1;
-    if (h2d(c, c->gcon + (size_t)geo * NPOS * 10 * KC, gcon, NPOS * 10 * KC * sizeof(double))) return -1;
-    if (h2d(c, c->gdet + (size_t)geo * NPOS * KC, gdet, NPOS * KC * sizeof(double))) return -1;
-    if (h2d(c, c->conn + (size_t)geo * 40 * KC, conn, 40 * KC * sizeof(double))) return -1;
-    if (h2d(c, c->rcen + (size_t)geo * KC, rcen, KC * sizeof(double))) return -1;
-    return 0;
-}
-
-int hamr_download_metric(hamr_ctx *c, int geo, double *gcov, double *gcon, double *gdet, double *conn, double *rcen)
-{
-    if (!c || geo < 0 || geo >= c->L.ngeo) return fail(c, "hamr_download_metric: bad geometry index");
-    const size_t KC = (size_t)c->L.kcol;
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    if (d2h(c, gcov, c->gcov + (size_t)geo * NPOS * 10 * KC, NPOS * 10 * KC * sizeof(double))) return -1;
-    if (d2h(c, gcon, c->gcon + (size_t)geo * NPOS * 10 * KC, NPOS * 10 * KC * sizeof(double))) return -1;
-    if (d2h(c, gdet, c->gdet + (size_t)geo * NPOS * KC, NPOS * KC * sizeof(double))) return -1;
-    if (d2h(c, conn, c->conn + (size_t)geo * 40 * KC, 40 * KC * sizeof(double))) return -1;
-    if (d2h(c, rcen, c->rcen + (size_t)geo * KC, KC * sizeof(double))) return -1;
     return 0;
 }
 
@@ -432,16 +792,15 @@ int hamr_init_torus(hamr_ctx *c, const hamr_torus_desc *t)
 {
     if (!c || !t) return -1;
     HIPCHK(c, hipSetDevice(c->device));
-    const Layout &L = c->L;
-    const size_t K = (size_t)L.ksize, nb = (size_t)L.nblk, KC = (size_t)L.kcol, ng = (size_t)L.ngeo;
-    std::vector<double> gcov(ng * NPOS * 10 * KC), gcon(ng * NPOS * 10 * KC), gdet(ng * NPOS * KC);
+    const size_t K = (size_t)c->cells, KC = (size_t)c->cols;
+    std::vector<double> gcov(NPOS * 10 * KC), gcon(NPOS * 10 * KC), gdet(NPOS * KC);
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (d2h(c, gcov.data(), c->gcov, gcov.size() * sizeof(double))) return -1;
     if (d2h(c, gcon.data(), c->gcon, gcon.size() * sizeof(double))) return -1;
     if (d2h(c, gdet.data(), c->gdet, gdet.size() * sizeof(double))) return -1;
-    std::vector<double> p(nb * NPR * K, 0.), ps(nb * 3 * K, 0.);
+    std::vector<double> p(NPR * K, 0.), ps(3 * K, 0.);
     TorusDesc td{t->rin, t->rmax, t->kappa, t->beta_min, t->aphi_cut, t->perturb, t->seed};
-    std::string e = init_torus(c->d, L, c->blocks, td, gcov.data(), gcon.data(), gdet.data(), p.data(), ps.data());
+    std::string e = init_torus(c->d, c->blocks, td, gcov.data(), gcon.data(), gdet.data(), p.data(), ps.data());
     if (!e.empty()) return fail(c, e);
     if (h2d(c, c->p, p.data(), p.size() * sizeof(double))) return -1;
     if (h2d(c, c->ps, ps.data(), ps.size() * sizeof(double))) return -1;
@@ -475,76 +834,65 @@ static bool is_float_array(const hamr_ctx *c, int which)
            (which == HAMR_F1 || which == HAMR_F2 || which == HAMR_F3 || which == HAMR_EMF || which == HAMR_ECENT);
 }
 
-int hamr_set_array(hamr_ctx *c, int which, const double *src)
+// Copy between the unpadded [block][component][cell] layout (ext) and the
+// device slabs, one 2D copy per block.
+static int copy_array(hamr_ctx *c, int which, double *ext, int to_device, hipMemcpyKind kind)
 {
-    if (!c || !src) return -1;
     double *d;
     int nc;
     if (array_info(c, which, &d, &nc)) return -1;
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell, rows = (size_t)c->L.nblk * nc;
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    if (is_float_array(c, which)) {
-        std::vector<float> t(rows * N);
-        for (size_t e = 0; e < t.size(); e++) t[e] = (float)src[e];
-        HIPCHK(c, hipMemcpy2DAsync(d, K * sizeof(float), t.data(), N * sizeof(float), N * sizeof(float), rows,
-                                   hipMemcpyHostToDevice, c->s));
-        HIPCHK(c, hipStreamSynchronize(c->s));
-        return 0;
+    const bool fl = is_float_array(c, which);
+    const size_t es = fl ? sizeof(float) : sizeof(double);
+    if (fl && kind != hipMemcpyHostToDevice && kind != hipMemcpyDeviceToHost)
+        return fail(c, "float array in the mixed step: host copies only");
+    size_t total = 0;
+    for (const BlockDev &B : c->blocks) total += (size_t)nc * ncell_of(B);
+    std::vector<float> tmp;
+    if (fl) {
+        tmp.resize(total);
+        if (to_device)
+            for (size_t e = 0; e < total; e++) tmp[e] = (float)ext[e];
     }
-    HIPCHK(c, hipMemcpy2DAsync(d, K * sizeof(double), src, N * sizeof(double), N * sizeof(double), rows,
-                               hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
+    size_t pos = 0;
+    for (const BlockDev &B : c->blocks) {
+        const size_t N = (size_t)ncell_of(B), K = (size_t)B.ks;
+        char *dev = (char *)d + (size_t)nc * B.off * es;
+        char *hst = fl ? (char *)(tmp.data() + pos) : (char *)(ext + pos);
+        if (to_device) HIPCHK(c, hipMemcpy2DAsync(dev, K * es, hst, N * es, N * es, nc, kind, c->s));
+        else HIPCHK(c, hipMemcpy2DAsync(hst, N * es, dev, K * es, N * es, nc, kind, c->s));
+        pos += (size_t)nc * N;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (fl && !to_device)
+        for (size_t e = 0; e < total; e++) ext[e] = (double)tmp[e];
     return 0;
+}
+
+int hamr_set_array(hamr_ctx *c, int which, const double *src)
+{
+    if (!c || !src) return -1;
+    return copy_array(c, which, const_cast<double *>(src), 1, hipMemcpyHostToDevice);
 }
 
 int hamr_get_array(hamr_ctx *c, int which, double *dst)
 {
     if (!c || !dst) return -1;
-    double *d;
-    int nc;
-    if (array_info(c, which, &d, &nc)) return -1;
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell, rows = (size_t)c->L.nblk * nc;
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    if (is_float_array(c, which)) {
-        std::vector<float> t(rows * N);
-        HIPCHK(c, hipMemcpy2DAsync(t.data(), N * sizeof(float), d, K * sizeof(float), N * sizeof(float), rows,
-                                   hipMemcpyDeviceToHost, c->s));
-        HIPCHK(c, hipStreamSynchronize(c->s));
-        for (size_t e = 0; e < t.size(); e++) dst[e] = (double)t[e];
-        return 0;
-    }
-    HIPCHK(c, hipMemcpy2DAsync(dst, N * sizeof(double), d, K * sizeof(double), N * sizeof(double), rows,
-                               hipMemcpyDeviceToHost, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    return 0;
+    return copy_array(c, which, dst, 0, hipMemcpyDeviceToHost);
 }
 
 int hamr_set_array_dev(hamr_ctx *c, int which, const double *dsrc)
 {
     if (!c || !dsrc) return -1;
-    double *d;
-    int nc;
-    if (array_info(c, which, &d, &nc)) return -1;
     if (is_float_array(c, which)) return fail(c, "hamr_set_array_dev: float array in the mixed step");
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell, rows = (size_t)c->L.nblk * nc;
-    HIPCHK(c, hipMemcpy2DAsync(d, K * sizeof(double), dsrc, N * sizeof(double), N * sizeof(double), rows,
-                               hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    return 0;
+    return copy_array(c, which, const_cast<double *>(dsrc), 1, hipMemcpyDeviceToDevice);
 }
 
 int hamr_get_array_dev(hamr_ctx *c, int which, double *ddst)
 {
     if (!c || !ddst) return -1;
-    double *d;
-    int nc;
-    if (array_info(c, which, &d, &nc)) return -1;
     if (is_float_array(c, which)) return fail(c, "hamr_get_array_dev: float array in the mixed step");
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell, rows = (size_t)c->L.nblk * nc;
-    HIPCHK(c, hipMemcpy2DAsync(ddst, N * sizeof(double), d, K * sizeof(double), N * sizeof(double), rows,
-                               hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    return 0;
+    return copy_array(c, which, ddst, 0, hipMemcpyDeviceToDevice);
 }
 
 int hamr_set_precision(hamr_ctx *c, int mode)
@@ -570,7 +918,8 @@ int hamr_get_precision(const hamr_ctx *c, int *mode)
 int hamr_upload(hamr_ctx *c, int n, const double *p_aos, const double *ps_aos)
 {
     if (!c || !valid_block(c, n)) return fail(c, "hamr_upload: bad block");
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell;
+    const BlockDev &B = c->blocks[n];
+    const size_t K = (size_t)B.ks, N = (size_t)ncell_of(B);
     std::vector<double> tp(NPR * K, 0.), ts(3 * K, 0.);
     for (size_t cell = 0; cell < N; cell++) {
         for (int k = 0; k < NPR; k++) tp[k * K + cell] = p_aos[cell * NPR + k];
@@ -578,19 +927,20 @@ int hamr_upload(hamr_ctx *c, int n, const double *p_aos, const double *ps_aos)
             for (int k = 0; k < 3; k++) ts[k * K + cell] = ps_aos[cell * 4 + 1 + k];
     }
     HIPCHK(c, hipStreamSynchronize(c->s));
-    if (h2d(c, c->p + (size_t)n * NPR * K, tp.data(), NPR * K * sizeof(double))) return -1;
-    if (ps_aos) if (h2d(c, c->ps + (size_t)n * 3 * K, ts.data(), 3 * K * sizeof(double))) return -1;
+    if (h2d(c, c->p + NPR * B.off, tp.data(), NPR * K * sizeof(double))) return -1;
+    if (ps_aos) if (h2d(c, c->ps + 3 * B.off, ts.data(), 3 * K * sizeof(double))) return -1;
     return 0;
 }
 
 int hamr_download(hamr_ctx *c, int n, double *p_aos, double *ps_aos)
 {
     if (!c || !valid_block(c, n)) return fail(c, "hamr_download: bad block");
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell;
+    const BlockDev &B = c->blocks[n];
+    const size_t K = (size_t)B.ks, N = (size_t)ncell_of(B);
     std::vector<double> tp(NPR * K), ts(3 * K);
     HIPCHK(c, hipStreamSynchronize(c->s));
-    if (d2h(c, tp.data(), c->p + (size_t)n * NPR * K, NPR * K * sizeof(double))) return -1;
-    if (d2h(c, ts.data(), c->ps + (size_t)n * 3 * K, 3 * K * sizeof(double))) return -1;
+    if (d2h(c, tp.data(), c->p + NPR * B.off, NPR * K * sizeof(double))) return -1;
+    if (d2h(c, ts.data(), c->ps + 3 * B.off, 3 * K * sizeof(double))) return -1;
     for (size_t cell = 0; cell < N; cell++) {
         if (p_aos)
             for (int k = 0; k < NPR; k++) p_aos[cell * NPR + k] = tp[k * K + cell];
@@ -604,7 +954,8 @@ int hamr_download(hamr_ctx *c, int n, double *p_aos, double *ps_aos)
 int hamr_fluxcalc(hamr_ctx *c, int dir, int flag, int n)
 {
     if (!c || !valid_block(c, n) || dir < 1 || dir > 3) return fail(c, "hamr_fluxcalc: bad arguments");
-    launch_flux(c->P, dir, c->ident + n, 1, flag ? c->ph : c->p, flag ? c->psh : c->ps, c->F[dir - 1], c->dtmin, c->prec, c->s);
+    launch_flux(c->P, dir, block_launch(c, KC_FLUX1 + dir - 1, n), flag ? c->ph : c->p, flag ? c->psh : c->ps,
+                c->F[dir - 1], c->dtmin, c->prec, c->s);
     return check_launch(c, "hamr_fluxcalc");
 }
 
@@ -625,39 +976,37 @@ int hamr_block_dt(hamr_ctx *c, int n, int dir, double *out)
 int hamr_ct_emf(hamr_ctx *c, int flag, int n)
 {
     if (!c || !valid_block(c, n)) return fail(c, "hamr_ct_emf: bad block");
-    launch_emf(c->P, c->ident + n, 1, flag ? c->ph : c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, c->s);
+    launch_emf(c->P, block_launch(c, KC_ECENT, n), block_launch(c, KC_CORNER, n), flag ? c->ph : c->p, c->F[0],
+               c->F[1], c->F[2], c->Ec, c->E, c->prec, c->s);
     return check_launch(c, "hamr_ct_emf");
 }
 
 int hamr_ct_bound(hamr_ctx *c, double dt)
 {
     if (!c) return -1;
-    launch_pole(c->P, c->ring, c->nring, dt, nullptr, c->E, c->prec, c->s);
+    pole(c, dt, nullptr);
     return check_launch(c, "hamr_ct_bound");
 }
 
 int hamr_ct_update(hamr_ctx *c, int flag, double Dt, int n)
 {
     if (!c || !valid_block(c, n)) return fail(c, "hamr_ct_update: bad block");
-    launch_ct_update(c->P, c->ident + n, 1, Dt, nullptr, 1.0, c->ps, flag ? c->ps : c->psh, c->E, c->prec, c->s);
+    launch_ct_update(c->P, block_launch(c, KC_CORNER, n), Dt, nullptr, 1.0, c->ps, flag ? c->ps : c->psh, c->E,
+                     c->prec, c->s);
     return check_launch(c, "hamr_ct_update");
 }
 
 int hamr_update(hamr_ctx *c, int flag, int n, double Dt)
 {
     if (!c || !valid_block(c, n)) return fail(c, "hamr_update: bad block");
+    const Launch L = block_launch(c, KC_INTERIOR, n);
     if (flag)
-        launch_update(c->P, c->ident + n, 1, 1, Dt, nullptr, 1.0, c->ph, c->ph, c->p, c->U0, c->Un, c->ps, c->F[0], c->F[1],
-                      c->F[2], c->pflag, c->failcnt, c->prec, c->s);
+        launch_update(c->P, L, 1, Dt, nullptr, 1.0, c->ph, c->ph, c->p, c->U0, c->Un, c->ps, c->F[0], c->F[1], c->F[2],
+                      c->pflag, c->failcnt, c->prec, c->s);
     else
-        launch_update(c->P, c->ident + n, 1, 0, Dt, nullptr, 1.0, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1],
-                      c->F[2], c->pflag, c->failcnt, c->prec, c->s);
+        launch_update(c->P, L, 0, Dt, nullptr, 1.0, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2],
+                      c->pflag, c->failcnt, c->prec, c->s);
     return check_launch(c, "hamr_update");
-}
-
-static void bound_all(hamr_ctx *c, double *p, double *ps)
-{
-    for (int dir = 1; dir <= 3; dir++) launch_bound(c->P, dir, c->zlist, c->L.nblk, p, ps, c->pflag, c->s);
 }
 
 int hamr_boundprim(hamr_ctx *c, int flag)
@@ -671,9 +1020,9 @@ int hamr_boundprim(hamr_ctx *c, int flag)
 int hamr_flags(hamr_ctx *c, int n, int *pflag, int *failimage)
 {
     if (!c || !valid_block(c, n)) return fail(c, "hamr_flags: bad block");
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell;
+    const BlockDev &B = c->blocks[n];
     HIPCHK(c, hipStreamSynchronize(c->s));
-    if (pflag) if (d2h(c, pflag, c->pflag + (size_t)n * K, N * sizeof(int))) return -1;
+    if (pflag) if (d2h(c, pflag, c->pflag + B.off, (size_t)ncell_of(B) * sizeof(int))) return -1;
     if (failimage) if (d2h(c, failimage, c->failcnt + (size_t)n * 4, 4 * sizeof(int))) return -1;
     return 0;
 }
@@ -681,7 +1030,7 @@ int hamr_flags(hamr_ctx *c, int n, int *pflag, int *failimage)
 int hamr_clear_flags(hamr_ctx *c)
 {
     if (!c) return -1;
-    HIPCHK(c, hipMemsetAsync(c->failcnt, 0, (size_t)c->L.nblk * 4 * sizeof(int), c->s));
+    HIPCHK(c, hipMemsetAsync(c->failcnt, 0, (size_t)nblk_of(c) * 4 * sizeof(int), c->s));
     return 0;
 }
 
@@ -689,24 +1038,23 @@ int hamr_clear_flags(hamr_ctx *c)
 static void enqueue_step(hamr_ctx *c)
 {
     const KParams &P = c->P;
-    const int nb = c->L.nblk;
-    const int *bl = c->zlist;
+    const int nb = nblk_of(c);
     hipStream_t s = c->s;
     // predictor: p/ps -> ph/psh with Dt = dt/2
-    for (int dir = 1; dir <= 3; dir++) launch_flux(P, dir, bl, nb, c->p, c->ps, c->F[dir - 1], nullptr, c->prec, s);
-    launch_emf(P, bl, nb, c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, s);
-    launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, c->prec, s);
-    launch_ct_update(P, bl, nb, 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, s);
-    launch_update(P, bl, nb, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2], c->pflag,
-                  c->failcnt, c->prec, s);
+    for (int dir = 1; dir <= 3; dir++)
+        launch_flux(P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->p, c->ps, c->F[dir - 1], nullptr, c->prec, s);
+    emf_all(c, c->p);
+    pole(c, 0., c->dt);
+    launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, s);
+    update_all(c, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->psh);
     bound_all(c, c->ph, c->psh);
     // corrector: U0 -> p/ps with Dt = dt, fluxes from ph/psh, Courant minima recorded
-    for (int dir = 1; dir <= 3; dir++) launch_flux(P, dir, bl, nb, c->ph, c->psh, c->F[dir - 1], c->dtmin, c->prec, s);
-    launch_emf(P, bl, nb, c->ph, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, s);
-    launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, c->prec, s);
-    launch_ct_update(P, bl, nb, 0., c->dt, 1.0, c->ps, c->ps, c->E, c->prec, s);
-    launch_update(P, bl, nb, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->U0, c->Un, c->ps, c->F[0], c->F[1], c->F[2], c->pflag,
-                  c->failcnt, c->prec, s);
+    for (int dir = 1; dir <= 3; dir++)
+        launch_flux(P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->ph, c->psh, c->F[dir - 1], c->dtmin, c->prec, s);
+    emf_all(c, c->ph);
+    pole(c, 0., c->dt);
+    launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 1.0, c->ps, c->ps, c->E, c->prec, s);
+    update_all(c, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->ps);
     bound_all(c, c->p, c->ps);
     launch_dt_finalize(nb, c->dtmin, c->dt, s);
 }
@@ -718,9 +1066,10 @@ int hamr_estimate_dt(hamr_ctx *c, double *dt)
     const double big[2] = {1e300, 1e300};
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (h2d(c, c->dt, big, sizeof big)) return -1;
-    launch_dt_reset(c->L.nblk, c->dtmin, c->s);
-    for (int dir = 1; dir <= 3; dir++) launch_flux(c->P, dir, c->zlist, c->L.nblk, c->p, c->ps, c->F[dir - 1], c->dtmin, c->prec, c->s);
-    launch_dt_finalize(c->L.nblk, c->dtmin, c->dt, c->s);
+    launch_dt_reset(nblk_of(c), c->dtmin, c->s);
+    for (int dir = 1; dir <= 3; dir++)
+        launch_flux(c->P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->p, c->ps, c->F[dir - 1], c->dtmin, c->prec, c->s);
+    launch_dt_finalize(nblk_of(c), c->dtmin, c->dt, c->s);
     if (check_launch(c, "hamr_estimate_dt")) return -1;
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (d2h(c, dt, c->dt, sizeof(double))) return -1;
@@ -741,7 +1090,7 @@ int hamr_step(hamr_ctx *c, int nsteps, double *dt_inout)
     if (!c) return -1;
     HIPCHK(c, hipSetDevice(c->device));
     if (set_dt(c, dt_inout)) return -1;
-    launch_dt_reset(c->L.nblk, c->dtmin, c->s);
+    launch_dt_reset(nblk_of(c), c->dtmin, c->s);
     for (int k = 0; k < nsteps; k++) enqueue_step(c);
     if (check_launch(c, "hamr_step")) return -1;
     if (dt_inout) {
@@ -756,7 +1105,7 @@ int hamr_step_graph(hamr_ctx *c, int nsteps, double *dt_inout)
     if (!c) return -1;
     HIPCHK(c, hipSetDevice(c->device));
     if (set_dt(c, dt_inout)) return -1;
-    launch_dt_reset(c->L.nblk, c->dtmin, c->s);
+    launch_dt_reset(nblk_of(c), c->dtmin, c->s);
     if (!c->graph) {
         hipGraph_t g;
         HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
@@ -773,14 +1122,139 @@ int hamr_step_graph(hamr_ctx *c, int nsteps, double *dt_inout)
     return 0;
 }
 
+// One launch of tune slot `slot` with launch configuration L (predictor-step
+// operands).  p and ps are read only, except by the ghost-fill kernels, which
+// rewrite ghost cells of a filled state with the same values.
+static void run_slot(hamr_ctx *c, int slot, const Launch &L, int T_ex, int G_ex)
+{
+    const KParams &P = c->P;
+    hipStream_t s = c->s;
+    switch (slot) {
+    case TS_FLUX1:
+    case TS_FLUX2:
+    case TS_FLUX3: launch_flux(P, slot - TS_FLUX1 + 1, L, c->p, c->ps, c->F[slot - TS_FLUX1], nullptr, c->prec, s); break;
+    case TS_ECENT: launch_emf_centre(P, L, c->p, c->Ec, c->prec, s); break;
+    case TS_ECORNER: launch_emf_corner(P, L, c->Ec, c->F[0], c->F[1], c->F[2], c->E, c->prec, s); break;
+    case TS_CT: launch_ct_update(P, L, 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, s); break;
+    case TS_CONS:
+    case TS_INVERT:
+    case TS_FALLBACK:
+        launch_update_part(P, slot - TS_CONS, L, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0],
+                           c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, s);
+        break;
+    case TS_BOUND1:
+    case TS_BOUND2:
+    case TS_BOUND3: launch_bound(P, slot - TS_BOUND1 + 1, L, c->p, c->ps, c->pflag, 0, s); break;
+    default:
+        for (int st = 0; st < 3; st++)
+            launch_amr_exchange(P, c->ex[st], c->nex[st], c->pro[st], c->npro[st], c->p, c->ps, c->pflag, T_ex, G_ex, s);
+        break;
+    }
+}
+
+static int time_launches(hamr_ctx *c, int slot, const Launch &L, int T_ex, int G_ex, int nrep, float *ms)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    run_slot(c, slot, L, T_ex, G_ex);  // warm-up
+    HIPCHK(c, hipEventRecord(e0, c->s));
+    for (int r = 0; r < nrep; r++) run_slot(c, slot, L, T_ex, G_ex);
+    HIPCHK(c, hipEventRecord(e1, c->s));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / nrep;
+    return check_launch(c, "autotune");
+}
+
+int hamr_autotune(hamr_ctx *c, int nrep)
+{
+    if (!c || nrep < 1) return fail(c, "hamr_autotune: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    // the tuning launches run the update on the current state: keep the
+    // failure counters and flags of the last step
+    const size_t nb = (size_t)nblk_of(c);
+    std::vector<int> fc(nb * 4), fl((size_t)c->cells);
+    if (d2h(c, fc.data(), c->failcnt, fc.size() * sizeof(int))) return -1;
+    if (d2h(c, fl.data(), c->pflag, fl.size() * sizeof(int))) return -1;
+    static const int grids[4] = {0, 1024, 2048, 4096};
+    for (int slot = 0; slot < TS_N; slot++) {
+        if (slot == TS_EXCHANGE && !c->amr) continue;
+        if (slot != TS_EXCHANGE && c->cl[kSlotClass[slot]].n[kDefaultTile] == 0) continue;
+        TuneCfg best;
+        for (int ti = 0; ti < 3; ti++)
+            for (int gi = 0; gi < 4; gi++)
+                for (int swz = 0; swz < 2; swz++) {
+                    if (slot == TS_EXCHANGE && swz) continue;
+                    TuneCfg cand;
+                    cand.tile = ti;
+                    cand.grid = grids[gi];
+                    cand.swz = swz;
+                    const TuneCfg keep = c->tune[slot];
+                    c->tune[slot] = cand;
+                    Launch L;
+                    if (slot != TS_EXCHANGE) L = all_launch(c, slot);
+                    float ms;
+                    const int r = time_launches(c, slot, L, kTiles[ti], grids[gi], nrep, &ms);
+                    c->tune[slot] = keep;
+                    if (r) return -1;
+                    cand.ms = ms;
+                    if (best.ms < 0.f || ms < best.ms) best = cand;
+                }
+        c->tune[slot] = best;
+    }
+    if (h2d(c, c->failcnt, fc.data(), fc.size() * sizeof(int))) return -1;
+    if (h2d(c, c->pflag, fl.data(), fl.size() * sizeof(int))) return -1;
+    // leave both states with consistent ghosts
+    bound_all(c, c->p, c->ps);
+    bound_all(c, c->ph, c->psh);
+    if (check_launch(c, "autotune")) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return 0;
+}
+
+int hamr_tune_get(const hamr_ctx *c, int slot, int *cfg, double *ms)
+{
+    if (!c || slot < 0 || slot >= TS_N || !cfg) return -1;
+    const TuneCfg &t = c->tune[slot];
+    cfg[0] = kTiles[t.tile];
+    cfg[1] = t.grid;
+    cfg[2] = t.swz;
+    if (ms) *ms = t.ms;
+    return 0;
+}
+
+int hamr_tune_set(hamr_ctx *c, int slot, const int *cfg)
+{
+    if (!c || slot < 0 || slot >= TS_N || !cfg) return -1;
+    int ti = -1;
+    for (int k = 0; k < 3; k++)
+        if (kTiles[k] == cfg[0]) ti = k;
+    if (ti < 0 || cfg[1] < 0 || (cfg[2] != 0 && cfg[2] != 1)) return fail(c, "hamr_tune_set: tile 64/128/256, grid >= 0, swz 0/1");
+    c->tune[slot].tile = ti;
+    c->tune[slot].grid = cfg[1];
+    c->tune[slot].swz = cfg[2];
+    c->tune[slot].ms = -1.f;
+    if (c->graph) {
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return 0;
+}
 
 int hamr_time_stage(hamr_ctx *c, int stage, int nrep, double *avg_ms)
 {
     if (!c || !avg_ms || nrep < 1 || stage < 0 || stage > 7) return fail(c, "hamr_time_stage: bad arguments");
     HIPCHK(c, hipSetDevice(c->device));
-    const KParams &P = c->P;
-    const int nb = c->L.nblk;
-    const int *bl = c->zlist;
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -789,14 +1263,13 @@ int hamr_time_stage(hamr_ctx *c, int stage, int nrep, double *avg_ms)
         switch (stage) {
         case 0:
         case 1:
-        case 2: launch_flux(P, stage + 1, bl, nb, c->p, c->ps, c->F[stage], nullptr, c->prec, c->s); break;
-        case 3: launch_emf(P, bl, nb, c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, c->s); break;
-        case 4: launch_pole(P, c->ring, c->nring, 0., c->dt, c->E, c->prec, c->s); break;
-        case 5: launch_ct_update(P, bl, nb, 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, c->s); break;
-        case 6:
-            launch_update(P, bl, nb, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2],
-                          c->pflag, c->failcnt, c->prec, c->s);
+        case 2:
+            launch_flux(c->P, stage + 1, all_launch(c, TS_FLUX1 + stage), c->p, c->ps, c->F[stage], nullptr, c->prec, c->s);
             break;
+        case 3: emf_all(c, c->p); break;
+        case 4: pole(c, 0., c->dt); break;
+        case 5: launch_ct_update(c->P, all_launch(c, TS_CT), 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, c->s); break;
+        case 6: update_all(c, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->psh); break;
         default: bound_all(c, c->ph, c->psh); break;
         }
     }
@@ -814,15 +1287,13 @@ int hamr_time_stage(hamr_ctx *c, int stage, int nrep, double *avg_ms)
 // ---- one forward stage and its adjoint ----
 static void enqueue_stage_forward(hamr_ctx *c, double Dt)
 {
-    const KParams &P = c->P;
-    const int nb = c->L.nblk;
-    const int *bl = c->zlist;
     bound_all(c, c->p, c->ps);
-    for (int dir = 1; dir <= 3; dir++) launch_flux(P, dir, bl, nb, c->p, c->ps, c->F[dir - 1], nullptr, c->prec, c->s);
-    launch_emf(P, bl, nb, c->p, c->F[0], c->F[1], c->F[2], c->Ec, c->E, c->prec, c->s);
-    launch_pole(P, c->ring, c->nring, Dt, nullptr, c->E, c->prec, c->s);
-    launch_ct_update(P, bl, nb, Dt, nullptr, 1.0, c->ps, c->psh, c->E, c->prec, c->s);
-    launch_cons(P, bl, nb, Dt, c->p, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2], c->prec, c->s);
+    for (int dir = 1; dir <= 3; dir++)
+        launch_flux(c->P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->p, c->ps, c->F[dir - 1], nullptr, c->prec, c->s);
+    emf_all(c, c->p);
+    pole(c, Dt, nullptr);
+    launch_ct_update(c->P, all_launch(c, TS_CT), Dt, nullptr, 1.0, c->ps, c->psh, c->E, c->prec, c->s);
+    launch_cons(c->P, all_launch(c, TS_CONS), Dt, c->p, c->U0, c->Un, c->psh, c->F[0], c->F[1], c->F[2], c->prec, c->s);
 }
 
 int hamr_stage_forward(hamr_ctx *c, double Dt)
@@ -838,10 +1309,11 @@ static int stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, do
                          hipMemcpyKind out)
 {
     if (!c || !w) return -1;
-    if (!c->d.minkowski || c->L.nblk != 1 || c->prec != HAMR_PREC_FP64)
+    if (!c->d.minkowski || c->amr || nblk_of(c) != 1 || c->prec != HAMR_PREC_FP64)
         return fail(c, "hamr_stage_adjoint: needs a flat periodic box of one block in FP64");
     HIPCHK(c, hipSetDevice(c->device));
-    const size_t K = (size_t)c->L.ksize, N = (size_t)c->L.ncell;
+    const BlockDev &B = c->blocks[0];
+    const size_t K = (size_t)B.ks, N = (size_t)ncell_of(B);
     if (!c->aw) {
         if (dalloc(c, &c->aw, NPR * K) || dalloc(c, &c->alP, NPR * K) || dalloc(c, &c->alps, 3 * K) ||
             dalloc(c, &c->alpsf, 3 * K) || dalloc(c, &c->alE, 3 * K) || dalloc(c, &c->alEc, 3 * K) ||
@@ -853,9 +1325,10 @@ static int stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, do
     // the fluxes of the stage (their mass-flux signs select the EMF upwinding)
     bound_all(c, c->p, c->ps);
     for (int dir = 1; dir <= 3; dir++)
-        launch_flux(c->P, dir, c->zlist, 1, c->p, c->ps, c->F[dir - 1], nullptr, HAMR_PREC_FP64, c->s);
-    launch_adjoint(c->P, c->blocks[0], Dt, c->p, c->ps, c->F[0], c->F[1], c->F[2], c->aw, c->alP, c->alps, c->alpsf,
-                   c->alE, c->alEc, c->alF[0], c->alF[1], c->alF[2], c->s);
+        launch_flux(c->P, dir, block_launch(c, KC_FLUX1 + dir - 1, 0), c->p, c->ps, c->F[dir - 1], nullptr,
+                    HAMR_PREC_FP64, c->s);
+    launch_adjoint(c->P, B, Dt, c->p, c->ps, c->F[0], c->F[1], c->F[2], c->aw, c->alP, c->alps, c->alpsf, c->alE,
+                   c->alEc, c->alF[0], c->alF[1], c->alF[2], c->s);
     if (check_launch(c, "hamr_stage_adjoint")) return -1;
     if (gp)
         HIPCHK(c, hipMemcpy2DAsync(gp, N * sizeof(double), c->alP, K * sizeof(double), N * sizeof(double), NPR, out, c->s));

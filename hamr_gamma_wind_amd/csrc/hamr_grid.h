@@ -19,19 +19,46 @@ namespace hamr {
 
 constexpr int NG = 3;
 
-// Face kinds of the ghost fill.
-enum FaceKind : int { FK_COPY = 0, FK_OUTFLOW = 1, FK_POLE = 2 };
+// Face kinds of the ghost fill.  FK_AMR: the face's ghosts come from the
+// multi-level exchange tables (neighbours of other sizes or levels).
+enum FaceKind : int { FK_COPY = 0, FK_OUTFLOW = 1, FK_POLE = 2, FK_AMR = 3 };
 
-// Per-block record used by the kernels (plain data, 72 bytes).
+// Per-block record used by the kernels (plain data, 160 bytes).  Component k of
+// an array with ncomp components starts at ncomp*off + k*ks for this block.
 struct BlockDev {
     int geo;        // geometry slab
-    int c1, c2, c3; // block coordinates on the base level
+    int c1, c2, c3; // block coordinates (uniform grid: block index; multi-level: origin / 16)
     int pole;       // bit0: lower x2 face is the polar axis, bit1: upper x2 face
     int nbr[6];     // source block for faces -x1,+x1,-x2,+x2,-x3,+x3 (-1: none)
     int kind[6];    // FaceKind of each face
-    int pad;
+    int level;      // refinement level (0: base)
+    int o[3];       // first interior cell, in cells of the block's level
+    int bs[3];      // interior cells
+    int D[3];       // bs + 2 NG
+    int ks;         // cells per component slab (padded to 64)
+    int kcol;       // columns per metric slab (padded to 64)
+    int pad2;
+    int64_t off;    // cells per component of all earlier blocks' slabs
+    int64_t moff;   // metric columns of all earlier geometry slabs
+    double dx[3];   // code-coordinate spacing of the block's level
 };
-static_assert(sizeof(BlockDev) == 72, "BlockDev layout");
+static_assert(sizeof(BlockDev) == 160, "BlockDev layout");
+
+// Multi-level exchange table entries (one per ghost cell).  A source code is
+// cell | kind << 28 with kind AS_CELL (that cell) or AS_AVG8 (the mean of the
+// 2x2x2 cells whose lowest corner is that cell); AS_NONE marks a missing
+// neighbour (no slope along that direction).
+enum AmrSrcKind : int { AS_NONE = 0, AS_CELL = 1, AS_AVG8 = 2 };
+struct AmrEntry {      // copy or restriction
+    int dblk, dcell;   // ghost cell
+    int sblk, scode;   // source
+};
+struct AmrProlong {    // prolongation from the coarse cell (cblk, ccell)
+    int dblk, dcell;
+    int cblk, ccell;
+    int nb[6][2];      // (block, code) of the coarse cell's -x1,+x1,-x2,+x2,-x3,+x3 neighbours
+};
+static_assert(sizeof(AmrEntry) == 16 && sizeof(AmrProlong) == 64, "exchange entry layout");
 
 // Static description of a run (mirrors the reference's compile-time config).
 struct GridDesc {
@@ -79,8 +106,15 @@ void conn_code(const GridDesc &d, const double *X, const double gcon[4][4], doub
 // Code coordinates of position `pos` of global cell (i, j) (interior-relative,
 // may be a ghost index), following the reference's coord() mirroring.
 void coord_of(const GridDesc &d, const GridCoords &gc, int i, int j, int pos, double X[4]);
+// The same on refinement level `level` (spacing halved per level; i, j in the
+// level's cells).
+void coord_of_level(const GridDesc &d, const GridCoords &gc, int level, int i, int j, int pos, double X[4]);
 void fill_geometry(const GridDesc &d, const Layout &L, int c1, int c2, double *gcov, double *gcon,
                    double *gdet, double *conn, double *rcen);
+// One geometry slab of D1 x D2 columns (padded to kcol) whose first interior
+// column is cell (o1, o2) of level `level`.
+void fill_geometry_level(const GridDesc &d, int level, int o1, int o2, int D1, int D2, int64_t kcol, double *gcov,
+                         double *gcon, double *gdet, double *conn, double *rcen);
 
 // Torus initial state on the whole grid, written per block in the device layout
 // (p: NPR components, ps: 3).  Returns an empty string or an error message.
@@ -91,9 +125,10 @@ struct TorusDesc {
     double perturb;            // u *= 1 + perturb (U[0,1) - 1/2)
     uint64_t seed;
 };
-std::string init_torus(const GridDesc &d, const Layout &L, const std::vector<BlockDev> &blocks,
-                       const TorusDesc &t, const double *gcov, const double *gcon, const double *gdet,
-                       double *p, double *ps);
+// Works for any block table (uniform or multi-level): per-block sizes, levels,
+// slab and metric offsets come from the rows.
+std::string init_torus(const GridDesc &d, const std::vector<BlockDev> &blocks, const TorusDesc &t,
+                       const double *gcov, const double *gcon, const double *gdet, double *p, double *ps);
 
 uint64_t hash64(uint64_t x);
 

@@ -8,38 +8,82 @@
 
 namespace hamr {
 
-// Passed by value to every kernel.
+// Passed by value to every kernel.  Everything that depends on the block (its
+// size, slab offsets, metric slab, spacing) is read from the block table.
 struct KParams {
-    int D[3];              // block extent including ghosts
-    int bs[3];             // block interior
-    int64_t ksize;         // cells per component slab
-    int64_t kcol;          // columns per metric slab
-    int nb3;               // blocks around the axis (pole partner shift)
     double gam, cour;
-    double dx[3];
     double rho_floor, u_floor;
     const BlockDev *blk;
     const double *gcov, *gcon, *gdet, *conn, *rcen;
 };
 
+// Work list of one launch: `items` are (block, tile) pairs, a tile being
+// blockDim.x consecutive threads of the block's iteration space.  A grid of
+// any size walks the list with a stride of gridDim.x (a persistent launch when
+// the grid is smaller than the list).  With swz = 1 the item order is permuted
+// so that the workgroups one XCD runs (ids congruent mod 8) take one contiguous
+// range of the list: the tiles of a block, whose stencils share halo data, then
+// meet in the same L2.
+struct WorkList {
+    const int2 *items;
+    int n;
+    int swz;
+};
+
+// One launch configuration: the work list built for `threads` per tile, the
+// workgroup count (<= wl.n) and the LDS the update kernel stages per tile.
+struct Launch {
+    WorkList wl;
+    int threads;
+    int grid;
+    int lds_cols;  // metric columns one tile of interior cells can touch (k_cons)
+};
+
+// Kernel classes: each has its own iteration space per block.
+enum KClass : int {
+    KC_FLUX1 = 0, KC_FLUX2, KC_FLUX3,  // faces of one direction (+ one transverse ghost layer)
+    KC_ECENT,                          // cells [NG-1, NG+BS]
+    KC_CORNER,                         // corners/faces [NG, NG+BS]   (corner EMF, face-B update)
+    KC_INTERIOR,                       // interior cells               (update, inversion, fallback)
+    KC_BOUND1, KC_BOUND2, KC_BOUND3,   // ghost layers of one direction
+    KC_N
+};
+// Threads a block needs in a kernel class.
+int64_t class_work(const BlockDev &B, int cls);
+// Metric columns a tile of `threads` interior cells of a block with bs3 cells
+// along z can touch.
+inline int cons_max_cols(int bs3, int threads) { return (threads - 1) / bs3 + 2; }
+
 // `mixed` selects the arithmetic of reconstruction, fluxes and EMFs: 0 = FP64
 // (F, Ec, E are double arrays), 1 = FP32 (float arrays with the same slab layout).
-void launch_flux(const KParams &P, int dir, const int *blist, int nb, const double *p, const double *ps, void *F,
+void launch_flux(const KParams &P, int dir, const Launch &L, const double *p, const double *ps, void *F,
                  unsigned long long *dtmin, int mixed, hipStream_t s);
-void launch_emf(const KParams &P, const int *blist, int nb, const double *p, const void *F1, const void *F2,
-                const void *F3, void *Ec, void *E, int mixed, hipStream_t s);
-void launch_pole(const KParams &P, const int *ring, int nring, double dt_val, const double *dt_ptr, void *E, int mixed,
-                 hipStream_t s);
-void launch_ct_update(const KParams &P, const int *blist, int nb, double Dt_val, const double *dt_ptr,
-                      double dt_mult, const double *psi, double *psf, const void *E, int mixed, hipStream_t s);
-void launch_update(const KParams &P, const int *blist, int nb, int full, double Dt_val, const double *dt_ptr,
-                   double dt_mult, const double *pi, const double *pb, double *pf, double *U0, double *Unew,
-                   const double *psf, const void *F1, const void *F2, const void *F3, int *pflag, int *failcnt,
-                   int mixed, hipStream_t s);
+void launch_emf(const KParams &P, const Launch &centre, const Launch &corner, const double *p, const void *F1,
+                const void *F2, const void *F3, void *Ec, void *E, int mixed, hipStream_t s);
+// Its two kernels one at a time (for timing and tuning).
+void launch_emf_centre(const KParams &P, const Launch &L, const double *p, void *Ec, int mixed, hipStream_t s);
+void launch_emf_corner(const KParams &P, const Launch &L, const void *Ec, const void *F1, const void *F2,
+                       const void *F3, void *E, int mixed, hipStream_t s);
+// Polar E1 ring average.  ring[r*len + b] lists the blocks of ring r in phi
+// order, rside[r] its side (0: lower x2 face is the axis, 1: upper); ni_max is
+// the largest bs1 + 1 over the rings.
+void launch_pole(const KParams &P, const int *ring, const int *rside, int nring, int len, int ni_max, double dt_val,
+                 const double *dt_ptr, void *E, int mixed, hipStream_t s);
+void launch_ct_update(const KParams &P, const Launch &L, double Dt_val, const double *dt_ptr, double dt_mult,
+                      const double *psi, double *psf, const void *E, int mixed, hipStream_t s);
+void launch_update(const KParams &P, const Launch &L, int full, double Dt_val, const double *dt_ptr, double dt_mult,
+                   const double *pi, const double *pb, double *pf, double *U0, double *Unew, const double *psf,
+                   const void *F1, const void *F2, const void *F3, int *pflag, int *failcnt, int mixed,
+                   hipStream_t s);
 // Conserved update only (k_cons, predictor form with step Dt): U(p) -> U0,
 // U(p) - Dt div F + Dt S(p) with the face average of psf -> Unew.
-void launch_cons(const KParams &P, const int *blist, int nb, double Dt, const double *p, double *U0, double *Unew,
+void launch_cons(const KParams &P, const Launch &L, double Dt, const double *p, double *U0, double *Unew,
                  const double *psf, const void *F1, const void *F2, const void *F3, int mixed, hipStream_t s);
+// The kernels of the update one at a time (k_cons, k_invert, k_fallback), for timing and tuning.
+void launch_update_part(const KParams &P, int part, const Launch &L, int full, double Dt_val, const double *dt_ptr,
+                        double dt_mult, const double *pi, const double *pb, double *pf, double *U0, double *Unew,
+                        const double *psf, const void *F1, const void *F2, const void *F3, int *pflag, int *failcnt,
+                        int mixed, hipStream_t s);
 // Adjoint of one forward stage of a single periodic flat block (FP64): from the
 // weights w of U' (9 slabs) to lP (9 slabs, dL/dP) and lps (3 slabs, dL/d face
 // field); F1-F3 are the stage's fluxes; lpsf, lE, lEc, lF1-lF3 are scratch.
@@ -47,10 +91,20 @@ void launch_adjoint(const KParams &P, const BlockDev &B, double Dt, const double
                     const double *F1, const double *F2, const double *F3, const double *w, double *lP, double *lps,
                     double *lpsf, double *lE, double *lEc, double *lF1, double *lF2, double *lF3, hipStream_t s);
 
-void launch_bound(const KParams &P, int dir, const int *blist, int nb, double *p, double *ps, int *pflag,
+// Ghost fill of one direction over the faces whose kind is FK_COPY, FK_OUTFLOW
+// or FK_POLE (FK_AMR faces are left to the exchange tables).  With
+// outflow_only = 1 only radial outflow faces are filled, and not at the ghost
+// rows of a polar face (the final pass of the multi-level fill).
+void launch_bound(const KParams &P, int dir, const Launch &L, double *p, double *ps, int *pflag, int outflow_only,
                   hipStream_t s);
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);
 void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s);
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
+
+// Multi-level ghost exchange of one stage: copy/restriction entries, then
+// prolongation entries (they write disjoint ghost cells and read only interior
+// cells and ghosts of earlier stages).  grid <= 0: one thread per entry.
+void launch_amr_exchange(const KParams &P, const AmrEntry *e, int64_t n, const AmrProlong *pe, int64_t npro,
+                         double *p, double *ps, int *pflag, int threads, int grid, hipStream_t s);
 
 }  // namespace hamr

@@ -69,6 +69,25 @@ std::vector<BlockDev> build_block_table(const GridDesc &d, int *ngeo_out)
                 b.nbr[5] = id(c1, c2, (c3 + 1) % nb3);
                 b.kind[5] = FK_COPY;
             }
+    // per-block layout: uniform sizes, slabs in natural order
+    const Layout L = make_layout(d);
+    const GridCoords gc = grid_coords(d);
+    for (int n = 0; n < (int)t.size(); n++) {
+        BlockDev &b = t[n];
+        b.level = 0;
+        b.o[0] = b.c1 * d.bs[0];
+        b.o[1] = b.c2 * d.bs[1];
+        b.o[2] = b.c3 * d.bs[2];
+        for (int k = 0; k < 3; k++) {
+            b.bs[k] = d.bs[k];
+            b.D[k] = L.D[k];
+            b.dx[k] = gc.dx[k];
+        }
+        b.ks = (int)L.ksize;
+        b.kcol = (int)L.kcol;
+        b.off = (int64_t)n * L.ksize;
+        b.moff = (int64_t)b.geo * L.kcol;
+    }
     if (ngeo_out) *ngeo_out = nb1 * nb2;
     return t;
 }
@@ -302,22 +321,30 @@ void conn_code(const GridDesc &d, const double *X, const double gcon[4][4], doub
 
 void coord_of(const GridDesc &d, const GridCoords &gc, int i, int j, int pos, double X[4])
 {
-    const int N2 = d.nb[1] * d.bs[1];
+    coord_of_level(d, gc, 0, i, j, pos, X);
+}
+
+void coord_of_level(const GridDesc &d, const GridCoords &gc, int level, int i, int j, int pos, double X[4])
+{
+    // spacings of the level: exact halvings of the base spacing
+    const double sc = std::ldexp(1.0, -level);
+    const double dx0 = gc.dx[0] * sc, dx1 = gc.dx[1] * sc, dx2 = gc.dx[2] * sc;
+    const int N2 = (d.nb[1] * d.bs[1]) << level;
     X[0] = 0.;
-    X[3] = gc.startx[2] + 0.5 * gc.dx[2];
+    X[3] = gc.startx[2] + 0.5 * dx2;
     if (d.minkowski) {
-        X[1] = gc.startx[0] + ((pos == FACE1 || pos == CORN) ? i : i + 0.5) * gc.dx[0];
-        X[2] = gc.startx[1] + ((pos == FACE2 || pos == CORN) ? j : j + 0.5) * gc.dx[1];
+        X[1] = gc.startx[0] + ((pos == FACE1 || pos == CORN) ? i : i + 0.5) * dx0;
+        X[2] = gc.startx[1] + ((pos == FACE2 || pos == CORN) ? j : j + 0.5) * dx1;
         return;
     }
     int jl = j;
     if (j < 0) jl = -j - 1;
     if (j >= N2) jl = 2 * N2 - 1 - j;
     if (j == N2 && pos == FACE2) jl = j;
-    if (pos == FACE1) { X[1] = gc.startx[0] + i * gc.dx[0]; X[2] = gc.startx[1] + (jl + 0.5) * gc.dx[1]; }
-    else if (pos == FACE2) { X[1] = gc.startx[0] + (i + 0.5) * gc.dx[0]; X[2] = gc.startx[1] + jl * gc.dx[1]; }
-    else if (pos == CORN) { X[1] = gc.startx[0] + i * gc.dx[0]; X[2] = gc.startx[1] + jl * gc.dx[1]; }
-    else { X[1] = gc.startx[0] + (i + 0.5) * gc.dx[0]; X[2] = gc.startx[1] + (jl + 0.5) * gc.dx[1]; }
+    if (pos == FACE1) { X[1] = gc.startx[0] + i * dx0; X[2] = gc.startx[1] + (jl + 0.5) * dx1; }
+    else if (pos == FACE2) { X[1] = gc.startx[0] + (i + 0.5) * dx0; X[2] = gc.startx[1] + jl * dx1; }
+    else if (pos == CORN) { X[1] = gc.startx[0] + i * dx0; X[2] = gc.startx[1] + jl * dx1; }
+    else { X[1] = gc.startx[0] + (i + 0.5) * dx0; X[2] = gc.startx[1] + (jl + 0.5) * dx1; }
     if (j < 0) {
         X[2] = X[2] + 1;
         X[2] = -X[2];
@@ -339,26 +366,32 @@ static void pack10(const double g[4][4], double *o, int64_t stride)
 void fill_geometry(const GridDesc &d, const Layout &L, int c1, int c2, double *gcov, double *gcon,
                    double *gdet, double *conn, double *rcen)
 {
+    fill_geometry_level(d, 0, c1 * d.bs[0], c2 * d.bs[1], L.D[0], L.D[1], L.kcol, gcov, gcon, gdet, conn, rcen);
+}
+
+void fill_geometry_level(const GridDesc &d, int level, int o1, int o2, int D1, int D2, int64_t kcol, double *gcov,
+                         double *gcon, double *gdet, double *conn, double *rcen)
+{
     GridCoords gc = grid_coords(d);
     static const int pi[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, pj[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
 #pragma omp parallel for collapse(2) schedule(dynamic, 4)
-    for (int il = 0; il < L.D[0]; il++)
-        for (int jl = 0; jl < L.D[1]; jl++) {
-            const int i = c1 * d.bs[0] + il - NG, j = c2 * d.bs[1] + jl - NG;
-            const int64_t col = (int64_t)il * L.D[1] + jl;
+    for (int il = 0; il < D1; il++)
+        for (int jl = 0; jl < D2; jl++) {
+            const int i = o1 + il - NG, j = o2 + jl - NG;
+            const int64_t col = (int64_t)il * D2 + jl;
             for (int pos = 0; pos < NPOS; pos++) {
                 double X[4], g[4][4], gi[4][4];
-                coord_of(d, gc, i, j, pos, X);
+                coord_of_level(d, gc, level, i, j, pos, X);
                 metric_code(d, X, g);
                 invert4(g, gi);
-                pack10(g, gcov + ((int64_t)pos * 10) * L.kcol + col, L.kcol);
-                pack10(gi, gcon + ((int64_t)pos * 10) * L.kcol + col, L.kcol);
-                gdet[(int64_t)pos * L.kcol + col] = gdet_of(g);
+                pack10(g, gcov + ((int64_t)pos * 10) * kcol + col, kcol);
+                pack10(gi, gcon + ((int64_t)pos * 10) * kcol + col, kcol);
+                gdet[(int64_t)pos * kcol + col] = gdet_of(g);
                 if (pos == CENT) {
                     double cn[4][4][4], r, th;
                     conn_code(d, X, gi, cn);
                     for (int lam = 0; lam < 4; lam++)
-                        for (int s = 0; s < 10; s++) conn[((int64_t)lam * 10 + s) * L.kcol + col] = cn[lam][pi[s]][pj[s]];
+                        for (int s = 0; s < 10; s++) conn[((int64_t)lam * 10 + s) * kcol + col] = cn[lam][pi[s]][pj[s]];
                     if (d.minkowski) rcen[col] = 1.;
                     else {
                         bl_coord(d, X, &r, &th);
@@ -460,43 +493,41 @@ static void torus_velocity(const GridDesc &d, const TorusEval &T, const double *
     (void)gcov;
 }
 
-std::string init_torus(const GridDesc &d, const Layout &L, const std::vector<BlockDev> &blocks,
-                       const TorusDesc &t, const double *gcovA, const double *gconA, const double *gdetA,
-                       double *p, double *ps)
+std::string init_torus(const GridDesc &d, const std::vector<BlockDev> &blocks, const TorusDesc &t,
+                       const double *gcovA, const double *gconA, const double *gdetA, double *p, double *ps)
 {
     if (d.minkowski) return "init_torus: grid is a Minkowski box";
     const GridCoords gc = grid_coords(d);
     const TorusEval T = make_torus(d, t);
     const int N2 = d.nb[1] * d.bs[1], N3 = d.nb[2] * d.bs[2];
-    const int D1 = L.D[0], D2 = L.D[1], D3 = L.D[2];
-    auto cell = [&](int i, int j, int z) { return ((int64_t)i * D2 + j) * D3 + z; };
-    auto P = [&](int b, int k) { return p + ((int64_t)b * NPR + k) * L.ksize; };
-    auto PS = [&](int b, int k) { return ps + ((int64_t)b * 3 + k) * L.ksize; };
-    auto metric_at = [&](int geo, int pos, int64_t col, double g[4][4], double gi[4][4], double *gd) {
+    auto cell = [&](const BlockDev &B, int i, int j, int z) { return ((int64_t)i * B.D[1] + j) * B.D[2] + z; };
+    auto P = [&](const BlockDev &B, int k) { return p + NPR * B.off + (int64_t)k * B.ks; };
+    auto PS = [&](const BlockDev &B, int k) { return ps + 3 * B.off + (int64_t)k * B.ks; };
+    auto gdet_at = [&](const BlockDev &B, int pos, int64_t col) { return gdetA[B.moff * NPOS + (int64_t)pos * B.kcol + col]; };
+    auto metric_at = [&](const BlockDev &B, int pos, int64_t col, double g[4][4], double gi[4][4]) {
         static const int pi[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, pj[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
         for (int s = 0; s < 10; s++) {
-            g[pi[s]][pj[s]] = g[pj[s]][pi[s]] = gcovA[(((int64_t)geo * NPOS + pos) * 10 + s) * L.kcol + col];
-            gi[pi[s]][pj[s]] = gi[pj[s]][pi[s]] = gconA[(((int64_t)geo * NPOS + pos) * 10 + s) * L.kcol + col];
+            const int64_t e = B.moff * NPOS * 10 + ((int64_t)pos * 10 + s) * B.kcol + col;
+            g[pi[s]][pj[s]] = g[pj[s]][pi[s]] = gcovA[e];
+            gi[pi[s]][pj[s]] = gi[pj[s]][pi[s]] = gconA[e];
         }
-        if (gd) *gd = gdetA[((int64_t)geo * NPOS + pos) * L.kcol + col];
     };
-    (void)N2;
+    const int nblk = (int)blocks.size();
     // pass 1: densities and internal energies, rho_max
     double rhomax = 0.;
-    const int nblk = L.nblk;
     for (int b = 0; b < nblk; b++) {
         const BlockDev &B = blocks[b];
-        for (int il = NG; il < NG + d.bs[0]; il++)
-            for (int jl = NG; jl < NG + d.bs[1]; jl++) {
-                const int i = B.c1 * d.bs[0] + il - NG, j = B.c2 * d.bs[1] + jl - NG;
+        for (int il = NG; il < NG + B.bs[0]; il++)
+            for (int jl = NG; jl < NG + B.bs[1]; jl++) {
+                const int i = B.o[0] + il - NG, j = B.o[1] + jl - NG;
                 double X[4], r, th, u;
-                coord_of(d, gc, i, j, CENT, X);
+                coord_of_level(d, gc, B.level, i, j, CENT, X);
                 bl_coord(d, X, &r, &th);
                 double rho = torus_rho(T, r, th, &u);
-                for (int zl = NG; zl < NG + d.bs[2]; zl++) {
-                    const int64_t c = cell(il, jl, zl);
-                    P(b, RHO)[c] = rho;
-                    P(b, UU)[c] = u;
+                for (int zl = NG; zl < NG + B.bs[2]; zl++) {
+                    const int64_t c = cell(B, il, jl, zl);
+                    P(B, RHO)[c] = rho;
+                    P(B, UU)[c] = u;
                 }
                 rhomax = std::max(rhomax, rho);
             }
@@ -505,47 +536,50 @@ std::string init_torus(const GridDesc &d, const Layout &L, const std::vector<Blo
     // pass 2: velocities, perturbation, normalisation, atmosphere
     for (int b = 0; b < nblk; b++) {
         const BlockDev &B = blocks[b];
-        for (int il = NG; il < NG + d.bs[0]; il++)
-            for (int jl = NG; jl < NG + d.bs[1]; jl++) {
-                const int i = B.c1 * d.bs[0] + il - NG, j = B.c2 * d.bs[1] + jl - NG;
-                const int64_t col = (int64_t)il * D2 + jl;
+        const int N2l = N2 << B.level, N3l = N3 << B.level;
+        // the perturbation's cell key: global cell index on the block's level
+        const uint64_t lkey = B.level ? hash64(0xA5A5A5A5ULL + (uint64_t)B.level) : 0ULL;
+        for (int il = NG; il < NG + B.bs[0]; il++)
+            for (int jl = NG; jl < NG + B.bs[1]; jl++) {
+                const int i = B.o[0] + il - NG, j = B.o[1] + jl - NG;
+                const int64_t col = (int64_t)il * B.D[1] + jl;
                 double X[4], r, th, g[4][4], gi[4][4], ut[4] = {0., 0., 0., 0.};
-                coord_of(d, gc, i, j, CENT, X);
+                coord_of_level(d, gc, B.level, i, j, CENT, X);
                 bl_coord(d, X, &r, &th);
-                metric_at(B.geo, CENT, col, g, gi, nullptr);
-                const double rho_t = P(b, RHO)[cell(il, jl, NG)];
+                metric_at(B, CENT, col, g, gi);
+                const double rho_t = P(B, RHO)[cell(B, il, jl, NG)];
                 const bool in = rho_t > 0.;
                 if (in) torus_velocity(d, T, X, r, th, g, gi, ut);
                 const double sr = std::sqrt(r);
                 const double rhoflr = d.rho_floor / (r * sr), uflr = d.u_floor / (r * r * sr);
-                for (int zl = NG; zl < NG + d.bs[2]; zl++) {
-                    const int64_t c = cell(il, jl, zl);
-                    const int zg = B.c3 * d.bs[2] + zl - NG;
+                for (int zl = NG; zl < NG + B.bs[2]; zl++) {
+                    const int64_t c = cell(B, il, jl, zl);
+                    const int zg = B.o[2] + zl - NG;
                     double rho, u;
                     if (in) {
-                        const uint64_t gid = ((uint64_t)(i + 8) * (uint64_t)(N2 + 16) + (uint64_t)(j + 8)) * (uint64_t)(N3 + 16) + (uint64_t)zg;
-                        const double U01 = (double)(hash64(t.seed ^ (gid * 0x9E3779B97F4A7C15ULL)) >> 11) * (1.0 / 9007199254740992.0);
-                        rho = P(b, RHO)[c] / rhomax;
-                        u = P(b, UU)[c] * (1. + t.perturb * (U01 - 0.5)) / rhomax;
+                        const uint64_t gid = ((uint64_t)(i + 8) * (uint64_t)(N2l + 16) + (uint64_t)(j + 8)) * (uint64_t)(N3l + 16) + (uint64_t)zg;
+                        const double U01 = (double)(hash64(t.seed ^ (gid * 0x9E3779B97F4A7C15ULL) ^ lkey) >> 11) * (1.0 / 9007199254740992.0);
+                        rho = P(B, RHO)[c] / rhomax;
+                        u = P(B, UU)[c] * (1. + t.perturb * (U01 - 0.5)) / rhomax;
                     } else {
                         rho = 0.;
                         u = 0.;
                     }
                     if (!(rho >= rhoflr)) rho = rhoflr;
                     if (!(u >= uflr)) u = uflr;
-                    P(b, RHO)[c] = rho;
-                    P(b, UU)[c] = u;
-                    P(b, U1)[c] = in ? ut[1] : 0.;
-                    P(b, U2)[c] = in ? ut[2] : 0.;
-                    P(b, U3)[c] = in ? ut[3] : 0.;
-                    P(b, KTOT)[c] = (d.gam - 1.) * u * std::pow(rho, -d.gam);
+                    P(B, RHO)[c] = rho;
+                    P(B, UU)[c] = u;
+                    P(B, U1)[c] = in ? ut[1] : 0.;
+                    P(B, U2)[c] = in ? ut[2] : 0.;
+                    P(B, U3)[c] = in ? ut[3] : 0.;
+                    P(B, KTOT)[c] = (d.gam - 1.) * u * std::pow(rho, -d.gam);
                 }
             }
     }
     // pass 3: vector potential on corners and face fields (unnormalised)
-    auto aphi = [&](int i, int j) {
+    auto aphi = [&](int level, int i, int j) {
         double X[4], r, th;
-        coord_of(d, gc, i, j, CORN, X);
+        coord_of_level(d, gc, level, i, j, CORN, X);
         bl_coord(d, X, &r, &th);
         double rho = torus_rho(T, r, th, nullptr);
         if (rho < 0.) return 0.;
@@ -553,63 +587,63 @@ std::string init_torus(const GridDesc &d, const Layout &L, const std::vector<Blo
     };
     for (int b = 0; b < nblk; b++) {
         const BlockDev &B = blocks[b];
-        for (int il = NG; il <= NG + d.bs[0]; il++)
-            for (int jl = NG; jl <= NG + d.bs[1]; jl++) {
-                const int i = B.c1 * d.bs[0] + il - NG, j = B.c2 * d.bs[1] + jl - NG;
-                const int64_t col = (int64_t)il * D2 + jl;
-                const double A00 = aphi(i, j), A01 = aphi(i, j + 1), A10 = aphi(i + 1, j);
-                const double gF1 = gdetA[((int64_t)B.geo * NPOS + FACE1) * L.kcol + col];
-                const double gF2 = gdetA[((int64_t)B.geo * NPOS + FACE2) * L.kcol + col];
-                const double b1 = (A01 - A00) / (gc.dx[1] * gF1);
-                const double b2 = -(A10 - A00) / (gc.dx[0] * gF2);
-                for (int zl = NG; zl <= NG + d.bs[2]; zl++) {
-                    const int64_t c = cell(il, jl, zl);
-                    if (jl < NG + d.bs[1] && zl < NG + d.bs[2]) PS(b, 0)[c] = b1;
-                    if (il < NG + d.bs[0] && zl < NG + d.bs[2]) PS(b, 1)[c] = b2;
-                    if (il < NG + d.bs[0] && jl < NG + d.bs[1]) PS(b, 2)[c] = 0.;
+        const double sc = std::ldexp(1.0, -B.level);
+        const double dx1 = gc.dx[0] * sc, dx2 = gc.dx[1] * sc;
+        for (int il = NG; il <= NG + B.bs[0]; il++)
+            for (int jl = NG; jl <= NG + B.bs[1]; jl++) {
+                const int i = B.o[0] + il - NG, j = B.o[1] + jl - NG;
+                const int64_t col = (int64_t)il * B.D[1] + jl;
+                const double A00 = aphi(B.level, i, j), A01 = aphi(B.level, i, j + 1), A10 = aphi(B.level, i + 1, j);
+                const double gF1 = gdet_at(B, FACE1, col);
+                const double gF2 = gdet_at(B, FACE2, col);
+                const double b1 = (A01 - A00) / (dx2 * gF1);
+                const double b2 = -(A10 - A00) / (dx1 * gF2);
+                for (int zl = NG; zl <= NG + B.bs[2]; zl++) {
+                    const int64_t c = cell(B, il, jl, zl);
+                    if (jl < NG + B.bs[1] && zl < NG + B.bs[2]) PS(B, 0)[c] = b1;
+                    if (il < NG + B.bs[0] && zl < NG + B.bs[2]) PS(B, 1)[c] = b2;
+                    if (il < NG + B.bs[0] && jl < NG + B.bs[1]) PS(B, 2)[c] = 0.;
                 }
             }
     }
     // cell-centred field from the faces, as the update kernel forms it
-    auto centre_field = [&](int b) {
-        const BlockDev &B = blocks[b];
-        for (int il = NG; il < NG + d.bs[0]; il++)
-            for (int jl = NG; jl < NG + d.bs[1]; jl++) {
-                const int64_t col = (int64_t)il * D2 + jl;
-                const double *gd = gdetA + (int64_t)B.geo * NPOS * L.kcol;
-                const double gF1a = gd[FACE1 * L.kcol + col], gF1b = gd[FACE1 * L.kcol + col + D2];
-                const double gF2a = gd[FACE2 * L.kcol + col], gF2b = gd[FACE2 * L.kcol + col + 1];
-                const double gF3 = gd[FACE3 * L.kcol + col], gC = gd[CENT * L.kcol + col];
-                for (int zl = NG; zl < NG + d.bs[2]; zl++) {
-                    const int64_t c = cell(il, jl, zl);
-                    const int64_t si = (int64_t)D2 * D3, sj = D3;
-                    double U1f = (PS(b, 0)[c] * gF1a + PS(b, 0)[c + si] * gF1b) / 2.0;
-                    double U2f = (PS(b, 1)[c] * gF2a + PS(b, 1)[c + sj] * gF2b) / 2.0;
-                    double U3f = (PS(b, 2)[c] * gF3 + PS(b, 2)[c + 1] * gF3) / 2.0;
-                    P(b, B1)[c] = U1f / gC;
-                    P(b, B2)[c] = U2f / gC;
-                    P(b, B3)[c] = U3f / gC;
+    auto centre_field = [&](const BlockDev &B) {
+        for (int il = NG; il < NG + B.bs[0]; il++)
+            for (int jl = NG; jl < NG + B.bs[1]; jl++) {
+                const int64_t col = (int64_t)il * B.D[1] + jl;
+                const double gF1a = gdet_at(B, FACE1, col), gF1b = gdet_at(B, FACE1, col + B.D[1]);
+                const double gF2a = gdet_at(B, FACE2, col), gF2b = gdet_at(B, FACE2, col + 1);
+                const double gF3 = gdet_at(B, FACE3, col), gC = gdet_at(B, CENT, col);
+                for (int zl = NG; zl < NG + B.bs[2]; zl++) {
+                    const int64_t c = cell(B, il, jl, zl);
+                    const int64_t si = (int64_t)B.D[1] * B.D[2], sj = B.D[2];
+                    double U1f = (PS(B, 0)[c] * gF1a + PS(B, 0)[c + si] * gF1b) / 2.0;
+                    double U2f = (PS(B, 1)[c] * gF2a + PS(B, 1)[c + sj] * gF2b) / 2.0;
+                    double U3f = (PS(B, 2)[c] * gF3 + PS(B, 2)[c + 1] * gF3) / 2.0;
+                    P(B, B1)[c] = U1f / gC;
+                    P(B, B2)[c] = U2f / gC;
+                    P(B, B3)[c] = U3f / gC;
                 }
             }
     };
-    for (int b = 0; b < nblk; b++) centre_field(b);
+    for (int b = 0; b < nblk; b++) centre_field(blocks[b]);
     // pass 4: normalise min beta over field-carrying cells
     double beta_min = 1e300;
     for (int b = 0; b < nblk; b++) {
         const BlockDev &B = blocks[b];
-        for (int il = NG; il < NG + d.bs[0]; il++)
-            for (int jl = NG; jl < NG + d.bs[1]; jl++) {
-                const int64_t col = (int64_t)il * D2 + jl;
+        for (int il = NG; il < NG + B.bs[0]; il++)
+            for (int jl = NG; jl < NG + B.bs[1]; jl++) {
+                const int64_t col = (int64_t)il * B.D[1] + jl;
                 double g[4][4], gi[4][4];
-                metric_at(B.geo, CENT, col, g, gi, nullptr);
+                metric_at(B, CENT, col, g, gi);
                 Geom<double> G;
                 static const int pi[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, pj[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
                 for (int s = 0; s < 10; s++) { G.gcov[s] = g[pi[s]][pj[s]]; G.gcon[s] = gi[pi[s]][pj[s]]; }
                 G.g = 1.;
-                for (int zl = NG; zl < NG + d.bs[2]; zl++) {
-                    const int64_t c = cell(il, jl, zl);
+                for (int zl = NG; zl < NG + B.bs[2]; zl++) {
+                    const int64_t c = cell(B, il, jl, zl);
                     double pr[NPR];
-                    for (int k = 0; k < NPR; k++) pr[k] = P(b, k)[c];
+                    for (int k = 0; k < NPR; k++) pr[k] = P(B, k)[c];
                     State<double> q;
                     get_state(pr, G, q);
                     double bsq = dot4(q.bcon, q.bcov);
@@ -621,10 +655,10 @@ std::string init_torus(const GridDesc &d, const Layout &L, const std::vector<Blo
         const double norm = std::sqrt(beta_min / t.beta_min);
         for (int b = 0; b < nblk; b++)
             for (int k = 0; k < 3; k++) {
-                double *f = PS(b, k);
-                for (int64_t c = 0; c < L.ncell; c++) f[c] *= norm;
+                double *f = PS(blocks[b], k);
+                for (int64_t c = 0; c < blocks[b].ks; c++) f[c] *= norm;
             }
-        for (int b = 0; b < nblk; b++) centre_field(b);
+        for (int b = 0; b < nblk; b++) centre_field(blocks[b]);
     }
     return "";
 }

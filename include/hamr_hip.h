@@ -41,6 +41,21 @@ typedef struct hamr_desc {
     int device;         /* HIP device ordinal */
 } hamr_desc;
 
+/* Multi-level block set (no reference counterpart as a descriptor: the
+ * reference builds its refined block tree at run time, AMR.c:168-757, with
+ * uniform block sizes).  Level ratio 2; level l >= 1 covers
+ * [box[l][k][0], box[l][k][1]) cells of level l along direction k (multiples of
+ * 32); the base level is the grid of hamr_desc (nb * bs cells, a multiple of
+ * 64x32x32).  Each level is tiled by 64x32x32 superblocks, each kept whole
+ * (probability p_big), split into 32x16x16 blocks (p_mid) or into 16^3 blocks,
+ * by a seeded choice. */
+typedef struct hamr_amr_desc {
+    int nlev;
+    int box[3][3][2];
+    unsigned long long seed;
+    double p_big, p_mid;
+} hamr_amr_desc;
+
 /* Fishbone-Moncrief torus with a poloidal loop (init.c:808-1110, init_mag.c:6). */
 typedef struct hamr_torus_desc {
     double rin, rmax, kappa;  /* inner edge, pressure maximum, polytropic constant */
@@ -88,6 +103,24 @@ int hamr_desc_coords(const hamr_desc *desc, double *startx, double *dx);
  * pair s at lam*10+s, and Boyer-Lindquist r, theta (bl_coord, coord.c:111). */
 int hamr_desc_metric_at(const hamr_desc *desc, const double *X, double *gcov, double *gcon, double *gdet,
                         double *conn, double *r_th);
+/* The multi-level block set of `amr` without a device (as hamr_create_amr
+ * builds it): info[12] as hamr_amr_info; rows (15 ints per block, as
+ * hamr_block_layout), the copy/restriction entries of stages 1-3 one after
+ * another (4 ints: ghost block, ghost cell, source block, source cell |
+ * kind << 28 with kind 1 = cell, 2 = mean of the 2x2x2 cells from there) and
+ * the prolongation entries (16 ints: ghost block and cell, coarse block and
+ * cell, then (block, code) of the coarse cell's -x1 +x1 -x2 +x2 -x3 +x3
+ * neighbours, kind 0 = none); any array may be NULL.  Returns -2 if the
+ * description is rejected. */
+int hamr_desc_amr(const hamr_desc *desc, const hamr_amr_desc *amr, long long *info, int *rows, int *ex, int *pro);
+/* Geometry slab geo of that block set (device layout, kcol = its padded
+ * D1*D2; hamr_geometry_info gives the sizes on a context) and its torus initial
+ * state (unpadded whole-array layout, as hamr_get_array on a multi-level
+ * context), without a device. */
+int hamr_desc_amr_geometry(const hamr_desc *desc, const hamr_amr_desc *amr, int geo, double *gcov, double *gcon,
+                           double *gdet, double *conn, double *rcen);
+int hamr_desc_amr_torus(const hamr_desc *desc, const hamr_amr_desc *amr, const hamr_torus_desc *t, double *p,
+                        double *ps);
 /* One geometry slab in the device layout (as hamr_upload_metric takes it). */
 int hamr_desc_geometry(const hamr_desc *desc, int geo, double *gcov, double *gcon, double *gdet, double *conn,
                        double *rcen);
@@ -97,6 +130,26 @@ int hamr_desc_torus(const hamr_desc *desc, const hamr_torus_desc *t, double *p, 
 /* GPU_init (GPU_main.cu:10) + set_AMR (AMR.c:473): allocate the resident block
  * set and the block table. */
 int hamr_create(const hamr_desc *desc, hamr_ctx **out);
+/* A context over the multi-level block set of `amr` on the torus grid of
+ * `desc` (every block resident, one slab per block and array).  All entry
+ * points below work on it; the ghost fill (hamr_boundprim) then also exchanges
+ * across block sizes and levels: copies, restriction by 2x2x2 means
+ * (packsendaverage1, GPU_program2.cu:154-197) and MC-limited linear
+ * prolongation (unpackreceivecoarse1, GPU_program2.cu:615-803).  Host layout
+ * of whole arrays: the blocks' [component][cell] slabs one after another, each
+ * with its own (bs+6)^3 cell count. */
+int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *amr, hamr_ctx **out);
+/* info[12]: blocks, geometry slabs, padded cells, interior zones, exchange
+ * entries (copy/restriction) of stages 1-3, prolongation entries of stages
+ * 1-3, polar rings, 1 if multi-level. */
+int hamr_amr_info(const hamr_ctx *ctx, long long *info);
+/* row[15]: level, origin o1 o2 o3 (cells of its level), bs1 bs2 bs3, geometry
+ * slab, pole bits, kinds of faces -x1..+x3 (0 copy, 1 outflow, 2 polar mirror,
+ * 3 exchange table). */
+int hamr_block_layout(const hamr_ctx *ctx, int n, int *row);
+/* info[6]: level, first column o1 o2 (cells of its level, interior origin),
+ * D1, D2, kcol of geometry slab geo. */
+int hamr_geometry_info(const hamr_ctx *ctx, int geo, long long *info);
 int hamr_destroy(hamr_ctx *ctx);
 const char *hamr_last_error(const hamr_ctx *ctx);
 
@@ -184,6 +237,23 @@ int hamr_sync(hamr_ctx *ctx);
  * modified): 0-2 flux x1..x3, 3 EMF, 4 polar average, 5 face-B update,
  * 6 conserved update + inversion, 7 ghost fill (3 directions). */
 int hamr_time_stage(hamr_ctx *ctx, int stage, int nrep, double *avg_ms);
+/* ---- launch autotuner (no reference counterpart) ----
+ * Every kernel of the step walks a work list of (block, tile) items.  For each
+ * kernel (HAMR_TUNE_*) the tuner times, on the current state, the tile sizes
+ * 64/128/256 threads, grids of one workgroup per item or 1024/2048/4096
+ * persistent workgroups, and item orders plain or XCD-contiguous, and keeps the
+ * fastest (nrep timed launches each).  Results do not depend on the choice.
+ * The step graph is re-captured after a change. */
+enum {
+    HAMR_TUNE_FLUX1 = 0, HAMR_TUNE_FLUX2, HAMR_TUNE_FLUX3, HAMR_TUNE_ECENT, HAMR_TUNE_ECORNER, HAMR_TUNE_CT,
+    HAMR_TUNE_CONS, HAMR_TUNE_INVERT, HAMR_TUNE_FALLBACK, HAMR_TUNE_BOUND1, HAMR_TUNE_BOUND2, HAMR_TUNE_BOUND3,
+    HAMR_TUNE_EXCHANGE, HAMR_TUNE_N
+};
+int hamr_autotune(hamr_ctx *ctx, int nrep);
+/* cfg[3]: threads per workgroup, workgroups (0: one per item), XCD order; ms: tuned time (-1: not tuned). */
+int hamr_tune_get(const hamr_ctx *ctx, int slot, int *cfg, double *ms);
+int hamr_tune_set(hamr_ctx *ctx, int slot, const int *cfg);
+
 /* ---- one forward stage and its adjoint (no reference counterpart) ----
  * Forward stage: the predictor's flux, EMF, face-field and conserved updates
  * with the full step Dt, from p/ps (ghosts filled first):

@@ -1,0 +1,515 @@
+/*
+ * hamr_amr_oracle.c -- FP64 CPU restatement of the step on a multi-level set
+ * of blocks of uneven sizes (level ratio 2).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker for the HIP step; see hamr_oracle.h).
+ *
+ * Each block is advanced by the single-grid stage functions of hamr_oracle.c on
+ * its own ghost-inclusive arrays (its level's spacing, its metric); the polar
+ * E1 ring average runs over the blocks of each ring (E_average,
+ * const_trans.c:268-543) and the ghost fill is the multi-level exchange below.
+ * Everything is found by searching the block list (which leaf holds a
+ * position, which block is a polar block's partner, which blocks form a ring),
+ * independently of the device tables.
+ *
+ * Ghost fill, in the order of bound_prim (bounds.c:11-105), stage d filling the
+ * ghosts along d over the full extent of the earlier directions and the
+ * interior of the later ones:
+ *   1. radial outflow faces (inflow check at the ghost's geometry), then the
+ *      exchange of the internal x1 faces;
+ *   2. polar faces (mirror from the pi-shifted partner), then the exchange of
+ *      the internal x2 faces;
+ *   3. the exchange of the x3 faces (periodic);
+ *   4. radial outflow again over every row except the polar ghost rows (the
+ *      radial ghosts of stages 2 and 3 are the outflow image of the block's own
+ *      filled edge cells).
+ * Exchange of a ghost cell at position Y (cells of its block's level l, x3
+ * wrapped): in a polar ghost row (stage 3), a copy of the same-level leaf
+ * holding the row's clamped position, at Y; beyond the radial edge along an
+ * earlier direction, nothing (step 4); otherwise, for the leaf L holding Y:
+ *   level l    : copy of the cell;
+ *   level l+1  : restriction, the mean of the 2x2x2 children (packsendaverage1,
+ *                GPU_program2.cu:154-197); face fields: the mean of the four
+ *                children's faces that tile the coarse face;
+ *   level l-1  : prolongation (unpackreceivecoarse1, GPU_program2.cu:615-803):
+ *                coarse value + 0.25 s_d dq_d summed over d, s_d = -1 on the
+ *                lower and +1 on the upper child, dq_d the MC slope
+ *                (slope_lim_2, GPU_program2.cu:10-45) of the coarse cell and
+ *                its two neighbours along d (the neighbour's coarse-level value:
+ *                a cell, the mean of finer children, or the cell of a coarser
+ *                leaf; beyond the radial edge or the axis: no slope); a face
+ *                field along a has no slope term along a and starts from the
+ *                coarse face (even child) or the coarse cell-centred field (odd
+ *                child, the face that halves the coarse cell).
+ * pflag of a ghost: the source cell's (first child / coarse cell).
+ */
+#include "hamr_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NG ORC_NG
+#define NPR ORC_NPR
+enum { RHO = 0, UU = 1, U1 = 2, U2 = 3, U3 = 4, B1 = 5, B2 = 6, B3 = 7, KTOT = 8 };
+enum { FACE1 = 0, FACE2 = 1, CORN = 2, CENT = 3, FACE3 = 4 };
+#define NV (NPR + 3)
+
+typedef struct {
+    int D[3];
+    long ncell;
+} bgeo;
+
+static bgeo geo_of(const oracle_amr_block *B)
+{
+    bgeo r;
+    for (int k = 0; k < 3; k++) r.D[k] = B->bs[k] + 2 * NG;
+    r.ncell = (long)r.D[0] * r.D[1] * r.D[2];
+    return r;
+}
+
+static long cell_local(const oracle_amr_block *B, int i, int j, int z)
+{
+    return ((long)i * (B->bs[1] + 2 * NG) + j) * (B->bs[2] + 2 * NG) + z;
+}
+
+/* cell of block B at position Y of B's level (may be a ghost) */
+static long cell_at(const oracle_amr_block *B, const int Y[3])
+{
+    return cell_local(B, Y[0] - B->o[0] + NG, Y[1] - B->o[1] + NG, Y[2] - B->o[2] + NG);
+}
+
+static int inside(const oracle_amr_block *B, const int Y[3])
+{
+    for (int k = 0; k < 3; k++)
+        if (Y[k] < B->o[k] || Y[k] >= B->o[k] + B->bs[k]) return 0;
+    return 1;
+}
+
+/* the leaf holding cell Y of level l (Y inside the grid); *lc = its level */
+static int find_leaf(const oracle_amr *g, int l, const int Y[3], int *lc)
+{
+    for (int b = 0; b < g->nblk; b++) {
+        const oracle_amr_block *B = &g->blk[b];
+        int Z[3], s = B->level - l;
+        for (int k = 0; k < 3; k++) Z[k] = s >= 0 ? (Y[k] << s) : (Y[k] >> (-s));
+        if (inside(B, Z)) {
+            *lc = B->level;
+            return b;
+        }
+    }
+    return -1;
+}
+
+static int wrap(int y, int n) { return ((y % n) + n) % n; }
+
+/* all 12 values of one cell */
+static void cell_values(const oracle_amr *g, int b, long c, double **p, double **ps, double *v)
+{
+    const long N = geo_of(&g->blk[b]).ncell;
+    for (int k = 0; k < NPR; k++) v[k] = p[b][k * N + c];
+    for (int a = 0; a < 3; a++) v[NPR + a] = ps[b][a * N + c];
+}
+
+/* restriction: mean of the 2x2x2 cells of block b from cell c (lowest corner) */
+static void avg8_values(const oracle_amr *g, int b, long c, double **p, double **ps, double *v)
+{
+    const oracle_amr_block *B = &g->blk[b];
+    const long N = geo_of(B).ncell, sj = B->bs[2] + 2 * NG, si = (long)(B->bs[1] + 2 * NG) * sj;
+    for (int k = 0; k < NPR; k++) {
+        const double *q = p[b] + k * N + c;
+        v[k] = 0.125 * (q[0] + q[1] + q[sj] + q[sj + 1] + q[si] + q[si + 1] + q[si + sj] + q[si + sj + 1]);
+    }
+    const double *f0 = ps[b] + c, *f1 = ps[b] + N + c, *f2 = ps[b] + 2 * N + c;
+    v[NPR + 0] = 0.25 * (f0[0] + f0[1] + f0[sj] + f0[sj + 1]);
+    v[NPR + 1] = 0.25 * (f1[0] + f1[1] + f1[si] + f1[si + 1]);
+    v[NPR + 2] = 0.25 * (f2[0] + f2[sj] + f2[si] + f2[si + sj]);
+}
+
+/* slope_lim_2 (GPU_program2.cu:10-45, the MC limiter with factor 2) */
+static double slope_lim(double y1, double y2, double y3)
+{
+    double Dqm = 2.0 * (y2 - y1), Dqp = 2.0 * (y3 - y2), Dqc = 0.5 * (y3 - y1), s = Dqm * Dqp;
+    if (s <= 0.) return 0.;
+    if (fabs(Dqm) < fabs(Dqp) && fabs(Dqm) < fabs(Dqc)) return Dqm;
+    else if (fabs(Dqp) < fabs(Dqc)) return Dqp;
+    else return Dqc;
+}
+
+static void write_ghost(const oracle_amr *g, int b, long c, double **p, double **ps, int **pflag, const double *v,
+                        int flag)
+{
+    const long N = geo_of(&g->blk[b]).ncell;
+    for (int k = 0; k < NPR; k++) p[b][k * N + c] = v[k];
+    for (int a = 0; a < 3; a++) ps[b][a * N + c] = v[NPR + a];
+    if (pflag) pflag[b][c] = flag;
+}
+
+/* value at position Q of level lq's coarse neighbour of a prolongation; 0 if none */
+static int neighbour_values(const oracle_amr *g, int lq, int cblk, int Q[3], int d, double **p, double **ps,
+                            double *v)
+{
+    if (d == 2) Q[2] = wrap(Q[2], g->N[lq][2]);
+    if (Q[d] < 0 || Q[d] >= g->N[lq][d]) return 0;
+    if (inside(&g->blk[cblk], Q)) {
+        cell_values(g, cblk, cell_at(&g->blk[cblk], Q), p, ps, v);
+        return 1;
+    }
+    int ll, q = find_leaf(g, lq, Q, &ll);
+    if (q < 0) abort();
+    const oracle_amr_block *S = &g->blk[q];
+    if (ll == lq) {
+        cell_values(g, q, cell_at(S, Q), p, ps, v);
+    } else if (ll == lq + 1) {
+        int F[3] = {2 * Q[0], 2 * Q[1], 2 * Q[2]};
+        avg8_values(g, q, cell_at(S, F), p, ps, v);
+    } else {
+        int sh = lq - ll, C[3] = {Q[0] >> sh, Q[1] >> sh, Q[2] >> sh};
+        cell_values(g, q, cell_at(S, C), p, ps, v);
+    }
+    return 1;
+}
+
+static void exchange_cell(const oracle_amr *g, int b, int i, int j, int z, double **p, double **ps, int **pflag)
+{
+    const oracle_amr_block *B = &g->blk[b];
+    const int l = B->level;
+    const long dc = cell_local(B, i, j, z);
+    int Y[3] = {B->o[0] + i - NG, B->o[1] + j - NG, B->o[2] + z - NG};
+    Y[2] = wrap(Y[2], g->N[l][2]);
+    const int out1 = Y[0] < 0 || Y[0] >= g->N[l][0], out2 = Y[1] < 0 || Y[1] >= g->N[l][1];
+    double v[NV];
+    if (out2) {
+        int Yc[3] = {Y[0] < 0 ? 0 : (Y[0] >= g->N[l][0] ? g->N[l][0] - 1 : Y[0]), Y[1] < 0 ? 0 : g->N[l][1] - 1, Y[2]};
+        int lc, c = find_leaf(g, l, Yc, &lc);
+        if (c < 0 || lc != l) abort();
+        const long sc = cell_at(&g->blk[c], Y);
+        cell_values(g, c, sc, p, ps, v);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0);
+        return;
+    }
+    if (out1) return;
+    int lc, c = find_leaf(g, l, Y, &lc);
+    if (c < 0) abort();
+    const oracle_amr_block *C = &g->blk[c];
+    if (lc == l) {
+        const long sc = cell_at(C, Y);
+        cell_values(g, c, sc, p, ps, v);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0);
+    } else if (lc == l + 1) {
+        int F[3] = {2 * Y[0], 2 * Y[1], 2 * Y[2]};
+        const long sc = cell_at(C, F);
+        avg8_values(g, c, sc, p, ps, v);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0);
+    } else if (lc == l - 1) {
+        int P[3] = {Y[0] >> 1, Y[1] >> 1, Y[2] >> 1};
+        const long pc = cell_at(C, P);
+        double c0[NV], dq[3][NV], vm[NV], vp[NV], sgn[3];
+        cell_values(g, c, pc, p, ps, c0);
+        for (int d = 0; d < 3; d++) {
+            int Qm[3] = {P[0], P[1], P[2]}, Qp[3] = {P[0], P[1], P[2]};
+            Qm[d] -= 1;
+            Qp[d] += 1;
+            const int hm = neighbour_values(g, l - 1, c, Qm, d, p, ps, vm);
+            const int hp = neighbour_values(g, l - 1, c, Qp, d, p, ps, vp);
+            for (int k = 0; k < NV; k++) dq[d][k] = (hm && hp) ? slope_lim(vm[k], c0[k], vp[k]) : 0.;
+            sgn[d] = (Y[d] & 1) ? 1.0 : -1.0;
+        }
+        for (int k = 0; k < NPR; k++)
+            v[k] = c0[k] + 0.25 * sgn[0] * dq[0][k] + 0.25 * sgn[1] * dq[1][k] + 0.25 * sgn[2] * dq[2][k];
+        for (int a = 0; a < 3; a++) {
+            const int kk = (sgn[a] > 0.) ? B1 + a : NPR + a;
+            double w = c0[kk];
+            for (int d = 0; d < 3; d++)
+                if (d != a) w = w + 0.25 * sgn[d] * dq[d][kk];
+            v[NPR + a] = w;
+        }
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][pc] : 0);
+    } else {
+        abort();
+    }
+}
+
+static int radial_face(const oracle_amr *g, const oracle_amr_block *B, int side)
+{
+    return side ? (B->o[0] + B->bs[0] == g->N[B->level][0]) : (B->o[0] == 0);
+}
+
+static int polar_face(const oracle_amr *g, const oracle_amr_block *B, int side)
+{
+    return side ? (B->o[1] + B->bs[1] == g->N[B->level][1]) : (B->o[1] == 0);
+}
+
+void oracle_amr_exchange(const oracle_amr *g, int stage, double **p, double **ps, int **pflag)
+{
+    const int dd = stage - 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < g->nblk; b++) {
+        const oracle_amr_block *B = &g->blk[b];
+        const bgeo G = geo_of(B);
+        for (int side = 0; side < 2; side++) {
+            if ((dd == 0 && radial_face(g, B, side)) || (dd == 1 && polar_face(g, B, side))) continue;
+            int lo[3], hi[3];
+            for (int k = 0; k < 3; k++) {
+                lo[k] = k < dd ? 0 : NG;
+                hi[k] = k < dd ? G.D[k] : NG + B->bs[k];
+            }
+            lo[dd] = side ? NG + B->bs[dd] : 0;
+            hi[dd] = side ? G.D[dd] : NG;
+            for (int i = lo[0]; i < hi[0]; i++)
+                for (int j = lo[1]; j < hi[1]; j++)
+                    for (int z = lo[2]; z < hi[2]; z++) exchange_cell(g, b, i, j, z, p, ps, pflag);
+        }
+    }
+}
+
+/* metric of block b at position pos of column (i, j) */
+static void geom(const oracle_amr *g, int b, int pos, int i, int j, const double **gcov, const double **gcon)
+{
+    const oracle_amr_block *B = &g->blk[b];
+    const long ncol = (long)(B->bs[0] + 2 * NG) * (B->bs[1] + 2 * NG), col = (long)i * (B->bs[1] + 2 * NG) + j;
+    *gcov = g->m[b].gcov + ((long)pos * ncol + col) * 10;
+    *gcon = g->m[b].gcon + ((long)pos * ncol + col) * 10;
+}
+
+/* radial outflow of block b (GPU_bounds.h:3-175): every row, or (final pass)
+ * every row but the polar ghost rows */
+static void outflow(const oracle_amr *g, int b, double **p, double **ps, int **pflag, int skip_polar_rows)
+{
+    const oracle_amr_block *B = &g->blk[b];
+    const bgeo G = geo_of(B);
+    const long N = G.ncell;
+    for (int side = 0; side < 2; side++) {
+        if (!radial_face(g, B, side)) continue;
+        const int isrc = side ? NG + B->bs[0] - 1 : NG;
+        for (int j = 0; j < G.D[1]; j++) {
+            if (skip_polar_rows && ((polar_face(g, B, 0) && j < NG) || (polar_face(g, B, 1) && j >= NG + B->bs[1])))
+                continue;
+            for (int z = 0; z < G.D[2]; z++)
+                for (int gg = 0; gg < NG; gg++) {
+                    const int i = side ? NG + B->bs[0] + gg : gg;
+                    const long c = cell_local(B, i, j, z), cs = cell_local(B, isrc, j, z);
+                    double pr[NPR];
+                    const double *gcov, *gcon;
+                    for (int k = 0; k < NPR; k++) pr[k] = p[b][k * N + cs];
+                    geom(g, b, CENT, i, j, &gcov, &gcon);
+                    oracle_inflow_check(pr, gcov, gcon, side ? 1 : 0);
+                    for (int k = 0; k < NPR; k++) p[b][k * N + c] = pr[k];
+                    ps[b][1 * N + c] = ps[b][1 * N + cs];
+                    ps[b][2 * N + c] = ps[b][2 * N + cs];
+                    if (pflag) pflag[b][c] = pflag[b][cs];
+                }
+        }
+    }
+}
+
+/* polar partner: the block of the same level and shape half a turn away */
+static int partner(const oracle_amr *g, int b)
+{
+    const oracle_amr_block *B = &g->blk[b];
+    const int o3 = (B->o[2] + g->N[B->level][2] / 2) % g->N[B->level][2];
+    for (int q = 0; q < g->nblk; q++) {
+        const oracle_amr_block *Q = &g->blk[q];
+        if (Q->level == B->level && Q->o[0] == B->o[0] && Q->o[1] == B->o[1] && Q->o[2] == o3 && Q->bs[0] == B->bs[0] &&
+            Q->bs[1] == B->bs[1] && Q->bs[2] == B->bs[2])
+            return q;
+    }
+    return -1;
+}
+
+/* polar mirror (GPU_bounds.h:510-606) over every i and z of block b */
+static void polar(const oracle_amr *g, int b, double **p, double **ps, int **pflag)
+{
+    const oracle_amr_block *B = &g->blk[b];
+    const bgeo G = geo_of(B);
+    const long N = G.ncell;
+    for (int side = 0; side < 2; side++) {
+        if (!polar_face(g, B, side)) continue;
+        const int q = partner(g, b);
+        if (q < 0) abort();
+        for (int i = 0; i < G.D[0]; i++)
+            for (int z = 0; z < G.D[2]; z++) {
+                int zs = z;
+                if (q == b) zs = (((z - NG + B->bs[2] / 2) % B->bs[2]) + B->bs[2]) % B->bs[2] + NG;
+                for (int gg = 0; gg < NG; gg++) {
+                    const int j = side ? NG + B->bs[1] + gg : NG - 1 - gg;
+                    const int jsrc = side ? NG + B->bs[1] - 1 - gg : NG + gg;
+                    const long c = cell_local(B, i, j, z), cs = cell_local(B, i, jsrc, zs);
+                    for (int k = 0; k < NPR; k++) p[b][k * N + c] = p[q][k * N + cs];
+                    p[b][(long)U2 * N + c] *= -1.0;
+                    p[b][(long)U3 * N + c] *= -1.0;
+                    p[b][(long)B2 * N + c] *= -1.0;
+                    p[b][(long)B3 * N + c] *= -1.0;
+                    ps[b][0 * N + c] = ps[q][0 * N + cs];
+                    ps[b][2 * N + c] = -ps[q][2 * N + cs];
+                    if (pflag) pflag[b][c] = pflag[q][cs];
+                }
+            }
+    }
+}
+
+void oracle_amr_bound(const oracle_amr *g, double **p, double **ps, int **pflag)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < g->nblk; b++) outflow(g, b, p, ps, pflag, 0);
+    oracle_amr_exchange(g, 1, p, ps, pflag);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < g->nblk; b++) polar(g, b, p, ps, pflag);
+    oracle_amr_exchange(g, 2, p, ps, pflag);
+    oracle_amr_exchange(g, 3, p, ps, pflag);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < g->nblk; b++) outflow(g, b, p, ps, pflag, 1);
+}
+
+static oracle_params block_params(const oracle_amr *g, int b)
+{
+    const oracle_amr_block *B = &g->blk[b];
+    oracle_params par;
+    memset(&par, 0, sizeof par);
+    par.n1 = par.bs1 = B->bs[0];
+    par.n2 = par.bs2 = B->bs[1];
+    par.n3 = par.bs3 = B->bs[2];
+    par.bc[0] = (radial_face(g, B, 0) || radial_face(g, B, 1)) ? ORC_BC_OUTFLOW : ORC_BC_NONE;
+    const int lo = polar_face(g, B, 0), hi = polar_face(g, B, 1);
+    par.bc[1] = (lo && hi) ? ORC_BC_POLE : lo ? ORC_BC_POLE_LO : hi ? ORC_BC_POLE_HI : ORC_BC_NONE;
+    par.bc[2] = ORC_BC_NONE;
+    par.gam = g->gam;
+    par.cour = g->cour;
+    for (int k = 0; k < 3; k++) par.dx[k] = g->dx0[k] * ldexp(1.0, -B->level);
+    par.rho_floor = g->rho_floor;
+    par.u_floor = g->u_floor;
+    return par;
+}
+
+/* polar E1 average over each ring of blocks (same side, level, radial range and
+ * shape, ordered in phi), as the single-grid oracle averages over its phi-blocks */
+static void ring_average(const oracle_amr *g, double **E, double dt)
+{
+    int *done = calloc((size_t)g->nblk * 2, sizeof(int));
+    int *ring = malloc(sizeof(int) * (size_t)g->nblk);
+    for (int b0 = 0; b0 < g->nblk; b0++)
+        for (int side = 0; side < 2; side++) {
+            const oracle_amr_block *A = &g->blk[b0];
+            if (!polar_face(g, A, side) || done[b0 * 2 + side]) continue;
+            int len = 0;
+            for (int q = 0; q < g->nblk; q++) {
+                const oracle_amr_block *Q = &g->blk[q];
+                if (polar_face(g, Q, side) && Q->level == A->level && Q->o[0] == A->o[0] && Q->bs[0] == A->bs[0] &&
+                    Q->bs[1] == A->bs[1] && Q->bs[2] == A->bs[2])
+                    ring[len++] = q;
+            }
+            /* phi order */
+            for (int x = 1; x < len; x++)
+                for (int y = x; y > 0 && g->blk[ring[y]].o[2] < g->blk[ring[y - 1]].o[2]; y--) {
+                    int t = ring[y];
+                    ring[y] = ring[y - 1];
+                    ring[y - 1] = t;
+                }
+            for (int x = 0; x < len; x++) done[ring[x] * 2 + side] = 1;
+            const int j = side ? NG + A->bs[1] : NG;
+            for (int i = NG; i <= NG + A->bs[0]; i++) {
+                double avg = 0.;
+                for (int b = 0; b < len; b++) {
+                    const oracle_amr_block *B = &g->blk[ring[b]];
+                    double s = 0.;
+                    for (int zz = 0; zz < B->bs[2]; zz++) s += dt * E[ring[b]][cell_local(B, i, j, NG + zz)];
+                    s /= (double)(B->bs[2]);
+                    if (b == 0) avg = s / ((double)len);
+                    else avg += s / ((double)len);
+                }
+                for (int b = 0; b < len; b++) {
+                    const oracle_amr_block *B = &g->blk[ring[b]];
+                    const long N = geo_of(B).ncell;
+                    for (int z = NG; z <= NG + B->bs[2]; z++) {
+                        E[ring[b]][cell_local(B, i, j, z)] = avg / dt;
+                        E[ring[b]][2 * N + cell_local(B, i, j, z)] = 0.;
+                    }
+                }
+            }
+        }
+    free(done);
+    free(ring);
+}
+
+double oracle_amr_step(const oracle_amr *g, double **p, double **ps, double dt, int **pflag, long long *failcount)
+{
+    const int nb = g->nblk;
+    double **ph = calloc(nb, sizeof(double *)), **psh = calloc(nb, sizeof(double *));
+    double **F1 = calloc(nb, sizeof(double *)), **F2 = calloc(nb, sizeof(double *)), **F3 = calloc(nb, sizeof(double *));
+    double **U0 = calloc(nb, sizeof(double *)), **E = calloc(nb, sizeof(double *)), **Ec = calloc(nb, sizeof(double *));
+    double **d1 = calloc(nb, sizeof(double *)), **d2 = calloc(nb, sizeof(double *)), **d3 = calloc(nb, sizeof(double *));
+    double ndt = -1.;
+    int ok = ph && psh && F1 && F2 && F3 && U0 && E && Ec && d1 && d2 && d3;
+    for (int b = 0; ok && b < nb; b++) {
+        const long N = geo_of(&g->blk[b]).ncell;
+        ph[b] = malloc(sizeof(double) * N * NPR);
+        psh[b] = malloc(sizeof(double) * N * 3);
+        F1[b] = calloc(N * NPR, sizeof(double));
+        F2[b] = calloc(N * NPR, sizeof(double));
+        F3[b] = calloc(N * NPR, sizeof(double));
+        U0[b] = calloc(N * NPR, sizeof(double));
+        E[b] = calloc(N * 3, sizeof(double));
+        Ec[b] = calloc(N * 3, sizeof(double));
+        d1[b] = malloc(sizeof(double) * N);
+        d2[b] = malloc(sizeof(double) * N);
+        d3[b] = malloc(sizeof(double) * N);
+        ok = ph[b] && psh[b] && F1[b] && F2[b] && F3[b] && U0[b] && E[b] && Ec[b] && d1[b] && d2[b] && d3[b];
+        if (ok) {
+            memcpy(ph[b], p[b], sizeof(double) * N * NPR);
+            memcpy(psh[b], ps[b], sizeof(double) * N * 3);
+        }
+    }
+    if (ok) {
+        /* predictor: p -> ph with Dt = dt/2 */
+        for (int b = 0; b < nb; b++) {
+            oracle_params par = block_params(g, b);
+            oracle_flux(&par, 1, p[b], ps[b], &g->m[b], F1[b], NULL);
+            oracle_flux(&par, 2, p[b], ps[b], &g->m[b], F2[b], NULL);
+            oracle_flux(&par, 3, p[b], ps[b], &g->m[b], F3[b], NULL);
+            oracle_emf(&par, p[b], F1[b], F2[b], F3[b], &g->m[b], dt, E[b], Ec[b]);
+        }
+        ring_average(g, E, dt);
+        for (int b = 0; b < nb; b++) {
+            oracle_params par = block_params(g, b);
+            oracle_ct_update(&par, 0.5 * dt, ps[b], psh[b], E[b], &g->m[b]);
+            oracle_update(&par, 0, 0.5 * dt, p[b], p[b], ph[b], U0[b], psh[b], F1[b], F2[b], F3[b], &g->m[b],
+                          pflag[b], failcount);
+        }
+        oracle_amr_bound(g, ph, psh, pflag);
+        /* corrector: U0 -> p with Dt = dt, fluxes from ph */
+        for (int b = 0; b < nb; b++) {
+            oracle_params par = block_params(g, b);
+            oracle_flux(&par, 1, ph[b], psh[b], &g->m[b], F1[b], d1[b]);
+            oracle_flux(&par, 2, ph[b], psh[b], &g->m[b], F2[b], d2[b]);
+            oracle_flux(&par, 3, ph[b], psh[b], &g->m[b], F3[b], d3[b]);
+            oracle_emf(&par, ph[b], F1[b], F2[b], F3[b], &g->m[b], dt, E[b], Ec[b]);
+        }
+        ring_average(g, E, dt);
+        ndt = 1e9;
+        for (int b = 0; b < nb; b++) {
+            oracle_params par = block_params(g, b);
+            oracle_ct_update(&par, dt, ps[b], ps[b], E[b], &g->m[b]);
+            oracle_update(&par, 1, dt, p[b], ph[b], p[b], U0[b], ps[b], F1[b], F2[b], F3[b], &g->m[b],
+                          pflag[b], failcount);
+            const double bd = oracle_block_dt(&par, d1[b], d2[b], d3[b]);
+            if (bd < ndt) ndt = bd;
+        }
+        oracle_amr_bound(g, p, ps, pflag);
+        if (ndt > 1.3 * dt) ndt = 1.3 * dt;
+    }
+    for (int b = 0; b < nb; b++) {
+        if (ph) free(ph[b]);
+        if (psh) free(psh[b]);
+        if (F1) free(F1[b]);
+        if (F2) free(F2[b]);
+        if (F3) free(F3[b]);
+        if (U0) free(U0[b]);
+        if (E) free(E[b]);
+        if (Ec) free(Ec[b]);
+        if (d1) free(d1[b]);
+        if (d2) free(d2[b]);
+        if (d3) free(d3[b]);
+    }
+    free(ph); free(psh); free(F1); free(F2); free(F3); free(U0); free(E); free(Ec); free(d1); free(d2); free(d3);
+    return ndt;
+}

@@ -710,6 +710,8 @@ void oracle_flux(const oracle_params *par, int dir, const double *p, const doubl
     int lo[3], hi[3], d;
     double factor = par->cour * par->dx[dir - 1];
     int pos = (dir == 1) ? FACE1 : (dir == 2) ? FACE2 : FACE3;
+    const int pole_lo = par->bc[1] == ORC_BC_POLE || par->bc[1] == ORC_BC_POLE_LO;
+    const int pole_hi = par->bc[1] == ORC_BC_POLE || par->bc[1] == ORC_BC_POLE_HI;
     const int n[3] = {par->n1, par->n2, par->n3};
     for (d = 0; d < 3; d++) {
         if (d == dir - 1) { lo[d] = NG; hi[d] = NG + n[d]; }
@@ -739,7 +741,7 @@ void oracle_flux(const oracle_params *par, int dir, const double *p, const doubl
                 }
                 pl[B1 + dir - 1] = ps[(long)(dir - 1) * G.ncell + c];
                 pr[B1 + dir - 1] = ps[(long)(dir - 1) * G.ncell + c];
-                if (dir == 2 && par->bc[1] == ORC_BC_POLE && (j == NG || j == NG + par->n2)) {
+                if (dir == 2 && ((pole_lo && j == NG) || (pole_hi && j == NG + par->n2))) {
                     pl[B1] = 0.;
                     pr[B1] = 0.;
                 }
@@ -807,6 +809,8 @@ void oracle_emf(const oracle_params *par, const double *pb, const double *F1, co
     const int NG = ORC_NG;
     const long N = G.ncell, si = G.s1, sj = G.s2;
     const int pole = (par->bc[1] == ORC_BC_POLE);
+    const int pole_lo = pole || par->bc[1] == ORC_BC_POLE_LO;
+    const int pole_hi = pole || par->bc[1] == ORC_BC_POLE_HI;
     /* E_cent over cells [NG-1, NG+n] */
 #pragma omp parallel for collapse(2) schedule(static)
     for (int i = NG - 1; i <= NG + par->n1; i++)
@@ -870,7 +874,7 @@ void oracle_emf(const oracle_params *par, const double *pb, const double *F1, co
                     + (F2B1[g - si] + (dER31_1 * (double)(F2R[g - si] <= 0.0) + dER31_2 * (double)(F2R[g - si] > 0.0)))
                     + (-F1B2[g] - (dEL32_1 * (double)(F1R[g] <= 0.0) + dEL32_2 * (double)(F1R[g] > 0.0)))
                     + (-F1B2[g - sj] + (dER32_1 * (double)(F1R[g - sj] <= 0.0) + dER32_2 * (double)(F1R[g - sj] > 0.0))));
-                if (pole && (j == NG || j == NG + par->n2)) {
+                if ((pole_lo && j == NG) || (pole_hi && j == NG + par->n2)) {
                     e3 = 0.;
                     e1 = -0.5 * (F2B3[g] + F2B3[g - 1]);
                 }
@@ -1066,7 +1070,13 @@ void oracle_update(const oracle_params *par, int full, double Dt, const double *
 
 /* inflow_check, GPU_bounds.h:1368-1400 (type 0: no outflow from the inner
  * ghost into the grid; type 1: no inflow at the outer ghost). */
+void oracle_inflow_check(double *pr, const double *gcov, const double *gcon, int type);
 static void inflow_check(double *pr, const double *gcov, const double *gcon, int type)
+{
+    oracle_inflow_check(pr, gcov, gcon, type);
+}
+
+void oracle_inflow_check(double *pr, const double *gcov, const double *gcon, int type)
 {
     double ucon[4], alpha, beta1, gamma, vsq;
     ucon_calc(pr, gcov, gcon, ucon);

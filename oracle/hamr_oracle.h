@@ -34,7 +34,12 @@ extern "C" {
 #define ORC_NPR 9
 #define ORC_NG 3
 
-enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_POLE = 2 };
+/* ORC_BC_POLE: both x2 faces of the grid are on the axis (with the polar E1
+ * ring average over the grid's phi range).  ORC_BC_POLE_LO / _HI: only the
+ * lower / upper x2 face is on the axis and the ring average is left to the
+ * caller (one block of a multi-block ring); ORC_BC_NONE: a block-internal face. */
+enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_POLE = 2, ORC_BC_POLE_LO = 3, ORC_BC_POLE_HI = 4,
+       ORC_BC_NONE = 5 };
 
 typedef struct oracle_params {
     int n1, n2, n3;        /* interior cells of the global grid */
@@ -94,6 +99,37 @@ double oracle_block_dt(const oracle_params *par, const double *dt1, const double
  * (min over blocks, clamped to 1.3*dt), or a negative value on allocation failure. */
 double oracle_step(const oracle_params *par, double *p, double *ps, const oracle_metric *m,
                    double dt, int *pflag, long long *failcount);
+
+/* inflow_check (GPU_bounds.h:1368-1400) on one primitive vector. */
+void oracle_inflow_check(double *pr, const double *gcov, const double *gcon, int type);
+
+/* ---- multi-level block sets (hamr_amr_oracle.c) ----
+ * Blocks of uneven sizes on levels of ratio 2, each its own ghost-inclusive
+ * grid in the layout above (p[b], ps[b], pflag[b]; metric m[b] in the layout
+ * above with ncol = D1*D2 of the block).  The ghost fill is the build's
+ * specification of the multi-level exchange (copies, 2x2x2 means, MC-limited
+ * linear prolongation; see hamr_amr_oracle.c), found here by searching the
+ * block list for the leaf that holds each position. */
+typedef struct oracle_amr_block {
+    int level, o[3], bs[3];   /* level, first interior cell and size in cells of the level */
+} oracle_amr_block;
+
+typedef struct oracle_amr {
+    int nblk, nlev;
+    int N[3][3];              /* cells of each level */
+    const oracle_amr_block *blk;
+    const oracle_metric *m;   /* per block */
+    double gam, cour, dx0[3], rho_floor, u_floor;
+} oracle_amr;
+
+/* Ghost fill of every block (p, ps, pflag: arrays of per-block pointers). */
+void oracle_amr_bound(const oracle_amr *g, double **p, double **ps, int **pflag);
+/* Only the exchange part of stage `stage` (1..3): the ghosts of internal faces. */
+void oracle_amr_exchange(const oracle_amr *g, int stage, double **p, double **ps, int **pflag);
+/* One predictor + corrector step of every block (ghosts filled on input;
+ * pflag required);
+ * returns the next dt (min over blocks, clamped to 1.3 dt), < 0 on allocation failure. */
+double oracle_amr_step(const oracle_amr *g, double **p, double **ps, double dt, int **pflag, long long *failcount);
 
 #ifdef __cplusplus
 }

@@ -92,3 +92,34 @@ def relerr_by_var(a, b, axis_comp=1):
     scale = np.max(np.abs(b), axis=1)
     scale[scale == 0] = 1.0
     return np.max(np.abs(a - b), axis=1) / scale
+
+
+def device_to_oracle_metric(m, D1, D2):
+    """One geometry slab of the device layout -> the oracle's layout (D1 x D2 columns)."""
+    ncol = D1 * D2
+    return dict(gcov=np.ascontiguousarray(m["gcov"][:, :, :ncol].transpose(0, 2, 1)),
+                gcon=np.ascontiguousarray(m["gcon"][:, :, :ncol].transpose(0, 2, 1)),
+                gdet=np.ascontiguousarray(m["gdet"][:, :ncol]),
+                conn=np.ascontiguousarray(m["conn"][:, :ncol].T),
+                rcen=np.ascontiguousarray(m["rcen"][:ncol]))
+
+
+def amr_oracle(oracle, desc, amr):
+    """The oracle's multi-level grid for a block set (metric per block from the
+    host geometry of the same set)."""
+    info = H.desc_amr(desc, amr)
+    rows = info["rows"]
+    geos = {}
+    metrics = []
+    for r in rows:
+        geo = int(r[7])
+        D1, D2 = int(r[4]) + 2 * NG, int(r[5]) + 2 * NG
+        if geo not in geos:
+            geos[geo] = device_to_oracle_metric(H.desc_amr_geometry(desc, amr, geo, D1, D2), D1, D2)
+        metrics.append(geos[geo])
+    n0 = [desc.bs[k] * desc.nb[k] for k in range(3)]
+    nlev = amr.nlev
+    N = [tuple(x << l for x in n0) for l in range(nlev)]
+    _, dx0 = H.desc_coords(desc)
+    return O.AmrGrid(oracle, [r[:7] for r in rows], N, metrics, desc.gam, desc.cour, dx0, desc.rho_floor,
+                     desc.u_floor), rows

@@ -184,3 +184,78 @@ def load():
     if not os.path.exists(LIB):
         raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
     return Oracle(C.CDLL(LIB))
+
+
+# ---- multi-level block sets ----
+class AmrBlock(C.Structure):
+    _fields_ = [("level", C.c_int), ("o", C.c_int * 3), ("bs", C.c_int * 3)]
+
+
+class AmrGridC(C.Structure):
+    _fields_ = [("nblk", C.c_int), ("nlev", C.c_int), ("N", (C.c_int * 3) * 3), ("blk", C.POINTER(AmrBlock)),
+                ("m", C.POINTER(Metric)), ("gam", C.c_double), ("cour", C.c_double), ("dx0", C.c_double * 3),
+                ("rho_floor", C.c_double), ("u_floor", C.c_double)]
+
+
+def _setup_amr(lib):
+    if getattr(lib, "_amr_ready", False):
+        return
+    G = C.POINTER(AmrGridC)
+    pp = C.POINTER(_dp)
+    ipp = C.POINTER(_ip)
+    lib.oracle_amr_bound.argtypes = [G, pp, pp, ipp]
+    lib.oracle_amr_exchange.argtypes = [G, C.c_int, pp, pp, ipp]
+    lib.oracle_amr_step.argtypes = [G, pp, pp, C.c_double, ipp, C.POINTER(C.c_longlong)]
+    lib.oracle_amr_step.restype = C.c_double
+    lib._amr_ready = True
+
+
+class AmrGrid:
+    """A multi-level block set stepped by the oracle.  blocks: rows (level, o1,
+    o2, o3, bs1, bs2, bs3); N[l] cells of level l; metrics: per block, the
+    oracle layout (as Grid's metric)."""
+
+    def __init__(self, oracle, blocks, N, metrics, gam, cour, dx0, rho_floor=0.0, u_floor=0.0):
+        self.o = oracle
+        _setup_amr(oracle.lib)
+        nb = len(blocks)
+        self._blk = (AmrBlock * nb)()
+        for b, r in enumerate(blocks):
+            self._blk[b].level = int(r[0])
+            for k in range(3):
+                self._blk[b].o[k] = int(r[1 + k])
+                self._blk[b].bs[k] = int(r[4 + k])
+        self.D = [tuple(int(r[4 + k]) + 2 * NG for k in range(3)) for r in blocks]
+        self._m = [{k: np.ascontiguousarray(v, dtype=float) for k, v in m.items()} for m in metrics]
+        self._mc = (Metric * nb)()
+        for b, m in enumerate(self._m):
+            self._mc[b] = Metric(_ptr(m["gcov"]), _ptr(m["gcon"]), _ptr(m["gdet"]), _ptr(m["conn"]), _ptr(m["rcen"]))
+        self.g = AmrGridC()
+        self.g.nblk = nb
+        self.g.nlev = len(N)
+        for l, n in enumerate(N):
+            for k in range(3):
+                self.g.N[l][k] = int(n[k])
+        self.g.blk = self._blk
+        self.g.m = self._mc
+        self.g.gam, self.g.cour = gam, cour
+        for k in range(3):
+            self.g.dx0[k] = dx0[k]
+        self.g.rho_floor, self.g.u_floor = rho_floor, u_floor
+        self.failcount = (C.c_longlong * 4)()
+
+    @staticmethod
+    def _pp(arrs, t=_dp):
+        for a in arrs:
+            assert a.flags["C_CONTIGUOUS"]
+        return (C.POINTER(t._type_) * len(arrs))(*[a.ctypes.data_as(t) for a in arrs])
+
+    def bound(self, p, ps, pflag):
+        self.o.lib.oracle_amr_bound(C.byref(self.g), self._pp(p), self._pp(ps), self._pp(pflag, _ip))
+
+    def exchange(self, stage, p, ps, pflag):
+        self.o.lib.oracle_amr_exchange(C.byref(self.g), stage, self._pp(p), self._pp(ps), self._pp(pflag, _ip))
+
+    def step(self, p, ps, pflag, dt):
+        return self.o.lib.oracle_amr_step(C.byref(self.g), self._pp(p), self._pp(ps), dt, self._pp(pflag, _ip),
+                                          self.failcount)

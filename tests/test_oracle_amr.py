@@ -1,0 +1,211 @@
+"""The oracle's multi-level step (oracle/hamr_amr_oracle.c), pinned on the CPU.
+
+- On one level with blocks of mixed sizes (64x32x32, 32x16x16, 16^3), the
+  block-wise ghost fill equals the single-grid ghost fill (oracle_bound, itself
+  pinned by test_oracle_invariants) cell for cell, and one step of the block set
+  equals one step of the single grid bit for bit: the exchange, the per-block
+  polar rules and ring average and the final outflow pass reproduce the
+  single-grid semantics.
+- On three levels, the exchange (restriction, prolongation, copies) keeps a
+  constant state exactly and a state linear in x2 (or in x1, away from the
+  radial edge) to rounding, which fixes the child signs, the quarter slopes, the
+  MC limiter on linear data and the face rule (odd fine faces from the coarse
+  cell-centred field).
+- The block-set builder (hamr_desc_amr) gives the multi-level torus workload
+  its stated structure.
+"""
+import numpy as np
+import pytest
+
+import hamr_gamma_wind_amd as H
+from tests import gridtools as GT
+
+NG = 3
+
+# single level, mixed sizes: 128x64x32 cells as 2x2x1 superblocks
+DESC1 = dict(bs=(64, 32, 32), nb=(2, 2, 1))
+# three levels: base 64x64x32; level 1 covers x1 [0,64) x2 [32,96) of 128x128x64;
+# level 2 covers x1 [0,64) x2 [96,160) of 256x256x128
+BOXES3 = [((0, 64), (32, 96), (0, 64)), ((0, 64), (96, 160), (0, 128))]
+DESC3 = dict(bs=(64, 32, 32), nb=(1, 2, 1))
+
+
+def _desc(d):
+    return H.make_desc(d["bs"], d["nb"])
+
+
+def _blocks_from_global(rows, G):
+    """Each block's ghost-inclusive slice of a global level-0 array."""
+    out = []
+    for r in rows:
+        o, bs = r[1:4], r[4:7]
+        out.append(np.ascontiguousarray(G[:, o[0]:o[0] + bs[0] + 2 * NG, o[1]:o[1] + bs[1] + 2 * NG,
+                                        o[2]:o[2] + bs[2] + 2 * NG]))
+    return out
+
+
+def _interior_only(rows, arrs):
+    out = []
+    for r, a in zip(rows, arrs):
+        b = np.zeros_like(a)
+        bs = r[4:7]
+        I = (slice(None),) + tuple(slice(NG, NG + int(k)) for k in bs)
+        b[I] = a[I]
+        out.append(b)
+    return out
+
+
+def _random_global(n, seed):
+    rng = np.random.default_rng(seed)
+    D = tuple(k + 2 * NG for k in n)
+    P = np.zeros((9,) + D)
+    S = np.zeros((3,) + D)
+    I = (slice(None),) + tuple(slice(NG, NG + k) for k in n)
+    shape = (len(n) and n)
+    P[0][I[1:]] = rng.lognormal(0., 0.3, size=n)
+    P[1][I[1:]] = rng.lognormal(-1., 0.3, size=n)
+    for k in (2, 3, 4):
+        P[k][I[1:]] = rng.normal(0., 0.3, size=n)
+    for k in (5, 6, 7):
+        P[k][I[1:]] = rng.normal(0., 0.1, size=n)
+    P[8][I[1:]] = rng.lognormal(0., 0.2, size=n)
+    for k in range(3):
+        S[k][I[1:]] = rng.normal(0., 0.1, size=n)
+    return P, S
+
+
+def test_block_set_structure_of_the_multilevel_torus():
+    # the multi-level torus workload: base 128x96x32, levels 1 and 2 refining
+    # r <= 80 M, 30-150 deg and r <= 34.6 M, 60-120 deg
+    desc = H.make_desc((64, 32, 32), (2, 3, 1))
+    amr = H.make_amr_desc([((0, 160), (32, 160), (0, 64)), ((0, 256), (128, 256), (0, 128))], seed=2)
+    r = H.desc_amr(desc, amr, tables=True)
+    info, rows = r["info"], r["rows"]
+    assert 650 <= info["blocks"] <= 800
+    sizes = {tuple(int(x) for x in row[4:7]) for row in rows}
+    assert sizes == {(64, 32, 32), (32, 16, 16), (16, 16, 16)}
+    assert set(rows[:, 0].tolist()) == {0, 1, 2}
+    # the leaves tile each level's region exactly: zones per level
+    N0 = (128, 96, 32)
+    vol = [N0[0] * N0[1] * N0[2] - 80 * 64 * 32, 160 * 128 * 64 - 128 * 64 * 64, 256 * 128 * 128]
+    for l in range(3):
+        sel = rows[rows[:, 0] == l]
+        assert int(np.prod(sel[:, 4:7], axis=1).sum()) == vol[l]
+    # polar blocks are level 0 and their rings close
+    polar = rows[(rows[:, 8] & 3) != 0]
+    assert (polar[:, 0] == 0).all() and info["rings"] > 0
+    # table entries name cells inside their blocks (also checked on the host before upload)
+    dims = H.amr_block_dims(rows)
+    for ex in r["ex"]:
+        assert ((ex[:, 1] >= 0) & (ex[:, 1] < [int(np.prod(dims[b])) for b in ex[:, 0]])).all()
+        assert set(np.unique(ex[:, 3] >> 28).tolist()) <= {1, 2}
+    assert sum(len(p) for p in r["pro"]) > 0 and sum(len(e) for e in r["ex"]) > 0
+
+
+def test_single_level_fill_equals_single_grid(oracle):
+    desc = _desc(DESC1)
+    amr = H.make_amr_desc([], seed=3, p_big=0.3, p_mid=0.4)
+    g, rows = GT.amr_oracle(oracle, desc, amr)
+    assert len({tuple(int(x) for x in r[4:7]) for r in rows}) >= 2  # mixed sizes
+    n = tuple(desc.bs[k] * desc.nb[k] for k in range(3))
+    P, S = _random_global(n, seed=7)
+    grid = GT.oracle_grid(oracle, desc)
+    bp = _blocks_from_global(rows, P)
+    bs_ = _blocks_from_global(rows, S)
+    want_p = [a.copy() for a in bp]
+    want_s = [a.copy() for a in bs_]
+    grid.bound(P, S)
+    want_p = _blocks_from_global(rows, P)
+    want_s = _blocks_from_global(rows, S)
+    p = _interior_only(rows, bp)
+    ps = _interior_only(rows, bs_)
+    flags = [np.zeros(a.shape[1:], dtype=np.int32) for a in p]
+    g.bound(p, ps, flags)
+    for b in range(len(rows)):
+        assert np.array_equal(p[b], want_p[b]), b
+        assert np.array_equal(ps[b], want_s[b]), b
+
+
+def test_single_level_step_equals_single_grid(oracle):
+    desc = _desc(DESC1)
+    amr = H.make_amr_desc([], seed=3, p_big=0.3, p_mid=0.4)
+    g, rows = GT.amr_oracle(oracle, desc, amr)
+    # single grid with the polar blocks' phi size (the ring average runs over 16-cell blocks)
+    sdesc = H.make_desc((64, 32, 16), (2, 2, 2))
+    grid = GT.oracle_grid(oracle, sdesc)
+    p0, ps0 = H.desc_torus(desc, H.torus_desc())
+    P = GT.blocks_to_global(desc, p0)
+    S = GT.blocks_to_global(desc, ps0)
+    grid.bound(P, S)
+    bp = _blocks_from_global(rows, P)
+    bs_ = _blocks_from_global(rows, S)
+    dt = 0.5 * min(H.desc_coords(desc)[1])
+    flags = [np.zeros(a.shape[1:], dtype=np.int32) for a in bp]
+    g.step(bp, bs_, flags, dt)
+    grid.step(P, S, dt)
+    want_p = _blocks_from_global(rows, P)
+    want_s = _blocks_from_global(rows, S)
+    for b in range(len(rows)):
+        assert np.array_equal(bp[b], want_p[b]), b
+        assert np.array_equal(bs_[b], want_s[b]), b
+
+
+def _level_positions(r, dx0, sx):
+    """Code coordinates X1, X2 of the cell centres of block row r (ghost-inclusive)."""
+    l = int(r[0])
+    dx = [d * 2.0 ** -l for d in dx0]
+    i = np.arange(int(r[4]) + 2 * NG) + int(r[1]) - NG
+    j = np.arange(int(r[5]) + 2 * NG) + int(r[2]) - NG
+    return sx[0] + (i + 0.5) * dx[0], sx[1] + (j + 0.5) * dx[1], dx
+
+
+@pytest.mark.parametrize("axis", [1, 0])
+def test_multilevel_exchange_linear_and_constant_states(oracle, axis):
+    desc = _desc(DESC3)
+    amr = H.make_amr_desc(BOXES3, seed=5)
+    g, rows = GT.amr_oracle(oracle, desc, amr)
+    assert set(rows[:, 0].tolist()) == {0, 1, 2}
+    sx, dx0 = H.desc_coords(desc)
+    # dyadic coefficients: sums of eight equal values are then exact
+    coef = np.array([1.0, 0.5, 0.25, -0.25, 0.125, 0.75, -0.375, 0.25, 2.0])
+
+    def fill(linear):
+        p, ps = [], []
+        for r in rows:
+            X1, X2, dx = _level_positions(r, dx0, sx)
+            D = tuple(int(b) + 2 * NG for b in r[4:7])
+            Xc = (X1 if axis == 0 else X2)
+            Xf = (X1 - 0.5 * dx[0]) if axis == 0 else (X2 - 0.5 * dx[1])  # lower face along the varying axis
+            shape = [1, 1, 1]
+            shape[axis] = D[axis]
+            base = np.broadcast_to(Xc.reshape(shape), D) if linear else np.ones(D)
+            face = np.broadcast_to(Xf.reshape(shape), D) if linear else np.ones(D)
+            P = np.stack([1.0 + c * base for c in coef])
+            F = np.stack([0.25 + 0.5 * (face if a == axis else base) for a in range(3)])
+            # the cell-centred field B_a matches face a's function
+            for a in range(3):
+                P[5 + a] = 0.25 + 0.5 * base
+            p.append(np.ascontiguousarray(P))
+            ps.append(np.ascontiguousarray(F))
+        return p, ps
+
+    for linear in (False, True):
+        p, ps = fill(linear)
+        want_p = [a.copy() for a in p]
+        want_s = [a.copy() for a in ps]
+        flags = [np.zeros(a.shape[1:], dtype=np.int32) for a in p]
+        for stage in (1, 2, 3):
+            g.exchange(stage, p, ps, flags)
+        for b, r in enumerate(rows):
+            if not linear:
+                assert np.array_equal(p[b], want_p[b]) and np.array_equal(ps[b], want_s[b]), b
+                continue
+            # prolongations at the radial edge have no slope across it: skip the
+            # cells within 8 cells of r_in when x1 varies
+            m = np.ones(p[b].shape[1:], bool)
+            if axis == 0:
+                i0 = int(r[1]) - NG
+                m[: max(0, 8 - i0), :, :] = False
+            ep = np.abs(p[b] - want_p[b])[:, m].max()
+            es = np.abs(ps[b] - want_s[b])[:, m].max()
+            assert ep < 1e-12 and es < 1e-12, (b, ep, es)
