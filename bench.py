@@ -16,6 +16,12 @@ metric, face-field update, compensated conserved update and inversions).
 the error of the mixed state against the FP64 state after the same number of
 steps (max and L1, relative, for rho and the field) are reported beside it.
 
+Launch shapes come from the autotuner (hamr_autotune) for each arithmetic.
+Beside the batched steps the line reports the reference-structured FP64 step
+(the C host's per-block call sequence with a host dt read-back per block), the
+multi-level torus (3 levels, ~700 blocks of uneven sizes, both arithmetics) and
+the adjoint of one stage on the flat 128^3 box.
+
 Prints ONE JSON line with the contract's fields plus `roofline` (the mixed
 step's dominant stage, timed with HIP events on the step's stream) and
 `cpu_baseline` (the FP64 CPU oracle on the host cores, on a bounded sample of
@@ -105,9 +111,14 @@ def cpu_baseline(desc, nsteps):
 
 
 def run_mode(H, ctx, p0, ps0, mode, args):
-    """Time `steps` graph-replayed steps after `warmup` steps from (p0, ps0);
-    return (zone-updates/s, ms/step, final state, per-stage ms)."""
+    """Time `steps` graph-replayed steps after `warmup` steps from (p0, ps0),
+    with the launch shapes the autotuner picks for this arithmetic; return
+    (seconds, final state, per-stage ms, dt, failure counters, tune table)."""
     ctx.set_precision(mode)
+    ctx.set(H.HAMR_P, p0)
+    ctx.set(H.HAMR_PS, ps0)
+    ctx.clear_flags()
+    tune = ctx.autotune(3) if not args.no_tune else None
     ctx.set(H.HAMR_P, p0)
     ctx.set(H.HAMR_PS, ps0)
     ctx.clear_flags()
@@ -120,9 +131,78 @@ def run_mode(H, ctx, p0, ps0, mode, args):
     ctx.sync()
     el = time.perf_counter() - t0
     p = ctx.get(H.HAMR_P)
-    _, fi = ctx.all_flags()
+    if isinstance(ctx, H.AmrContext):
+        fi = np.array([f[1] for f in ctx.all_flags()])
+    else:
+        _, fi = ctx.all_flags()
     stage_ms = {s: ctx.time_stage(i, 5) for i, s in enumerate(STAGES)}
-    return el, p, stage_ms, dt, fi
+    return el, p, stage_ms, dt, fi, tune
+
+
+def host_sequence(H, ctx, p0, ps0, nsteps):
+    """The reference-structured FP64 step: the C host (hamr_host_advance in
+    libhamr_host.so) calling the per-block entry points, each block's dt read
+    back to the host; seconds for nsteps steps."""
+    ctx.set_precision(H.HAMR_PREC_FP64)
+    ctx.set(H.HAMR_P, p0)
+    ctx.set(H.HAMR_PS, ps0)
+    dt = ctx.estimate_dt()
+    dt = H.host_advance(ctx, dt)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        dt = H.host_advance(ctx, dt)
+    return time.perf_counter() - t0
+
+
+# The multi-level torus: base 128x96x32 (64x32x32 superblocks); level 1 over
+# x1 [0,160) x2 [32,160) of its 256x192x64 cells (r <= 80 M, 30-150 deg), level 2
+# over x1 [0,256) x2 [128,256) of its 512x384x128 cells (r <= 34.6 M, 60-120 deg);
+# blocks of 64x32x32, 32x16x16 and 16^3 by the seeded choice (seed 2).
+AMR_DESC = ((64, 32, 32), (2, 3, 1))
+AMR_BOXES = [((0, 160), (32, 160), (0, 64)), ((0, 256), (128, 256), (0, 128))]
+
+
+def amr_bench(H, args):
+    """Both arithmetics on the multi-level block set: tuned launches, graph replay."""
+    import numpy as np
+
+    desc = H.make_desc(*AMR_DESC)
+    amr = H.make_amr_desc(AMR_BOXES, seed=2)
+    ctx = H.AmrContext(desc, amr)
+    try:
+        info = ctx.info()
+        ctx.set_grid()
+        ctx.init_torus(H.torus_desc())
+        ctx.boundprim(1)
+        p0, ps0 = ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS)
+        steps = args.amr_steps
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = steps, args.warmup
+        out = {"workload": "torus on 3 levels (ratio 2; level 1: r <= 80 M, 30-150 deg; level 2: r <= 34.6 M, "
+                           "60-120 deg), %d blocks of 64x32x32, 32x16x16 and 16^3, multi-level ghost exchange"
+                           % info["blocks"],
+               "blocks": info["blocks"], "zones": info["zones"],
+               "exchange_entries": sum(info[k] for k in ("ex1", "ex2", "ex3", "pro1", "pro2", "pro3")),
+               "steps": steps}
+        res = {}
+        for name, mode in (("fp64", H.HAMR_PREC_FP64), ("mixed", H.HAMR_PREC_MIXED)):
+            el, p, stage_ms, dt, fi, tune = run_mode(H, ctx, p0, ps0, mode, sub)
+            res[name] = p
+            out[name] = {"value": info["zones"] * steps / el, "ms_per_step": el / steps * 1e3,
+                         "stages_ms_per_step": {k: round(2 * v, 4) for k, v in stage_ms.items()},
+                         "inversion_failures_2d": int(fi[:, 0].sum()),
+                         "tuned": {k: [v["threads"], v["grid"], v["xcd_order"]] for k, v in tune.items()}
+                         if tune else None}
+        out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+        err = {"steps": args.warmup + steps}
+        for name, k in (("rho", 0), ("B1", 5), ("B2", 6), ("B3", 7), ("u", 1)):
+            d = max(float(np.abs(a[k] - b[k])[3:-3, 3:-3, 3:-3].max()) for a, b in zip(res["mixed"], res["fp64"]))
+            m = max(float(np.abs(b[k])[3:-3, 3:-3, 3:-3].max()) for b in res["fp64"])
+            err[name + "_max"] = d / m
+        out["error_vs_fp64"] = err
+        return out
+    finally:
+        ctx.close()
 
 
 def adjoint_bench(H, n=128, reps=5):
@@ -181,6 +261,11 @@ def main():
                     help="also run one streaming copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None, help="run one arithmetic only")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement (128^3 flat box)")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuner's")
+    ap.add_argument("--no-amr", action="store_true", help="skip the multi-level block set")
+    ap.add_argument("--amr-steps", type=int, default=40)
+    ap.add_argument("--host-steps", type=int, default=3,
+                    help="steps of the reference-structured per-block FP64 sequence (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (each GPU box holds one MI355X); "
@@ -206,7 +291,14 @@ def main():
         modes = {args.only: modes[args.only]}
     res = {name: run_mode(H, ctx, p0, ps0, m, args) for name, m in modes.items()}
     head = "mixed" if "mixed" in res else "fp64"
-    el, p_head, stage_ms, dt, fi = res[head]
+    el, p_head, stage_ms, dt, fi, tune = res[head]
+    host = None
+    if args.host_steps > 0:
+        el_host = host_sequence(H, ctx, p0, ps0, args.host_steps)
+        host = {"value": nz * args.host_steps / el_host, "ms_per_step": el_host / args.host_steps * 1e3,
+                "steps": args.host_steps,
+                "what": "FP64, the C host's per-block call sequence of advance_GPU (libhamr_host.so), "
+                        "each block's Courant dt read back to the host"}
     value = nz * args.steps / el
     mixed = head == "mixed"
 
@@ -237,10 +329,16 @@ def main():
         el64 = res["fp64"][0]
         fp64 = {"value": nz * args.steps / el64, "ms_per_step": el64 / args.steps * 1e3,
                 "stages_ms_per_step": {k: round(2 * v, 4) for k, v in res["fp64"][2].items()}}
+        for name, k0, k1 in (("velocity", 2, 5), ("field", 5, 8)):
+            d = np.abs(a[:, k0:k1] - b[:, k0:k1]).max()
+            err[name + "_max_of_vector"] = float(d / np.abs(b[:, k0:k1]).max())
 
     adj = None
     if not args.no_adjoint:
         adj = adjoint_bench(H)
+    amr_res = None
+    if not args.no_amr:
+        amr_res = amr_bench(H, args)
 
     cpu = None
     if args.cpu_steps > 0:
@@ -275,6 +373,10 @@ def main():
         "stages_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
         "fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
+        "fp64_per_block_host": host,
+        "speedup_vs_fp64_per_block_host": (value / host["value"]) if host else None,
+        "tuned": {k: [v["threads"], v["grid"], v["xcd_order"]] for k, v in tune.items()} if tune else None,
+        "amr": amr_res,
         "error_vs_fp64": err,
         "adjoint": adj,
         "copy_probe": copy,

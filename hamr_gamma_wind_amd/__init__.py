@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhamr_hip.so")
+HOST_LIB_PATH = os.path.join(HERE, "libhamr_host.so")
 
 NPR = 9
 NG = 3
@@ -89,6 +90,33 @@ def _ptr(a, t=_dp):
     if not a.flags["C_CONTIGUOUS"]:
         raise ValueError("array must be C-contiguous")
     return a.ctypes.data_as(t)
+
+
+_host = None
+
+
+def load_host_library():
+    """libhamr_host.so: the reference's per-block host step in C over the C ABI."""
+    global _host
+    if _host is not None:
+        return _host
+    load_library()
+    if not os.path.exists(HOST_LIB_PATH):
+        raise HamrError("libhamr_host.so is not built: run __graft_entry__.build()")
+    L = C.CDLL(HOST_LIB_PATH)
+    L.hamr_host_advance.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
+    L.hamr_host_advance.restype = C.c_int
+    _host = L
+    return L
+
+
+def host_advance(ctx, dt):
+    """One step through hamr_host_advance (the C host's per-block sequence); returns the next dt."""
+    v = C.c_double()
+    r = load_host_library().hamr_host_advance(ctx.h, ctx.nblk, dt, C.byref(v))
+    if r:
+        raise HamrError("hamr_host_advance: %s" % ctx.L.hamr_last_error(ctx.h).decode())
+    return v.value
 
 
 def load_library(path=None):

@@ -1,4 +1,5 @@
-"""Builds libhamr_hip.so (gfx950) in-tree with hipcc.
+"""Builds libhamr_hip.so (gfx950) in-tree with hipcc, and libhamr_host.so (the
+C host of the reference's per-block step sequence, gcc, linked to it).
 
 The FP64 step is compiled without FMA contraction (-ffp-contract=off) so that
 its arithmetic follows the reference's expression order; packed-FP32 VALU
@@ -11,6 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libhamr_hip.so")
+HOST_OUT = os.path.join(HERE, "libhamr_host.so")
 BUILD = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -52,6 +54,10 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)
+    tmp = HOST_OUT + ".tmp"
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-o", tmp, os.path.join(CSRC, "hamr_host.c"), "-L" + HERE,
+          "-l:libhamr_hip.so", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_OUT)
     return OUT
 
 

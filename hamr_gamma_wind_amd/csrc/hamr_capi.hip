@@ -102,13 +102,16 @@ static int check_launch(hamr_ctx *c, const char *what)
     return 0;
 }
 
+// Zero-filled on the context's stream: every later copy and kernel of the
+// context is ordered after the fill (the stream is non-blocking, so a fill on
+// the null stream could land after an upload to the same memory).
 template <class T> static int dalloc(hamr_ctx *c, T **ptr, size_t n)
 {
     void *v = nullptr;
     hipError_t e = hipMalloc(&v, n * sizeof(T) + 256);
     if (e != hipSuccess) return fail(c, std::string("hipMalloc: ") + hipGetErrorString(e));
-    e = hipMemset(v, 0, n * sizeof(T) + 256);
-    if (e != hipSuccess) return fail(c, std::string("hipMemset: ") + hipGetErrorString(e));
+    e = hipMemsetAsync(v, 0, n * sizeof(T) + 256, c->s);
+    if (e != hipSuccess) return fail(c, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
     c->allocs.push_back(v);
     *ptr = (T *)v;
     return 0;

@@ -268,6 +268,13 @@ int hamr_stage_forward(hamr_ctx *ctx, double Dt);
 int hamr_stage_adjoint(hamr_ctx *ctx, double Dt, const double *w, double *grad_p, double *grad_ps);
 int hamr_stage_adjoint_dev(hamr_ctx *ctx, double Dt, const double *w_dev, double *grad_p_dev, double *grad_ps_dev);
 
+/* ---- libhamr_host.so: the reference's host-side step in C ----
+ * advance_GPU / GPU_step_ch (step_ch.c:916-1336) over the per-block entry
+ * points above for blocks 0..nblk-1 (every block at timelevel 1), each block's
+ * Courant dt read back to the host as fluxcalc_GPU does; *ndt_out: the next dt
+ * (min over blocks of the per-block harmonic, clamped to 1.3 dt). */
+int hamr_host_advance(hamr_ctx *ctx, int nblk, double dt, double *ndt_out);
+
 /* Streaming copy of `bytes` bytes (read once, written once), nrep launches:
  * the known-size kernel that calibrates HBM byte counters; average ms. */
 int hamr_copy_probe(hamr_ctx *ctx, long long bytes, int nrep, double *avg_ms);

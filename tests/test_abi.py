@@ -30,7 +30,9 @@ def test_header_declares_the_boundary():
 
 def test_library_exports_every_declared_symbol():
     lib = H.load_library()
-    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    host = H.load_host_library()
+    # hamr_host_* live in libhamr_host.so (the C host over the ABI), the rest in libhamr_hip.so
+    missing = [n for n in declared_functions() if not hasattr(host if n.startswith("hamr_host_") else lib, n)]
     assert not missing, missing
 
 
@@ -45,8 +47,9 @@ def test_header_is_plain_c(tmp_path):
 def test_struct_layouts_match_header(tmp_path):
     src = tmp_path / "s.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "hamr_hip.h"\n'
-                   'int main(void){ printf("%zu %zu %zu %zu\\n", sizeof(hamr_desc), offsetof(hamr_desc, minkowski),'
-                   ' sizeof(hamr_torus_desc), offsetof(hamr_torus_desc, seed)); return 0; }\n')
+                   'int main(void){ printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(hamr_desc), offsetof(hamr_desc, minkowski),'
+                   ' sizeof(hamr_torus_desc), offsetof(hamr_torus_desc, seed), sizeof(hamr_amr_desc),'
+                   ' offsetof(hamr_amr_desc, p_mid)); return 0; }\n')
     exe = tmp_path / "s"
     subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
@@ -54,6 +57,8 @@ def test_struct_layouts_match_header(tmp_path):
     assert int(out[1]) == H.Desc.minkowski.offset
     assert int(out[2]) == C.sizeof(H.TorusDesc)
     assert int(out[3]) == H.TorusDesc.seed.offset
+    assert int(out[4]) == C.sizeof(H.AmrDescC)
+    assert int(out[5]) == H.AmrDescC.p_mid.offset
 
 
 def test_host_entry_points_reject_bad_descriptors():

@@ -13,8 +13,9 @@ Bars: the ghost fill (exchange tables + outflow + polar mirror + final
 outflow pass) and the flags bit-exact; after one FP64 step every variable
 within 1e-12 of the oracle (relative to its largest magnitude) and dt within
 1e-12; graph replay, tuned launches and the per-block stage calls give the
-batched step's state bit for bit; the mixed step within 1e-5 of the FP64 step
-after one step.
+batched step's state bit for bit; after one mixed step, rho and u within 1e-5
+of the FP64 step (relative to their largest magnitude), velocity and field
+within 1e-5 of the largest component of their vector.
 """
 import numpy as np
 import pytest
@@ -169,7 +170,18 @@ def test_mixed_step_close_to_fp64():
         ctx.step(1, dt=dt)
         got = _state(ctx)
         ep = _relerr(got[0], ref[0])
-        print("mixed vs FP64 after one step, per var:", np.array2string(ep, precision=2), flush=True)
-        assert np.all(ep[:8] <= 1e-5)
+        # velocity and field components relative to the largest component of
+        # their vector (U2 and B3 start at zero: their own maxima after one
+        # step are the small generated values)
+        err = np.zeros(9)
+        sc = np.zeros(9)
+        for a, b in zip(got[0], ref[0]):
+            err = np.maximum(err, np.abs(a - b).reshape(9, -1).max(axis=1))
+            sc = np.maximum(sc, np.abs(b).reshape(9, -1).max(axis=1))
+        ev = err[2:5].max() / sc[2:5].max()
+        eb = err[5:8].max() / sc[5:8].max()
+        print("mixed vs FP64 after one step, per var:", np.array2string(ep, precision=2),
+              "velocity %.2e field %.2e (of the vector's largest component)" % (ev, eb), flush=True)
+        assert ep[0] <= 1e-5 and ep[1] <= 1e-5 and eb <= 1e-5 and ev <= 1e-5
     finally:
         ctx.close()
