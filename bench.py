@@ -48,26 +48,35 @@ CONFIGS = {
     "c1": ((16, 8, 8), (4, 4, 4)),
 }
 
-# Algorithmic HBM bytes per interior zone per launch (each array element once
-# per stage; halo re-reads and intermediates excluded).  Values are FP64 (8 B)
-# except the fluxes and EMFs of the mixed step (FP32, 4 B):
-#   flux (one direction): 9 prims + 1 face B read, 9 fluxes written
-#   EMF: U1-3, B1-3, 6 flux B-components, 3 mass fluxes read, 3 EMFs written
-#   face-B update: 3 EMFs + 3 face B read, 3 written
-#   update + inversion (predictor operands): prims 9, fluxes 27, face B 3 read;
-#     U^n 9, prims 9 written; 4 B flag
-#   ghost fill: (NPR+3) values per ghost cell, both ways, 24*g with g the ghost/interior ratio
 STAGES = ["flux_x1", "flux_x2", "flux_x3", "emf", "pole", "ct_update", "update", "bound"]
 
 
-def stage_bytes(stage, bs, mixed=False):
+KERNELS = ["flux1", "flux2", "flux3", "emf_centre", "emf_corner", "ct_update", "cons", "invert", "fallback",
+           "bound1", "bound2", "bound3", "exchange"]
+
+
+def kernel_bytes(kernel, bs, mixed=False):
+    """Algorithmic HBM bytes per interior zone of one launch of a kernel (each
+    array element it must read or write once; halo re-reads excluded).  FP64
+    values are 8 B; the fluxes and EMFs of the mixed step 4 B."""
     b1, b2, b3 = bs
-    g = ((b1 + 6) * (b2 + 6) * (b3 + 6) - b1 * b2 * b3) / (b1 * b2 * b3)
     w = 4 if mixed else 8
-    per_zone = {"flux_x1": 10 * 8 + 9 * w, "flux_x2": 10 * 8 + 9 * w, "flux_x3": 10 * 8 + 9 * w,
-                "emf": 6 * 8 + 9 * w + 3 * w, "pole": 0.0, "ct_update": 3 * w + 6 * 8,
-                "update": 12 * 8 + 27 * w + 18 * 8 + 4, "bound": 24 * g * 8}
-    return per_zone[stage]
+    nz = b1 * b2 * b3
+    ghosts = {"bound1": 2 * 3 * (b2 + 6) * (b3 + 6), "bound2": 2 * 3 * (b1 + 6) * (b3 + 6),
+              "bound3": 2 * 3 * (b1 + 6) * (b2 + 6)}
+    table = {
+        "flux1": 10 * 8 + 9 * w, "flux2": 10 * 8 + 9 * w, "flux3": 10 * 8 + 9 * w,
+        "emf_centre": 6 * 8 + 3 * w,        # U1-3, B1-3 read; 3 centre EMFs written
+        "emf_corner": 15 * w,               # 3 centre EMFs, 6 flux B-components, 3 mass fluxes read; 3 EMFs written
+        "ct_update": 3 * w + 6 * 8,         # 3 EMFs + 3 face B read, 3 face B written
+        "cons": 12 * 8 + 27 * w + 18 * 8,   # 9 prims + 3 face B, 27 fluxes read; U^n and U' written
+        "invert": 18 * 8 + 9 * 8 + 4,       # U' and the guess read; 9 prims and a flag written
+        "fallback": 4,                      # the flags (the few failed cells are negligible)
+        "exchange": 0,
+    }
+    if kernel in ghosts:
+        return ghosts[kernel] * 12 * 8 * 2 / nz
+    return table[kernel]
 
 
 def read_traffic(kernel, workload):
@@ -136,7 +145,8 @@ def run_mode(H, ctx, p0, ps0, mode, args):
     else:
         _, fi = ctx.all_flags()
     stage_ms = {s: ctx.time_stage(i, 5) for i, s in enumerate(STAGES)}
-    return el, p, stage_ms, dt, fi, tune
+    kern_ms = {k: ctx.time_kernel(i, 5) for i, k in enumerate(KERNELS)}
+    return el, p, stage_ms, dt, fi, tune, kern_ms
 
 
 def host_sequence(H, ctx, p0, ps0, nsteps):
@@ -186,10 +196,11 @@ def amr_bench(H, args):
                "steps": steps}
         res = {}
         for name, mode in (("fp64", H.HAMR_PREC_FP64), ("mixed", H.HAMR_PREC_MIXED)):
-            el, p, stage_ms, dt, fi, tune = run_mode(H, ctx, p0, ps0, mode, sub)
+            el, p, stage_ms, dt, fi, tune, kern_ms = run_mode(H, ctx, p0, ps0, mode, sub)
             res[name] = p
             out[name] = {"value": info["zones"] * steps / el, "ms_per_step": el / steps * 1e3,
                          "stages_ms_per_step": {k: round(2 * v, 4) for k, v in stage_ms.items()},
+                         "kernels_ms_per_launch": {k: round(v, 4) for k, v in kern_ms.items()},
                          "inversion_failures_2d": int(fi[:, 0].sum()),
                          "tuned": {k: [v["threads"], v["grid"], v["xcd_order"]] for k, v in tune.items()}
                          if tune else None}
@@ -198,7 +209,7 @@ def amr_bench(H, args):
         for name, k in (("rho", 0), ("B1", 5), ("B2", 6), ("B3", 7), ("u", 1)):
             d = max(float(np.abs(a[k] - b[k])[3:-3, 3:-3, 3:-3].max()) for a, b in zip(res["mixed"], res["fp64"]))
             m = max(float(np.abs(b[k])[3:-3, 3:-3, 3:-3].max()) for b in res["fp64"])
-            err[name + "_max"] = d / m
+            err[name + "_max"] = d / m if m > 0 else d
         out["error_vs_fp64"] = err
         return out
     finally:
@@ -264,6 +275,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--no-amr", action="store_true", help="skip the multi-level block set")
     ap.add_argument("--amr-steps", type=int, default=40)
+    ap.add_argument("--amr-only", action="store_true",
+                    help="measure only the multi-level block set (prints its JSON object; for profiling)")
     ap.add_argument("--host-steps", type=int, default=3,
                     help="steps of the reference-structured per-block FP64 sequence (0: skip)")
     args = ap.parse_args()
@@ -276,6 +289,9 @@ def main():
 
     import hamr_gamma_wind_amd as H
 
+    if args.amr_only:
+        print(json.dumps({"amr": amr_bench(H, args)}), flush=True)
+        return
     bs, nb = CONFIGS[args.config]
     desc = H.make_desc(bs, nb)
     nz = bs[0] * bs[1] * bs[2] * nb[0] * nb[1] * nb[2]
@@ -291,7 +307,7 @@ def main():
         modes = {args.only: modes[args.only]}
     res = {name: run_mode(H, ctx, p0, ps0, m, args) for name, m in modes.items()}
     head = "mixed" if "mixed" in res else "fp64"
-    el, p_head, stage_ms, dt, fi, tune = res[head]
+    el, p_head, stage_ms, dt, fi, tune, kern_ms = res[head]
     host = None
     if args.host_steps > 0:
         el_host = host_sequence(H, ctx, p0, ps0, args.host_steps)
@@ -302,11 +318,12 @@ def main():
     value = nz * args.steps / el
     mixed = head == "mixed"
 
-    # one step runs every stage twice (predictor, corrector)
+    # one step runs every stage and kernel twice (predictor, corrector); the
+    # roofline is that of the kernel with the largest share of the step
     per_step = {s: 2 * ms for s, ms in stage_ms.items()}
-    dom = max(per_step, key=per_step.get)
-    bpz = stage_bytes(dom, bs, mixed)
-    achieved = nz * bpz / (stage_ms[dom] * 1e-3) / 1e9 if bpz > 0 else 0.0
+    dom = max(kern_ms, key=kern_ms.get)
+    bpz = kernel_bytes(dom, bs, mixed)
+    achieved = nz * bpz / (kern_ms[dom] * 1e-3) / 1e9 if bpz > 0 else 0.0
     traffic = read_traffic(dom, args.config)
     copy = None
     if args.copy_probe_mib > 0:
@@ -369,7 +386,10 @@ def main():
                    "floor_hits": int(fi[:, 3].sum())},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": nz * bpz, "avg_launch_ms": stage_ms[dom]},
+                     "algorithmic_bytes_per_launch": nz * bpz, "avg_launch_ms": kern_ms[dom]},
+        "kernels_ms_per_launch": {k: round(v, 4) for k, v in kern_ms.items()},
+        "kernels_hbm_frac": {k: round(nz * kernel_bytes(k, bs, mixed) / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                             for k, v in kern_ms.items() if v > 0 and kernel_bytes(k, bs, mixed) > 4},
         "stages_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
         "fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,

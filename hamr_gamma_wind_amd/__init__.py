@@ -191,6 +191,7 @@ def load_library(path=None):
         "hamr_autotune": [ctxp, C.c_int],
         "hamr_tune_get": [ctxp, C.c_int, _ip, _dp],
         "hamr_tune_set": [ctxp, C.c_int, _ip],
+        "hamr_time_kernel": [ctxp, C.c_int, C.c_int, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -524,6 +525,11 @@ class Context:
             self._chk(self.L.hamr_tune_get(self.h, k, _ptr(cfg, _ip), C.byref(ms)), "hamr_tune_get")
             out[name] = dict(threads=int(cfg[0]), grid=int(cfg[1]), xcd_order=int(cfg[2]), ms=ms.value)
         return out
+
+    def time_kernel(self, slot, nrep=5):
+        v = C.c_double()
+        self._chk(self.L.hamr_time_kernel(self.h, slot, nrep, C.byref(v)), "hamr_time_kernel")
+        return v.value
 
     def tune_set(self, slot, threads, grid, xcd_order):
         cfg = np.array([threads, grid, xcd_order], dtype=np.int32)

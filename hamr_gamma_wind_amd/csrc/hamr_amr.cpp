@@ -362,6 +362,7 @@ std::string build_amr(const GridDesc &d, const AmrDesc &a, AmrGrid &g)
         const int l = B.level;
         const Level &L = b.lev[l];
         for (int d0 = 0; d0 < 3; d0++) {
+            const size_t ex0 = g.ex[d0].size(), pro0 = g.pro[d0].size();
             int lo[3], hi[3];
             for (int k = 0; k < 3; k++) {
                 if (k < d0) { lo[k] = 0; hi[k] = B.D[k]; }
@@ -441,6 +442,12 @@ std::string build_amr(const GridDesc &d, const AmrDesc &a, AmrGrid &g)
                             }
                         }
             }
+            // the block's entries of this stage in memory order of the ghost cells
+            // (the upper ghosts of one row next to the lower ghosts of the next)
+            std::sort(g.ex[d0].begin() + ex0, g.ex[d0].end(),
+                      [](const AmrEntry &x, const AmrEntry &y) { return x.dcell < y.dcell; });
+            std::sort(g.pro[d0].begin() + pro0, g.pro[d0].end(),
+                      [](const AmrProlong &x, const AmrProlong &y) { return x.dcell < y.dcell; });
         }
     }
     g.blocks = b.blk;

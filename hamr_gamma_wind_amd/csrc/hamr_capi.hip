@@ -1224,6 +1224,22 @@ int hamr_autotune(hamr_ctx *c, int nrep)
     return 0;
 }
 
+int hamr_time_kernel(hamr_ctx *c, int slot, int nrep, double *avg_ms)
+{
+    if (!c || !avg_ms || nrep < 1 || slot < 0 || slot >= TS_N) return fail(c, "hamr_time_kernel: bad arguments");
+    if (slot == TS_EXCHANGE && !c->amr) {
+        *avg_ms = 0.;
+        return 0;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    Launch L{};
+    if (slot != TS_EXCHANGE) L = all_launch(c, slot);
+    float ms = 0.f;
+    if (time_launches(c, slot, L, kTiles[c->tune[TS_EXCHANGE].tile], c->tune[TS_EXCHANGE].grid, nrep, &ms)) return -1;
+    *avg_ms = ms;
+    return 0;
+}
+
 int hamr_tune_get(const hamr_ctx *c, int slot, int *cfg, double *ms)
 {
     if (!c || slot < 0 || slot >= TS_N || !cfg) return -1;

@@ -253,6 +253,10 @@ int hamr_autotune(hamr_ctx *ctx, int nrep);
 /* cfg[3]: threads per workgroup, workgroups (0: one per item), XCD order; ms: tuned time (-1: not tuned). */
 int hamr_tune_get(const hamr_ctx *ctx, int slot, int *cfg, double *ms);
 int hamr_tune_set(hamr_ctx *ctx, int slot, const int *cfg);
+/* Average device time (ms, HIP events on the context stream) of one launch of
+ * kernel `slot` over all blocks with its tuned shape and predictor-step
+ * operands (HAMR_TUNE_EXCHANGE: the three exchange stages of a multi-level set). */
+int hamr_time_kernel(hamr_ctx *ctx, int slot, int nrep, double *avg_ms);
 
 /* ---- one forward stage and its adjoint (no reference counterpart) ----
  * Forward stage: the predictor's flux, EMF, face-field and conserved updates

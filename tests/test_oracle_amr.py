@@ -95,9 +95,9 @@ def test_block_set_structure_of_the_multilevel_torus():
     polar = rows[(rows[:, 8] & 3) != 0]
     assert (polar[:, 0] == 0).all() and info["rings"] > 0
     # table entries name cells inside their blocks (also checked on the host before upload)
-    dims = H.amr_block_dims(rows)
+    ncell = np.array([int(np.prod(D)) for D in H.amr_block_dims(rows)])
     for ex in r["ex"]:
-        assert ((ex[:, 1] >= 0) & (ex[:, 1] < [int(np.prod(dims[b])) for b in ex[:, 0]])).all()
+        assert ((ex[:, 1] >= 0) & (ex[:, 1] < ncell[ex[:, 0]])).all()
         assert set(np.unique(ex[:, 3] >> 28).tolist()) <= {1, 2}
     assert sum(len(p) for p in r["pro"]) > 0 and sum(len(e) for e in r["ex"]) > 0
 
