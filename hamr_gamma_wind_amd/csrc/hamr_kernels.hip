@@ -89,8 +89,10 @@ int64_t class_work(const BlockDev &B, int cls)
 // R is the arithmetic of reconstruction, state, fluxes and speeds: double for
 // the FP64 step, float for the mixed step (primitives, face fields and metric
 // are read in FP64 and rounded once; fluxes are stored as R).
+// At least 3 waves per SIMD (<= 168 VGPRs): the FP64 flux needs 169 registers
+// unconstrained, one too many for the third wave.
 template <int DIR, class R>
-__global__ void __launch_bounds__(256) k_flux(KParams P, WorkList wl, const double *__restrict__ p,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_flux(KParams P, WorkList wl, const double *__restrict__ p,
                                               const double *__restrict__ ps, R *__restrict__ F,
                                               unsigned long long *__restrict__ dtmin)
 {
@@ -462,16 +464,17 @@ __global__ void __launch_bounds__(256) k_cons(KParams P, WorkList wl, int full, 
                 for (int k = 0; k < NPR; k++) pf[k] = pb[base + k * K];
                 get_state(pf, G, q);
             }
-            double f1l[NPR], f1h[NPR], f2l[NPR], f2h[NPR], f3l[NPR], f3h[NPR];
+            // fluxes kept in their storage type until used (half the registers in the mixed step)
+            R f1l[NPR], f1h[NPR], f2l[NPR], f2h[NPR], f3l[NPR], f3h[NPR];
 #pragma unroll
             for (int k = 0; k < NPR; k++) {
                 const R *f1 = F1 + base + k * K, *f2 = F2 + base + k * K, *f3 = F3 + base + k * K;
-                f1l[k] = (double)f1[0];
-                f1h[k] = (double)f1[si];
-                f2l[k] = (double)f2[0];
-                f2h[k] = (double)f2[sj];
-                f3l[k] = (double)f3[0];
-                f3h[k] = (double)f3[1];
+                f1l[k] = f1[0];
+                f1h[k] = f1[si];
+                f2l[k] = f2[0];
+                f2h[k] = f2[sj];
+                f3l[k] = f3[0];
+                f3h[k] = f3[1];
             }
             source(pf, q, [&](int lam, int s) { return mv[(MV_CONN + lam * 10 + s) * ncw]; }, G.g, P.gam, dU);
             if (sizeof(R) == sizeof(double)) {
@@ -487,9 +490,9 @@ __global__ void __launch_bounds__(256) k_cons(KParams P, WorkList wl, int full, 
 #pragma unroll
                 for (int k = 0; k < NPR; k++) {
                     double sum = U[k], cmp = 0.;
-                    comp_add(sum, cmp, -(Dt * (f1h[k] - f1l[k]) / B.dx[0]));
-                    comp_add(sum, cmp, -(Dt * (f2h[k] - f2l[k]) / B.dx[1]));
-                    comp_add(sum, cmp, -(Dt * (f3h[k] - f3l[k]) / B.dx[2]));
+                    comp_add(sum, cmp, -(Dt * ((double)f1h[k] - (double)f1l[k]) / B.dx[0]));
+                    comp_add(sum, cmp, -(Dt * ((double)f2h[k] - (double)f2l[k]) / B.dx[1]));
+                    comp_add(sum, cmp, -(Dt * ((double)f3h[k] - (double)f3l[k]) / B.dx[2]));
                     comp_add(sum, cmp, Dt * dU[k]);
                     U[k] = sum + cmp;
                 }
