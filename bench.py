@@ -80,7 +80,9 @@ def kernel_bytes(kernel, bs, mixed=False):
 
 
 def read_traffic(kernel, workload):
-    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
+    """HBM bytes per launch of `kernel` on `workload` measured with rocprofv3
+    --pmc (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel; committed
+    summary profiles/<kernel>_<workload>_traffic.txt), or None."""
     path = os.path.join(ROOT, "profiles", "%s_%s_traffic.txt" % (kernel, workload))
     if not os.path.exists(path):
         return None
@@ -267,7 +269,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-graph", action="store_true", help="enqueue each step instead of replaying a HIP graph")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU oracle baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU oracle baseline (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one streaming copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None, help="run one arithmetic only")
@@ -324,7 +326,8 @@ def main():
     dom = max(kern_ms, key=kern_ms.get)
     bpz = kernel_bytes(dom, bs, mixed)
     achieved = nz * bpz / (kern_ms[dom] * 1e-3) / 1e9 if bpz > 0 else 0.0
-    traffic = read_traffic(dom, args.config)
+    traffic = read_traffic(dom, "%s_torus%dx%dx%d" % ("mixed" if mixed else "fp64", bs[0] * nb[0], bs[1] * nb[1],
+                                                       bs[2] * nb[2]))
     copy = None
     if args.copy_probe_mib > 0:
         nbytes = args.copy_probe_mib << 20

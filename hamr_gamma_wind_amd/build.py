@@ -1,10 +1,12 @@
 """Builds libhamr_hip.so (gfx950) in-tree with hipcc, and libhamr_host.so (the
 C host of the reference's per-block step sequence, gcc, linked to it).
 
-The FP64 step is compiled without FMA contraction (-ffp-contract=off) so that
-its arithmetic follows the reference's expression order; packed-FP32 VALU
-instructions are disabled for all device code.
+Everything is compiled without FMA contraction (-ffp-contract=off): the FP64
+step follows the reference's expression order, and the FP32 fluxes of the mixed
+step keep the face-field error of the CT update at FP32 roundoff.  Packed-FP32
+VALU instructions are disabled for all device code.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -24,6 +26,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 
 SOURCES = [
     ("hamr_kernels.hip", True),
+    ("hamr_kernels_f32.hip", True),
     ("hamr_adjoint.hip", True),
     ("hamr_capi.hip", True),
     ("hamr_setup.cpp", False),
@@ -42,15 +45,20 @@ def _run(cmd):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    jobs = []
     for src, dev in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC] + COMMON + (DEVICE if dev else ["-x", "hip", "--offload-arch=" + ARCH])
         cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
-        objs.append(obj)
+        jobs.append((cmd, obj))
+    # the units compile concurrently (one hipcc per unit)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1, 8)) as ex:
+        outs = list(ex.map(lambda j: _run(j[0]), jobs))
+    if verbose:
+        for out in outs:
+            if out.strip():
+                print(out)
+    objs = [obj for _, obj in jobs]
     tmp = OUT + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)

@@ -81,6 +81,8 @@ struct hamr_ctx {
     double *aw = nullptr, *alP = nullptr, *alps = nullptr, *alpsf = nullptr, *alE = nullptr, *alEc = nullptr;
     double *alF[3] = {nullptr, nullptr, nullptr};
     std::vector<void *> allocs;
+    void *stage = nullptr;         // pinned host staging of whole-array copies
+    size_t stage_bytes = 0;
 };
 
 static int fail(hamr_ctx *c, const std::string &m)
@@ -121,17 +123,37 @@ static int nblk_of(const hamr_ctx *c) { return (int)c->blocks.size(); }
 static bool valid_block(const hamr_ctx *c, int n) { return n >= 0 && n < nblk_of(c); }
 static int64_t ncell_of(const BlockDev &B) { return (int64_t)B.D[0] * B.D[1] * B.D[2]; }
 
+// Pinned host staging buffer of at least `bytes` (grown on demand, kept until
+// hamr_destroy).  Asynchronous copies on the context stream touch only this
+// buffer, never pageable memory.
+static int stage_reserve(hamr_ctx *c, size_t bytes)
+{
+    if (c->stage_bytes >= bytes) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->stage) HIPCHK(c, hipHostFree(c->stage));
+    c->stage = nullptr;
+    c->stage_bytes = 0;
+    HIPCHK(c, hipHostMalloc(&c->stage, bytes, hipHostMallocDefault));
+    c->stage_bytes = bytes;
+    return 0;
+}
+
 // Host<->device copies ordered on the context stream and completed before return.
 static int h2d(hamr_ctx *c, void *dst, const void *src, size_t bytes)
 {
-    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (stage_reserve(c, bytes)) return -1;
+    std::memcpy(c->stage, src, bytes);
+    HIPCHK(c, hipMemcpyAsync(dst, c->stage, bytes, hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return 0;
 }
 static int d2h(hamr_ctx *c, void *dst, const void *src, size_t bytes)
 {
-    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->s));
+    if (stage_reserve(c, bytes)) return -1;
+    HIPCHK(c, hipMemcpyAsync(c->stage, src, bytes, hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
+    std::memcpy(dst, c->stage, bytes);
     return 0;
 }
 
@@ -449,6 +471,7 @@ int hamr_destroy(hamr_ctx *c)
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->s) (void)hipStreamSynchronize(c->s);
     for (void *v : c->allocs) (void)hipFree(v);
+    if (c->stage) (void)hipHostFree(c->stage);
     if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
     return 0;
@@ -839,6 +862,9 @@ static bool is_float_array(const hamr_ctx *c, int which)
 
 // Copy between the unpadded [block][component][cell] layout (ext) and the
 // device slabs, one 2D copy per block.
+// Whole-array copy between the caller's buffer (blocks' unpadded slabs one
+// after another) and the device slabs.  Host buffers pass through a pinned
+// staging buffer: the asynchronous 2D copies never touch pageable memory.
 static int copy_array(hamr_ctx *c, int which, double *ext, int to_device, hipMemcpyKind kind)
 {
     double *d;
@@ -846,29 +872,38 @@ static int copy_array(hamr_ctx *c, int which, double *ext, int to_device, hipMem
     if (array_info(c, which, &d, &nc)) return -1;
     const bool fl = is_float_array(c, which);
     const size_t es = fl ? sizeof(float) : sizeof(double);
-    if (fl && kind != hipMemcpyHostToDevice && kind != hipMemcpyDeviceToHost)
-        return fail(c, "float array in the mixed step: host copies only");
+    const bool host = kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToHost;
+    if (fl && !host) return fail(c, "float array in the mixed step: host copies only");
     size_t total = 0;
     for (const BlockDev &B : c->blocks) total += (size_t)nc * ncell_of(B);
-    std::vector<float> tmp;
-    if (fl) {
-        tmp.resize(total);
-        if (to_device)
-            for (size_t e = 0; e < total; e++) tmp[e] = (float)ext[e];
-    }
     HIPCHK(c, hipStreamSynchronize(c->s));
+    char *buf = (char *)ext;
+    if (host) {
+        if (stage_reserve(c, total * es)) return -1;
+        buf = (char *)c->stage;
+        if (to_device) {
+            if (fl)
+                for (size_t e = 0; e < total; e++) ((float *)buf)[e] = (float)ext[e];
+            else
+                std::memcpy(buf, ext, total * es);
+        }
+    }
     size_t pos = 0;
     for (const BlockDev &B : c->blocks) {
         const size_t N = (size_t)ncell_of(B), K = (size_t)B.ks;
         char *dev = (char *)d + (size_t)nc * B.off * es;
-        char *hst = fl ? (char *)(tmp.data() + pos) : (char *)(ext + pos);
+        char *hst = buf + pos * es;
         if (to_device) HIPCHK(c, hipMemcpy2DAsync(dev, K * es, hst, N * es, N * es, nc, kind, c->s));
         else HIPCHK(c, hipMemcpy2DAsync(hst, N * es, dev, K * es, N * es, nc, kind, c->s));
         pos += (size_t)nc * N;
     }
     HIPCHK(c, hipStreamSynchronize(c->s));
-    if (fl && !to_device)
-        for (size_t e = 0; e < total; e++) ext[e] = (double)tmp[e];
+    if (host && !to_device) {
+        if (fl)
+            for (size_t e = 0; e < total; e++) ext[e] = (double)((const float *)buf)[e];
+        else
+            std::memcpy(ext, buf, total * es);
+    }
     return 0;
 }
 
@@ -1340,7 +1375,14 @@ static int stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, do
             return -1;
     }
     HIPCHK(c, hipStreamSynchronize(c->s));
-    HIPCHK(c, hipMemcpy2DAsync(c->aw, K * sizeof(double), w, N * sizeof(double), N * sizeof(double), NPR, in, c->s));
+    // host buffers pass through the pinned staging buffer (w in, then the gradients out)
+    const bool hin = in == hipMemcpyHostToDevice, hout = out == hipMemcpyDeviceToHost;
+    if ((hin || hout) && stage_reserve(c, (NPR + 3) * N * sizeof(double))) return -1;
+    double *sw = (double *)c->stage;
+    if (hin) std::memcpy(sw, w, NPR * N * sizeof(double));
+    HIPCHK(c, hipMemcpy2DAsync(c->aw, K * sizeof(double), hin ? sw : w, N * sizeof(double), N * sizeof(double), NPR,
+                               in, c->s));
+    if (hin) HIPCHK(c, hipStreamSynchronize(c->s));
     // the fluxes of the stage (their mass-flux signs select the EMF upwinding)
     bound_all(c, c->p, c->ps);
     for (int dir = 1; dir <= 3; dir++)
@@ -1349,11 +1391,16 @@ static int stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, do
     launch_adjoint(c->P, B, Dt, c->p, c->ps, c->F[0], c->F[1], c->F[2], c->aw, c->alP, c->alps, c->alpsf, c->alE,
                    c->alEc, c->alF[0], c->alF[1], c->alF[2], c->s);
     if (check_launch(c, "hamr_stage_adjoint")) return -1;
+    double *sp = hout ? (double *)c->stage : gp, *sps = hout ? (double *)c->stage + NPR * N : gps;
     if (gp)
-        HIPCHK(c, hipMemcpy2DAsync(gp, N * sizeof(double), c->alP, K * sizeof(double), N * sizeof(double), NPR, out, c->s));
+        HIPCHK(c, hipMemcpy2DAsync(sp, N * sizeof(double), c->alP, K * sizeof(double), N * sizeof(double), NPR, out, c->s));
     if (gps)
-        HIPCHK(c, hipMemcpy2DAsync(gps, N * sizeof(double), c->alps, K * sizeof(double), N * sizeof(double), 3, out, c->s));
+        HIPCHK(c, hipMemcpy2DAsync(sps, N * sizeof(double), c->alps, K * sizeof(double), N * sizeof(double), 3, out, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
+    if (hout) {
+        if (gp) std::memcpy(gp, sp, NPR * N * sizeof(double));
+        if (gps) std::memcpy(gps, sps, 3 * N * sizeof(double));
+    }
     return 0;
 }
 

@@ -8,61 +8,11 @@
 // layout) so that every global access of a wavefront is one or two contiguous
 // 512-byte segments.  Built without FMA contraction so that the FP64 arithmetic
 // matches the reference's expression by expression.
-#include "hamr_kernels.h"
+#include "hamr_recon.h"
 
 #include <cstdio>
 
 namespace hamr {
-
-__device__ __forceinline__ void region_index(int t, int n2, int n3, int &a, int &b, int &c)
-{
-    c = t % n3;
-    int q = t / n3;
-    b = q % n2;
-    a = q / n2;
-}
-
-// Item w of the list in visiting order (see WorkList).
-__device__ __forceinline__ int2 wl_get(const WorkList &wl, int w)
-{
-    if (wl.swz) {
-        const int q = wl.n >> 3, r = wl.n & 7, x = w & 7, k = w >> 3;
-        w = x * q + (x < r ? x : r) + k;
-    }
-    return wl.items[w];
-}
-
-#define FOR_ITEMS(wl)                                                                  \
-    for (int w_ = blockIdx.x; w_ < (wl).n; w_ += gridDim.x)
-
-// Metric addressing of a block's geometry slab (axisymmetric: one column per
-// (i, j) of the block, padded to kcol; moff columns of earlier slabs).
-__device__ __forceinline__ int64_t mg10(const BlockDev &B, int pos, int s) { return B.moff * (NPOS * 10) + ((int64_t)pos * 10 + s) * B.kcol; }
-__device__ __forceinline__ int64_t mg1(const BlockDev &B, int pos) { return B.moff * NPOS + (int64_t)pos * B.kcol; }
-
-// Metric at one position of one column, rounded to R (the metric is stored in FP64).
-template <class R = double>
-__device__ __forceinline__ Geom<R> load_geom(const KParams &P, const BlockDev &B, int pos, int64_t col)
-{
-    Geom<R> G;
-    const double *gc = P.gcov + mg10(B, pos, 0) + col;
-    const double *gi = P.gcon + mg10(B, pos, 0) + col;
-#pragma unroll
-    for (int s = 0; s < 10; s++) {
-        G.gcov[s] = (R)gc[s * B.kcol];
-        G.gcon[s] = (R)gi[s * B.kcol];
-    }
-    G.g = (R)P.gdet[mg1(B, pos) + col];
-    if constexpr (sizeof(R) == 4) G.oma = (R)(1. - 1. / sqrt(-gi[0]));
-    return G;
-}
-
-__device__ __forceinline__ double wave_min(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
-    return v;
-}
 
 int64_t class_work(const BlockDev &B, int cls)
 {
@@ -77,198 +27,6 @@ int64_t class_work(const BlockDev &B, int cls)
     case KC_BOUND1: return 2 * NG * d2 * d3;
     case KC_BOUND2: return 2 * NG * d1 * d3;
     default: return 2 * NG * (d1 * d2 + 1);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Fused PPM reconstruction + HLL flux through the lower dir-faces of a block:
-// fluxcalcprep (GPU_program1.cu:78-190) and fluxcalc2D2 (:791-1068) in one pass,
-// without the dq1/dq2 round trip through HBM.  Face range: [NG, NG+BS] along
-// dir, [NG-1, NG+BS] across (the transverse ghost faces feed the corner EMFs).
-//
-// R is the arithmetic of reconstruction, state, fluxes and speeds: double for
-// the FP64 step, float for the mixed step (primitives, face fields and metric
-// are read in FP64 and rounded once; fluxes are stored as R).
-// At least 3 waves per SIMD (<= 168 VGPRs): the FP64 flux needs 169 registers
-// unconstrained, one too many for the third wave.
-template <int DIR, class R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_flux(KParams P, WorkList wl, const double *__restrict__ p,
-                                              const double *__restrict__ ps, R *__restrict__ F,
-                                              unsigned long long *__restrict__ dtmin)
-{
-    FOR_ITEMS(wl)
-    {
-        const int2 it = wl_get(wl, w_);
-        const int n = it.x;
-        const BlockDev &B = P.blk[n];
-        const int e1 = B.bs[0] + (DIR == 1 ? 1 : 2), e2 = B.bs[1] + (DIR == 2 ? 1 : 2), e3 = B.bs[2] + (DIR == 3 ? 1 : 2);
-        const int t = it.y * blockDim.x + threadIdx.x;
-        double vdt = 1.e9;
-        if (t < e1 * e2 * e3) {
-            int a, b, c;
-            region_index(t, e2, e3, a, b, c);
-            const int i = a + (DIR == 1 ? NG : NG - 1);
-            const int j = b + (DIR == 2 ? NG : NG - 1);
-            const int z = c + (DIR == 3 ? NG : NG - 1);
-            const int64_t K = B.ks;
-            const int64_t cell = ((int64_t)i * B.D[1] + j) * B.D[2] + z;
-            const int64_t sd = (DIR == 1) ? (int64_t)B.D[1] * B.D[2] : (DIR == 2) ? (int64_t)B.D[2] : 1;
-            const double *pb = p + NPR * B.off;
-            R pl[NPR], pr[NPR];
-#pragma unroll
-            for (int k = 0; k < NPR; k++) {
-                const double *q = pb + (int64_t)k * K + cell;
-                const R ym3 = (R)q[-3 * sd], ym2 = (R)q[-2 * sd], ym1 = (R)q[-sd], y0 = (R)q[0], yp1 = (R)q[sd],
-                        yp2 = (R)q[2 * sd];
-                R l, r;
-                para(ym3, ym2, ym1, y0, yp1, l, r);
-                pl[k] = r;
-                para(ym2, ym1, y0, yp1, yp2, l, r);
-                pr[k] = l;
-            }
-            const R bn = (R)ps[3 * B.off + (DIR - 1) * K + cell];
-            pl[B1 + DIR - 1] = bn;
-            pr[B1 + DIR - 1] = bn;
-            if (DIR == 2 && (((B.pole & 1) && j == NG) || ((B.pole & 2) && j == NG + B.bs[1]))) {
-                pl[B1] = R(0.);
-                pr[B1] = R(0.);
-            }
-            const int64_t col = (int64_t)i * B.D[1] + j;
-            const Geom<R> G = load_geom<R>(P, B, DIR == 1 ? FACE1 : DIR == 2 ? FACE2 : FACE3, col);
-            const R gam = (R)P.gam;
-            R t1[NPR], t2[NPR], t3[NPR], t4[NPR], cmax_l = R(0.), cmin_l = R(0.), cmax_r = R(0.), cmin_r = R(0.), dum;
-            State<R> q;
-            get_state(pl, G, q);
-            primtoflux(pl, q, DIR, G, gam, t1, &cmax_l, &cmin_l);
-            primtoflux(pl, q, 0, G, gam, t2, &dum, &dum);
-            get_state(pr, G, q);
-            primtoflux(pr, q, DIR, G, gam, t3, &cmax_r, &cmin_r);
-            primtoflux(pr, q, 0, G, gam, t4, &dum, &dum);
-            const R cmax = fabs(mymax(mymax(R(0.), cmax_l), cmax_r));
-            const R cmin = fabs(mymax(mymax(R(0.), -cmin_l), -cmin_r));
-            const R ctop = mymax(cmax, cmin);
-            R *Fb = F + NPR * B.off + cell;
-#pragma unroll
-            for (int k = 0; k < NPR; k++)
-                Fb[(int64_t)k * K] = dv(cmax * t1[k] + cmin * t3[k] - cmax * cmin * (t4[k] - t2[k]), cmax + cmin + R(kSMALL));
-            const double v = P.cour * B.dx[DIR - 1] / (double)ctop;
-            if (v < 1.e9 && v > 1.e-6) vdt = v;
-        }
-        if (dtmin) {
-            vdt = wave_min(vdt);
-            if ((threadIdx.x & 63) == 0 && vdt < 1.e9)
-                atomicMin(&dtmin[n * 3 + DIR - 1], (unsigned long long)__double_as_longlong(vdt));
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Cell-centre EMF, consttransport1 (GPU_const_trans.h:63-119), cells [NG-1, NG+BS].
-template <class R>
-__global__ void __launch_bounds__(256) k_emf_centre(KParams P, WorkList wl, const double *__restrict__ p,
-                                                    R *__restrict__ Ec)
-{
-    FOR_ITEMS(wl)
-    {
-        const int2 it = wl_get(wl, w_);
-        const BlockDev &B = P.blk[it.x];
-        const int e1 = B.bs[0] + 2, e2 = B.bs[1] + 2, e3 = B.bs[2] + 2;
-        const int t = it.y * blockDim.x + threadIdx.x;
-        if (t >= e1 * e2 * e3) continue;
-        int a, b, c;
-        region_index(t, e2, e3, a, b, c);
-        const int i = a + NG - 1, j = b + NG - 1, z = c + NG - 1;
-        const int64_t K = B.ks;
-        const int64_t cell = ((int64_t)i * B.D[1] + j) * B.D[2] + z;
-        const double *pb = p + NPR * B.off + cell;
-        R pr[NPR];
-#pragma unroll
-        for (int k = 0; k < NPR; k++) pr[k] = (R)pb[(int64_t)k * K];
-        const Geom<R> G = load_geom<R>(P, B, CENT, (int64_t)i * B.D[1] + j);
-        R ucon[4], ucov[4], bcon[4];
-        ucon_calc(pr, G, ucon);
-        lower10(ucon, G.gcov, ucov);
-        bcon_calc(pr, ucon, ucov, bcon);
-        R *E = Ec + 3 * B.off + cell;
-        E[0] = -G.g * (ucon[2] * bcon[3] - ucon[3] * bcon[2]);
-        E[K] = -G.g * (ucon[3] * bcon[1] - ucon[1] * bcon[3]);
-        E[2 * K] = -G.g * (ucon[1] * bcon[2] - ucon[2] * bcon[1]);
-    }
-}
-
-// Corner EMF upwinded on the mass-flux sign, consttransport2
-// (GPU_const_trans.h:121-339) with the polar rule (:233-236); corners [NG, NG+BS].
-template <class R>
-__global__ void __launch_bounds__(256) k_emf_corner(KParams P, WorkList wl, const R *__restrict__ Ec,
-                                                    const R *__restrict__ F1, const R *__restrict__ F2,
-                                                    const R *__restrict__ F3, R *__restrict__ E)
-{
-    FOR_ITEMS(wl)
-    {
-        const int2 it = wl_get(wl, w_);
-        const BlockDev &B = P.blk[it.x];
-        const int e1 = B.bs[0] + 1, e2 = B.bs[1] + 1, e3 = B.bs[2] + 1;
-        const int t = it.y * blockDim.x + threadIdx.x;
-        if (t >= e1 * e2 * e3) continue;
-        int a, b, c;
-        region_index(t, e2, e3, a, b, c);
-        const int i = a + NG, j = b + NG, z = c + NG;
-        const int64_t g = ((int64_t)i * B.D[1] + j) * B.D[2] + z;
-        const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2], sj = B.D[2];
-        const R *E1c = Ec + 3 * B.off, *E2c = E1c + K, *E3c = E2c + K;
-        const R *f1 = F1 + NPR * B.off, *f2 = F2 + NPR * B.off, *f3 = F3 + NPR * B.off;
-        const R *F1R = f1 + RHO * K, *F2R = f2 + RHO * K, *F3R = f3 + RHO * K;
-        const R *F1B2 = f1 + B2 * K, *F1B3 = f1 + B3 * K;
-        const R *F2B1 = f2 + B1 * K, *F2B3 = f2 + B3 * K;
-        const R *F3B1 = f3 + B1 * K, *F3B2 = f3 + B2 * K;
-        const R dEL13_1 = E1c[g] - F3B2[g];
-        const R dEL13_2 = E1c[g - sj] - F3B2[g - sj];
-        const R dER13_1 = F3B2[g] - E1c[g - 1];
-        const R dER13_2 = F3B2[g - sj] - E1c[g - sj - 1];
-        const R dEL12_1 = E1c[g] + F2B3[g];
-        const R dEL12_2 = E1c[g - 1] + F2B3[g - 1];
-        const R dER12_1 = -F2B3[g] - E1c[g - sj];
-        const R dER12_2 = -F2B3[g - 1] - E1c[g - sj - 1];
-        const R dEL21_1 = E2c[g] - F1B3[g];
-        const R dEL21_2 = E2c[g - 1] - F1B3[g - 1];
-        const R dER21_1 = F1B3[g] - E2c[g - si];
-        const R dER21_2 = F1B3[g - 1] - E2c[g - si - 1];
-        const R dEL23_1 = E2c[g] + F3B1[g];
-        const R dEL23_2 = E2c[g - si] + F3B1[g - si];
-        const R dER23_1 = -F3B1[g] - E2c[g - 1];
-        const R dER23_2 = -F3B1[g - si] - E2c[g - si - 1];
-        const R dEL31_1 = E3c[g] + F1B2[g];
-        const R dEL31_2 = E3c[g - sj] + F1B2[g - sj];
-        const R dER31_1 = -F1B2[g] - E3c[g - si];
-        const R dER31_2 = -F1B2[g - sj] - E3c[g - si - sj];
-        const R dEL32_1 = E3c[g] - F2B1[g];
-        const R dEL32_2 = E3c[g - si] - F2B1[g - si];
-        const R dER32_1 = F2B1[g] - E3c[g - sj];
-        const R dER32_2 = F2B1[g - si] - E3c[g - si - sj];
-#define LE(x) ((R)((x) <= R(0.)))
-#define GT(x) ((R)((x) > R(0.)))
-        R e1v = R(0.25) * ((-F2B3[g] - (dEL13_1 * LE(F2R[g]) + dEL13_2 * GT(F2R[g])))
-            + (-F2B3[g - 1] + (dER13_1 * LE(F2R[g - 1]) + dER13_2 * GT(F2R[g - 1])))
-            + (F3B2[g] - (dEL12_1 * LE(F3R[g]) + dEL12_2 * GT(F3R[g])))
-            + (F3B2[g - sj] + (dER12_1 * LE(F3R[g - sj]) + dER12_2 * GT(F3R[g - sj]))));
-        const R e2v = R(0.25) * ((-F3B1[g] - (dEL21_1 * LE(F3R[g]) + dEL21_2 * GT(F3R[g])))
-            + (-F3B1[g - si] + (dER21_1 * LE(F3R[g - si]) + dER21_2 * GT(F3R[g - si])))
-            + (F1B3[g] - (dEL23_1 * LE(F1R[g]) + dEL23_2 * GT(F1R[g])))
-            + (F1B3[g - 1] + (dER23_1 * LE(F1R[g - 1]) + dER23_2 * GT(F1R[g - 1]))));
-        R e3v = R(0.25) * ((F2B1[g] - (dEL31_1 * LE(F2R[g]) + dEL31_2 * GT(F2R[g])))
-            + (F2B1[g - si] + (dER31_1 * LE(F2R[g - si]) + dER31_2 * GT(F2R[g - si])))
-            + (-F1B2[g] - (dEL32_1 * LE(F1R[g]) + dEL32_2 * GT(F1R[g])))
-            + (-F1B2[g - sj] + (dER32_1 * LE(F1R[g - sj]) + dER32_2 * GT(F1R[g - sj]))));
-#undef LE
-#undef GT
-        if (((B.pole & 1) && j == NG) || ((B.pole & 2) && j == NG + B.bs[1])) {
-            e3v = R(0.);
-            e1v = R(-0.5) * (F2B3[g] + F2B3[g - 1]);
-        }
-        R *Eb = E + 3 * B.off + g;
-        Eb[0] = e1v;
-        Eb[K] = e2v;
-        Eb[2 * K] = e3v;
     }
 }
 
@@ -512,6 +270,19 @@ __global__ void __launch_bounds__(256) k_cons(KParams P, WorkList wl, int full, 
     }
 }
 
+// Adds the number of active lanes with `pred` set to *cnt with one atomic per
+// wavefront.  All lanes of a wavefront work on the same block (a tile never
+// spans two blocks), so they share the counter.  Most atmosphere cells sit on
+// the floor, so a per-cell atomic would issue millions of device-scope atomics
+// per half step.
+__device__ __forceinline__ void wave_count(int *cnt, bool pred)
+{
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    const unsigned long long act = __ballot(1);
+    if (__lane_id() == __ffsll((unsigned long long)act) - 1) atomicAdd(cnt, __popcll(m));
+}
+
 // Floors (the build's specification: rho >= rho_floor r^-1.5, u >= u_floor
 // r^-2.5, flag -333 as GPU_fixup.h:399-415) and the result of one cell.
 __device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B, int n, int64_t col, int64_t g,
@@ -529,10 +300,8 @@ __device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B,
         const double uflr = P.u_floor / (r * r * sr);
         if (pf[UU] < uflr) { pf[UU] = uflr; hit = 1; }
     }
-    if (hit) {
-        flag = -333;
-        atomicAdd(&failcnt[n * 4 + 3], 1);
-    }
+    if (hit) flag = -333;
+    wave_count(&failcnt[n * 4 + 3], hit != 0);
     pflag[B.off + g] = flag;
 #pragma unroll
     for (int k = 0; k < NPR; k++) pf_out[base + k * K] = pf[k];
@@ -881,63 +650,39 @@ __global__ void k_fill_u64(unsigned long long *a, int n, unsigned long long v)
 }
 
 // ---------------------------------------------------------------------------
-// launchers
-static inline dim3 gdim(const Launch &L) { return dim3((unsigned)L.grid); }
-static inline dim3 bdim(const Launch &L) { return dim3((unsigned)L.threads); }
-
-template <class R>
-static void flux_t(const KParams &P, int dir, const Launch &L, const double *p, const double *ps, R *F,
-                   unsigned long long *dtmin, hipStream_t s)
-{
-    if (L.wl.n <= 0) return;
-    if (dir == 1) hipLaunchKernelGGL((k_flux<1, R>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
-    else if (dir == 2) hipLaunchKernelGGL((k_flux<2, R>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
-    else hipLaunchKernelGGL((k_flux<3, R>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
-}
+// launchers.  The FP32 reconstruction, flux and EMF kernels are instantiated in
+// hamr_kernels_f32.hip, the FP64 ones here.
 
 void launch_flux(const KParams &P, int dir, const Launch &L, const double *p, const double *ps, void *F,
                  unsigned long long *dtmin, int mixed, hipStream_t s)
 {
-    if (mixed) flux_t(P, dir, L, p, ps, (float *)F, dtmin, s);
-    else flux_t(P, dir, L, p, ps, (double *)F, dtmin, s);
-}
-
-template <class R>
-static void emf_t(const KParams &P, const Launch &Lc, const Launch &Lk, const double *p, const R *F1, const R *F2,
-                  const R *F3, R *Ec, R *E, hipStream_t s)
-{
-    if (Lc.wl.n <= 0) return;
-    hipLaunchKernelGGL((k_emf_centre<R>), gdim(Lc), bdim(Lc), 0, s, P, Lc.wl, p, Ec);
-    hipLaunchKernelGGL((k_emf_corner<R>), gdim(Lk), bdim(Lk), 0, s, P, Lk.wl, Ec, F1, F2, F3, E);
-}
-
-void launch_emf(const KParams &P, const Launch &centre, const Launch &corner, const double *p, const void *F1,
-                const void *F2, const void *F3, void *Ec, void *E, int mixed, hipStream_t s)
-{
-    if (mixed)
-        emf_t(P, centre, corner, p, (const float *)F1, (const float *)F2, (const float *)F3, (float *)Ec, (float *)E, s);
-    else
-        emf_t(P, centre, corner, p, (const double *)F1, (const double *)F2, (const double *)F3, (double *)Ec, (double *)E,
-              s);
+    if (mixed) launch_flux_f32(P, dir, L, p, ps, (float *)F, dtmin, s);
+    else recon_flux(P, dir, L, p, ps, (double *)F, dtmin, s);
 }
 
 void launch_emf_centre(const KParams &P, const Launch &L, const double *p, void *Ec, int mixed, hipStream_t s)
 {
-    if (L.wl.n <= 0) return;
-    if (mixed) hipLaunchKernelGGL((k_emf_centre<float>), gdim(L), bdim(L), 0, s, P, L.wl, p, (float *)Ec);
-    else hipLaunchKernelGGL((k_emf_centre<double>), gdim(L), bdim(L), 0, s, P, L.wl, p, (double *)Ec);
+    if (mixed) launch_emf_centre_f32(P, L, p, (float *)Ec, s);
+    else recon_emf_centre(P, L, p, (double *)Ec, s);
 }
 
 void launch_emf_corner(const KParams &P, const Launch &L, const void *Ec, const void *F1, const void *F2,
                        const void *F3, void *E, int mixed, hipStream_t s)
 {
-    if (L.wl.n <= 0) return;
     if (mixed)
-        hipLaunchKernelGGL((k_emf_corner<float>), gdim(L), bdim(L), 0, s, P, L.wl, (const float *)Ec, (const float *)F1,
-                           (const float *)F2, (const float *)F3, (float *)E);
+        launch_emf_corner_f32(P, L, (const float *)Ec, (const float *)F1, (const float *)F2, (const float *)F3,
+                              (float *)E, s);
     else
-        hipLaunchKernelGGL((k_emf_corner<double>), gdim(L), bdim(L), 0, s, P, L.wl, (const double *)Ec,
-                           (const double *)F1, (const double *)F2, (const double *)F3, (double *)E);
+        recon_emf_corner(P, L, (const double *)Ec, (const double *)F1, (const double *)F2, (const double *)F3,
+                         (double *)E, s);
+}
+
+void launch_emf(const KParams &P, const Launch &centre, const Launch &corner, const double *p, const void *F1,
+                const void *F2, const void *F3, void *Ec, void *E, int mixed, hipStream_t s)
+{
+    if (centre.wl.n <= 0) return;
+    launch_emf_centre(P, centre, p, Ec, mixed, s);
+    launch_emf_corner(P, corner, Ec, F1, F2, F3, E, mixed, s);
 }
 
 void launch_pole(const KParams &P, const int *ring, const int *rside, int nring, int len, int ni_max, double dt_val,

@@ -1,0 +1,64 @@
+// hamr_kernels_dev.h -- device helpers shared by the kernel translation units:
+// work-list walking, metric addressing and loads, wavefront reductions.
+#pragma once
+
+#include "hamr_kernels.h"
+
+namespace hamr {
+
+
+__device__ __forceinline__ void region_index(int t, int n2, int n3, int &a, int &b, int &c)
+{
+    c = t % n3;
+    int q = t / n3;
+    b = q % n2;
+    a = q / n2;
+}
+
+// Item w of the list in visiting order (see WorkList).
+__device__ __forceinline__ int2 wl_get(const WorkList &wl, int w)
+{
+    if (wl.swz) {
+        const int q = wl.n >> 3, r = wl.n & 7, x = w & 7, k = w >> 3;
+        w = x * q + (x < r ? x : r) + k;
+    }
+    return wl.items[w];
+}
+
+#define FOR_ITEMS(wl)                                                                  \
+    for (int w_ = blockIdx.x; w_ < (wl).n; w_ += gridDim.x)
+
+// Metric addressing of a block's geometry slab (axisymmetric: one column per
+// (i, j) of the block, padded to kcol; moff columns of earlier slabs).
+__device__ __forceinline__ int64_t mg10(const BlockDev &B, int pos, int s) { return B.moff * (NPOS * 10) + ((int64_t)pos * 10 + s) * B.kcol; }
+__device__ __forceinline__ int64_t mg1(const BlockDev &B, int pos) { return B.moff * NPOS + (int64_t)pos * B.kcol; }
+
+// Metric at one position of one column, rounded to R (the metric is stored in FP64).
+template <class R = double>
+__device__ __forceinline__ Geom<R> load_geom(const KParams &P, const BlockDev &B, int pos, int64_t col)
+{
+    Geom<R> G;
+    const double *gc = P.gcov + mg10(B, pos, 0) + col;
+    const double *gi = P.gcon + mg10(B, pos, 0) + col;
+#pragma unroll
+    for (int s = 0; s < 10; s++) {
+        G.gcov[s] = (R)gc[s * B.kcol];
+        G.gcon[s] = (R)gi[s * B.kcol];
+    }
+    G.g = (R)P.gdet[mg1(B, pos) + col];
+    if constexpr (sizeof(R) == 4) G.oma = (R)(1. - 1. / sqrt(-gi[0]));
+    return G;
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// launch geometry of a work-list launch
+static inline dim3 gdim(const Launch &L) { return dim3((unsigned)L.grid); }
+static inline dim3 bdim(const Launch &L) { return dim3((unsigned)L.threads); }
+
+}  // namespace hamr

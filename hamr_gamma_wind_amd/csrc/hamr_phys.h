@@ -228,12 +228,17 @@ template <class R> HD R mc_slope(R ym, R y0, R yp)
     return (Dqm * Dqp > R(0.)) ? v : R(0.);
 }
 
-// GPU_interp.h:60-111: CW84 PPM with MC (factor 2) slopes; l = a_L, r = a_R.
-template <class R> HD void para(R x1, R x2, R x3, R x4, R x5, R &lout, R &rout)
+// PPM interface value between cells with values ym, yp and MC slopes dqm, dqp
+// (GPU_interp.h:89-90): a_{c+1/2} = (y_c + y_{c+1})/2 - (dq_{c+1} - dq_c)/6.
+template <class R> HD R ppm_face(R ym, R yp, R dqm, R dqp)
 {
-    const R dq1 = mc_slope(x1, x2, x3), dq2 = mc_slope(x2, x3, x4), dq3 = mc_slope(x3, x4, x5);
-    R l = R(0.5) * (x3 + x2) - dv(dq2 - dq1, R(6.0));
-    R r = R(0.5) * (x4 + x3) - dv(dq3 - dq2, R(6.0));
+    return R(0.5) * (yp + ym) - dv(dqp - dqm, R(6.0));
+}
+
+// CW84 monotonicity limits of the parabola of a cell with mean x3 and interface
+// values l, r (GPU_interp.h:92-108).
+template <class R> HD void ppm_limit(R x3, R l, R r, R &lout, R &rout)
+{
     const R qa = (r - x3) * (x3 - l);
     const R qd = (r - l);
     const R qe = R(6.0) * (x3 - R(0.5) * (l + r));
@@ -242,6 +247,13 @@ template <class R> HD void para(R x1, R x2, R x3, R x4, R x5, R &lout, R &rout)
     else if (qd * (qd + qe) < R(0.0)) r = R(3.0) * x3 - R(2.0) * l;
     lout = l;
     rout = r;
+}
+
+// GPU_interp.h:60-111: CW84 PPM with MC (factor 2) slopes; l = a_L, r = a_R.
+template <class R> HD void para(R x1, R x2, R x3, R x4, R x5, R &lout, R &rout)
+{
+    const R dq1 = mc_slope(x1, x2, x3), dq2 = mc_slope(x2, x3, x4), dq3 = mc_slope(x3, x4, x5);
+    ppm_limit(x3, ppm_face(x2, x3, dq1, dq2), ppm_face(x3, x4, dq2, dq3), lout, rout);
 }
 
 // ---------------------------------------------------------------------------
