@@ -161,6 +161,7 @@ def load_library(path=None):
         "hamr_get_array": [ctxp, C.c_int, _dp],
         "hamr_fluxcalc": [ctxp, C.c_int, C.c_int, C.c_int],
         "hamr_block_dt": [ctxp, C.c_int, C.c_int, _dp],
+        "hamr_flux_correct": [ctxp, C.c_int, C.c_double],
         "hamr_ct_emf": [ctxp, C.c_int, C.c_int],
         "hamr_ct_bound": [ctxp, C.c_double],
         "hamr_ct_update": [ctxp, C.c_int, C.c_double, C.c_int],
@@ -269,14 +270,15 @@ def desc_torus(desc, tdesc):
     return p, ps
 
 
-AMR_INFO_KEYS = ("blocks", "geometries", "cells", "zones", "ex1", "ex2", "ex3", "pro1", "pro2", "pro3", "rings", "amr")
+AMR_INFO_KEYS = ("blocks", "geometries", "cells", "zones", "ex1", "ex2", "ex3", "pro1", "pro2", "pro3", "rings", "amr",
+                 "reflux1", "reflux2", "reflux3")
 
 
 def desc_amr(desc, amr, tables=False):
     """The multi-level block set (host only): info, block rows (level, o, bs,
     geo, pole, face kinds) and, with tables=True, the exchange entries per stage."""
     L = load_library()
-    info = np.zeros(12, dtype=np.int64)
+    info = np.zeros(len(AMR_INFO_KEYS), dtype=np.int64)
     r = L.hamr_desc_amr(C.byref(desc), C.byref(amr), _ptr(info, _llp), None, None, None)
     if r:
         raise HamrError("hamr_desc_amr: description rejected (%d)" % r)
@@ -439,6 +441,9 @@ class Context:
         self._chk(self.L.hamr_block_dt(self.h, n, d, C.byref(v)), "hamr_block_dt")
         return v.value
 
+    def flux_correct(self, flag, dt):
+        self._chk(self.L.hamr_flux_correct(self.h, flag, dt), "hamr_flux_correct")
+
     def ct_emf(self, flag, n):
         self._chk(self.L.hamr_ct_emf(self.h, flag, n), "hamr_ct_emf")
 
@@ -561,7 +566,7 @@ class AmrContext(Context):
         self.Ds = [tuple(int(b) + 2 * NG for b in self.layout[n, 4:7]) for n in range(self.nblk)]
 
     def info(self):
-        v = np.zeros(12, dtype=np.int64)
+        v = np.zeros(len(AMR_INFO_KEYS), dtype=np.int64)
         self._chk(self.L.hamr_amr_info(self.h, _ptr(v, _llp)), "hamr_amr_info")
         return {k: int(x) for k, x in zip(AMR_INFO_KEYS, v)}
 
@@ -610,8 +615,9 @@ class AmrContext(Context):
 def advance_step(ctx, dt):
     """One predictor + corrector step through the per-block stage API, in the
     order of advance_GPU (step_ch.c:997-1336) with every block at timelevel 1:
-    fluxes dir 3, 2, 1; CT EMF; polar E1 average; face-B update; conserved
-    update + inversion; ghost fill.  Returns the next dt (per-block harmonic of
+    fluxes dir 3, 2, 1; flux correction at level jumps (full step); CT EMF;
+    polar E1 average; face-B update; conserved update + inversion; ghost fill.
+    Returns the next dt (per-block harmonic of
     the corrector's Courant minima, minimum over blocks, clamped to 1.3 dt)."""
     nb = ctx.nblk
     for flag, Dt in ((0, 0.5 * dt), (1, dt)):
@@ -624,6 +630,7 @@ def advance_step(ctx, dt):
             for n in range(nb):
                 for d in (1, 2, 3):
                     ctx.block_dt(n, d)
+        ctx.flux_correct(flag, dt)
         for n in range(nb):
             ctx.ct_emf(flag, n)
         ctx.ct_bound(dt)

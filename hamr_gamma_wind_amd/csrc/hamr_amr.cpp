@@ -450,6 +450,42 @@ std::string build_amr(const GridDesc &d, const AmrDesc &a, AmrGrid &g)
                       [](const AmrProlong &x, const AmrProlong &y) { return x.dcell < y.dcell; });
         }
     }
+    // flux correction at level jumps (flux_rec1-3 with calc_corr = 1,
+    // boundcomF.c:583-1315): every interior face cell of a block face whose
+    // neighbour across it is one level finer, with the finer block's face that
+    // tiles its lowest transverse quarter
+    for (int d = 0; d < 3; d++) {
+        g.rfx[d].clear();
+        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;  // transverse, slower first
+        for (int n = 0; n < nblk; n++) {
+            const BlockDev &C = b.blk[n];
+            const Level &L = b.lev[C.level];
+            for (int side = 0; side < 2; side++) {
+                int Y[3];
+                Y[d] = side ? C.o[d] + C.bs[d] : C.o[d] - 1;
+                if (d < 2 && (Y[d] < 0 || Y[d] >= L.N[d])) continue;  // radial edge or axis
+                if (d == 2) Y[2] = ((Y[2] % L.N[2]) + L.N[2]) % L.N[2];
+                for (int ya = C.o[ta]; ya < C.o[ta] + C.bs[ta]; ya++)
+                    for (int yb = C.o[tb]; yb < C.o[tb] + C.bs[tb]; yb++) {
+                        Y[ta] = ya;
+                        Y[tb] = yb;
+                        int lc = -1;
+                        const int f = b.find(C.level, Y, &lc);
+                        if (f < 0 || lc != C.level + 1) continue;
+                        const BlockDev &Fb = b.blk[f];
+                        int Z[3];
+                        Z[d] = side ? 2 * Y[d] : 2 * Y[d] + 1;  // the finer cell beside the face
+                        Z[ta] = 2 * ya;
+                        Z[tb] = 2 * yb;
+                        if (!interior_of(Fb, Z)) return "amr: flux correction source outside its block";
+                        const int sd = d == 0 ? Fb.D[1] * Fb.D[2] : d == 1 ? Fb.D[2] : 1;
+                        int K[3] = {Y[0], Y[1], Y[2]};
+                        K[d] = side ? C.o[d] + C.bs[d] : C.o[d];
+                        g.rfx[d].push_back({n, cell_in(C, K), f, cell_in(Fb, Z) + (side ? 0 : sd)});
+                    }
+            }
+        }
+    }
     g.blocks = b.blk;
     g.nlev = a.nlev;
     for (int l = 0; l < a.nlev; l++)
@@ -491,6 +527,15 @@ std::string validate_amr(const AmrGrid &g)
                 if (!src_ok(blk, e.nb[k][0], e.nb[k][1])) return "amr: prolongation neighbour out of range";
         }
     }
+    for (int d = 0; d < 3; d++)
+        for (const AmrReflux &e : g.rfx[d]) {
+            if (!dst_ok(e.cb, e.cf)) return "amr: flux correction entry out of range";
+            if (e.fb < 0 || e.fb >= (int)blk.size()) return "amr: flux correction source out of range";
+            const BlockDev &F = blk[e.fb];
+            const int sa = d == 0 ? F.D[2] : F.D[1] * F.D[2], sb = d == 2 ? F.D[2] : 1;
+            if (!dst_ok(e.fb, e.ff) || !dst_ok(e.fb, e.ff + sa + sb))
+                return "amr: flux correction source out of range";
+        }
     for (size_t n = 0; n < blk.size(); n++) {
         const BlockDev &B = blk[n];
         for (int f = 0; f < 6; f++)

@@ -75,6 +75,8 @@ struct hamr_ctx {
     AmrEntry *ex[3] = {nullptr, nullptr, nullptr};
     AmrProlong *pro[3] = {nullptr, nullptr, nullptr};
     int64_t nex[3] = {0, 0, 0}, npro[3] = {0, 0, 0};
+    AmrReflux *rfx[3] = {nullptr, nullptr, nullptr};  // flux correction at level jumps, per direction
+    int64_t nrfx[3] = {0, 0, 0};
     hipGraphExec_t graph = nullptr;
     int prec = HAMR_PREC_FP64;     // arithmetic of reconstruction, fluxes and EMFs
     // adjoint scratch (allocated by the first adjoint call)
@@ -300,6 +302,15 @@ static void bound_all(hamr_ctx *c, double *p, double *ps)
     launch_bound(c->P, 1, all_launch(c, TS_BOUND1), p, ps, c->pflag, 1, c->s);
 }
 
+// Flux correction at level jumps of the corrector's fluxes (flux_rec1-3 with
+// calc_corr = 1 after the fluxes of every block, step_ch.c:1142-1196); dt from
+// the device when dt_ptr is set.  Nothing on a single-level set.
+static void reflux_all(hamr_ctx *c, double dt_val, const double *dt_ptr)
+{
+    for (int d = 0; d < 3; d++)
+        launch_amr_reflux(c->P, c->rfx[d], c->nrfx[d], d + 1, c->F[d], dt_val, dt_ptr, c->prec, c->s);
+}
+
 static void emf_all(hamr_ctx *c, const double *p)
 {
     launch_emf(c->P, all_launch(c, TS_ECENT), all_launch(c, TS_ECORNER), p, c->F[0], c->F[1], c->F[2], c->Ec, c->E,
@@ -426,6 +437,11 @@ int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *a, hamr_ctx **ou
             if (dalloc(c, &c->pro[s], g.pro[s].size())) return -1;
             if (h2d(c, c->pro[s], g.pro[s].data(), g.pro[s].size() * sizeof(AmrProlong))) return -1;
         }
+        c->nrfx[s] = (int64_t)g.rfx[s].size();
+        if (c->nrfx[s]) {
+            if (dalloc(c, &c->rfx[s], g.rfx[s].size())) return -1;
+            if (h2d(c, c->rfx[s], g.rfx[s].data(), g.rfx[s].size() * sizeof(AmrReflux))) return -1;
+        }
     }
     c->err.clear();
     return 0;
@@ -446,6 +462,7 @@ int hamr_amr_info(const hamr_ctx *c, long long *info)
     }
     info[10] = c->nring;
     info[11] = c->amr ? 1 : 0;
+    for (int d = 0; d < 3; d++) info[12 + d] = c->nrfx[d];
     return 0;
 }
 
@@ -657,6 +674,7 @@ int hamr_desc_amr(const hamr_desc *desc, const hamr_amr_desc *a, long long *info
     }
     info[10] = (long long)g.rside.size();
     info[11] = 1;
+    for (int d = 0; d < 3; d++) info[12 + d] = (long long)g.rfx[d].size();
     if (rows)
         for (size_t n = 0; n < g.blocks.size(); n++) {
             const BlockDev &B = g.blocks[n];
@@ -997,6 +1015,13 @@ int hamr_fluxcalc(hamr_ctx *c, int dir, int flag, int n)
     return check_launch(c, "hamr_fluxcalc");
 }
 
+int hamr_flux_correct(hamr_ctx *c, int flag, double dt)
+{
+    if (!c) return -1;
+    if (flag) reflux_all(c, dt, nullptr);
+    return check_launch(c, "hamr_flux_correct");
+}
+
 int hamr_block_dt(hamr_ctx *c, int n, int dir, double *out)
 {
     if (!c || !valid_block(c, n) || dir < 1 || dir > 3 || !out) return fail(c, "hamr_block_dt: bad arguments");
@@ -1089,6 +1114,7 @@ static void enqueue_step(hamr_ctx *c)
     // corrector: U0 -> p/ps with Dt = dt, fluxes from ph/psh, Courant minima recorded
     for (int dir = 1; dir <= 3; dir++)
         launch_flux(P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->ph, c->psh, c->F[dir - 1], c->dtmin, c->prec, s);
+    reflux_all(c, 0., c->dt);
     emf_all(c, c->ph);
     pole(c, 0., c->dt);
     launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 1.0, c->ps, c->ps, c->E, c->prec, s);

@@ -58,7 +58,15 @@ struct AmrProlong {    // prolongation from the coarse cell (cblk, ccell)
     int cblk, ccell;
     int nb[6][2];      // (block, code) of the coarse cell's -x1,+x1,-x2,+x2,-x3,+x3 neighbours
 };
-static_assert(sizeof(AmrEntry) == 16 && sizeof(AmrProlong) == 64, "exchange entry layout");
+// Flux correction at a level jump: the face cf (cell index of the face's upper
+// cell) of block cb along one direction takes the mean of the four faces of
+// the one level finer block fb that tile it: ff, ff + sb, ff + sa, ff + sa + sb,
+// with sa, sb fb's strides of the two transverse directions (the slower first).
+struct AmrReflux {
+    int cb, cf;
+    int fb, ff;
+};
+static_assert(sizeof(AmrEntry) == 16 && sizeof(AmrProlong) == 64 && sizeof(AmrReflux) == 16, "table entry layout");
 
 // Static description of a run (mirrors the reference's compile-time config).
 struct GridDesc {

@@ -5,7 +5,8 @@
  * hamr_host_advance is advance_GPU / GPU_step_ch (step_ch.c:916-1336) with
  * every block at timelevel 1: per half step, the fluxes of each block
  * (GPU_fluxcalc2D, dir 3, 2, 1), each block's Courant dt read back to the
- * host (fluxcalc_GPU), the EMFs of each block, the polar E1 average, the
+ * host (fluxcalc_GPU), the flux correction at level jumps in the full step
+ * (flux_send / flux_rec), the EMFs of each block, the polar E1 average, the
  * face-field update of each block, the conserved update and inversion of each
  * block, and the ghost fill; the next dt is the minimum over blocks of the
  * per-block harmonic of the corrector's direction minima (step_ch.c:1313-1322),
@@ -29,6 +30,7 @@ int hamr_host_advance(hamr_ctx *ctx, int nblk, double dt, double *ndt_out)
             for (n = 0; n < nblk && !r; n++) r = hamr_fluxcalc(ctx, d, flag, n);
         for (n = 0; n < nblk && !r; n++)
             for (d = 1; d <= 3 && !r; d++) r = hamr_block_dt(ctx, n, d, &bdt[3 * n + d - 1]);
+        if (!r) r = hamr_flux_correct(ctx, flag, dt);
         for (n = 0; n < nblk && !r; n++) r = hamr_ct_emf(ctx, flag, n);
         if (!r) r = hamr_ct_bound(ctx, dt);
         for (n = 0; n < nblk && !r; n++) r = hamr_ct_update(ctx, flag, Dt, n);

@@ -97,6 +97,11 @@ void launch_adjoint(const KParams &P, const BlockDev &B, double Dt, const double
 // rows of a polar face (the final pass of the multi-level fill).
 void launch_bound(const KParams &P, int dir, const Launch &L, double *p, double *ps, int *pflag, int outflow_only,
                   hipStream_t s);
+// Flux correction at level jumps along direction dir (1..3): each entry's
+// coarse face takes the mean of its four finer faces (F: the direction's flux
+// array, FP32 when mixed); dt from dt_ptr if non-null.
+void launch_amr_reflux(const KParams &P, const AmrReflux *e, int64_t n, int dir, void *F, double dt_val,
+                       const double *dt_ptr, int mixed, hipStream_t s);
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);
 void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s);
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);

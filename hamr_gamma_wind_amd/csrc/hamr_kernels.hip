@@ -10,6 +10,7 @@
 // matches the reference's expression by expression.
 #include "hamr_recon.h"
 
+#include <algorithm>
 #include <cstdio>
 
 namespace hamr {
@@ -609,6 +610,33 @@ __global__ void __launch_bounds__(256) k_amr_prolong(KParams P, const AmrProlong
         if (pflag) pflag[B.off + E.dcell] = pflag[C.off + E.ccell];
     }
 }
+// Flux correction at level jumps (unpackreceive1flux-3flux with calc_corr = 1,
+// GPU_program2.cu:1258-1617, fed by packsendfluxaverage1-3, :1618-1771): the
+// coarse face takes ((0.25 dt) (f00 + f01 + f10 + f11)) / dt of the four finer
+// faces that tile it, f_ab at transverse offsets a (slower) and b.  One thread
+// per coarse face; the corrector's fluxes only (the reference sends them from
+// blocks doing their full step).  Coarse faces written and finer faces read
+// are distinct faces, so the entries of one direction need no ordering.
+template <class R>
+__global__ void __launch_bounds__(256) k_amr_reflux(KParams P, const AmrReflux *__restrict__ e, int64_t n, int dir,
+                                                    R *__restrict__ F, double dt_val, const double *__restrict__ dt_ptr)
+{
+    const double dt = dt_ptr ? dt_ptr[0] : dt_val;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const AmrReflux x = e[t];
+        const BlockDev &C = P.blk[x.cb], &Fb = P.blk[x.fb];
+        const int64_t sa = dir == 1 ? Fb.D[2] : (int64_t)Fb.D[1] * Fb.D[2], sb = dir == 3 ? Fb.D[2] : 1;
+        const R *f = F + NPR * Fb.off + x.ff;
+        R *c = F + NPR * C.off + x.cf;
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            const R *q = f + (int64_t)k * Fb.ks;
+            const double sum = (double)q[0] + (double)q[sb] + (double)q[sa] + (double)q[sa + sb];
+            c[(int64_t)k * C.ks] = (R)(((0.25 * dt) * sum) / dt);
+        }
+    }
+}
+
 
 // Courant dt of the step: per-block harmonic of the direction minima, minimum
 // over blocks (step_ch.c:1313-1322), clamped to 1.3 dt (step_ch.c:974).  Resets
@@ -784,6 +812,17 @@ void launch_amr_exchange(const KParams &P, const AmrEntry *e, int64_t n, const A
         const unsigned g = (unsigned)(grid > 0 && grid < need ? grid : need);
         hipLaunchKernelGGL(k_amr_prolong, dim3(g), dim3(threads), 0, s, P, pe, npro, p, ps, pflag);
     }
+}
+
+void launch_amr_reflux(const KParams &P, const AmrReflux *e, int64_t n, int dir, void *F, double dt_val,
+                       const double *dt_ptr, int mixed, hipStream_t s)
+{
+    if (n <= 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+    if (mixed)
+        hipLaunchKernelGGL(k_amr_reflux<float>, dim3(g), dim3(256), 0, s, P, e, n, dir, (float *)F, dt_val, dt_ptr);
+    else
+        hipLaunchKernelGGL(k_amr_reflux<double>, dim3(g), dim3(256), 0, s, P, e, n, dir, (double *)F, dt_val, dt_ptr);
 }
 
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s)

@@ -104,7 +104,7 @@ int hamr_desc_coords(const hamr_desc *desc, double *startx, double *dx);
 int hamr_desc_metric_at(const hamr_desc *desc, const double *X, double *gcov, double *gcon, double *gdet,
                         double *conn, double *r_th);
 /* The multi-level block set of `amr` without a device (as hamr_create_amr
- * builds it): info[12] as hamr_amr_info; rows (15 ints per block, as
+ * builds it): info[15] as hamr_amr_info; rows (15 ints per block, as
  * hamr_block_layout), the copy/restriction entries of stages 1-3 one after
  * another (4 ints: ghost block, ghost cell, source block, source cell |
  * kind << 28 with kind 1 = cell, 2 = mean of the 2x2x2 cells from there) and
@@ -139,9 +139,10 @@ int hamr_create(const hamr_desc *desc, hamr_ctx **out);
  * of whole arrays: the blocks' [component][cell] slabs one after another, each
  * with its own (bs+6)^3 cell count. */
 int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *amr, hamr_ctx **out);
-/* info[12]: blocks, geometry slabs, padded cells, interior zones, exchange
+/* info[15]: blocks, geometry slabs, padded cells, interior zones, exchange
  * entries (copy/restriction) of stages 1-3, prolongation entries of stages
- * 1-3, polar rings, 1 if multi-level. */
+ * 1-3, polar rings, 1 if multi-level, coarse faces at level jumps along x1,
+ * x2, x3 (flux correction entries). */
 int hamr_amr_info(const hamr_ctx *ctx, long long *info);
 /* row[15]: level, origin o1 o2 o3 (cells of its level), bs1 bs2 bs3, geometry
  * slab, pole bits, kinds of faces -x1..+x3 (0 copy, 1 outflow, 2 polar mirror,
@@ -203,6 +204,13 @@ int hamr_fluxcalc(hamr_ctx *ctx, int dir, int flag, int n);
 /* fluxcalc_GPU (GPU_main.cu:1091): the block's minimum cour*dx/ctop of the last
  * fluxcalc in direction dir (1e9 if none); blocking; resets the minimum. */
 int hamr_block_dt(hamr_ctx *ctx, int n, int dir, double *out);
+/* flux_send1-3 / flux_rec1-3 with calc_corr = 1 (boundcomF.c:5-1315; called
+ * between the fluxes and consttransport1, step_ch.c:1142-1196): after every
+ * block's fluxes of the full step (flag 1), each face of a block whose
+ * neighbour across it is one level finer takes ((0.25 dt) (f00 + f01 + f10 +
+ * f11)) / dt of the four finer faces that tile it.  flag 0 (the half step) and
+ * single-level sets: nothing. */
+int hamr_flux_correct(hamr_ctx *ctx, int flag, double dt);
 /* GPU_consttransport1 + GPU_consttransport2 (GPU_main.cu:1365, :1393). */
 int hamr_ct_emf(hamr_ctx *ctx, int flag, int n);
 /* GPU_consttransport_bound (GPU_main.cu:1144): polar E1 average over each ring

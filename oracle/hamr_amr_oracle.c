@@ -431,6 +431,76 @@ static void ring_average(const oracle_amr *g, double **E, double dt)
     free(ring);
 }
 
+/* Flux correction at level jumps (flux_send / flux_rec with calc_corr = 1,
+ * boundcomF.c:5-1315; packsendfluxaverage1-3, GPU_program2.cu:1618-1771;
+ * unpackreceive1flux-3flux, :1258-1617): a face of a block whose neighbour
+ * across it is one level finer takes the flux ((0.25 dt) (f00 + f01 + f10 +
+ * f11)) / dt of the four finer faces that tile it, f_ab at transverse offsets
+ * a (the slower direction) and b; the finer faces keep their own fluxes
+ * (AVG_FLUXES 0).  The leaf across each face cell is found by searching the
+ * block list. */
+void oracle_amr_reflux(const oracle_amr *g, double **F1, double **F2, double **F3, double dt)
+{
+    double **F[3] = {F1, F2, F3};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < g->nblk; b++) {
+        const oracle_amr_block *C = &g->blk[b];
+        const long NC = geo_of(C).ncell;
+        for (int d = 0; d < 3; d++) {
+            const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2; /* transverse directions, slower first */
+            for (int side = 0; side < 2; side++) {
+                int Y[3];
+                Y[d] = side ? C->o[d] + C->bs[d] : C->o[d] - 1;
+                if (d < 2 && (Y[d] < 0 || Y[d] >= g->N[C->level][d])) continue; /* radial edge or axis */
+                if (d == 2) Y[2] = wrap(Y[2], g->N[C->level][2]);
+                for (int ya = C->o[ta]; ya < C->o[ta] + C->bs[ta]; ya++)
+                    for (int yb = C->o[tb]; yb < C->o[tb] + C->bs[tb]; yb++) {
+                        Y[ta] = ya;
+                        Y[tb] = yb;
+                        int lc;
+                        const int f = find_leaf(g, C->level, Y, &lc);
+                        if (f < 0 || lc != C->level + 1) continue;
+                        const oracle_amr_block *Fb = &g->blk[f];
+                        const long NF = geo_of(Fb).ncell;
+                        /* the finer cell beside the face (lowest transverse corner) and its face */
+                        int Z[3], L[3], K[3];
+                        Z[d] = side ? 2 * Y[d] : 2 * Y[d] + 1;
+                        Z[ta] = 2 * ya;
+                        Z[tb] = 2 * yb;
+                        for (int k = 0; k < 3; k++) L[k] = Z[k] - Fb->o[k] + NG;
+                        L[d] += side ? 0 : 1;
+                        for (int k = 0; k < 3; k++) K[k] = (k == d) ? (side ? NG + C->bs[d] : NG) : Y[k] - C->o[k] + NG;
+                        int Lb[3] = {L[0], L[1], L[2]}, La[3] = {L[0], L[1], L[2]}, Lab[3] = {L[0], L[1], L[2]};
+                        Lb[tb] += 1;
+                        La[ta] += 1;
+                        Lab[ta] += 1;
+                        Lab[tb] += 1;
+                        const long f00 = cell_local(Fb, L[0], L[1], L[2]), f01 = cell_local(Fb, Lb[0], Lb[1], Lb[2]);
+                        const long f10 = cell_local(Fb, La[0], La[1], La[2]), f11 = cell_local(Fb, Lab[0], Lab[1], Lab[2]);
+                        const long cc = cell_local(C, K[0], K[1], K[2]);
+                        for (int k = 0; k < NPR; k++) {
+                            const double *q = F[d][f] + k * NF;
+                            const double sum = q[f00] + q[f01] + q[f10] + q[f11];
+                            F[d][b][k * NC + cc] = ((0.25 * dt) * sum) / dt;
+                        }
+                    }
+            }
+        }
+    }
+}
+
+void oracle_amr_fluxes(const oracle_amr *g, double **p, double **ps, double dt, double **F1, double **F2, double **F3)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < g->nblk; b++) {
+        oracle_params par = block_params(g, b);
+        oracle_flux(&par, 1, p[b], ps[b], &g->m[b], F1[b], NULL);
+        oracle_flux(&par, 2, p[b], ps[b], &g->m[b], F2[b], NULL);
+        oracle_flux(&par, 3, p[b], ps[b], &g->m[b], F3[b], NULL);
+    }
+    if (g->reflux) oracle_amr_reflux(g, F1, F2, F3, dt);
+}
+
 double oracle_amr_step(const oracle_amr *g, double **p, double **ps, double dt, int **pflag, long long *failcount)
 {
     const int nb = g->nblk;
@@ -476,12 +546,17 @@ double oracle_amr_step(const oracle_amr *g, double **p, double **ps, double dt, 
                           pflag[b], failcount);
         }
         oracle_amr_bound(g, ph, psh, pflag);
-        /* corrector: U0 -> p with Dt = dt, fluxes from ph */
+        /* corrector: U0 -> p with Dt = dt, fluxes from ph, corrected at level
+         * jumps before the EMFs use them (step_ch.c:1142-1196) */
         for (int b = 0; b < nb; b++) {
             oracle_params par = block_params(g, b);
             oracle_flux(&par, 1, ph[b], psh[b], &g->m[b], F1[b], d1[b]);
             oracle_flux(&par, 2, ph[b], psh[b], &g->m[b], F2[b], d2[b]);
             oracle_flux(&par, 3, ph[b], psh[b], &g->m[b], F3[b], d3[b]);
+        }
+        if (g->reflux) oracle_amr_reflux(g, F1, F2, F3, dt);
+        for (int b = 0; b < nb; b++) {
+            oracle_params par = block_params(g, b);
             oracle_emf(&par, ph[b], F1[b], F2[b], F3[b], &g->m[b], dt, E[b], Ec[b]);
         }
         ring_average(g, E, dt);

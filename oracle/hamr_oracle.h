@@ -120,12 +120,20 @@ typedef struct oracle_amr {
     const oracle_amr_block *blk;
     const oracle_metric *m;   /* per block */
     double gam, cour, dx0[3], rho_floor, u_floor;
+    int reflux;               /* 1: flux correction at level jumps in the corrector */
 } oracle_amr;
 
 /* Ghost fill of every block (p, ps, pflag: arrays of per-block pointers). */
 void oracle_amr_bound(const oracle_amr *g, double **p, double **ps, int **pflag);
 /* Only the exchange part of stage `stage` (1..3): the ghosts of internal faces. */
 void oracle_amr_exchange(const oracle_amr *g, int stage, double **p, double **ps, int **pflag);
+/* Flux correction at level jumps: each face of a block whose neighbour
+ * across it is one level finer takes the mean of the four finer faces
+ * (F1..F3: per-block flux arrays, dt the step's). */
+void oracle_amr_reflux(const oracle_amr *g, double **F1, double **F2, double **F3, double dt);
+/* Fluxes of every block from (p, ps) (ghosts filled), corrected at level
+ * jumps when g->reflux (the corrector's fluxes of oracle_amr_step). */
+void oracle_amr_fluxes(const oracle_amr *g, double **p, double **ps, double dt, double **F1, double **F2, double **F3);
 /* One predictor + corrector step of every block (ghosts filled on input;
  * pflag required);
  * returns the next dt (min over blocks, clamped to 1.3 dt), < 0 on allocation failure. */

@@ -194,7 +194,7 @@ class AmrBlock(C.Structure):
 class AmrGridC(C.Structure):
     _fields_ = [("nblk", C.c_int), ("nlev", C.c_int), ("N", (C.c_int * 3) * 3), ("blk", C.POINTER(AmrBlock)),
                 ("m", C.POINTER(Metric)), ("gam", C.c_double), ("cour", C.c_double), ("dx0", C.c_double * 3),
-                ("rho_floor", C.c_double), ("u_floor", C.c_double)]
+                ("rho_floor", C.c_double), ("u_floor", C.c_double), ("reflux", C.c_int)]
 
 
 def _setup_amr(lib):
@@ -207,6 +207,8 @@ def _setup_amr(lib):
     lib.oracle_amr_exchange.argtypes = [G, C.c_int, pp, pp, ipp]
     lib.oracle_amr_step.argtypes = [G, pp, pp, C.c_double, ipp, C.POINTER(C.c_longlong)]
     lib.oracle_amr_step.restype = C.c_double
+    lib.oracle_amr_reflux.argtypes = [G, pp, pp, pp, C.c_double]
+    lib.oracle_amr_fluxes.argtypes = [G, pp, pp, C.c_double, pp, pp, pp]
     lib._amr_ready = True
 
 
@@ -215,7 +217,7 @@ class AmrGrid:
     o2, o3, bs1, bs2, bs3); N[l] cells of level l; metrics: per block, the
     oracle layout (as Grid's metric)."""
 
-    def __init__(self, oracle, blocks, N, metrics, gam, cour, dx0, rho_floor=0.0, u_floor=0.0):
+    def __init__(self, oracle, blocks, N, metrics, gam, cour, dx0, rho_floor=0.0, u_floor=0.0, reflux=True):
         self.o = oracle
         _setup_amr(oracle.lib)
         nb = len(blocks)
@@ -242,6 +244,7 @@ class AmrGrid:
         for k in range(3):
             self.g.dx0[k] = dx0[k]
         self.g.rho_floor, self.g.u_floor = rho_floor, u_floor
+        self.g.reflux = 1 if reflux else 0
         self.failcount = (C.c_longlong * 4)()
 
     @staticmethod
@@ -255,6 +258,14 @@ class AmrGrid:
 
     def exchange(self, stage, p, ps, pflag):
         self.o.lib.oracle_amr_exchange(C.byref(self.g), stage, self._pp(p), self._pp(ps), self._pp(pflag, _ip))
+
+    def fluxes(self, p, ps, dt):
+        """The corrector's fluxes of every block from (p, ps) (ghosts filled),
+        corrected at level jumps when the grid refluxes: [F1, F2, F3] lists."""
+        F = [[np.zeros_like(a) for a in p] for _ in range(3)]
+        self.o.lib.oracle_amr_fluxes(C.byref(self.g), self._pp(p), self._pp(ps), dt, self._pp(F[0]), self._pp(F[1]),
+                                     self._pp(F[2]))
+        return F
 
     def step(self, p, ps, pflag, dt):
         return self.o.lib.oracle_amr_step(C.byref(self.g), self._pp(p), self._pp(ps), dt, self._pp(pflag, _ip),

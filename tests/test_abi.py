@@ -22,7 +22,7 @@ def declared_functions():
 
 def test_header_declares_the_boundary():
     names = declared_functions()
-    for must in ("hamr_create", "hamr_destroy", "hamr_fluxcalc", "hamr_block_dt", "hamr_ct_emf", "hamr_ct_bound",
+    for must in ("hamr_create", "hamr_destroy", "hamr_fluxcalc", "hamr_block_dt", "hamr_flux_correct", "hamr_ct_emf", "hamr_ct_bound",
                  "hamr_ct_update", "hamr_update", "hamr_boundprim", "hamr_flags", "hamr_step", "hamr_upload",
                  "hamr_download", "hamr_upload_metric"):
         assert must in names
