@@ -287,11 +287,16 @@ static void pole(hamr_ctx *c, double dt_val, const double *dt_ptr)
 // Multi-level set: outflow faces, exchange stage 1, polar faces, exchange
 // stages 2 and 3, and the final outflow pass over the radial ghosts that the
 // later stages left (see hamr_amr.cpp).
-static void bound_all(hamr_ctx *c, double *p, double *ps)
+// Ghost fill of every block.  x3_rows_done: the update and face-field kernels
+// of the batched step already wrote the x3 ghosts of the rows inside each
+// block's x1 and x2 range from the periodic neighbour (only the remaining rows
+// are filled along x3; the result is the full fill's, cell for cell).
+static void bound_all(hamr_ctx *c, double *p, double *ps, bool x3_rows_done = false)
 {
     const int T = kTiles[c->tune[TS_EXCHANGE].tile], G = c->tune[TS_EXCHANGE].grid;
     if (!c->amr) {
-        for (int dir = 1; dir <= 3; dir++) launch_bound(c->P, dir, all_launch(c, TS_BOUND1 + dir - 1), p, ps, c->pflag, 0, c->s);
+        for (int dir = 1; dir <= 3; dir++)
+            launch_bound(c->P, dir, all_launch(c, TS_BOUND1 + dir - 1), p, ps, c->pflag, 0, c->s, x3_rows_done ? 1 : 0);
         return;
     }
     launch_bound(c->P, 1, all_launch(c, TS_BOUND1), p, ps, c->pflag, 0, c->s);
@@ -1110,7 +1115,7 @@ static void enqueue_step(hamr_ctx *c)
     pole(c, 0., c->dt);
     launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, s);
     update_all(c, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->psh);
-    bound_all(c, c->ph, c->psh);
+    bound_all(c, c->ph, c->psh, true);
     // corrector: U0 -> p/ps with Dt = dt, fluxes from ph/psh, Courant minima recorded
     for (int dir = 1; dir <= 3; dir++)
         launch_flux(P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->ph, c->psh, c->F[dir - 1], c->dtmin, c->prec, s);
@@ -1119,7 +1124,7 @@ static void enqueue_step(hamr_ctx *c)
     pole(c, 0., c->dt);
     launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 1.0, c->ps, c->ps, c->E, c->prec, s);
     update_all(c, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->ps);
-    bound_all(c, c->p, c->ps);
+    bound_all(c, c->p, c->ps, true);
     launch_dt_finalize(nb, c->dtmin, c->dt, s);
 }
 

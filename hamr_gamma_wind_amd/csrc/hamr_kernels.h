@@ -95,8 +95,11 @@ void launch_adjoint(const KParams &P, const BlockDev &B, double Dt, const double
 // or FK_POLE (FK_AMR faces are left to the exchange tables).  With
 // outflow_only = 1 only radial outflow faces are filled, and not at the ghost
 // rows of a polar face (the final pass of the multi-level fill).
+// With x3_rows_done = 1 the x3 fill skips the rows inside the block's x1 and
+// x2 range whose ghosts the update (k_invert, k_fallback) and face-field
+// (k_ct_update) kernels of the step wrote from the periodic neighbour.
 void launch_bound(const KParams &P, int dir, const Launch &L, double *p, double *ps, int *pflag, int outflow_only,
-                  hipStream_t s);
+                  hipStream_t s, int x3_rows_done = 0);
 // Flux correction at level jumps along direction dir (1..3): each entry's
 // coarse face takes the mean of its four finer faces (F: the direction's flux
 // array, FP32 when mixed); dt from dt_ptr if non-null.

@@ -108,18 +108,30 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
         double *sf = psf + 3 * B.off + g;
         const double dx1 = B.dx[0], dx2 = B.dx[1], dx3 = B.dx[2];
         double temp;
+        // the x3 ghosts of the periodic neighbour, as in finish_cell: B1 and B2
+        // of the NG lowest / highest cells of the z-row, B3 of the faces that
+        // are not the block's own x3 boundary faces (each block forms those)
+        const int bs3 = B.bs[2];
+        const bool inner = i < NG + B.bs[0] && j < NG + B.bs[1];
+        const int m = !inner ? -1 : z < 2 * NG ? x3_copy_partner(P, B, it.x, 4)
+                                              : (z >= bs3 && z < NG + bs3) ? x3_copy_partner(P, B, it.x, 5) : -1;
+        double *sm = nullptr;
+        if (m >= 0) sm = psf + 3 * P.blk[m].off + g + (z < 2 * NG ? bs3 : -bs3);
+        const int64_t KM = m >= 0 ? (int64_t)P.blk[m].ks : 0;
         if (j < NG + B.bs[1] && z < NG + B.bs[2]) {
             temp = 0.;
             temp += 1.0 / 1.0 * si_[0] * gF1;
             temp += -Dt / (1.0 * dx2) * (Eb(2 * K + sj) - Eb(2 * K));
             temp += Dt / (1.0 * dx3) * (Eb(K + 1) - Eb(K));
             sf[0] = temp / gF1;
+            if (sm) sm[0] = sf[0];
         }
         if (i < NG + B.bs[0] && j < NG + B.bs[1]) {
             temp = si_[2 * K] * gF3;
             temp += -Dt / dx1 * (Eb(K + si) - Eb(K));
             temp += Dt / dx2 * (Eb(sj) - Eb(0));
             sf[2 * K] = temp / gF3;
+            if (sm && z != NG) sm[2 * KM] = sf[2 * K];
         }
         if (i < NG + B.bs[0] && z < NG + B.bs[2]) {
             temp = 0.;
@@ -127,6 +139,7 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
             temp += Dt / (1.0 * dx1) * (Eb(2 * K + si) - Eb(2 * K));
             temp += -Dt / (1.0 * dx3) * (Eb(1) - Eb(0));
             sf[K] = temp / gF2;
+            if (sm) sm[KM] = sf[K];
         }
     }
 }
@@ -299,8 +312,13 @@ __device__ __forceinline__ void wave_count(int *cnt, bool pred)
 }
 
 // Floors (the build's specification: rho >= rho_floor r^-1.5, u >= u_floor
-// r^-2.5, flag -333 as GPU_fixup.h:399-415) and the result of one cell.
-__device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B, int n, int64_t col, int64_t g,
+// r^-2.5, flag -333 as GPU_fixup.h:399-415) and the result of one cell.  A
+// cell among the NG lowest or highest interior cells of its z-row is also
+// written into the x3 ghost cell of the periodic neighbour that the ghost fill
+// would copy it to (the neighbour's upper ghosts from this block's lowest
+// cells, its lower ghosts from the highest), so that the ghost fill after the
+// update only has the rows beyond the block's x1 and x2 edges left along x3.
+__device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B, int n, int64_t col, int64_t g, int z,
                                             int64_t base, int flag, double *pf, double *__restrict__ pf_out,
                                             int *__restrict__ pflag, int *__restrict__ failcnt)
 {
@@ -320,6 +338,15 @@ __device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B,
     pflag[B.off + g] = flag;
 #pragma unroll
     for (int k = 0; k < NPR; k++) pf_out[base + k * K] = pf[k];
+    const int bs3 = B.bs[2];
+    const int m = z < 2 * NG ? x3_copy_partner(P, B, n, 4) : z >= bs3 ? x3_copy_partner(P, B, n, 5) : -1;
+    if (m >= 0) {
+        const BlockDev &M = P.blk[m];
+        const int64_t gm = g + (z < 2 * NG ? bs3 : -bs3);
+        pflag[M.off + gm] = flag;
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pf_out[NPR * M.off + gm + k * M.ks] = pf[k];
+    }
 }
 
 // Marks a cell whose 2D inversion failed until k_fallback has run the 1D
@@ -357,7 +384,7 @@ __global__ void __launch_bounds__(256) k_invert(KParams P, WorkList wl, int full
         }
         const int flag = utoprim(0, U, G.gcov, G.gcon, G.g, P.gam, pf);
         if (flag == 0) {
-            finish_cell(P, B, n, col, g, base, 0, pf, pf_out, pflag, failcnt);
+            finish_cell(P, B, n, col, g, z, base, 0, pf, pf_out, pflag, failcnt);
             continue;
         }
         atomicAdd(&failcnt[n * 4 + 0], 1);
@@ -400,7 +427,7 @@ __global__ void __launch_bounds__(256) k_fallback(KParams P, WorkList wl, const 
             flag = utoprim(2, U, G.gcov, G.gcon, G.g, P.gam, pf);
             if (flag) atomicAdd(&failcnt[n * 4 + 2], 1);
         }
-        finish_cell(P, B, n, col, g, base, flag, pf, pf_out, pflag, failcnt);
+        finish_cell(P, B, n, col, g, z, base, flag, pf, pf_out, pflag, failcnt);
     }
 }
 
@@ -416,7 +443,7 @@ __global__ void __launch_bounds__(256) k_fallback(KParams P, WorkList wl, const 
 // in memory, so each group stores one contiguous 48-byte segment.
 template <int DIR>
 __global__ void __launch_bounds__(256) k_bound(KParams P, WorkList wl, double *p, double *ps, int *pflag,
-                                               int outflow_only)
+                                               int outflow_only, int x3_rows_done)
 {
     FOR_ITEMS(wl)
     {
@@ -447,6 +474,11 @@ __global__ void __launch_bounds__(256) k_bound(KParams P, WorkList wl, double *p
         const int face = 2 * (DIR - 1) + side;
         const int kind = B.kind[face];
         if (kind == FK_AMR) continue;
+        // rows inside the block's x1 and x2 range whose x3 ghosts the update
+        // and face-field kernels already wrote from the periodic neighbour
+        if (DIR == 3 && x3_rows_done && aa >= NG && aa < NG + B.bs[0] && bb >= NG && bb < NG + B.bs[1] &&
+            x3_copy_partner(P, B, n, face) >= 0)
+            continue;
         if (outflow_only) {
             if (kind != FK_OUTFLOW) continue;
             // rows beyond a polar face keep the mirror image of the partner's ghosts
@@ -791,12 +823,12 @@ void launch_cons(const KParams &P, const Launch &L, double Dt, const double *p, 
 }
 
 void launch_bound(const KParams &P, int dir, const Launch &L, double *p, double *ps, int *pflag, int outflow_only,
-                  hipStream_t s)
+                  hipStream_t s, int x3_rows_done)
 {
     if (L.wl.n <= 0) return;
-    if (dir == 1) hipLaunchKernelGGL(k_bound<1>, gdim(L), bdim(L), 0, s, P, L.wl, p, ps, pflag, outflow_only);
-    else if (dir == 2) hipLaunchKernelGGL(k_bound<2>, gdim(L), bdim(L), 0, s, P, L.wl, p, ps, pflag, outflow_only);
-    else hipLaunchKernelGGL(k_bound<3>, gdim(L), bdim(L), 0, s, P, L.wl, p, ps, pflag, outflow_only);
+    if (dir == 1) hipLaunchKernelGGL(k_bound<1>, gdim(L), bdim(L), 0, s, P, L.wl, p, ps, pflag, outflow_only, 0);
+    else if (dir == 2) hipLaunchKernelGGL(k_bound<2>, gdim(L), bdim(L), 0, s, P, L.wl, p, ps, pflag, outflow_only, 0);
+    else hipLaunchKernelGGL(k_bound<3>, gdim(L), bdim(L), 0, s, P, L.wl, p, ps, pflag, outflow_only, x3_rows_done);
 }
 
 void launch_amr_exchange(const KParams &P, const AmrEntry *e, int64_t n, const AmrProlong *pe, int64_t npro,

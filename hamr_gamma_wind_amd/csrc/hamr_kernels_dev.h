@@ -57,6 +57,17 @@ __device__ __forceinline__ double wave_min(double v)
     return v;
 }
 
+// x3 neighbour m of block n across face f (4: lower, 5: upper) when it is a
+// same-level copy neighbour whose own face back is n: the blocks whose x3
+// ghost rows the step writes directly (the periodic x3 fill of a uniform grid).
+__device__ __forceinline__ int x3_copy_partner(const KParams &P, const BlockDev &B, int n, int f)
+{
+    if (B.kind[f] != FK_COPY) return -1;
+    const int m = B.nbr[f];
+    const BlockDev &M = P.blk[m];
+    return (M.kind[f ^ 1] == FK_COPY && M.nbr[f ^ 1] == n) ? m : -1;
+}
+
 // launch geometry of a work-list launch
 static inline dim3 gdim(const Launch &L) { return dim3((unsigned)L.grid); }
 static inline dim3 bdim(const Launch &L) { return dim3((unsigned)L.threads); }
