@@ -19,8 +19,9 @@ steps (max and L1, relative, for rho and the field) are reported beside it.
 Launch shapes come from the autotuner (hamr_autotune) for each arithmetic.
 Beside the batched steps the line reports the reference-structured FP64 step
 (the C host's per-block call sequence with a host dt read-back per block), the
-multi-level torus (3 levels, ~700 blocks of uneven sizes, both arithmetics) and
-the adjoint of one stage on the flat 128^3 box.
+multi-level torus (3 levels, ~700 blocks of uneven sizes, both arithmetics),
+the adjoint of one stage on the flat 128^3 box and the restart records of the
+whole grid (the rdump pack and unpack, device to device).
 
 Prints ONE JSON line with the contract's fields plus `roofline` (the mixed
 step's dominant stage, timed with HIP events on the step's stream) and
@@ -262,6 +263,35 @@ def adjoint_bench(H, n=128, reps=5):
         ctx.close()
 
 
+def rdump_bench(H, ctx, bs, reps=5):
+    """Restart records of every block (rdump_block_write / rdump_block_read,
+    restart.c:52, :178) packed from / unpacked into the resident state, device
+    to device in one launch each: ms per call (wall time between device
+    synchronisations, launch included) and the algorithmic rate.  Write: each
+    ghost-inclusive cell reads 12 doubles and writes 13; read: each cell of
+    [NG, NG+BS]^3 reads 12 and writes 12 (the column's gdet values aside)."""
+    import torch
+
+    off = ctx.rdump_offsets()
+    buf = torch.empty(int(off[-1]), dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    nb = ctx.nblk
+    D = [b + 6 for b in bs]
+    cells_w = nb * D[0] * D[1] * D[2]
+    cells_r = nb * (bs[0] + 1) * (bs[1] + 1) * (bs[2] + 1)
+    out = {"record_bytes": int(off[-1]) * 8}
+    for name, fn, nbytes in (("write", lambda: ctx.rdump_write_dev(buf.data_ptr()), cells_w * (12 + 13) * 8),
+                             ("read", lambda: ctx.rdump_read_dev(buf.data_ptr()), cells_r * 24 * 8)):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        out[name + "_ms"] = ms
+        out[name + "_algorithmic_GBps"] = nbytes / (ms * 1e-3) / 1e9
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,6 +358,7 @@ def main():
     achieved = nz * bpz / (kern_ms[dom] * 1e-3) / 1e9 if bpz > 0 else 0.0
     traffic = read_traffic(dom, "%s_torus%dx%dx%d" % ("mixed" if mixed else "fp64", bs[0] * nb[0], bs[1] * nb[1],
                                                        bs[2] * nb[2]))
+    rdump = rdump_bench(H, ctx, bs)
     copy = None
     if args.copy_probe_mib > 0:
         nbytes = args.copy_probe_mib << 20
@@ -403,6 +434,7 @@ def main():
         "error_vs_fp64": err,
         "adjoint": adj,
         "copy_probe": copy,
+        "rdump": rdump,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

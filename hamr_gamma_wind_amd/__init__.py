@@ -193,6 +193,12 @@ def load_library(path=None):
         "hamr_tune_get": [ctxp, C.c_int, _ip, _dp],
         "hamr_tune_set": [ctxp, C.c_int, _ip],
         "hamr_time_kernel": [ctxp, C.c_int, C.c_int, _dp],
+        "hamr_rdump_block_size": [ctxp, C.c_int, C.c_int, C.c_int, C.c_int, _llp],
+        "hamr_rdump_block_write": [ctxp, C.c_int, _dp],
+        "hamr_rdump_block_read": [ctxp, C.c_int, _dp, C.c_int, C.c_int, C.c_int],
+        "hamr_rdump_offsets": [ctxp, _llp],
+        "hamr_rdump_write_dev": [ctxp, C.c_void_p],
+        "hamr_rdump_read_dev": [ctxp, C.c_void_p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -468,6 +474,55 @@ class Context:
 
     def get_dev(self, which, ptr):
         self._chk(self.L.hamr_get_array_dev(self.h, which, C.c_void_p(ptr)), "hamr_get_array_dev")
+
+    # restart records: the rdump files of restart.c (rdump_block_write :52,
+    # rdump_block_read :178); NPR + 4 doubles per ghost-inclusive cell
+    def rdump_size(self, n, red=(1, 1, 1)):
+        v = C.c_longlong()
+        self._chk(self.L.hamr_rdump_block_size(self.h, n, *[int(r) for r in red], C.byref(v)),
+                  "hamr_rdump_block_size")
+        return v.value
+
+    def rdump_block_write(self, n):
+        rec = np.zeros(self.rdump_size(n))
+        self._chk(self.L.hamr_rdump_block_write(self.h, n, _ptr(rec)), "hamr_rdump_block_write")
+        return rec
+
+    def rdump_block_read(self, n, rec, red=(1, 1, 1)):
+        rec = np.ascontiguousarray(rec, dtype=float).reshape(-1)
+        if rec.size != self.rdump_size(n, red):
+            raise HamrError("rdump record of block %d: %d doubles, expected %d" % (n, rec.size,
+                                                                                    self.rdump_size(n, red)))
+        self._chk(self.L.hamr_rdump_block_read(self.h, n, _ptr(rec), *[int(r) for r in red]),
+                  "hamr_rdump_block_read")
+
+    def rdump_offsets(self):
+        off = np.zeros(self.nblk + 1, dtype=np.int64)
+        self._chk(self.L.hamr_rdump_offsets(self.h, _ptr(off, _llp)), "hamr_rdump_offsets")
+        return off
+
+    def rdump_write_dev(self, ptr):
+        self._chk(self.L.hamr_rdump_write_dev(self.h, C.c_void_p(ptr)), "hamr_rdump_write_dev")
+
+    def rdump_read_dev(self, ptr):
+        self._chk(self.L.hamr_rdump_read_dev(self.h, C.c_void_p(ptr)), "hamr_rdump_read_dev")
+
+    def rdump_write(self, directory):
+        """restart_write's block files (restart.c:40-45): directory/rdump<n>
+        holds block n's record as raw doubles."""
+        os.makedirs(directory, exist_ok=True)
+        for n in range(self.nblk):
+            self.rdump_block_write(n).tofile(os.path.join(directory, "rdump%d" % n))
+
+    def rdump_read(self, directory, red=(1, 1, 1)):
+        """restart_read (restart.c:94-175): every block from directory/rdump<n>
+        (records of blocks red times finer), then the ghost fill."""
+        for n in range(self.nblk):
+            path = os.path.join(directory, "rdump%d" % n)
+            if not os.path.exists(path):
+                raise HamrError("cannot open restart file %s" % path)
+            self.rdump_block_read(n, np.fromfile(path, dtype=np.float64), red)
+        self.boundprim(1)
 
     # one forward stage and its adjoint
     def stage_forward(self, Dt):

@@ -28,6 +28,7 @@ SOURCES = [
     ("hamr_kernels.hip", True),
     ("hamr_kernels_f32.hip", True),
     ("hamr_adjoint.hip", True),
+    ("hamr_rdump.hip", True),
     ("hamr_capi.hip", True),
     ("hamr_setup.cpp", False),
     ("hamr_amr.cpp", False),

@@ -85,6 +85,12 @@ struct hamr_ctx {
     std::vector<void *> allocs;
     void *stage = nullptr;         // pinned host staging of whole-array copies
     size_t stage_bytes = 0;
+    // restart records: record n of the whole-grid layout starts at roff[n] doubles
+    std::vector<int64_t> roff;
+    int *io_blist = nullptr;       // 0 .. nblk-1
+    int64_t *io_roff = nullptr;
+    double *io = nullptr;          // device record of one block (grown on demand)
+    size_t io_bytes = 0;
 };
 
 static int fail(hamr_ctx *c, const std::string &m)
@@ -248,6 +254,15 @@ static int init_device(hamr_ctx *c)
         dalloc(c, &c->conn, 40 * KC) || dalloc(c, &c->rcen, KC) || dalloc(c, &c->blk, nb))
         return -1;
     if (h2d(c, c->blk, c->blocks.data(), nb * sizeof(BlockDev))) return -1;
+    std::vector<int> bl(nb);
+    c->roff.assign(nb + 1, 0);
+    for (size_t n = 0; n < nb; n++) {
+        bl[n] = (int)n;
+        c->roff[n + 1] = c->roff[n] + RREC * ncell_of(c->blocks[n]);
+    }
+    if (dalloc(c, &c->io_blist, nb) || dalloc(c, &c->io_roff, nb + 1)) return -1;
+    if (h2d(c, c->io_blist, bl.data(), nb * sizeof(int))) return -1;
+    if (h2d(c, c->io_roff, c->roff.data(), (nb + 1) * sizeof(int64_t))) return -1;
     if (build_class_lists(c)) return -1;
     KParams &P = c->P;
     P.gam = c->d.gam;
@@ -494,6 +509,7 @@ int hamr_destroy(hamr_ctx *c)
     if (c->s) (void)hipStreamSynchronize(c->s);
     for (void *v : c->allocs) (void)hipFree(v);
     if (c->stage) (void)hipHostFree(c->stage);
+    if (c->io) (void)hipFree(c->io);
     if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
     return 0;
@@ -1008,6 +1024,117 @@ int hamr_download(hamr_ctx *c, int n, double *p_aos, double *ps_aos)
         if (ps_aos)
             for (int k = 0; k < 3; k++) ps_aos[cell * 4 + 1 + k] = ts[k * K + cell];
     }
+    return 0;
+}
+
+// ---- restart records (rdump files) ----
+// Device buffer of at least `bytes` for one block's record (grown on demand,
+// kept until hamr_destroy).
+static int io_reserve(hamr_ctx *c, size_t bytes)
+{
+    if (c->io_bytes >= bytes) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->io) HIPCHK(c, hipFree(c->io));
+    c->io = nullptr;
+    c->io_bytes = 0;
+    HIPCHK(c, hipMalloc((void **)&c->io, bytes));
+    c->io_bytes = bytes;
+    return 0;
+}
+
+static bool red_ok(const int *red) { return red[0] >= 1 && red[1] >= 1 && red[2] >= 1 && red[0] <= 64 && red[1] <= 64 && red[2] <= 64; }
+
+// Doubles of a record whose block is red[d] times finer than B along d.
+static int64_t rdump_count(const BlockDev &B, const int *red)
+{
+    int64_t n = RREC;
+    for (int k = 0; k < 3; k++) n *= (int64_t)B.bs[k] * red[k] + 2 * NG;
+    return n;
+}
+
+// The reader's parameters: the record's refinement and, on the torus grid
+// (theta range 1 - 2/N2 of the full wedge, restart.c:249-257 with
+// coord.c:397), the two grids' theta fractions.
+static RdumpRead rdump_desc(const hamr_ctx *c, const int *red)
+{
+    RdumpRead R;
+    for (int k = 0; k < 3; k++) R.red[k] = red[k];
+    R.spherical = c->d.minkowski ? 0 : 1;
+    const int N2 = c->d.nb[1] * c->d.bs[1];
+    R.fractheta = grid_coords(c->d).fractheta;
+    R.fractheta_old = 1.0 - 2.0 / ((double)N2 * red[1]);
+    return R;
+}
+
+static int64_t unpack_cells(const BlockDev &B) { return (int64_t)(B.bs[0] + 1) * (B.bs[1] + 1) * (B.bs[2] + 1); }
+
+int hamr_rdump_block_size(const hamr_ctx *c, int n, int red1, int red2, int red3, long long *count)
+{
+    const int red[3] = {red1, red2, red3};
+    if (!c || !valid_block(c, n) || !red_ok(red) || !count) return -1;
+    *count = rdump_count(c->blocks[n], red);
+    return 0;
+}
+
+int hamr_rdump_offsets(const hamr_ctx *c, long long *off)
+{
+    if (!c || !off) return -1;
+    for (size_t n = 0; n < c->roff.size(); n++) off[n] = c->roff[n];
+    return 0;
+}
+
+int hamr_rdump_block_write(hamr_ctx *c, int n, double *rec)
+{
+    if (!c || !valid_block(c, n) || !rec) return fail(c, "hamr_rdump_block_write: bad arguments");
+    const BlockDev &B = c->blocks[n];
+    const size_t bytes = (size_t)(RREC * ncell_of(B)) * sizeof(double);
+    if (io_reserve(c, bytes)) return -1;
+    launch_rdump_pack(c->P, c->io_blist, c->io_roff, c->roff.data(), n, 1, ncell_of(B), c->p, c->ps, c->io, c->s);
+    if (check_launch(c, "hamr_rdump_block_write")) return -1;
+    return d2h(c, rec, c->io, bytes);
+}
+
+int hamr_rdump_block_read(hamr_ctx *c, int n, const double *rec, int red1, int red2, int red3)
+{
+    const int red[3] = {red1, red2, red3};
+    if (!c || !valid_block(c, n) || !rec) return fail(c, "hamr_rdump_block_read: bad arguments");
+    if (!red_ok(red)) return fail(c, "hamr_rdump_block_read: refinement factors must be in [1, 64]");
+    const BlockDev &B = c->blocks[n];
+    const size_t bytes = (size_t)rdump_count(B, red) * sizeof(double);
+    if (io_reserve(c, bytes)) return -1;
+    if (h2d(c, c->io, rec, bytes)) return -1;
+    launch_rdump_unpack(c->P, c->io_blist, c->io_roff, c->roff.data(), n, 1, unpack_cells(B), rdump_desc(c, red),
+                        c->io, c->p, c->ps, c->s);
+    if (check_launch(c, "hamr_rdump_block_read")) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+static int64_t max_over_blocks(const hamr_ctx *c, int64_t (*f)(const BlockDev &))
+{
+    int64_t m = 0;
+    for (const BlockDev &B : c->blocks) m = std::max(m, f(B));
+    return m;
+}
+
+int hamr_rdump_write_dev(hamr_ctx *c, double *drec)
+{
+    if (!c || !drec) return fail(c, "hamr_rdump_write_dev: bad arguments");
+    launch_rdump_pack(c->P, c->io_blist, c->io_roff, c->roff.data(), 0, nblk_of(c), max_over_blocks(c, ncell_of),
+                      c->p, c->ps, drec, c->s);
+    if (check_launch(c, "hamr_rdump_write_dev")) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int hamr_rdump_read_dev(hamr_ctx *c, const double *drec)
+{
+    if (!c || !drec) return fail(c, "hamr_rdump_read_dev: bad arguments");
+    const int red[3] = {1, 1, 1};
+    launch_rdump_unpack(c->P, c->io_blist, c->io_roff, c->roff.data(), 0, nblk_of(c),
+                        max_over_blocks(c, unpack_cells), rdump_desc(c, red), drec, c->p, c->ps, c->s);
+    if (check_launch(c, "hamr_rdump_read_dev")) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
     return 0;
 }
 

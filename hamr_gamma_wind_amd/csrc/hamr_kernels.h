@@ -105,6 +105,21 @@ void launch_bound(const KParams &P, int dir, const Launch &L, double *p, double 
 // array, FP32 when mixed); dt from dt_ptr if non-null.
 void launch_amr_reflux(const KParams &P, const AmrReflux *e, int64_t n, int dir, void *F, double dt_val,
                        const double *dt_ptr, int mixed, hipStream_t s);
+// Restart records (rdump files): RREC doubles per ghost-inclusive cell.  Item
+// q of a launch is block blist[q], whose record starts at out + roff[q] -
+// roff_host[first]; max_cells bounds the items' cell counts (pack: D1 D2 D3;
+// unpack: (bs1+1)(bs2+1)(bs3+1)).
+constexpr int RREC = NPR + 4;
+struct RdumpRead {
+    int red[3];            // the record's block is red[d] times finer along direction d
+    int spherical;         // B^2 scaled by fractheta / fractheta_old
+    double fractheta, fractheta_old;
+};
+void launch_rdump_pack(const KParams &P, const int *blist, const int64_t *roff, const int64_t *roff_host, int first,
+                       int count, int64_t max_cells, const double *p, const double *ps, double *out, hipStream_t s);
+void launch_rdump_unpack(const KParams &P, const int *blist, const int64_t *roff, const int64_t *roff_host, int first,
+                         int count, int64_t max_cells, const RdumpRead &R, const double *rec, double *p, double *ps,
+                         hipStream_t s);
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);
 void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s);
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);

@@ -197,6 +197,33 @@ int hamr_get_array(hamr_ctx *ctx, int which, double *dst);
 int hamr_set_array_dev(hamr_ctx *ctx, int which, const double *dsrc);
 int hamr_get_array_dev(hamr_ctx *ctx, int which, double *ddst);
 
+/* ---- restart records (the rdump files of restart.c) ----
+ * Record of one block: its ghost-inclusive cells in index_3D order (z fastest),
+ * NPR+4 doubles per cell: p[0..8], 0 (face-field component 0, never read
+ * back), then B^1 gdet(FACE1), B^2 gdet(FACE2), B^3 gdet(FACE3), gdet of the
+ * cell's (i, j) column.  This is the content of the file rdump<n>
+ * (MPI_File_iwrite of array_rdump, restart.c:64-66, raw doubles, no header). */
+/* Doubles of block n's record when its block is red_d times finer along d
+ * (red = 1: the block itself; restart.c:181 npr_file and :233 loop bounds). */
+int hamr_rdump_block_size(const hamr_ctx *ctx, int n, int red1, int red2, int red3, long long *count);
+/* rdump_block_write (restart.c:52-67): block n's record of the current state
+ * into rec (host memory). */
+int hamr_rdump_block_write(hamr_ctx *ctx, int n, double *rec);
+/* rdump_block_read (restart.c:178-269): block n's state from a record whose
+ * block is red_d (1..64) times finer along d (BS_read / BS, restart.c:228-232).
+ * Sets p and the face field of the cells [NG, NG+BS] of each direction (the
+ * interior and the first upper ghost row, restart.c:244) to the means of the
+ * fine cells and faces they cover; the other ghosts are left as they were
+ * (restart_read bounds them afterwards: hamr_boundprim(ctx, 1)). */
+int hamr_rdump_block_read(hamr_ctx *ctx, int n, const double *rec, int red1, int red2, int red3);
+/* The records of all blocks back to back in block order, in device memory of
+ * the context's device: off[n] (n = 0..nblk, nblk+1 entries) is the first
+ * double of block n's record, off[nblk] the total.  The write packs every block
+ * in one launch; the read takes records of the same block sizes (red = 1). */
+int hamr_rdump_offsets(const hamr_ctx *ctx, long long *off);
+int hamr_rdump_write_dev(hamr_ctx *ctx, double *drec);
+int hamr_rdump_read_dev(hamr_ctx *ctx, const double *drec);
+
 /* ---- per-block stages, the call sequence of advance_GPU (step_ch.c:997-1336) ---- */
 /* GPU_fluxcalc2D (GPU_main.cu:560): PPM + HLL flux through the lower dir-faces
  * of block n; records the block's Courant dt for hamr_block_dt. */

@@ -139,6 +139,15 @@ void oracle_amr_fluxes(const oracle_amr *g, double **p, double **ps, double dt, 
  * returns the next dt (min over blocks, clamped to 1.3 dt), < 0 on allocation failure. */
 double oracle_amr_step(const oracle_amr *g, double **p, double **ps, double dt, int **pflag, long long *failcount);
 
+/* ---- restart record of one block (hamr_rdump_oracle.c) ----
+ * One block's ghost-inclusive arrays in the layout above (bs: interior size;
+ * gdet with ncol = D1*D2).  The record holds NPR+4 doubles per cell. */
+void oracle_rdump_block_write(const int bs[3], const double *p, const double *ps, const double *gdet, double *rec);
+/* rec: the record of a block red[d] times finer along d; n2: this grid's total
+ * x2 cells; spherical: the torus grid (B^2 theta-fraction ratio). */
+void oracle_rdump_block_read(const int bs[3], const int red[3], int spherical, int n2, const double *rec,
+                             const double *gdet, double *p, double *ps);
+
 #ifdef __cplusplus
 }
 #endif

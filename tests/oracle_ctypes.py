@@ -65,6 +65,8 @@ class Oracle:
         L.oracle_block_dt.restype = C.c_double
         L.oracle_step.argtypes = [P, _dp, _dp, M, C.c_double, _ip, C.POINTER(C.c_longlong)]
         L.oracle_step.restype = C.c_double
+        L.oracle_rdump_block_write.argtypes = [_ip, _dp, _dp, _dp, _dp]
+        L.oracle_rdump_block_read.argtypes = [_ip, _ip, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
 
     # ---- pointwise ----
     def get_state(self, pr, gcov, gcon):
@@ -100,6 +102,29 @@ class Oracle:
                                       _ptr(np.ascontiguousarray(gcov, float)),
                                       _ptr(np.ascontiguousarray(gcon, float)), g, gam, _ptr(prim))
         return ret, prim
+
+
+    # ---- restart record of one block ----
+    def rdump_write(self, bs, p, ps, gdet):
+        """Record of one block (p: 9 x ncell, ps: 3 x ncell, gdet: 5 x ncol)."""
+        D = [b + 2 * NG for b in bs]
+        rec = np.zeros(D[0] * D[1] * D[2] * (NPR + 4))
+        b = np.array(bs, dtype=np.int32)
+        self.lib.oracle_rdump_block_write(_ptr(b, _ip), _ptr(np.ascontiguousarray(p, float)),
+                                          _ptr(np.ascontiguousarray(ps, float)),
+                                          _ptr(np.ascontiguousarray(gdet, float)), _ptr(rec))
+        return rec
+
+    def rdump_read(self, bs, red, spherical, n2, rec, gdet, p, ps):
+        """Reads a record (of a block red times finer) into copies of p, ps."""
+        p = np.array(p, dtype=float, copy=True)
+        ps = np.array(ps, dtype=float, copy=True)
+        b = np.array(bs, dtype=np.int32)
+        r = np.array(red, dtype=np.int32)
+        self.lib.oracle_rdump_block_read(_ptr(b, _ip), _ptr(r, _ip), int(spherical), int(n2),
+                                         _ptr(np.ascontiguousarray(rec, float)),
+                                         _ptr(np.ascontiguousarray(gdet, float)), _ptr(p), _ptr(ps))
+        return p, ps
 
 
 class Grid:

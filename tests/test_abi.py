@@ -24,7 +24,8 @@ def test_header_declares_the_boundary():
     names = declared_functions()
     for must in ("hamr_create", "hamr_destroy", "hamr_fluxcalc", "hamr_block_dt", "hamr_flux_correct", "hamr_ct_emf", "hamr_ct_bound",
                  "hamr_ct_update", "hamr_update", "hamr_boundprim", "hamr_flags", "hamr_step", "hamr_upload",
-                 "hamr_download", "hamr_upload_metric"):
+                 "hamr_download", "hamr_upload_metric", "hamr_rdump_block_write", "hamr_rdump_block_read",
+                 "hamr_rdump_block_size", "hamr_rdump_offsets", "hamr_rdump_write_dev", "hamr_rdump_read_dev"):
         assert must in names
 
 
