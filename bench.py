@@ -20,8 +20,9 @@ Launch shapes come from the autotuner (hamr_autotune) for each arithmetic.
 Beside the batched steps the line reports the reference-structured FP64 step
 (the C host's per-block call sequence with a host dt read-back per block), the
 multi-level torus (3 levels, ~700 blocks of uneven sizes, both arithmetics),
-the adjoint of one stage on the flat 128^3 box and the restart records of the
-whole grid (the rdump pack and unpack, device to device).
+the adjoint of one stage on the flat 128^3 box, the FP64 step with HLLC fluxes
+and the restart records of the whole grid (the rdump pack and unpack, device
+to device).
 
 Prints ONE JSON line with the contract's fields plus `roofline` (the mixed
 step's dominant stage, timed with HIP events on the step's stream) and
@@ -263,6 +264,35 @@ def adjoint_bench(H, n=128, reps=5):
         ctx.close()
 
 
+def hllc_bench(H, ctx, p0, ps0, nz, args, steps=20):
+    """The FP64 step with the HLLC Riemann solver (hamr_set_riemann) on the same
+    state: tuned launches, graph replay, 3 warm-up and `steps` timed steps, and
+    the flux kernels' ms per launch."""
+    ctx.set_precision(H.HAMR_PREC_FP64)
+    ctx.set_riemann(H.HAMR_RIEMANN_HLLC)
+    try:
+        ctx.set(H.HAMR_P, p0)
+        ctx.set(H.HAMR_PS, ps0)
+        if not args.no_tune:
+            ctx.autotune(3)
+        ctx.set(H.HAMR_P, p0)
+        ctx.set(H.HAMR_PS, ps0)
+        ctx.clear_flags()
+        graph = not args.no_graph
+        dt = ctx.step(3, ctx.estimate_dt(), graph=graph)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(steps, None, graph=graph)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        kern = {k: round(ctx.time_kernel(i, 5), 4) for i, k in enumerate(KERNELS[:3])}
+    finally:
+        ctx.set_riemann(H.HAMR_RIEMANN_HLL)
+    return {"value": nz * steps / el, "unit": "zone-updates/s", "ms_per_step": el / steps * 1e3, "steps": steps,
+            "flux_ms_per_launch": kern,
+            "what": "FP64 step with HLLC fluxes in the face's orthonormal frame (the reference's HLLC = 1)"}
+
+
 def rdump_bench(H, ctx, bs, reps=5):
     """Restart records of every block (rdump_block_write / rdump_block_read,
     restart.c:52, :178) packed from / unpacked into the resident state, device
@@ -306,6 +336,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement (128^3 flat box)")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--no-amr", action="store_true", help="skip the multi-level block set")
+    ap.add_argument("--no-hllc", action="store_true", help="skip the FP64 step with HLLC fluxes")
     ap.add_argument("--amr-steps", type=int, default=40)
     ap.add_argument("--amr-only", action="store_true",
                     help="measure only the multi-level block set (prints its JSON object; for profiling)")
@@ -359,6 +390,7 @@ def main():
     traffic = read_traffic(dom, "%s_torus%dx%dx%d" % ("mixed" if mixed else "fp64", bs[0] * nb[0], bs[1] * nb[1],
                                                        bs[2] * nb[2]))
     rdump = rdump_bench(H, ctx, bs)
+    hllc = hllc_bench(H, ctx, p0, ps0, nz, args) if not args.no_hllc else None
     copy = None
     if args.copy_probe_mib > 0:
         nbytes = args.copy_probe_mib << 20
@@ -435,6 +467,7 @@ def main():
         "adjoint": adj,
         "copy_probe": copy,
         "rdump": rdump,
+        "fp64_hllc": hllc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

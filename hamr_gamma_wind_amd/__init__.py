@@ -24,6 +24,7 @@ FACE1, FACE2, CORN, CENT, FACE3 = range(5)
 NCOMP = {HAMR_P: 9, HAMR_PS: 3, HAMR_PH: 9, HAMR_PSH: 3, HAMR_F1: 9, HAMR_F2: 9, HAMR_F3: 9, HAMR_EMF: 3,
          HAMR_ECENT: 3, HAMR_U0: 9, HAMR_UNEW: 9}
 HAMR_PREC_FP64, HAMR_PREC_MIXED = 0, 1
+HAMR_RIEMANN_HLL, HAMR_RIEMANN_HLLC = 0, 1
 
 
 class HamrError(RuntimeError):
@@ -193,6 +194,10 @@ def load_library(path=None):
         "hamr_tune_get": [ctxp, C.c_int, _ip, _dp],
         "hamr_tune_set": [ctxp, C.c_int, _ip],
         "hamr_time_kernel": [ctxp, C.c_int, C.c_int, _dp],
+        "hamr_set_riemann": [ctxp, C.c_int],
+        "hamr_get_riemann": [ctxp, _ip],
+        "hamr_face_tetrad": [_dp, _dp, C.c_int, _dp, _dp],
+        "hamr_download_tetrads": [ctxp, C.c_int, _dp],
         "hamr_rdump_block_size": [ctxp, C.c_int, C.c_int, C.c_int, C.c_int, _llp],
         "hamr_rdump_block_write": [ctxp, C.c_int, _dp],
         "hamr_rdump_block_read": [ctxp, C.c_int, _dp, C.c_int, C.c_int, C.c_int],
@@ -213,6 +218,18 @@ def load_library(path=None):
 
 # ---------------------------------------------------------------------------
 # host-only set-up
+def face_tetrad(gcov10, gcon10, d):
+    """set_Mud (hllc.c:20-178) at one face of direction d from the packed metric
+    there: (mud, mudi) as 4x4 arrays, mud[mu, a] = e_(a)^mu, mudi[a, mu] = e^(a)_mu."""
+    L = load_library()
+    mud, mudi = np.zeros(16), np.zeros(16)
+    r = L.hamr_face_tetrad(_ptr(np.ascontiguousarray(gcov10, float)), _ptr(np.ascontiguousarray(gcon10, float)),
+                           int(d), _ptr(mud), _ptr(mudi))
+    if r:
+        raise HamrError("hamr_face_tetrad: bad arguments")
+    return mud.reshape(4, 4), mudi.reshape(4, 4)
+
+
 def desc_layout(desc):
     L = load_library()
     D = (C.c_int * 3)()
@@ -544,6 +561,20 @@ class Context:
     def set_precision(self, mode):
         """HAMR_PREC_FP64 or HAMR_PREC_MIXED (FP32 reconstruction, fluxes, EMFs)."""
         self._chk(self.L.hamr_set_precision(self.h, mode), "hamr_set_precision")
+
+    def set_riemann(self, solver):
+        self._chk(self.L.hamr_set_riemann(self.h, solver), "hamr_set_riemann")
+
+    def riemann(self):
+        v = C.c_int()
+        self._chk(self.L.hamr_get_riemann(self.h, C.byref(v)), "hamr_get_riemann")
+        return v.value
+
+    def download_tetrads(self, geo, kcol=None):
+        kc = kcol if kcol is not None else self.kcol
+        t = np.zeros(96 * kc)
+        self._chk(self.L.hamr_download_tetrads(self.h, geo, _ptr(t)), "hamr_download_tetrads")
+        return t.reshape(3, 32, kc)
 
     def precision(self):
         v = C.c_int()

@@ -66,6 +66,7 @@ struct hamr_ctx {
     unsigned long long *dtmin = nullptr;
     double *dt = nullptr;          // [0] dt of the next step, [1] previous
     double *gcov = nullptr, *gcon = nullptr, *gdet = nullptr, *conn = nullptr, *rcen = nullptr;
+    double *tet = nullptr;         // face tetrads of the HLLC solver (96 per metric column)
     BlockDev *blk = nullptr;
     int *ring = nullptr, *rside = nullptr;  // polar rings
     int nring = 0, ring_len = 0, ni_max = 0;
@@ -251,7 +252,8 @@ static int init_device(hamr_ctx *c)
         return -1;
     const size_t KC = (size_t)c->cols;
     if (dalloc(c, &c->gcov, NPOS * 10 * KC) || dalloc(c, &c->gcon, NPOS * 10 * KC) || dalloc(c, &c->gdet, NPOS * KC) ||
-        dalloc(c, &c->conn, 40 * KC) || dalloc(c, &c->rcen, KC) || dalloc(c, &c->blk, nb))
+        dalloc(c, &c->conn, 40 * KC) || dalloc(c, &c->rcen, KC) || dalloc(c, &c->tet, 96 * KC) ||
+        dalloc(c, &c->blk, nb))
         return -1;
     if (h2d(c, c->blk, c->blocks.data(), nb * sizeof(BlockDev))) return -1;
     std::vector<int> bl(nb);
@@ -275,6 +277,8 @@ static int init_device(hamr_ctx *c)
     P.gdet = c->gdet;
     P.conn = c->conn;
     P.rcen = c->rcen;
+    P.tet = c->tet;
+    P.riemann = HAMR_RIEMANN_HLL;
     launch_dt_reset((int)nb, c->dtmin, c->s);
     if (check_launch(c, "dt reset")) return -1;
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -818,6 +822,47 @@ int hamr_upload_metric(hamr_ctx *c, int geo, const double *gcov, const double *g
     if (h2d(c, c->gdet + mo * NPOS, gdet, NPOS * KC * sizeof(double))) return -1;
     if (h2d(c, c->conn + mo * 40, conn, 40 * KC * sizeof(double))) return -1;
     if (h2d(c, c->rcen + mo, rcen, KC * sizeof(double))) return -1;
+    // the HLLC solver's face tetrads follow the metric (set_Mud after set_grid)
+    std::vector<double> tet(96 * KC);
+    fill_tetrads(gcov, gcon, (int64_t)G.D1 * G.D2, (int64_t)KC, tet.data());
+    if (h2d(c, c->tet + mo * 96, tet.data(), 96 * KC * sizeof(double))) return -1;
+    return 0;
+}
+
+int hamr_download_tetrads(hamr_ctx *c, int geo, double *tet)
+{
+    if (!c || !tet || geo < 0 || geo >= (int)c->geos.size()) return fail(c, "hamr_download_tetrads: bad arguments");
+    const AmrGeo &G = c->geos[geo];
+    const size_t KC = (size_t)G.kcol, mo = (size_t)G.moff;
+    return d2h(c, tet, c->tet + mo * 96, 96 * KC * sizeof(double));
+}
+
+int hamr_face_tetrad(const double *gcov10, const double *gcon10, int dir, double *mud, double *mudi)
+{
+    if (!gcov10 || !gcon10 || !mud || !mudi || dir < 1 || dir > 3) return -1;
+    face_tetrad(gcov10, gcon10, dir, mud, mudi);
+    return 0;
+}
+
+int hamr_set_riemann(hamr_ctx *c, int solver)
+{
+    if (!c) return -1;
+    if (solver != HAMR_RIEMANN_HLL && solver != HAMR_RIEMANN_HLLC) return fail(c, "hamr_set_riemann: unknown solver");
+    if (solver == HAMR_RIEMANN_HLLC && c->prec != HAMR_PREC_FP64)
+        return fail(c, "hamr_set_riemann: HLLC runs in the FP64 step only");
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (solver != c->P.riemann && c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    c->P.riemann = solver;
+    return 0;
+}
+
+int hamr_get_riemann(const hamr_ctx *c, int *solver)
+{
+    if (!c || !solver) return -1;
+    *solver = c->P.riemann;
     return 0;
 }
 
@@ -976,6 +1021,8 @@ int hamr_set_precision(hamr_ctx *c, int mode)
 {
     if (!c) return -1;
     if (mode != HAMR_PREC_FP64 && mode != HAMR_PREC_MIXED) return fail(c, "hamr_set_precision: unknown mode");
+    if (mode == HAMR_PREC_MIXED && c->P.riemann != HAMR_RIEMANN_HLL)
+        return fail(c, "hamr_set_precision: the mixed step uses HLL (HLLC runs in the FP64 step only)");
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (mode != c->prec && c->graph) {
         (void)hipGraphExecDestroy(c->graph);
@@ -1521,8 +1568,8 @@ static int stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, do
                          hipMemcpyKind out)
 {
     if (!c || !w) return -1;
-    if (!c->d.minkowski || c->amr || nblk_of(c) != 1 || c->prec != HAMR_PREC_FP64)
-        return fail(c, "hamr_stage_adjoint: needs a flat periodic box of one block in FP64");
+    if (!c->d.minkowski || c->amr || nblk_of(c) != 1 || c->prec != HAMR_PREC_FP64 || c->P.riemann != HAMR_RIEMANN_HLL)
+        return fail(c, "hamr_stage_adjoint: needs a flat periodic box of one block in FP64 with HLL fluxes");
     HIPCHK(c, hipSetDevice(c->device));
     const BlockDev &B = c->blocks[0];
     const size_t K = (size_t)B.ks, N = (size_t)ncell_of(B);

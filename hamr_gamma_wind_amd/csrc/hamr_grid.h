@@ -124,6 +124,16 @@ void fill_geometry(const GridDesc &d, const Layout &L, int c1, int c2, double *g
 void fill_geometry_level(const GridDesc &d, int level, int o1, int o2, int D1, int D2, int64_t kcol, double *gcov,
                          double *gcon, double *gdet, double *conn, double *rcen);
 
+// Orthonormal tetrad at a face of direction dir (1..3) from the packed gcov /
+// gcon there (pairs 00,01,02,03,11,12,13,22,23,33), set_Mud (hllc.c:20-178):
+// mud[mu*4 + a] = e_(a)^mu (e_(0) the normal observer, e_(dir) in the span of
+// the normal and grad x^dir), mudi[a*4 + mu] = e^(a)_mu, its inverse.
+void face_tetrad(const double *gcov, const double *gcon, int dir, double *mud, double *mudi);
+// The tetrads of one geometry slab in the device layout tet[(t*32 + e)*kcol +
+// col] (t = dir - 1, e < 16: mud, e >= 16: mudi) from the slab's packed metric
+// gcov/gcon[(pos*10 + s)*kcol + col]; columns >= ncol (padding) are zero.
+void fill_tetrads(const double *gcov, const double *gcon, int64_t ncol, int64_t kcol, double *tet);
+
 // Torus initial state on the whole grid, written per block in the device layout
 // (p: NPR components, ps: 3).  Returns an empty string or an error message.
 struct TorusDesc {

@@ -15,6 +15,8 @@ struct KParams {
     double rho_floor, u_floor;
     const BlockDev *blk;
     const double *gcov, *gcon, *gdet, *conn, *rcen;
+    const double *tet;  // face tetrads: tet[moff*96 + ((dir-1)*32 + e)*kcol + col] (e < 16: e_(a)^mu, else e^(a)_mu)
+    int riemann;        // HAMR_RIEMANN_HLL (0) or HAMR_RIEMANN_HLLC (1); HLLC in the FP64 step only
 };
 
 // Work list of one launch: `items` are (block, tile) pairs, a tile being

@@ -120,6 +120,51 @@ template <class R> HD void get_state(const R *pr, const Geom<R> &G, State<R> &q)
 
 template <class R> HD R dot4(const R *a, const R *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; }
 
+// Fast magnetosonic speeds along dir (the vchar part of GPU_phys_mhd.h:550-617)
+// given w = rho + P + u, bsq and eta = w + bsq of the state.
+template <class R> HD void vchar_coord(const R *pr, const State<R> &q, int dir, const Geom<R> &G, R gam, R w,
+                                       R bsq, R eta, R *vmax, R *vmin)
+{
+    R discr, vp, vm, va2, cs2, cms2, Acon_0, Acon_js, Asq, Bsq, Au, Bu, AB, Au2, Bu2, AuBu, A, B, C;
+    if (dir == 1) { Acon_0 = G.gcon[1]; Acon_js = G.gcon[4]; }
+    else if (dir == 2) { Acon_0 = G.gcon[2]; Acon_js = G.gcon[7]; }
+    else { Acon_0 = G.gcon[3]; Acon_js = G.gcon[9]; }
+    cs2 = dv(gam * (gam - R(1.)) * pr[UU], w);
+    va2 = dv(bsq, eta);
+    cms2 = cs2 + va2 - cs2 * va2;
+    if (cms2 < R(0.)) cms2 = R(kSMALL);
+    if (cms2 > R(1.)) cms2 = R(1.);
+    Asq = Acon_js;
+    Bsq = G.gcon[0];
+    Au = q.ucon[dir];
+    Bu = q.ucon[0];
+    AB = Acon_0;
+    Au2 = Au * Au;
+    Bu2 = Bu * Bu;
+    AuBu = Au * Bu;
+    A = Bu2 - (Bsq + Bu2) * cms2;
+    B = R(2.) * (AuBu - (AB + AuBu) * cms2);
+    C = Au2 - (Asq + Au2) * cms2;
+    discr = B * B - R(4.) * A * C;
+    if ((discr < R(0.0)) && (discr > R(-1.e-10))) discr = R(0.0);
+    else if (discr < R(-1.e-10)) discr = R(0.);
+    discr = sqrt(discr);
+    vp = dv(-(-B + discr), R(2.) * A);
+    vm = dv(-(-B - discr), R(2.) * A);
+    *vmax = mymax(vp, vm);
+    *vmin = mymin(vp, vm);
+}
+
+// The standalone vchar (GPU_phys_mhd.h:779-885): the same speeds with the
+// enthalpy formed as w = rho + gam u.
+template <class R> HD void vchar_state(const R *pr, const State<R> &q, int dir, const Geom<R> &G, R gam, R *vmax,
+                                       R *vmin)
+{
+    const R w = pr[RHO] + gam * pr[UU];
+    const R bsq = dot4(q.bcon, q.bcov);
+    vchar_coord(pr, q, dir, G, gam, w, bsq, w + bsq, vmax, vmin);
+}
+
 // GPU_phys_mhd.h:432-623 (ideal gas).  Speeds are written only for dir != 0.
 template <class R> HD void primtoflux(const R *pr, const State<R> &q, int dir, const Geom<R> &G, R gam,
                                       R *flux, R *vmax, R *vmin)
@@ -151,36 +196,7 @@ template <class R> HD void primtoflux(const R *pr, const State<R> &q, int dir, c
         flux[KTOT] = flux[RHO] * entr;
     }
     for (int k = 0; k < NPR; k++) flux[k] *= G.g;
-    if (dir != 0) {
-        R discr, vp, vm, va2, cs2, cms2, Acon_0, Acon_js, Asq, Bsq, Au, Bu, AB, Au2, Bu2, AuBu, A, B, C;
-        if (dir == 1) { Acon_0 = G.gcon[1]; Acon_js = G.gcon[4]; }
-        else if (dir == 2) { Acon_0 = G.gcon[2]; Acon_js = G.gcon[7]; }
-        else { Acon_0 = G.gcon[3]; Acon_js = G.gcon[9]; }
-        cs2 = dv(gam * (gam - R(1.)) * pr[UU], w);
-        va2 = dv(bsq, eta);
-        cms2 = cs2 + va2 - cs2 * va2;
-        if (cms2 < R(0.)) cms2 = R(kSMALL);
-        if (cms2 > R(1.)) cms2 = R(1.);
-        Asq = Acon_js;
-        Bsq = G.gcon[0];
-        Au = q.ucon[dir];
-        Bu = q.ucon[0];
-        AB = Acon_0;
-        Au2 = Au * Au;
-        Bu2 = Bu * Bu;
-        AuBu = Au * Bu;
-        A = Bu2 - (Bsq + Bu2) * cms2;
-        B = R(2.) * (AuBu - (AB + AuBu) * cms2);
-        C = Au2 - (Asq + Au2) * cms2;
-        discr = B * B - R(4.) * A * C;
-        if ((discr < R(0.0)) && (discr > R(-1.e-10))) discr = R(0.0);
-        else if (discr < R(-1.e-10)) discr = R(0.);
-        discr = sqrt(discr);
-        vp = dv(-(-B + discr), R(2.) * A);
-        vm = dv(-(-B - discr), R(2.) * A);
-        *vmax = mymax(vp, vm);
-        *vmin = mymin(vp, vm);
-    }
+    if (dir != 0) vchar_coord(pr, q, dir, G, gam, w, bsq, eta, vmax, vmin);
 }
 
 // GPU_phys_mhd.h:34-197 (ideal gas): dU_nu = gdet T^kap_lam Gamma^lam_{nu kap}.

@@ -8,6 +8,7 @@
 // equal size).
 #pragma once
 
+#include "hamr_hllc.h"
 #include "hamr_kernels_dev.h"
 
 namespace hamr {
@@ -61,6 +62,49 @@ __device__ __forceinline__ double hll_face(const KParams &P, const BlockDev &B, 
     return (v < 1.e9 && v > 1.e-6) ? v : 1.e9;
 }
 
+// HLLC flux through the same face (fluxcalc2D_FT, GPU_hllc.h:3-247): the
+// Courant dt from the coordinate speeds as in hll_face, the flux from the
+// orthonormal-frame solver of hamr_hllc.h with the face's tetrad.
+template <int DIR, class R>
+__device__ __forceinline__ double hllc_face(const KParams &P, const BlockDev &B, const double *__restrict__ ps, R *pl,
+                                            R *pr, int i, int j, int64_t cell, R *__restrict__ F)
+{
+    const int64_t K = B.ks;
+    const R bn = (R)ps[3 * B.off + (DIR - 1) * K + cell];
+    pl[B1 + DIR - 1] = bn;
+    pr[B1 + DIR - 1] = bn;
+    if (DIR == 2 && (((B.pole & 1) && j == NG) || ((B.pole & 2) && j == NG + B.bs[1]))) {
+        pl[B1] = R(0.);
+        pr[B1] = R(0.);
+    }
+    const int64_t col = (int64_t)i * B.D[1] + j;
+    const Geom<R> G = load_geom<R>(P, B, DIR == 1 ? FACE1 : DIR == 2 ? FACE2 : FACE3, col);
+    const R gam = (R)P.gam;
+    State<R> ql, qr;
+    get_state(pl, G, ql);
+    get_state(pr, G, qr);
+    R cmax_l, cmin_l, cmax_r, cmin_r;
+    vchar_state(pl, ql, DIR, G, gam, &cmax_l, &cmin_l);
+    vchar_state(pr, qr, DIR, G, gam, &cmax_r, &cmin_r);
+    const R cmax = fabs(mymax(mymax(R(0.), cmax_l), cmax_r));
+    const R cmin = fabs(mymax(mymax(R(0.), -cmin_l), -cmin_r));
+    const R ctop = mymax(cmax, cmin);
+    R mud[16], mudi[16];
+    const double *T = P.tet + B.moff * 96 + (int64_t)(DIR - 1) * 32 * B.kcol + col;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        mud[e] = (R)T[(int64_t)e * B.kcol];
+        mudi[e] = (R)T[(int64_t)(16 + e) * B.kcol];
+    }
+    R F1[NPR];
+    hllc_frame_flux<DIR>(pl, pr, ql.ucon, ql.bcon, qr.ucon, qr.bcon, mud, mudi, face_velocity<DIR>(G.gcon), gam, F1);
+    R *Fb = F + NPR * B.off + cell;
+#pragma unroll
+    for (int k = 0; k < NPR; k++) Fb[(int64_t)k * K] = G.g * F1[k];
+    const double v = P.cour * B.dx[DIR - 1] / (double)ctop;
+    return (v < 1.e9 && v > 1.e-6) ? v : 1.e9;
+}
+
 __device__ __forceinline__ void dt_reduce(double vdt, unsigned long long *__restrict__ dtmin, int slot)
 {
     vdt = wave_min(vdt);
@@ -73,8 +117,9 @@ __device__ __forceinline__ void dt_reduce(double vdt, unsigned long long *__rest
 // each thread through a segment of faces along x1 or x2 and forms each cell's
 // parabola once issues 29% fewer FP32 VALU instructions per face, but its
 // carried state takes 157 registers (three waves) and it ran 7-35% slower.
-template <int DIR, class R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+// RS selects the Riemann solver: 0 HLL, 1 HLLC.
+template <int DIR, class R, int RS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RS ? 1 : 3)))
 k_flux(KParams P, WorkList wl, const double *__restrict__ p, const double *__restrict__ ps, R *__restrict__ F,
        unsigned long long *__restrict__ dtmin)
 {
@@ -109,7 +154,8 @@ k_flux(KParams P, WorkList wl, const double *__restrict__ p, const double *__res
                 para(ym2, ym1, y0, yp1, yp2, l, r);
                 pr[k] = l;
             }
-            vdt = hll_face<DIR>(P, B, ps, pl, pr, i, j, cell, F);
+            if constexpr (RS == 1) vdt = hllc_face<DIR>(P, B, ps, pl, pr, i, j, cell, F);
+            else vdt = hll_face<DIR>(P, B, ps, pl, pr, i, j, cell, F);
         }
         if (dtmin) dt_reduce(vdt, dtmin, n * 3 + DIR - 1);
     }
@@ -232,9 +278,17 @@ void recon_flux(const KParams &P, int dir, const Launch &L, const double *p, con
                 unsigned long long *dtmin, hipStream_t s)
 {
     if (L.wl.n <= 0) return;
-    if (dir == 1) hipLaunchKernelGGL((k_flux<1, R>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
-    else if (dir == 2) hipLaunchKernelGGL((k_flux<2, R>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
-    else hipLaunchKernelGGL((k_flux<3, R>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
+    if constexpr (sizeof(R) == 8) {
+        if (P.riemann == 1) {
+            if (dir == 1) hipLaunchKernelGGL((k_flux<1, R, 1>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
+            else if (dir == 2) hipLaunchKernelGGL((k_flux<2, R, 1>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
+            else hipLaunchKernelGGL((k_flux<3, R, 1>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
+            return;
+        }
+    }
+    if (dir == 1) hipLaunchKernelGGL((k_flux<1, R, 0>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
+    else if (dir == 2) hipLaunchKernelGGL((k_flux<2, R, 0>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
+    else hipLaunchKernelGGL((k_flux<3, R, 0>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
 }
 
 template <class R>

@@ -404,6 +404,75 @@ void fill_geometry_level(const GridDesc &d, int level, int o1, int o2, int D1, i
 
 // ---------------------------------------------------------------------------
 // Fishbone-Moncrief torus
+// ---------------------------------------------------------------------------
+// Face tetrads (set_Mud, hllc.c:20-178).  For face direction a the other two
+// directions are taken cyclically, b = a % 3 + 1 and c = b % 3 + 1:
+//   e_(0) = A g^{0mu} (A = -(-g^00)^-1/2: the normal observer),
+//   e_(a) from the normal and g^{a mu}, e_(b) in the (b, c) plane orthogonal to
+//   e_(c) = g_cc^-1/2 d_c.
+// The products keep the reference's operand order (same roundings).
+static int pair10(int x, int y)
+{
+    static const int t[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+    return t[x][y];
+}
+
+void face_tetrad(const double *gcov, const double *gcon, int dir, double *mud, double *mudi)
+{
+    const int a = dir, b = dir % 3 + 1, c = b % 3 + 1;
+    auto gu = [&](int x, int y) { return gcon[pair10(x, y)]; };
+    auto gd = [&](int x, int y) { return gcov[pair10(x, y)]; };
+    const double A = -std::pow(-gu(0, 0), -0.5);
+    const double B = std::pow((gu(0, 0)) * (gu(0, 0) * gu(a, a) - gu(0, a) * gu(0, a)), -0.5);
+    const double C = std::pow(gd(c, c), -0.5);
+    const double D = std::pow((gd(c, c)) * (gd(b, b) * gd(c, c) - gd(b, c) * gd(b, c)), -0.5);
+    for (int k = 0; k < 16; k++) mud[k] = mudi[k] = 0.;
+    double *M = mud, *Mi = mudi;
+    M[0 * 4 + 0] = A * gu(0, 0);
+    M[a * 4 + 0] = A * gu(0, a);
+    M[a * 4 + a] = B * (gu(0, a) * gu(0, a) - gu(0, 0) * gu(a, a));
+    M[b * 4 + 0] = A * gu(0, b);
+    M[b * 4 + a] = B * (gu(0, a) * gu(0, b) - gu(0, 0) * gu(a, b));
+    M[b * 4 + b] = D * gd(c, c);
+    M[c * 4 + 0] = A * gu(0, c);
+    M[c * 4 + a] = B * (gu(0, a) * gu(0, c) - gu(0, 0) * gu(a, c));
+    M[c * 4 + b] = -D * gd(b, c);
+    M[c * 4 + c] = C;
+    const double E = gu(0, a) * gu(a, b) - gu(a, a) * gu(0, b);
+    const double F = gu(0, a) * gu(0, b) - gu(0, 0) * gu(a, b);
+    const double G = gu(0, a) * gu(a, c) - gu(a, a) * gu(0, c);
+    const double H = gu(0, a) * gu(0, c) - gu(0, 0) * gu(a, c);
+    Mi[0 * 4 + 0] = -A;
+    Mi[a * 4 + 0] = B * gu(0, a);
+    Mi[a * 4 + a] = -B * gu(0, 0);
+    Mi[b * 4 + 0] = B * B * E * gu(0, 0) / (D * gd(c, c));
+    Mi[b * 4 + a] = B * B * F * gu(0, 0) / (D * gd(c, c));
+    Mi[b * 4 + b] = 1. / (D * gd(c, c));
+    Mi[c * 4 + 0] = (B * B / C) * gu(0, 0) * (G + E * gd(b, c) / gd(c, c));
+    Mi[c * 4 + a] = (B * B / C) * gu(0, 0) * (H + F * gd(b, c) / gd(c, c));
+    Mi[c * 4 + b] = (1. / C) * gd(b, c) / gd(c, c);
+    Mi[c * 4 + c] = 1. / C;
+}
+
+void fill_tetrads(const double *gcov, const double *gcon, int64_t ncol, int64_t kcol, double *tet)
+{
+    std::memset(tet, 0, sizeof(double) * 96 * (size_t)kcol);
+    static const int posof[3] = {FACE1, FACE2, FACE3};
+    for (int t = 0; t < 3; t++)
+        for (int64_t col = 0; col < ncol; col++) {
+            double gc[10], gi[10], m[16], mi[16];
+            for (int s = 0; s < 10; s++) {
+                gc[s] = gcov[((int64_t)posof[t] * 10 + s) * kcol + col];
+                gi[s] = gcon[((int64_t)posof[t] * 10 + s) * kcol + col];
+            }
+            face_tetrad(gc, gi, t + 1, m, mi);
+            for (int e = 0; e < 16; e++) {
+                tet[((int64_t)t * 32 + e) * kcol + col] = m[e];
+                tet[((int64_t)t * 32 + 16 + e) * kcol + col] = mi[e];
+            }
+        }
+}
+
 static double lfish_calc(double a, double r)
 {
     return (((a * a - 2. * a * std::sqrt(r) + r * r) *

@@ -89,6 +89,11 @@ enum {
  * FP64. */
 enum { HAMR_PREC_FP64 = 0, HAMR_PREC_MIXED = 1 };
 
+/* Riemann solver of the fluxes (hamr_set_riemann): HLL (fluxcalc2D2,
+ * GPU_program1.cu:791-1068), or HLLC in the face's orthonormal frame
+ * (fluxcalc2D_FT, GPU_hllc.h:3-247; the reference's HLLC = 1, config.h:480). */
+enum { HAMR_RIEMANN_HLL = 0, HAMR_RIEMANN_HLLC = 1 };
+
 /* ---- host-only set-up (no device needed; for drivers and tests) ---- */
 /* Layout of a descriptor: D[3], cells per slab, padded slab length, metric
  * columns per slab and padded length, block and geometry counts. */
@@ -158,6 +163,20 @@ const char *hamr_last_error(const hamr_ctx *ctx);
  * counterpart: the reference's precision is fixed at compile time). */
 int hamr_set_precision(hamr_ctx *ctx, int mode);
 int hamr_get_precision(const hamr_ctx *ctx, int *mode);
+/* Select the Riemann solver of every later flux stage and step (the
+ * reference's HLLC / FRAME_TRANSFORM compile-time switch, config.h:480 and
+ * :1434-1438).  HLLC runs in the FP64 step only: it is refused in the mixed
+ * arithmetic, and the mixed arithmetic is refused while HLLC is selected. */
+int hamr_set_riemann(hamr_ctx *ctx, int solver);
+int hamr_get_riemann(const hamr_ctx *ctx, int *solver);
+/* set_Mud (hllc.c:20-178): the orthonormal tetrad at a face of direction dir
+ * (1..3) from the packed gcov/gcon there (pairs 00,01,02,03,11,12,13,22,23,33):
+ * mud[mu*4 + a] = e_(a)^mu, mudi[a*4 + mu] = e^(a)_mu.  Host only. */
+int hamr_face_tetrad(const double *gcov10, const double *gcon10, int dir, double *mud, double *mudi);
+/* The device copy of geometry slab geo's tetrads (computed by
+ * hamr_upload_metric / hamr_set_grid): tet[((dir-1)*32 + e)*kcol + col], e < 16
+ * mud, e >= 16 mudi; 96*kcol doubles. */
+int hamr_download_tetrads(hamr_ctx *ctx, int geo, double *tet);
 
 /* Layout: D[3] = BS+6 extents, ncell, padded slab length, block and geometry counts. */
 int hamr_layout(const hamr_ctx *ctx, int *D, long long *ncell, long long *ksize, int *nblk, int *ngeo);

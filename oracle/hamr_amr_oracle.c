@@ -378,6 +378,7 @@ static oracle_params block_params(const oracle_amr *g, int b)
     for (int k = 0; k < 3; k++) par.dx[k] = g->dx0[k] * ldexp(1.0, -B->level);
     par.rho_floor = g->rho_floor;
     par.u_floor = g->u_floor;
+    par.riemann = g->riemann;
     return par;
 }
 

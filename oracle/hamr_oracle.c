@@ -161,6 +161,307 @@ static void primtoflux(const double *pr, const ostate *q, int dir, const double 
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* HLLC in the face's orthonormal frame (HLLC = 1 builds of the reference).   */
+
+/* packed index of the symmetric pair (x, y) */
+static int pk10(int x, int y)
+{
+    static const int t[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+    return t[x][y];
+}
+
+/* set_Mud (hllc.c:20-178).  For face direction a the reference writes out the
+ * three cases; they are one construction with the other two directions taken
+ * cyclically (b = a % 3 + 1, c = b % 3 + 1), restated here in that form. */
+void oracle_tetrad(const double *gcov, const double *gcon, int dir, double *mud, double *mudi)
+{
+    const int a = dir, b = dir % 3 + 1, c = b % 3 + 1;
+#define GU(x, y) gcon[pk10(x, y)]
+#define GD(x, y) gcov[pk10(x, y)]
+    double A = -pow(-GU(0, 0), -0.5);
+    double B = pow((GU(0, 0)) * (GU(0, 0) * GU(a, a) - GU(0, a) * GU(0, a)), -0.5);
+    double C = pow(GD(c, c), -0.5);
+    double D = pow((GD(c, c)) * (GD(b, b) * GD(c, c) - GD(b, c) * GD(b, c)), -0.5);
+    double E, F, G, H;
+    int k;
+    for (k = 0; k < 16; k++) mud[k] = mudi[k] = 0.;
+    mud[0] = A * GU(0, 0);
+    mud[a * 4 + 0] = A * GU(0, a);
+    mud[a * 4 + a] = B * (GU(0, a) * GU(0, a) - GU(0, 0) * GU(a, a));
+    mud[b * 4 + 0] = A * GU(0, b);
+    mud[b * 4 + a] = B * (GU(0, a) * GU(0, b) - GU(0, 0) * GU(a, b));
+    mud[b * 4 + b] = D * GD(c, c);
+    mud[c * 4 + 0] = A * GU(0, c);
+    mud[c * 4 + a] = B * (GU(0, a) * GU(0, c) - GU(0, 0) * GU(a, c));
+    mud[c * 4 + b] = -D * GD(b, c);
+    mud[c * 4 + c] = C;
+    E = GU(0, a) * GU(a, b) - GU(a, a) * GU(0, b);
+    F = GU(0, a) * GU(0, b) - GU(0, 0) * GU(a, b);
+    G = GU(0, a) * GU(a, c) - GU(a, a) * GU(0, c);
+    H = GU(0, a) * GU(0, c) - GU(0, 0) * GU(a, c);
+    mudi[0] = -A;
+    mudi[a * 4 + 0] = B * GU(0, a);
+    mudi[a * 4 + a] = -B * GU(0, 0);
+    mudi[b * 4 + 0] = B * B * E * GU(0, 0) / (D * GD(c, c));
+    mudi[b * 4 + a] = B * B * F * GU(0, 0) / (D * GD(c, c));
+    mudi[b * 4 + b] = 1. / (D * GD(c, c));
+    mudi[c * 4 + 0] = (B * B / C) * GU(0, 0) * (G + E * GD(b, c) / GD(c, c));
+    mudi[c * 4 + a] = (B * B / C) * GU(0, 0) * (H + F * GD(b, c) / GD(c, c));
+    mudi[c * 4 + b] = (1. / C) * GD(b, c) / GD(c, c);
+    mudi[c * 4 + c] = 1. / C;
+#undef GU
+#undef GD
+}
+
+/* primtoflux_FT (hllc.c:180-213; GPU_hllc.h:903-997): flat-frame row dir of
+ * the MHD stress tensor (no rest mass added), number, field and entropy-tracer
+ * fluxes; dir = 0 gives the densities. */
+static void ft_flux(const double *pr, const double *uc, const double *bc, int dir, double gam, double *fl)
+{
+    double P = (gam - 1.) * pr[UU];
+    double w = P + pr[RHO] + pr[UU];
+    double bsq = -bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2] + bc[3] * bc[3];
+    double eta = w + bsq, ptot = P + 0.5 * bsq;
+    int j;
+    fl[RHO] = pr[RHO] * uc[dir];
+    fl[UU] = -eta * uc[dir] * uc[0] + ptot * (dir == 0 ? 1. : 0.) + bc[dir] * bc[0];
+    for (j = 1; j < 4; j++) fl[UU + j] = eta * uc[dir] * uc[j] + ptot * (dir == j ? 1. : 0.) - bc[dir] * bc[j];
+    for (j = B1; j <= B3; j++) fl[j] = bc[j - 4] * uc[dir] - bc[dir] * uc[j - 4];
+    fl[KTOT] = fl[RHO] * P * pow(pr[RHO], -gam);
+}
+
+/* vchar_FT (hllc.c:201-274; GPU_hllc.h:999-1111, FULL_DISP off) */
+static void ft_speeds(const double *pr, const double *uc, const double *bc, int dir, double gam, double *vmax,
+                      double *vmin)
+{
+    double bsq = -bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2] + bc[3] * bc[3];
+    double EF = pr[RHO] + gam * pr[UU];
+    double cs2 = gam * (gam - 1.) * pr[UU] / EF;
+    double EE = bsq + EF, va2 = bsq / EE;
+    double cms2 = cs2 + va2 - cs2 * va2;
+    double Au, Bu, Au2, Bu2, AuBu, A, B, C, discr, vp, vm;
+    if (cms2 < 0.) cms2 = SMALL;
+    if (cms2 > 1.) cms2 = 1.;
+    Au = uc[dir];
+    Bu = uc[0];
+    Au2 = Au * Au;
+    Bu2 = Bu * Bu;
+    AuBu = Au * Bu;
+    A = Bu2 - (-1. + Bu2) * cms2;
+    B = 2. * (AuBu - (AuBu)*cms2);
+    C = Au2 - (1. + Au2) * cms2;
+    discr = B * B - 4. * A * C;
+    if ((discr < 0.0) && (discr > -1.e-10)) discr = 0.0;
+    else if (discr < -1.e-10) discr = 0.;
+    discr = sqrt(discr);
+    vp = -(-B + discr) / (2. * A);
+    vm = -(-B - discr) / (2. * A);
+    if (vp > vm) { *vmax = vp; *vmin = vm; }
+    else { *vmax = vm; *vmin = vp; }
+}
+
+/* the standalone coordinate vchar (GPU_phys_mhd.h:779-885): as the speeds of
+ * primtoflux but with w = rho + gam u */
+static void coord_speeds(const double *pr, const ostate *q, int dir, const double *gcon, double gam, double *vmax,
+                         double *vmin)
+{
+    double w = pr[RHO] + gam * pr[UU];
+    double cs2 = gam * (gam - 1.) * pr[UU] / w;
+    double bsq = DOT4(q->bcon, q->bcov), eta = w + bsq, va2 = bsq / eta;
+    double cms2 = cs2 + va2 - cs2 * va2;
+    int dd = dir == 1 ? 4 : dir == 2 ? 7 : 9;
+    double Asq = gcon[dd], Bsq = gcon[0], AB = gcon[dir], Au = q->ucon[dir], Bu = q->ucon[0];
+    double Au2, Bu2, AuBu, A, B, C, discr, vp, vm;
+    if (cms2 < 0.) cms2 = SMALL;
+    if (cms2 > 1.) cms2 = 1.;
+    Au2 = Au * Au;
+    Bu2 = Bu * Bu;
+    AuBu = Au * Bu;
+    A = Bu2 - (Bsq + Bu2) * cms2;
+    B = 2. * (AuBu - (AB + AuBu) * cms2);
+    C = Au2 - (Asq + Au2) * cms2;
+    discr = B * B - 4. * A * C;
+    if ((discr < 0.0) && (discr > -1.e-10)) discr = 0.0;
+    else if (discr < -1.e-10) discr = 0.;
+    discr = sqrt(discr);
+    vp = -(-B + discr) / (2. * A);
+    vm = -(-B - discr) / (2. * A);
+    *vmax = MYMAX(vp, vm);
+    *vmin = MYMIN(vp, vm);
+}
+
+/* the contact's star state on the upwind side (index 0 of S) and the HLLC flux
+ * (index 1) for a wave speed c and contact speed vn, from that side's state U,
+ * flux Fs and 4-velocity (u0, ud): the parts the hydro and MHD variants share
+ * (calc_HLLC_hydro :279-311, calc_HLLC :360-394): rho and the entropy tracer
+ * scale with the upwind state. */
+static void star_common(double c, double vn, const double *U, double u0, double ud, double S[2][9])
+{
+    double s = (c - ud / u0) / (c - vn + SMALL);
+    S[0][RHO] = s * U[RHO];
+    S[0][KTOT] = s * U[KTOT];
+}
+
+/* calc_HLLC_hydro (GPU_hllc.h:250-315) */
+static void hllc_hydro_o(int dir, const double *lu, const double *ru, double vint, double cmin, double cmax,
+                         double S[2][9], double H[2][9], const double *Fl, const double *Fr, const double *Ul,
+                         const double *Ur)
+{
+    const int g1 = dir, g2 = dir % 3 + 1, g3 = g2 % 3 + 1;
+    const int u1 = UU + g1, u2 = UU + g2, u3 = UU + g3, b1 = B1 - 1 + g1, b2 = B1 - 1 + g2, b3 = B1 - 1 + g3;
+    double A = -H[1][UU];
+    double B = -H[1][u1] + H[0][UU];
+    double C = H[0][u1];
+    double D = B * B - 4. * A * C;
+    double vcon = (-B - sqrt(D)) / (2. * A);
+    double ptot = -(-H[1][UU]) * vcon + H[1][u1];
+    int fail = !(fabs(ptot) > 0.) || (vcon < cmin) || (vcon > cmax) || !(fabs(vcon) > 0.);
+    int k, side = 0;
+    if (cmax > vint && vcon <= vint && !fail) side = 1;
+    else if (cmin < vint && vcon >= vint && !fail) side = -1;
+    if (side) {
+        const double c = side > 0 ? cmax : cmin;
+        const double *U = side > 0 ? Ur : Ul, *Fs = side > 0 ? Fr : Fl, *uu = side > 0 ? ru : lu;
+        double s;
+        star_common(c, vcon, U, uu[0], uu[dir], S);
+        s = (c - uu[dir] / uu[0]) / (c - vcon + SMALL);
+        S[0][UU] = (c * U[UU] + U[u1] - ptot * vcon) / (c - vcon + SMALL);
+        S[0][u1] = (-S[0][UU] + ptot) * vcon;
+        S[0][u2] = s * U[u2];
+        S[0][u3] = s * U[u3];
+        S[0][b1] = H[0][b1];
+        S[0][b2] = s * U[b2];
+        S[0][b3] = s * U[b3];
+        for (k = 0; k < ORC_NPR; k++) S[1][k] = (Fs[k] + c * (S[0][k] - U[k]));
+    } else {
+        for (k = 0; k < ORC_NPR; k++) {
+            S[0][k] = H[0][k];
+            S[1][k] = H[1][k];
+        }
+    }
+}
+
+/* calc_HLLC (GPU_hllc.h:317-400) */
+static void hllc_mhd_o(int dir, const double *lu, const double *ru, double vint, double cmin, double cmax,
+                       double S[2][9], double H[2][9], const double *Fl, const double *Fr, const double *Ul,
+                       const double *Ur)
+{
+    const int g1 = dir, g2 = dir % 3 + 1, g3 = g2 % 3 + 1;
+    const int u1 = UU + g1, u2 = UU + g2, u3 = UU + g3, b1 = B1 - 1 + g1, b2 = B1 - 1 + g2, b3 = B1 - 1 + g3;
+    double v[4], A, B, C, D, gammasq, vdB, ptot;
+    int k, side = 0, fail;
+    A = -H[1][UU] - (H[0][b2] * H[1][b2] + H[0][b3] * H[1][b3]);
+    B = -H[1][u1] + H[0][UU] + (H[0][b2] * H[0][b2] + H[0][b3] * H[0][b3]) + (H[1][b2] * H[1][b2] + H[1][b3] * H[1][b3]);
+    C = H[0][u1] - (H[0][b2] * H[1][b2] + H[0][b3] * H[1][b3]);
+    D = B * B - 4. * A * C;
+    v[0] = 0.;
+    v[g1] = (-B - sqrt(D)) / (2. * A);
+    v[g2] = (H[0][b2] * v[g1] - H[1][b2]) / H[0][b1];
+    v[g3] = (H[0][b3] * v[g1] - H[1][b3]) / H[0][b1];
+    gammasq = 1. / (1. - (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]));
+    vdB = (v[1] * H[0][B1] + v[2] * H[0][B2] + v[3] * H[0][B3]);
+    /* pow(Bn, 2.0) of the reference is the correctly rounded square */
+    ptot = -(-H[1][UU] - H[0][b1] * (vdB)) * v[dir] + H[1][u1] + (H[0][b1] * H[0][b1]) / gammasq;
+    fail = !(fabs(ptot) > 0.) || (v[g1] < cmin) || (v[g1] > cmax) || !(fabs(v[g1]) > 0.);
+    if (cmax > vint && v[dir] <= vint && !fail) side = 1;
+    else if (cmin < vint && v[dir] >= vint && !fail) side = -1;
+    if (side) {
+        const double c = side > 0 ? cmax : cmin;
+        const double *U = side > 0 ? Ur : Ul, *Fs = side > 0 ? Fr : Fl, *uu = side > 0 ? ru : lu;
+        star_common(c, v[dir], U, uu[0], uu[dir], S);
+        S[0][UU] = (c * U[UU] + U[u1] - ptot * v[dir] + vdB * H[0][b1]) / (c - v[dir] + SMALL);
+        S[0][u1] = (-S[0][UU] + ptot) * v[dir] - vdB * H[0][b1];
+        S[0][u2] = (-H[0][b1] * (H[0][b2] / (gammasq) + vdB * v[g2]) + c * U[u2] - Fs[u2]) / (c - v[dir] + SMALL);
+        S[0][u3] = (-H[0][b1] * (H[0][b3] / (gammasq) + vdB * v[g3]) + c * U[u3] - Fs[u3]) / (c - v[dir] + SMALL);
+        S[0][b1] = H[0][b1];
+        S[0][b2] = H[0][b2];
+        S[0][b3] = H[0][b3];
+        for (k = 0; k < ORC_NPR; k++) S[1][k] = (Fs[k] + c * (S[0][k] - U[k]));
+    } else {
+        for (k = 0; k < ORC_NPR; k++) {
+            S[0][k] = H[0][k];
+            S[1][k] = H[1][k];
+        }
+    }
+}
+
+double oracle_hllc_face(const double *pl, const double *pr, const double *gcov, const double *gcon, double g,
+                        double gam, int dir, double *F)
+{
+    double mud[16], mudi[16], lu[4], lb[4], ru[4], rb[4];
+    double Fl[9], Fr[9], Ul[9], Ur[9], H[2][9], S[2][9], F1[9];
+    double cmax_l, cmin_l, cmax_r, cmin_r, cmax, cmin, ctop, cmaxr, cminr, vint, m0, md;
+    ostate ql, qr;
+    int i1, j1, k, dd = dir == 1 ? 4 : dir == 2 ? 7 : 9;
+    get_state(pl, gcov, gcon, &ql);
+    get_state(pr, gcov, gcon, &qr);
+    coord_speeds(pl, &ql, dir, gcon, gam, &cmax_l, &cmin_l);
+    coord_speeds(pr, &qr, dir, gcon, gam, &cmax_r, &cmin_r);
+    cmax = fabs(MYMAX(MYMAX(0., cmax_l), cmax_r));
+    cmin = fabs(MYMAX(MYMAX(0., -cmin_l), -cmin_r));
+    ctop = MYMAX(cmax, cmin);
+    oracle_tetrad(gcov, gcon, dir, mud, mudi);
+    for (i1 = 0; i1 < 4; i1++) {
+        lu[i1] = lb[i1] = ru[i1] = rb[i1] = 0.0;
+        for (j1 = 0; j1 < 4; j1++) {
+            lu[i1] += ql.ucon[j1] * mudi[i1 * 4 + j1];
+            lb[i1] += ql.bcon[j1] * mudi[i1 * 4 + j1];
+            ru[i1] += qr.ucon[j1] * mudi[i1 * 4 + j1];
+            rb[i1] += qr.bcon[j1] * mudi[i1 * 4 + j1];
+        }
+    }
+    ft_flux(pl, lu, lb, dir, gam, Fl);
+    ft_flux(pr, ru, rb, dir, gam, Fr);
+    ft_flux(pl, lu, lb, 0, gam, Ul);
+    ft_flux(pr, ru, rb, 0, gam, Ur);
+    ft_speeds(pl, lu, lb, dir, gam, &cmax_l, &cmin_l);
+    ft_speeds(pr, ru, rb, dir, gam, &cmax_r, &cmin_r);
+    cmaxr = MYMAX(cmax_r, cmax_l);
+    cminr = MYMIN(cmin_r, cmin_l);
+    /* GPU_hllc.h:88 */
+    vint = gcon[dir] / (sqrt(gcon[dir] * gcon[dir] - gcon[0] * gcon[dd]));
+    if (cmaxr <= vint) {
+        for (k = 0; k < 9; k++) { S[0][k] = Ur[k]; S[1][k] = Fr[k]; }
+    } else if (cminr >= vint) {
+        for (k = 0; k < 9; k++) { S[0][k] = Ul[k]; S[1][k] = Fl[k]; }
+    } else {
+        for (k = 0; k < 9; k++) H[0][k] = (Fl[k] - Fr[k] + cmaxr * Ur[k] - cminr * Ul[k]) / (cmaxr - cminr + SMALL);
+        for (k = 0; k < 9; k++)
+            H[1][k] = ((cmaxr * Fl[k] - cminr * Fr[k] + cmaxr * cminr * (Ur[k] - Ul[k])) / (cmaxr - cminr + SMALL));
+        if (fabs(H[0][dir + B1 - 1] * H[0][dir + B1 - 1] * lu[0] * ru[0]) < pow(10., -10.) * fabs(H[0][UU]))
+            hllc_hydro_o(dir, lu, ru, vint, cminr, cmaxr, S, H, Fl, Fr, Ul, Ur);
+        else
+            hllc_mhd_o(dir, lu, ru, vint, cminr, cmaxr, S, H, Fl, Fr, Ul, Ur);
+    }
+    /* back to the coordinate basis (GPU_hllc.h:206-230) */
+    m0 = mud[dir * 4 + 0];
+    md = mud[dir * 4 + dir];
+    for (j1 = 0; j1 < 4; j1++) {
+        F1[j1 + UU] = 0.;
+        for (k = 0; k < 4; k++) {
+            F1[j1 + UU] += S[0][k + UU] * m0 * mudi[k * 4 + j1];
+            F1[j1 + UU] += S[1][k + UU] * md * mudi[k * 4 + j1];
+        }
+    }
+    S[0][U3] = 0.;
+    S[1][U3] = -S[0][B1 - 1 + dir];
+    for (j1 = 1; j1 < 4; j1++) {
+        F1[j1 + U3] = 0.;
+        for (k = 0; k < 4; k++) {
+            F1[j1 + U3] += S[0][k + U3] * m0 * mud[j1 * 4 + k];
+            F1[j1 + U3] += S[1][k + U3] * md * mud[j1 * 4 + k];
+        }
+    }
+    F1[RHO] = S[0][RHO] * m0;
+    F1[RHO] += S[1][RHO] * md;
+    F1[KTOT] = S[0][KTOT] * m0;
+    F1[KTOT] += S[1][KTOT] * md;
+    F1[UU] += F1[RHO];
+    for (k = 0; k < 9; k++) F[k] = g * F1[k];
+    return ctop;
+}
+
 /* GPU_phys_mhd.h:34-197 source: dU_nu = gdet T^kap_lam Gamma^lam_{nu kap},
  * contracted over the packed symmetric (nu,kap) pairs in the reference's order. */
 static void source(const double *ph, const ostate *q, const double *conn, double g, double gam, double *dU)
@@ -744,6 +1045,13 @@ void oracle_flux(const oracle_params *par, int dir, const double *p, const doubl
                 if (dir == 2 && ((pole_lo && j == NG) || (pole_hi && j == NG + par->n2))) {
                     pl[B1] = 0.;
                     pr[B1] = 0.;
+                }
+                if (par->riemann == 1) {
+                    double Fh[ORC_NPR];
+                    ctop = oracle_hllc_face(pl, pr, gcov, gcon, g, par->gam, dir, Fh);
+                    for (k = 0; k < ORC_NPR; k++) F[(long)k * G.ncell + c] = Fh[k];
+                    if (dtface) dtface[c] = factor / ctop;
+                    continue;
                 }
                 get_state(pl, gcov, gcon, &q);
                 primtoflux(pl, &q, dir, gcon, g, par->gam, t1, &cmax_l, &cmin_l);

@@ -50,6 +50,7 @@ typedef struct oracle_params {
     double dx[3];          /* code-coordinate spacings dx1, dx2, dx3 */
     double rho_floor;      /* rho >= rho_floor * r^-1.5  (0 disables) */
     double u_floor;        /* u   >= u_floor   * r^-2.5  (0 disables) */
+    int riemann;           /* 0: HLL, 1: HLLC in the face's orthonormal frame */
 } oracle_params;
 
 typedef struct oracle_metric {
@@ -68,6 +69,15 @@ void oracle_para(const double *x5, double *l, double *r);
  * on input and the solution on success; returns the reference's return code. */
 int oracle_utoprim(int which, const double *U, const double *gcov, const double *gcon,
                    double g, double gam, double *prim);
+
+/* set_Mud (hllc.c:20-178): orthonormal tetrad at a face of direction dir from
+ * the packed gcov/gcon there; mud[mu*4 + a] = e_(a)^mu, mudi[a*4 + mu] = e^(a)_mu. */
+void oracle_tetrad(const double *gcov, const double *gcon, int dir, double *mud, double *mudi);
+/* fluxcalc2D_FT (GPU_hllc.h:3-247) at one face from the reconstructed states
+ * pl, pr (normal field already set) and the face's metric: the 9 coordinate
+ * fluxes F (gdet included); returns the Courant speed ctop. */
+double oracle_hllc_face(const double *pl, const double *pr, const double *gcov, const double *gcon, double g,
+                        double gam, int dir, double *F);
 
 /* ---- grid stages ---- */
 /* HLL flux through the lower dir-faces; dtface (may be NULL) gets cour*dx/ctop
@@ -121,6 +131,7 @@ typedef struct oracle_amr {
     const oracle_metric *m;   /* per block */
     double gam, cour, dx0[3], rho_floor, u_floor;
     int reflux;               /* 1: flux correction at level jumps in the corrector */
+    int riemann;              /* 0: HLL, 1: HLLC */
 } oracle_amr;
 
 /* Ghost fill of every block (p, ps, pflag: arrays of per-block pointers). */

@@ -35,7 +35,7 @@ def oracle_metric(desc):
     return dict(gcov=gcov, gcon=gcon, gdet=gdet, conn=conn, rcen=rcen)
 
 
-def oracle_grid(oracle, desc, metric=None):
+def oracle_grid(oracle, desc, metric=None, riemann=0):
     n = tuple(desc.bs[k] * desc.nb[k] for k in range(3))
     bs = tuple(desc.bs[k] for k in range(3))
     sx, dx = H.desc_coords(desc)
@@ -45,7 +45,7 @@ def oracle_grid(oracle, desc, metric=None):
         bc = (O.BC_OUTFLOW, O.BC_POLE, O.BC_PERIODIC)
     if metric is None:
         metric = oracle_metric(desc)
-    return O.Grid(oracle, n, bs, bc, desc.gam, desc.cour, dx, metric, desc.rho_floor, desc.u_floor)
+    return O.Grid(oracle, n, bs, bc, desc.gam, desc.cour, dx, metric, desc.rho_floor, desc.u_floor, riemann=riemann)
 
 
 def block_coords(desc):
@@ -104,7 +104,7 @@ def device_to_oracle_metric(m, D1, D2):
                 rcen=np.ascontiguousarray(m["rcen"][:ncol]))
 
 
-def amr_oracle(oracle, desc, amr):
+def amr_oracle(oracle, desc, amr, riemann=0):
     """The oracle's multi-level grid for a block set (metric per block from the
     host geometry of the same set)."""
     info = H.desc_amr(desc, amr)
@@ -122,4 +122,4 @@ def amr_oracle(oracle, desc, amr):
     N = [tuple(x << l for x in n0) for l in range(nlev)]
     _, dx0 = H.desc_coords(desc)
     return O.AmrGrid(oracle, [r[:7] for r in rows], N, metrics, desc.gam, desc.cour, dx0, desc.rho_floor,
-                     desc.u_floor), rows
+                     desc.u_floor, riemann=riemann), rows

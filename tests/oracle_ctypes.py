@@ -28,6 +28,7 @@ class Params(C.Structure):
         ("gam", C.c_double), ("cour", C.c_double),
         ("dx", C.c_double * 3),
         ("rho_floor", C.c_double), ("u_floor", C.c_double),
+        ("riemann", C.c_int),
     ]
 
 
@@ -65,6 +66,9 @@ class Oracle:
         L.oracle_block_dt.restype = C.c_double
         L.oracle_step.argtypes = [P, _dp, _dp, M, C.c_double, _ip, C.POINTER(C.c_longlong)]
         L.oracle_step.restype = C.c_double
+        L.oracle_tetrad.argtypes = [_dp, _dp, C.c_int, _dp, _dp]
+        L.oracle_hllc_face.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp]
+        L.oracle_hllc_face.restype = C.c_double
         L.oracle_rdump_block_write.argtypes = [_ip, _dp, _dp, _dp, _dp]
         L.oracle_rdump_block_read.argtypes = [_ip, _ip, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
 
@@ -104,6 +108,21 @@ class Oracle:
         return ret, prim
 
 
+    # ---- HLLC ----
+    def tetrad(self, gcov, gcon, d):
+        mud, mudi = np.zeros(16), np.zeros(16)
+        self.lib.oracle_tetrad(_ptr(np.ascontiguousarray(gcov, float)), _ptr(np.ascontiguousarray(gcon, float)), d,
+                               _ptr(mud), _ptr(mudi))
+        return mud.reshape(4, 4), mudi.reshape(4, 4)
+
+    def hllc_face(self, pl, pr, gcov, gcon, g, gam, d):
+        """Coordinate HLLC flux (gdet included) through one face and ctop."""
+        F = np.zeros(NPR)
+        ctop = self.lib.oracle_hllc_face(_ptr(np.ascontiguousarray(pl, float)), _ptr(np.ascontiguousarray(pr, float)),
+                                         _ptr(np.ascontiguousarray(gcov, float)),
+                                         _ptr(np.ascontiguousarray(gcon, float)), g, gam, d, _ptr(F))
+        return F, ctop
+
     # ---- restart record of one block ----
     def rdump_write(self, bs, p, ps, gdet):
         """Record of one block (p: 9 x ncell, ps: 3 x ncell, gdet: 5 x ncol)."""
@@ -130,9 +149,10 @@ class Oracle:
 class Grid:
     """A global ghost-inclusive grid stepped by the oracle."""
 
-    def __init__(self, oracle, n, bs, bc, gam, cour, dx, metric, rho_floor=0.0, u_floor=0.0):
+    def __init__(self, oracle, n, bs, bc, gam, cour, dx, metric, rho_floor=0.0, u_floor=0.0, riemann=0):
         self.o = oracle
         self.par = Params()
+        self.par.riemann = riemann
         self.par.n1, self.par.n2, self.par.n3 = n
         self.par.bs1, self.par.bs2, self.par.bs3 = bs
         for d in range(3):
@@ -219,7 +239,7 @@ class AmrBlock(C.Structure):
 class AmrGridC(C.Structure):
     _fields_ = [("nblk", C.c_int), ("nlev", C.c_int), ("N", (C.c_int * 3) * 3), ("blk", C.POINTER(AmrBlock)),
                 ("m", C.POINTER(Metric)), ("gam", C.c_double), ("cour", C.c_double), ("dx0", C.c_double * 3),
-                ("rho_floor", C.c_double), ("u_floor", C.c_double), ("reflux", C.c_int)]
+                ("rho_floor", C.c_double), ("u_floor", C.c_double), ("reflux", C.c_int), ("riemann", C.c_int)]
 
 
 def _setup_amr(lib):
@@ -242,7 +262,8 @@ class AmrGrid:
     o2, o3, bs1, bs2, bs3); N[l] cells of level l; metrics: per block, the
     oracle layout (as Grid's metric)."""
 
-    def __init__(self, oracle, blocks, N, metrics, gam, cour, dx0, rho_floor=0.0, u_floor=0.0, reflux=True):
+    def __init__(self, oracle, blocks, N, metrics, gam, cour, dx0, rho_floor=0.0, u_floor=0.0, reflux=True,
+                 riemann=0):
         self.o = oracle
         _setup_amr(oracle.lib)
         nb = len(blocks)
@@ -270,6 +291,7 @@ class AmrGrid:
             self.g.dx0[k] = dx0[k]
         self.g.rho_floor, self.g.u_floor = rho_floor, u_floor
         self.g.reflux = 1 if reflux else 0
+        self.g.riemann = riemann
         self.failcount = (C.c_longlong * 4)()
 
     @staticmethod
