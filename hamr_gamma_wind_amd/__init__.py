@@ -163,6 +163,7 @@ def load_library(path=None):
         "hamr_fluxcalc": [ctxp, C.c_int, C.c_int, C.c_int],
         "hamr_block_dt": [ctxp, C.c_int, C.c_int, _dp],
         "hamr_flux_correct": [ctxp, C.c_int, C.c_double],
+        "hamr_emf_correct": [ctxp, C.c_int, C.c_double],
         "hamr_ct_emf": [ctxp, C.c_int, C.c_int],
         "hamr_ct_bound": [ctxp, C.c_double],
         "hamr_ct_update": [ctxp, C.c_int, C.c_double, C.c_int],
@@ -294,7 +295,7 @@ def desc_torus(desc, tdesc):
 
 
 AMR_INFO_KEYS = ("blocks", "geometries", "cells", "zones", "ex1", "ex2", "ex3", "pro1", "pro2", "pro3", "rings", "amr",
-                 "reflux1", "reflux2", "reflux3")
+                 "reflux1", "reflux2", "reflux3", "emf")
 
 
 def desc_amr(desc, amr, tables=False):
@@ -466,6 +467,9 @@ class Context:
 
     def flux_correct(self, flag, dt):
         self._chk(self.L.hamr_flux_correct(self.h, flag, dt), "hamr_flux_correct")
+
+    def emf_correct(self, flag, dt):
+        self._chk(self.L.hamr_emf_correct(self.h, flag, dt), "hamr_emf_correct")
 
     def ct_emf(self, flag, n):
         self._chk(self.L.hamr_ct_emf(self.h, flag, n), "hamr_ct_emf")
@@ -702,7 +706,8 @@ def advance_step(ctx, dt):
     """One predictor + corrector step through the per-block stage API, in the
     order of advance_GPU (step_ch.c:997-1336) with every block at timelevel 1:
     fluxes dir 3, 2, 1; flux correction at level jumps (full step); CT EMF;
-    polar E1 average; face-B update; conserved update + inversion; ghost fill.
+    polar E1 average; EMF correction at level jumps (full step); face-B update;
+    conserved update + inversion; ghost fill.
     Returns the next dt (per-block harmonic of
     the corrector's Courant minima, minimum over blocks, clamped to 1.3 dt)."""
     nb = ctx.nblk
@@ -720,6 +725,7 @@ def advance_step(ctx, dt):
         for n in range(nb):
             ctx.ct_emf(flag, n)
         ctx.ct_bound(dt)
+        ctx.emf_correct(flag, dt)
         for n in range(nb):
             ctx.ct_update(flag, Dt, n)
         for n in range(nb):

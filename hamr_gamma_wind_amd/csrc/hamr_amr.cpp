@@ -486,6 +486,80 @@ std::string build_amr(const GridDesc &d, const AmrDesc &a, AmrGrid &g)
             }
         }
     }
+    // EMF correction at level jumps (E_send1-3 / E_rec1-3 with calc_corr = 1 and
+    // the edge exchanges E*_send_corn / E*_receive_corn, boundcomE.c): every
+    // edge EMF on a block's boundary that one of the (up to four) leaves around
+    // the edge refines by one level takes the mean of that leaf's two edges
+    // along it.  The leaf is the first finer one in a fixed order of the
+    // quadrants around the edge, so every block holding the edge gets the same
+    // value.
+    {
+        g.emf.clear();
+        std::vector<char> dest;  // (block, comp, cell) marks, to check that no source is rewritten
+        std::vector<int64_t> base(nblk + 1, 0);
+        for (int n = 0; n < nblk; n++)
+            base[n + 1] = base[n] + 3 * (int64_t)b.blk[n].D[0] * b.blk[n].D[1] * b.blk[n].D[2];
+        dest.assign((size_t)base[nblk], 0);
+        for (int n = 0; n < nblk; n++) {
+            const BlockDev &C = b.blk[n];
+            const int l = C.level;
+            if (l + 1 >= a.nlev) continue;
+            const Level &L = b.lev[l];
+            const Level &Lf = b.lev[l + 1];
+            for (int t = 0; t < 3; t++) {
+                const int p = (t + 1) % 3, q = (t + 2) % 3;
+                for (int kt = C.o[t]; kt < C.o[t] + C.bs[t]; kt++)
+                    for (int kp = C.o[p]; kp <= C.o[p] + C.bs[p]; kp++) {
+                        const bool pb = (kp == C.o[p] || kp == C.o[p] + C.bs[p]);
+                        for (int kq = C.o[q]; kq <= C.o[q] + C.bs[q]; kq += (pb ? 1 : C.bs[q])) {
+                            int f = -1;
+                            for (int qd = 0; qd < 4 && f < 0; qd++) {
+                                int Y[3];
+                                Y[t] = kt;
+                                Y[p] = kp - 1 + (qd >> 1);
+                                Y[q] = kq - 1 + (qd & 1);
+                                bool in = true;
+                                for (int x : {p, q}) {
+                                    if (x < 2 && (Y[x] < 0 || Y[x] >= L.N[x])) in = false;
+                                    if (x == 2) Y[2] = ((Y[2] % L.N[2]) + L.N[2]) % L.N[2];
+                                }
+                                if (!in) continue;
+                                int lc = -1;
+                                const int cand = b.find(l, Y, &lc);
+                                if (cand >= 0 && lc == l + 1) f = cand;
+                            }
+                            if (f < 0) continue;
+                            const BlockDev &Fb = b.blk[f];
+                            int Z[3];
+                            Z[t] = 2 * kt;
+                            Z[p] = 2 * kp;
+                            Z[q] = 2 * kq;
+                            if (Z[2] < Fb.o[2] || Z[2] > Fb.o[2] + Fb.bs[2])  // across the periodic x3 seam
+                                Z[2] += (Z[2] < Fb.o[2]) ? Lf.N[2] : -Lf.N[2];
+                            for (int x = 0; x < 3; x++) {
+                                const int hi = Fb.o[x] + Fb.bs[x] - (x == t ? 2 : 0);
+                                if (Z[x] < Fb.o[x] || Z[x] > hi) return "amr: EMF correction source outside its block";
+                            }
+                            int K[3];
+                            K[t] = kt;
+                            K[p] = kp;
+                            K[q] = kq;
+                            const int cc = cell_in(C, K), fc = cell_in(Fb, Z);
+                            dest[base[n] + (int64_t)t * C.D[0] * C.D[1] * C.D[2] + cc] = 1;
+                            g.emf.push_back({n, cc | (t << 28), f, fc});
+                        }
+                    }
+            }
+        }
+        for (const AmrEmf &e : g.emf) {
+            const int t = e.cc >> 28;
+            const BlockDev &Fb = b.blk[e.fb];
+            const int64_t nc = (int64_t)Fb.D[0] * Fb.D[1] * Fb.D[2];
+            const int st = t == 0 ? Fb.D[1] * Fb.D[2] : t == 1 ? Fb.D[2] : 1;
+            if (dest[base[e.fb] + t * nc + e.fc] || dest[base[e.fb] + t * nc + e.fc + st])
+                return "amr: an EMF correction source is itself corrected";
+        }
+    }
     g.blocks = b.blk;
     g.nlev = a.nlev;
     for (int l = 0; l < a.nlev; l++)
@@ -536,6 +610,14 @@ std::string validate_amr(const AmrGrid &g)
             if (!dst_ok(e.fb, e.ff) || !dst_ok(e.fb, e.ff + sa + sb))
                 return "amr: flux correction source out of range";
         }
+    for (const AmrEmf &e : g.emf) {
+        const int t = e.cc >> 28;
+        if (t < 0 || t > 2 || !dst_ok(e.cb, e.cc & 0x0FFFFFFF)) return "amr: EMF correction entry out of range";
+        if (e.fb < 0 || e.fb >= (int)blk.size()) return "amr: EMF correction source out of range";
+        const BlockDev &F = blk[e.fb];
+        const int st = t == 0 ? F.D[1] * F.D[2] : t == 1 ? F.D[2] : 1;
+        if (!dst_ok(e.fb, e.fc) || !dst_ok(e.fb, e.fc + st)) return "amr: EMF correction source out of range";
+    }
     for (size_t n = 0; n < blk.size(); n++) {
         const BlockDev &B = blk[n];
         for (int f = 0; f < 6; f++)

@@ -33,6 +33,7 @@ struct AmrGrid {
     std::vector<AmrEntry> ex[3];    // per stage: copies and restrictions
     std::vector<AmrProlong> pro[3]; // per stage: prolongations
     std::vector<AmrReflux> rfx[3];  // per direction: coarse faces at level jumps
+    std::vector<AmrEmf> emf;        // coarse edges at level jumps
     std::vector<int> ring, rside;   // polar rings (ring_len blocks each) and their sides
     int ring_len = 0, ni_max = 0;
     int nlev = 0;

@@ -78,6 +78,8 @@ struct hamr_ctx {
     int64_t nex[3] = {0, 0, 0}, npro[3] = {0, 0, 0};
     AmrReflux *rfx[3] = {nullptr, nullptr, nullptr};  // flux correction at level jumps, per direction
     int64_t nrfx[3] = {0, 0, 0};
+    AmrEmf *emfx = nullptr;        // EMF correction at level jumps
+    int64_t nemfx = 0;
     hipGraphExec_t graph = nullptr;
     int prec = HAMR_PREC_FP64;     // arithmetic of reconstruction, fluxes and EMFs
     // adjoint scratch (allocated by the first adjoint call)
@@ -335,6 +337,11 @@ static void reflux_all(hamr_ctx *c, double dt_val, const double *dt_ptr)
         launch_amr_reflux(c->P, c->rfx[d], c->nrfx[d], d + 1, c->F[d], dt_val, dt_ptr, c->prec, c->s);
 }
 
+static void emf_correct_all(hamr_ctx *c, double dt_val, const double *dt_ptr)
+{
+    launch_amr_emf(c->P, c->emfx, c->nemfx, c->E, dt_val, dt_ptr, c->prec, c->s);
+}
+
 static void emf_all(hamr_ctx *c, const double *p)
 {
     launch_emf(c->P, all_launch(c, TS_ECENT), all_launch(c, TS_ECORNER), p, c->F[0], c->F[1], c->F[2], c->Ec, c->E,
@@ -466,6 +473,11 @@ int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *a, hamr_ctx **ou
             if (dalloc(c, &c->rfx[s], g.rfx[s].size())) return -1;
             if (h2d(c, c->rfx[s], g.rfx[s].data(), g.rfx[s].size() * sizeof(AmrReflux))) return -1;
         }
+        if (s == 0 && !g.emf.empty()) {
+            c->nemfx = (int64_t)g.emf.size();
+            if (dalloc(c, &c->emfx, g.emf.size())) return -1;
+            if (h2d(c, c->emfx, g.emf.data(), g.emf.size() * sizeof(AmrEmf))) return -1;
+        }
     }
     c->err.clear();
     return 0;
@@ -487,6 +499,7 @@ int hamr_amr_info(const hamr_ctx *c, long long *info)
     info[10] = c->nring;
     info[11] = c->amr ? 1 : 0;
     for (int d = 0; d < 3; d++) info[12 + d] = c->nrfx[d];
+    info[15] = c->nemfx;
     return 0;
 }
 
@@ -700,6 +713,7 @@ int hamr_desc_amr(const hamr_desc *desc, const hamr_amr_desc *a, long long *info
     info[10] = (long long)g.rside.size();
     info[11] = 1;
     for (int d = 0; d < 3; d++) info[12 + d] = (long long)g.rfx[d].size();
+    info[15] = (long long)g.emf.size();
     if (rows)
         for (size_t n = 0; n < g.blocks.size(); n++) {
             const BlockDev &B = g.blocks[n];
@@ -1201,6 +1215,13 @@ int hamr_flux_correct(hamr_ctx *c, int flag, double dt)
     return check_launch(c, "hamr_flux_correct");
 }
 
+int hamr_emf_correct(hamr_ctx *c, int flag, double dt)
+{
+    if (!c) return -1;
+    if (flag) emf_correct_all(c, dt, nullptr);
+    return check_launch(c, "hamr_emf_correct");
+}
+
 int hamr_block_dt(hamr_ctx *c, int n, int dir, double *out)
 {
     if (!c || !valid_block(c, n) || dir < 1 || dir > 3 || !out) return fail(c, "hamr_block_dt: bad arguments");
@@ -1296,6 +1317,7 @@ static void enqueue_step(hamr_ctx *c)
     reflux_all(c, 0., c->dt);
     emf_all(c, c->ph);
     pole(c, 0., c->dt);
+    emf_correct_all(c, 0., c->dt);
     launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 1.0, c->ps, c->ps, c->E, c->prec, s);
     update_all(c, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->ps);
     bound_all(c, c->p, c->ps, true);

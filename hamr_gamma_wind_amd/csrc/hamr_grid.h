@@ -66,7 +66,17 @@ struct AmrReflux {
     int cb, cf;
     int fb, ff;
 };
-static_assert(sizeof(AmrEntry) == 16 && sizeof(AmrProlong) == 64 && sizeof(AmrReflux) == 16, "table entry layout");
+// EMF correction at a level jump: the edge EMF component comp (0..2: E1..E3)
+// at corner cell cc & 0x0FFFFFFF of block cb (comp = cc >> 28) takes the mean of
+// the two edges of the one level finer block fb that tile it: corners fc and fc
+// + s, s fb's cell stride along the edge.
+struct AmrEmf {
+    int cb, cc;
+    int fb, fc;
+};
+static_assert(sizeof(AmrEntry) == 16 && sizeof(AmrProlong) == 64 && sizeof(AmrReflux) == 16 &&
+                  sizeof(AmrEmf) == 16,
+              "table entry layout");
 
 // Static description of a run (mirrors the reference's compile-time config).
 struct GridDesc {

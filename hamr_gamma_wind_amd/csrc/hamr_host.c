@@ -33,6 +33,7 @@ int hamr_host_advance(hamr_ctx *ctx, int nblk, double dt, double *ndt_out)
         if (!r) r = hamr_flux_correct(ctx, flag, dt);
         for (n = 0; n < nblk && !r; n++) r = hamr_ct_emf(ctx, flag, n);
         if (!r) r = hamr_ct_bound(ctx, dt);
+        if (!r) r = hamr_emf_correct(ctx, flag, dt);
         for (n = 0; n < nblk && !r; n++) r = hamr_ct_update(ctx, flag, Dt, n);
         for (n = 0; n < nblk && !r; n++) r = hamr_update(ctx, flag, n, Dt);
         if (!r) r = hamr_boundprim(ctx, flag ? 1 : 0);

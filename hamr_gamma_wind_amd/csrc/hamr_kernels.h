@@ -107,6 +107,10 @@ void launch_bound(const KParams &P, int dir, const Launch &L, double *p, double 
 // array, FP32 when mixed); dt from dt_ptr if non-null.
 void launch_amr_reflux(const KParams &P, const AmrReflux *e, int64_t n, int dir, void *F, double dt_val,
                        const double *dt_ptr, int mixed, hipStream_t s);
+// EMF correction at level jumps: each entry's coarse edge EMF takes the mean
+// of its two finer edges (E: the corner EMF array, FP32 when mixed).
+void launch_amr_emf(const KParams &P, const AmrEmf *e, int64_t n, void *E, double dt_val, const double *dt_ptr,
+                    int mixed, hipStream_t s);
 // Restart records (rdump files): RREC doubles per ghost-inclusive cell.  Item
 // q of a launch is block blist[q], whose record starts at out + roff[q] -
 // roff_host[first]; max_cells bounds the items' cell counts (pack: D1 D2 D3;

@@ -670,6 +670,27 @@ __global__ void __launch_bounds__(256) k_amr_reflux(KParams P, const AmrReflux *
 }
 
 
+// EMF correction at level jumps (the receiving side of E_rec1-3 with calc_corr
+// = 1, unpackreceive1E, GPU_program2.cu:2106-2230, fed by packsendEaverage1-3,
+// :1875-2105): the coarse edge takes ((0.5 dt) (e0 + e1)) / dt of its two finer
+// edges, in the corrector after the polar average.  Coarse edges written and
+// finer edges read are distinct (a source is never corrected itself).
+template <class R>
+__global__ void __launch_bounds__(256) k_amr_emf(KParams P, const AmrEmf *__restrict__ e, int64_t n,
+                                                 R *__restrict__ E, double dt_val, const double *__restrict__ dt_ptr)
+{
+    const double dt = dt_ptr ? dt_ptr[0] : dt_val;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const AmrEmf x = e[t];
+        const int comp = x.cc >> 28;
+        const BlockDev &C = P.blk[x.cb], &Fb = P.blk[x.fb];
+        const int64_t st = comp == 0 ? (int64_t)Fb.D[1] * Fb.D[2] : comp == 1 ? (int64_t)Fb.D[2] : 1;
+        const R *f = E + 3 * Fb.off + (int64_t)comp * Fb.ks + x.fc;
+        const double sum = (double)f[0] + (double)f[st];
+        E[3 * C.off + (int64_t)comp * C.ks + (x.cc & 0x0FFFFFFF)] = (R)(((0.5 * dt) * sum) / dt);
+    }
+}
+
 // Courant dt of the step: per-block harmonic of the direction minima, minimum
 // over blocks (step_ch.c:1313-1322), clamped to 1.3 dt (step_ch.c:974).  Resets
 // the minima for the next corrector.  dt[0] <- new dt, dt[1] <- previous dt.
@@ -855,6 +876,17 @@ void launch_amr_reflux(const KParams &P, const AmrReflux *e, int64_t n, int dir,
         hipLaunchKernelGGL(k_amr_reflux<float>, dim3(g), dim3(256), 0, s, P, e, n, dir, (float *)F, dt_val, dt_ptr);
     else
         hipLaunchKernelGGL(k_amr_reflux<double>, dim3(g), dim3(256), 0, s, P, e, n, dir, (double *)F, dt_val, dt_ptr);
+}
+
+void launch_amr_emf(const KParams &P, const AmrEmf *e, int64_t n, void *E, double dt_val, const double *dt_ptr,
+                    int mixed, hipStream_t s)
+{
+    if (n <= 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+    if (mixed)
+        hipLaunchKernelGGL(k_amr_emf<float>, dim3(g), dim3(256), 0, s, P, e, n, (float *)E, dt_val, dt_ptr);
+    else
+        hipLaunchKernelGGL(k_amr_emf<double>, dim3(g), dim3(256), 0, s, P, e, n, (double *)E, dt_val, dt_ptr);
 }
 
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s)

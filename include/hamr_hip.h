@@ -109,7 +109,7 @@ int hamr_desc_coords(const hamr_desc *desc, double *startx, double *dx);
 int hamr_desc_metric_at(const hamr_desc *desc, const double *X, double *gcov, double *gcon, double *gdet,
                         double *conn, double *r_th);
 /* The multi-level block set of `amr` without a device (as hamr_create_amr
- * builds it): info[15] as hamr_amr_info; rows (15 ints per block, as
+ * builds it): info[16] as hamr_amr_info; rows (15 ints per block, as
  * hamr_block_layout), the copy/restriction entries of stages 1-3 one after
  * another (4 ints: ghost block, ghost cell, source block, source cell |
  * kind << 28 with kind 1 = cell, 2 = mean of the 2x2x2 cells from there) and
@@ -144,10 +144,11 @@ int hamr_create(const hamr_desc *desc, hamr_ctx **out);
  * of whole arrays: the blocks' [component][cell] slabs one after another, each
  * with its own (bs+6)^3 cell count. */
 int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *amr, hamr_ctx **out);
-/* info[15]: blocks, geometry slabs, padded cells, interior zones, exchange
+/* info[16]: blocks, geometry slabs, padded cells, interior zones, exchange
  * entries (copy/restriction) of stages 1-3, prolongation entries of stages
  * 1-3, polar rings, 1 if multi-level, coarse faces at level jumps along x1,
- * x2, x3 (flux correction entries). */
+ * x2, x3 (flux correction entries), coarse edges at level jumps (EMF
+ * correction entries). */
 int hamr_amr_info(const hamr_ctx *ctx, long long *info);
 /* row[15]: level, origin o1 o2 o3 (cells of its level), bs1 bs2 bs3, geometry
  * slab, pole bits, kinds of faces -x1..+x3 (0 copy, 1 outflow, 2 polar mirror,
@@ -257,6 +258,14 @@ int hamr_block_dt(hamr_ctx *ctx, int n, int dir, double *out);
  * f11)) / dt of the four finer faces that tile it.  flag 0 (the half step) and
  * single-level sets: nothing. */
 int hamr_flux_correct(hamr_ctx *ctx, int flag, double dt);
+/* EMF correction at level jumps: E_send1-3, E*_send_corn and E_rec1-3 /
+ * E*_receive_corn with calc_corr = 1 inside GPU_consttransport_bound
+ * (GPU_main.cu:1144-1330; boundcomE.c:5-2926, GPU_program2.cu:1875-2484).
+ * With flag = 1 (the full step) every edge EMF of a block boundary that a
+ * one level finer neighbour tiles takes the mean of the two finer edges; with
+ * flag = 0 nothing happens.  Call after hamr_ct_bound, before hamr_ct_update.
+ * A single-level set has no such edges. */
+int hamr_emf_correct(hamr_ctx *ctx, int flag, double dt);
 /* GPU_consttransport1 + GPU_consttransport2 (GPU_main.cu:1365, :1393). */
 int hamr_ct_emf(hamr_ctx *ctx, int flag, int n);
 /* GPU_consttransport_bound (GPU_main.cu:1144): polar E1 average over each ring

@@ -490,6 +490,58 @@ void oracle_amr_reflux(const oracle_amr *g, double **F1, double **F2, double **F
     }
 }
 
+/* EMF correction at level jumps (E_send1-3 / E_rec1-3 with calc_corr = 1 and
+ * the edge exchanges of boundcomE.c; packsendEaverage1-3 and unpackreceive1-3E,
+ * GPU_program2.cu:1875-2484): every edge EMF on a block's boundary that a leaf
+ * one level finer touches (one of the four cells around the edge, searched in
+ * the fixed order (-,-), (-,+), (+,-), (+,+) of the two directions across it)
+ * takes ((0.5 dt) (e0 + e1)) / dt of that leaf's two edges along it. */
+void oracle_amr_emf_correct(const oracle_amr *g, double **E, double dt)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < g->nblk; b++) {
+        const oracle_amr_block *C = &g->blk[b];
+        const int l = C->level;
+        if (l + 1 >= g->nlev) continue;
+        const long NC = geo_of(C).ncell;
+        for (int t = 0; t < 3; t++) {
+            const int p = (t + 1) % 3, q = (t + 2) % 3;
+            int K[3];
+            for (K[t] = C->o[t]; K[t] < C->o[t] + C->bs[t]; K[t]++)
+                for (K[p] = C->o[p]; K[p] <= C->o[p] + C->bs[p]; K[p]++)
+                    for (K[q] = C->o[q]; K[q] <= C->o[q] + C->bs[q]; K[q]++) {
+                        if (K[p] != C->o[p] && K[p] != C->o[p] + C->bs[p] && K[q] != C->o[q] &&
+                            K[q] != C->o[q] + C->bs[q])
+                            continue; /* not on the block's boundary */
+                        int f = -1, Z[3];
+                        for (int dp = -1; dp <= 0 && f < 0; dp++)
+                            for (int dq = -1; dq <= 0 && f < 0; dq++) {
+                                int Y[3] = {K[0], K[1], K[2]}, ok = 1, lc;
+                                Y[p] += dp;
+                                Y[q] += dq;
+                                for (int x = 0; x < 2; x++)
+                                    if (Y[x] < 0 || Y[x] >= g->N[l][x]) ok = 0; /* beyond the radial edge or the axis */
+                                Y[2] = wrap(Y[2], g->N[l][2]);
+                                if (!ok) continue;
+                                const int c = find_leaf(g, l, Y, &lc);
+                                if (c >= 0 && lc == l + 1) f = c;
+                            }
+                        if (f < 0) continue;
+                        const oracle_amr_block *Fb = &g->blk[f];
+                        for (int x = 0; x < 3; x++) Z[x] = 2 * K[x];
+                        /* the finer leaf across the periodic x3 seam */
+                        if (Z[2] < Fb->o[2]) Z[2] += g->N[l + 1][2];
+                        if (Z[2] > Fb->o[2] + Fb->bs[2]) Z[2] -= g->N[l + 1][2];
+                        int Z1[3] = {Z[0], Z[1], Z[2]};
+                        Z1[t] += 1;
+                        const double *e = E[f] + (long)t * geo_of(Fb).ncell;
+                        const double sum = e[cell_at(Fb, Z)] + e[cell_at(Fb, Z1)];
+                        E[b][(long)t * NC + cell_at(C, K)] = ((0.5 * dt) * sum) / dt;
+                    }
+        }
+    }
+}
+
 void oracle_amr_fluxes(const oracle_amr *g, double **p, double **ps, double dt, double **F1, double **F2, double **F3)
 {
 #pragma omp parallel for schedule(dynamic, 1)
@@ -500,6 +552,45 @@ void oracle_amr_fluxes(const oracle_amr *g, double **p, double **ps, double dt, 
         oracle_flux(&par, 3, p[b], ps[b], &g->m[b], F3[b], NULL);
     }
     if (g->reflux) oracle_amr_reflux(g, F1, F2, F3, dt);
+}
+
+void oracle_amr_ct(const oracle_amr *g, double **p, double **ps, double dt, double **psf)
+{
+    const int nb = g->nblk;
+    double **F1 = calloc(nb, sizeof(double *)), **F2 = calloc(nb, sizeof(double *)), **F3 = calloc(nb, sizeof(double *));
+    double **E = calloc(nb, sizeof(double *)), **Ec = calloc(nb, sizeof(double *));
+    int ok = F1 && F2 && F3 && E && Ec;
+    for (int b = 0; ok && b < nb; b++) {
+        const long N = geo_of(&g->blk[b]).ncell;
+        F1[b] = calloc(N * NPR, sizeof(double));
+        F2[b] = calloc(N * NPR, sizeof(double));
+        F3[b] = calloc(N * NPR, sizeof(double));
+        E[b] = calloc(N * 3, sizeof(double));
+        Ec[b] = calloc(N * 3, sizeof(double));
+        ok = F1[b] && F2[b] && F3[b] && E[b] && Ec[b];
+    }
+    if (ok) {
+        oracle_amr_fluxes(g, p, ps, dt, F1, F2, F3);
+        for (int b = 0; b < nb; b++) {
+            oracle_params par = block_params(g, b);
+            oracle_emf(&par, p[b], F1[b], F2[b], F3[b], &g->m[b], dt, E[b], Ec[b]);
+        }
+        ring_average(g, E, dt);
+        if (g->reflux) oracle_amr_emf_correct(g, E, dt);
+        for (int b = 0; b < nb; b++) {
+            oracle_params par = block_params(g, b);
+            memcpy(psf[b], ps[b], sizeof(double) * 3 * geo_of(&g->blk[b]).ncell);
+            oracle_ct_update(&par, dt, ps[b], psf[b], E[b], &g->m[b]);
+        }
+    }
+    for (int b = 0; b < nb; b++) {
+        if (F1) free(F1[b]);
+        if (F2) free(F2[b]);
+        if (F3) free(F3[b]);
+        if (E) free(E[b]);
+        if (Ec) free(Ec[b]);
+    }
+    free(F1); free(F2); free(F3); free(E); free(Ec);
 }
 
 double oracle_amr_step(const oracle_amr *g, double **p, double **ps, double dt, int **pflag, long long *failcount)
@@ -561,6 +652,7 @@ double oracle_amr_step(const oracle_amr *g, double **p, double **ps, double dt, 
             oracle_emf(&par, ph[b], F1[b], F2[b], F3[b], &g->m[b], dt, E[b], Ec[b]);
         }
         ring_average(g, E, dt);
+        if (g->reflux) oracle_amr_emf_correct(g, E, dt);
         ndt = 1e9;
         for (int b = 0; b < nb; b++) {
             oracle_params par = block_params(g, b);

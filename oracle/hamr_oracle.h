@@ -130,7 +130,7 @@ typedef struct oracle_amr {
     const oracle_amr_block *blk;
     const oracle_metric *m;   /* per block */
     double gam, cour, dx0[3], rho_floor, u_floor;
-    int reflux;               /* 1: flux correction at level jumps in the corrector */
+    int reflux;               /* 1: flux and EMF correction at level jumps in the corrector */
     int riemann;              /* 0: HLL, 1: HLLC */
 } oracle_amr;
 
@@ -142,6 +142,14 @@ void oracle_amr_exchange(const oracle_amr *g, int stage, double **p, double **ps
  * across it is one level finer takes the mean of the four finer faces
  * (F1..F3: per-block flux arrays, dt the step's). */
 void oracle_amr_reflux(const oracle_amr *g, double **F1, double **F2, double **F3, double dt);
+/* EMF correction at level jumps: every edge EMF of a block boundary that a
+ * one level finer leaf touches takes the mean of that leaf's two edges along
+ * it (E: per-block corner EMF arrays, dt the step's). */
+void oracle_amr_emf_correct(const oracle_amr *g, double **E, double dt);
+/* The corrector's face-field update of every block from (p, ps) (ghosts
+ * filled): fluxes and EMFs (corrected at level jumps when g->reflux), the polar
+ * ring average and the CT update with dt from ps into psf (no ghost fill). */
+void oracle_amr_ct(const oracle_amr *g, double **p, double **ps, double dt, double **psf);
 /* Fluxes of every block from (p, ps) (ghosts filled), corrected at level
  * jumps when g->reflux (the corrector's fluxes of oracle_amr_step). */
 void oracle_amr_fluxes(const oracle_amr *g, double **p, double **ps, double dt, double **F1, double **F2, double **F3);

@@ -254,6 +254,7 @@ def _setup_amr(lib):
     lib.oracle_amr_step.restype = C.c_double
     lib.oracle_amr_reflux.argtypes = [G, pp, pp, pp, C.c_double]
     lib.oracle_amr_fluxes.argtypes = [G, pp, pp, C.c_double, pp, pp, pp]
+    lib.oracle_amr_ct.argtypes = [G, pp, pp, C.c_double, pp]
     lib._amr_ready = True
 
 
@@ -313,6 +314,14 @@ class AmrGrid:
         self.o.lib.oracle_amr_fluxes(C.byref(self.g), self._pp(p), self._pp(ps), dt, self._pp(F[0]), self._pp(F[1]),
                                      self._pp(F[2]))
         return F
+
+    def ct(self, p, ps, dt):
+        """The corrector's face-field update (fluxes, EMFs, polar average and
+        the corrections at level jumps when the grid refluxes) from (p, ps):
+        the updated face fields, ghosts not filled."""
+        psf = [np.zeros_like(a) for a in ps]
+        self.o.lib.oracle_amr_ct(C.byref(self.g), self._pp(p), self._pp(ps), dt, self._pp(psf))
+        return psf
 
     def step(self, p, ps, pflag, dt):
         return self.o.lib.oracle_amr_step(C.byref(self.g), self._pp(p), self._pp(ps), dt, self._pp(pflag, _ip),
