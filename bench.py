@@ -295,11 +295,13 @@ def hllc_bench(H, ctx, p0, ps0, nz, args, steps=20):
 
 def rdump_bench(H, ctx, bs, reps=5):
     """Restart records of every block (rdump_block_write / rdump_block_read,
-    restart.c:52, :178) packed from / unpacked into the resident state, device
-    to device in one launch each: ms per call (wall time between device
-    synchronisations, launch included) and the algorithmic rate.  Write: each
-    ghost-inclusive cell reads 12 doubles and writes 13; read: each cell of
-    [NG, NG+BS]^3 reads 12 and writes 12 (the column's gdet values aside)."""
+    restart.c:52, :178) packed from / unpacked into the resident state, and the
+    dump records (dump_block, dump.c:276), device to device in one launch each:
+    ms per call (wall time between device synchronisations, launch included)
+    and the algorithmic rate.  Write: each ghost-inclusive cell reads 12
+    doubles and writes 13; read: each cell of [NG, NG+BS]^3 reads 12 and
+    writes 12; dump: each interior cell reads 9 doubles and writes 9 floats
+    (the column metric aside)."""
     import torch
 
     off = ctx.rdump_offsets()
@@ -309,9 +311,14 @@ def rdump_bench(H, ctx, bs, reps=5):
     D = [b + 6 for b in bs]
     cells_w = nb * D[0] * D[1] * D[2]
     cells_r = nb * (bs[0] + 1) * (bs[1] + 1) * (bs[2] + 1)
-    out = {"record_bytes": int(off[-1]) * 8}
+    doff = ctx.dump_offsets()
+    dbuf = torch.empty(int(doff[-1]), dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+    cells_d = nb * bs[0] * bs[1] * bs[2]
+    out = {"record_bytes": int(off[-1]) * 8, "dump_bytes": int(doff[-1]) * 4}
     for name, fn, nbytes in (("write", lambda: ctx.rdump_write_dev(buf.data_ptr()), cells_w * (12 + 13) * 8),
-                             ("read", lambda: ctx.rdump_read_dev(buf.data_ptr()), cells_r * 24 * 8)):
+                             ("read", lambda: ctx.rdump_read_dev(buf.data_ptr()), cells_r * 24 * 8),
+                             ("dump", lambda: ctx.dump_write_dev(dbuf.data_ptr()), cells_d * (9 * 8 + 9 * 4))):
         fn()
         t0 = time.perf_counter()
         for _ in range(reps):

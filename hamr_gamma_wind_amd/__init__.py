@@ -205,6 +205,10 @@ def load_library(path=None):
         "hamr_rdump_offsets": [ctxp, _llp],
         "hamr_rdump_write_dev": [ctxp, C.c_void_p],
         "hamr_rdump_read_dev": [ctxp, C.c_void_p],
+        "hamr_dump_block_size": [ctxp, C.c_int, _llp],
+        "hamr_dump_block_write": [ctxp, C.c_int, C.POINTER(C.c_float)],
+        "hamr_dump_offsets": [ctxp, _llp],
+        "hamr_dump_write_dev": [ctxp, C.c_void_p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -527,6 +531,30 @@ class Context:
 
     def rdump_read_dev(self, ptr):
         self._chk(self.L.hamr_rdump_read_dev(self.h, C.c_void_p(ptr)), "hamr_rdump_read_dev")
+
+    # dump records: the new_dump files of dump.c (dump_block :276)
+    def dump_block_write(self, n):
+        v = C.c_longlong()
+        self._chk(self.L.hamr_dump_block_size(self.h, n, C.byref(v)), "hamr_dump_block_size")
+        rec = np.zeros(v.value, dtype=np.float32)
+        self._chk(self.L.hamr_dump_block_write(self.h, n, rec.ctypes.data_as(C.POINTER(C.c_float))),
+                  "hamr_dump_block_write")
+        return rec
+
+    def dump_offsets(self):
+        off = np.zeros(self.nblk + 1, dtype=np.int64)
+        self._chk(self.L.hamr_dump_offsets(self.h, _ptr(off, _llp)), "hamr_dump_offsets")
+        return off
+
+    def dump_write_dev(self, ptr):
+        self._chk(self.L.hamr_dump_write_dev(self.h, C.c_void_p(ptr)), "hamr_dump_write_dev")
+
+    def dump_write(self, path, order=None):
+        """dump_new (dump.c:4-66) for one rank holding every block: the records
+        of the blocks in `order` (default: block index order) into one file."""
+        with open(path, "wb") as fh:
+            for n in (range(self.nblk) if order is None else order):
+                self.dump_block_write(int(n)).tofile(fh)
 
     def rdump_write(self, directory):
         """restart_write's block files (restart.c:40-45): directory/rdump<n>

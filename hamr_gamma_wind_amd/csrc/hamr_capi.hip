@@ -92,6 +92,8 @@ struct hamr_ctx {
     std::vector<int64_t> roff;
     int *io_blist = nullptr;       // 0 .. nblk-1
     int64_t *io_roff = nullptr;
+    std::vector<int64_t> doff;     // dump record n starts at doff[n] floats
+    int64_t *io_doff = nullptr;
     double *io = nullptr;          // device record of one block (grown on demand)
     size_t io_bytes = 0;
 };
@@ -260,13 +262,17 @@ static int init_device(hamr_ctx *c)
     if (h2d(c, c->blk, c->blocks.data(), nb * sizeof(BlockDev))) return -1;
     std::vector<int> bl(nb);
     c->roff.assign(nb + 1, 0);
+    c->doff.assign(nb + 1, 0);
     for (size_t n = 0; n < nb; n++) {
+        const BlockDev &B = c->blocks[n];
         bl[n] = (int)n;
-        c->roff[n + 1] = c->roff[n] + RREC * ncell_of(c->blocks[n]);
+        c->roff[n + 1] = c->roff[n] + RREC * ncell_of(B);
+        c->doff[n + 1] = c->doff[n] + DREC * (int64_t)B.bs[0] * B.bs[1] * B.bs[2];
     }
-    if (dalloc(c, &c->io_blist, nb) || dalloc(c, &c->io_roff, nb + 1)) return -1;
+    if (dalloc(c, &c->io_blist, nb) || dalloc(c, &c->io_roff, nb + 1) || dalloc(c, &c->io_doff, nb + 1)) return -1;
     if (h2d(c, c->io_blist, bl.data(), nb * sizeof(int))) return -1;
     if (h2d(c, c->io_roff, c->roff.data(), (nb + 1) * sizeof(int64_t))) return -1;
+    if (h2d(c, c->io_doff, c->doff.data(), (nb + 1) * sizeof(int64_t))) return -1;
     if (build_class_lists(c)) return -1;
     KParams &P = c->P;
     P.gam = c->d.gam;
@@ -1195,6 +1201,44 @@ int hamr_rdump_read_dev(hamr_ctx *c, const double *drec)
     launch_rdump_unpack(c->P, c->io_blist, c->io_roff, c->roff.data(), 0, nblk_of(c),
                         max_over_blocks(c, unpack_cells), rdump_desc(c, red), drec, c->p, c->ps, c->s);
     if (check_launch(c, "hamr_rdump_read_dev")) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+static int64_t interior_cells(const BlockDev &B) { return (int64_t)B.bs[0] * B.bs[1] * B.bs[2]; }
+
+int hamr_dump_block_size(const hamr_ctx *c, int n, long long *count)
+{
+    if (!c || !valid_block(c, n) || !count) return -1;
+    *count = DREC * interior_cells(c->blocks[n]);
+    return 0;
+}
+
+int hamr_dump_offsets(const hamr_ctx *c, long long *off)
+{
+    if (!c || !off) return -1;
+    for (size_t n = 0; n < c->doff.size(); n++) off[n] = c->doff[n];
+    return 0;
+}
+
+int hamr_dump_block_write(hamr_ctx *c, int n, float *rec)
+{
+    if (!c || !valid_block(c, n) || !rec) return fail(c, "hamr_dump_block_write: bad arguments");
+    const BlockDev &B = c->blocks[n];
+    const size_t bytes = (size_t)(DREC * interior_cells(B)) * sizeof(float);
+    if (io_reserve(c, bytes)) return -1;
+    launch_dump_pack(c->P, c->io_blist, c->io_doff, c->doff.data(), n, 1, interior_cells(B), c->p, (float *)c->io,
+                     c->s);
+    if (check_launch(c, "hamr_dump_block_write")) return -1;
+    return d2h(c, rec, c->io, bytes);
+}
+
+int hamr_dump_write_dev(hamr_ctx *c, float *drec)
+{
+    if (!c || !drec) return fail(c, "hamr_dump_write_dev: bad arguments");
+    launch_dump_pack(c->P, c->io_blist, c->io_doff, c->doff.data(), 0, nblk_of(c), max_over_blocks(c, interior_cells),
+                     c->p, drec, c->s);
+    if (check_launch(c, "hamr_dump_write_dev")) return -1;
     HIPCHK(c, hipStreamSynchronize(c->s));
     return 0;
 }

@@ -126,6 +126,12 @@ void launch_rdump_pack(const KParams &P, const int *blist, const int64_t *roff, 
 void launch_rdump_unpack(const KParams &P, const int *blist, const int64_t *roff, const int64_t *roff_host, int first,
                          int count, int64_t max_cells, const RdumpRead &R, const double *rec, double *p, double *ps,
                          hipStream_t s);
+// Dump records (new_dump files): DREC floats per interior cell; item q's
+// record starts at out + doff[q] - doff_host[first]; max_cells bounds the items'
+// interior cell counts.
+constexpr int DREC = 9;
+void launch_dump_pack(const KParams &P, const int *blist, const int64_t *doff, const int64_t *doff_host, int first,
+                      int count, int64_t max_cells, const double *p, float *out, hipStream_t s);
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);
 void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s);
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);

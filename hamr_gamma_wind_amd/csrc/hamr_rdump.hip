@@ -1,5 +1,6 @@
 // hamr_rdump.hip -- a block's restart record (the rdump files of restart.c)
-// packed from and unpacked into the device state.
+// packed from and unpacked into the device state, and its dump record (the
+// new_dump files of dump.c).
 //
 // Record of one block (restart.c:52-67, raw doubles, no header): the block's
 // ghost-inclusive cells in index_3D order (z fastest), RREC = NPR + 4 values per
@@ -109,6 +110,57 @@ __global__ void k_rdump_unpack(const KParams P, const int *blist, const int64_t 
     sc[0] = s1;
     sc[B.ks] = s2;
     sc[2 * (int64_t)B.ks] = s3;
+}
+
+// Dump record of a block (dump_block, dump.c:276-356, ideal gas, no optional
+// physics): the interior cells in index_3D order, DREC = 9 floats each: rho,
+// u, u^0..u^3 (ucon_calc at the cell centre, phys.c:309) and the cell-centred
+// B^1..B^3.  256 cells per workgroup, staged through LDS as the restart record.
+__global__ __launch_bounds__(RPACK_T) void k_dump_pack(const KParams P, const int *blist, const int64_t *doff,
+                                                       int first, int64_t base, const double *p, float *out)
+{
+    __shared__ float s[RPACK_T * DREC];
+    const int item = first + (int)blockIdx.y;
+    const BlockDev &B = P.blk[blist[item]];
+    const int64_t N = (int64_t)B.bs[0] * B.bs[1] * B.bs[2];
+    const int64_t c0 = (int64_t)blockIdx.x * RPACK_T;
+    if (c0 >= N) return;  // the whole workgroup leaves together
+    const int nv = (int)min((int64_t)RPACK_T, N - c0);
+    const int t = threadIdx.x;
+    if (t < nv) {
+        int a, b, c;
+        region_index((int)(c0 + t), B.bs[1], B.bs[2], a, b, c);
+        const int i = a + NG, j = b + NG, z = c + NG;
+        const int64_t cell = ((int64_t)i * B.D[1] + j) * B.D[2] + z;
+        const double *pc = p + NPR * B.off + cell;
+        double pr[NPR], ucon[4];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pr[k] = pc[(int64_t)k * B.ks];
+        const Geom<double> G = load_geom<double>(P, B, CENT, (int64_t)i * B.D[1] + j);
+        ucon_calc(pr, G, ucon);
+        float *r = s + t * DREC;
+        r[0] = (float)pr[RHO];
+        r[1] = (float)pr[UU];
+#pragma unroll
+        for (int k = 0; k < 4; k++) r[2 + k] = (float)ucon[k];
+        r[6] = (float)pr[B1];
+        r[7] = (float)pr[B2];
+        r[8] = (float)pr[B3];
+    }
+    __syncthreads();
+    float *o = out + (doff[item] - base) + c0 * DREC;
+    for (int e = t; e < nv * DREC; e += RPACK_T) o[e] = s[e];
+}
+
+void launch_dump_pack(const KParams &P, const int *blist, const int64_t *doff, const int64_t *doff_host, int first,
+                      int count, int64_t max_cells, const double *p, float *out, hipStream_t s)
+{
+    const unsigned gx = (unsigned)((max_cells + RPACK_T - 1) / RPACK_T);
+    for (int f = first; f < first + count; f += 65535) {
+        const int cnt = min(65535, first + count - f);
+        hipLaunchKernelGGL(k_dump_pack, dim3(gx, (unsigned)cnt), dim3(RPACK_T), 0, s, P, blist, doff, f,
+                           doff_host[first], p, out);
+    }
 }
 
 // Items [first, first + count) of (blist, roff); a grid's y extent is capped

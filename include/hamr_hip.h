@@ -244,6 +244,19 @@ int hamr_rdump_offsets(const hamr_ctx *ctx, long long *off);
 int hamr_rdump_write_dev(hamr_ctx *ctx, double *drec);
 int hamr_rdump_read_dev(hamr_ctx *ctx, const double *drec);
 
+/* ---- dump records (the new_dump files of dump.c) ----
+ * dump_block (dump.c:276-356, ideal gas, no optional physics): a block's
+ * interior cells in index_3D order, NPRDUMP = 9 floats each: rho, u, u^0..u^3
+ * (ucon_calc at the cell centre, phys.c:309) and the cell-centred B^1..B^3.
+ * The file dumps<k>/new_dump<rank> is the records of the rank's blocks in its
+ * block order (dump_new, dump.c:4-66). */
+int hamr_dump_block_size(const hamr_ctx *ctx, int n, long long *count);
+int hamr_dump_block_write(hamr_ctx *ctx, int n, float *rec);
+/* All blocks back to back in block order in device memory: off[n] (nblk+1
+ * entries) is the first float of block n's record, off[nblk] the total. */
+int hamr_dump_offsets(const hamr_ctx *ctx, long long *off);
+int hamr_dump_write_dev(hamr_ctx *ctx, float *drec);
+
 /* ---- per-block stages, the call sequence of advance_GPU (step_ch.c:997-1336) ---- */
 /* GPU_fluxcalc2D (GPU_main.cu:560): PPM + HLL flux through the lower dir-faces
  * of block n; records the block's Courant dt for hamr_block_dt. */

@@ -166,6 +166,9 @@ void oracle_rdump_block_write(const int bs[3], const double *p, const double *ps
  * x2 cells; spherical: the torus grid (B^2 theta-fraction ratio). */
 void oracle_rdump_block_read(const int bs[3], const int red[3], int spherical, int n2, const double *rec,
                              const double *gdet, double *p, double *ps);
+/* dump record of one block: 9 floats per interior cell (rho, u, u^0..u^3,
+ * B^1..B^3); gcov/gcon at the cell centres, packed [col*10 + s], ncol = D1*D2. */
+void oracle_dump_block(const int bs[3], const double *p, const double *gcov_cent, const double *gcon_cent, float *rec);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,7 @@
 /*
  * hamr_rdump_oracle.c -- CPU restatement of the restart record of one block
- * (rdump_block_write / rdump_block_read, restart.c:52-67 and :178-269).
+ * (rdump_block_write / rdump_block_read, restart.c:52-67 and :178-269) and of
+ * its dump record (dump_block, dump.c:276-356).
  *
  * TEST INFRASTRUCTURE ONLY (the checker for the HIP record kernels; see
  * hamr_oracle.h).
@@ -13,6 +14,8 @@
  * multiple of red), and scatters each fine cell into its coarse cell.
  */
 #include "hamr_oracle.h"
+
+#include <math.h>
 
 #define RREC (ORC_NPR + 4)
 
@@ -82,5 +85,40 @@ void oracle_rdump_block_read(const int bs[3], const int red[3], int spherical, i
                 }
                 rf = 1.0 / (double)(red[0] * red[1]);
                 if (r3 % red[2] == 0) ps[2 * nc + cell] += v[RREC - 1] * rf / gdet[4 * ncol + col];
+            }
+}
+
+/* dump_block (dump.c:276-356; ideal gas, CALC_MDOT and the optional physics
+ * off): 9 floats per interior cell in index_3D order: rho, u, ucon[0..3]
+ * (ucon_calc, phys.c:309-331, with gamma_calc :359 and the centre metric of the
+ * cell's column, gcov/gcon packed [col*10 + s]) and the cell-centred B^1..B^3. */
+void oracle_dump_block(const int bs[3], const double *p, const double *gcov_cent, const double *gcon_cent, float *rec)
+{
+    const int D1 = bs[0] + 2 * ORC_NG, D2 = bs[1] + 2 * ORC_NG, D3 = bs[2] + 2 * ORC_NG;
+    const long nc = (long)D1 * D2 * D3;
+    long o = 0;
+    (void)D1;
+    for (int i = ORC_NG; i < ORC_NG + bs[0]; i++)
+        for (int j = ORC_NG; j < ORC_NG + bs[1]; j++)
+            for (int z = ORC_NG; z < ORC_NG + bs[2]; z++, o += 9) {
+                const long cell = ((long)i * D2 + j) * D3 + z, col = (long)i * D2 + j;
+                const double *gc = gcov_cent + col * 10, *gi = gcon_cent + col * 10;
+                double pr[ORC_NPR], alpha, beta[4], qsq, gamma, ucon[4];
+                for (int k = 0; k < ORC_NPR; k++) pr[k] = p[k * nc + cell];
+                alpha = 1. / sqrt(-gi[0]);
+                for (int a = 1; a < 4; a++) beta[a] = gi[a] * alpha * alpha;
+                /* gcov pairs 11 12 13 22 23 33 at 4 5 6 7 8 9 */
+                qsq = gc[4] * pr[2] * pr[2] + gc[7] * pr[3] * pr[3] + gc[9] * pr[4] * pr[4] +
+                      2. * (gc[5] * pr[2] * pr[3] + gc[6] * pr[2] * pr[4] + gc[8] * pr[3] * pr[4]);
+                if (qsq < 0.) qsq = (fabs(qsq) > 1.E-10) ? -1. : 1.E-10; /* gamma = 1 on failure */
+                gamma = (qsq == -1.) ? 1. : sqrt(1. + qsq);
+                ucon[0] = gamma / alpha;
+                for (int a = 1; a < 4; a++) ucon[a] = pr[1 + a] - gamma * beta[a] / alpha;
+                rec[o + 0] = (float)pr[0];
+                rec[o + 1] = (float)pr[1];
+                for (int a = 0; a < 4; a++) rec[o + 2 + a] = (float)ucon[a];
+                rec[o + 6] = (float)pr[5];
+                rec[o + 7] = (float)pr[6];
+                rec[o + 8] = (float)pr[7];
             }
 }

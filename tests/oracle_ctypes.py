@@ -70,6 +70,7 @@ class Oracle:
         L.oracle_hllc_face.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp]
         L.oracle_hllc_face.restype = C.c_double
         L.oracle_rdump_block_write.argtypes = [_ip, _dp, _dp, _dp, _dp]
+        L.oracle_dump_block.argtypes = [_ip, _dp, _dp, _dp, C.POINTER(C.c_float)]
         L.oracle_rdump_block_read.argtypes = [_ip, _ip, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
 
     # ---- pointwise ----
@@ -132,6 +133,16 @@ class Oracle:
         self.lib.oracle_rdump_block_write(_ptr(b, _ip), _ptr(np.ascontiguousarray(p, float)),
                                           _ptr(np.ascontiguousarray(ps, float)),
                                           _ptr(np.ascontiguousarray(gdet, float)), _ptr(rec))
+        return rec
+
+    def dump_block(self, bs, p, gcov_cent, gcon_cent):
+        """Dump record (9 floats per interior cell); gcov/gcon_cent: (ncol, 10)."""
+        rec = np.zeros(int(np.prod(bs)) * 9, dtype=np.float32)
+        b = np.array(bs, dtype=np.int32)
+        self.lib.oracle_dump_block(_ptr(b, _ip), _ptr(np.ascontiguousarray(p, float)),
+                                   _ptr(np.ascontiguousarray(gcov_cent, float)),
+                                   _ptr(np.ascontiguousarray(gcon_cent, float)),
+                                   rec.ctypes.data_as(C.POINTER(C.c_float)))
         return rec
 
     def rdump_read(self, bs, red, spherical, n2, rec, gdet, p, ps):
