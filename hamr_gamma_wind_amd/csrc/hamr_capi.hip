@@ -1134,6 +1134,8 @@ static RdumpRead rdump_desc(const hamr_ctx *c, const int *red)
 }
 
 static int64_t unpack_cells(const BlockDev &B) { return (int64_t)(B.bs[0] + 1) * (B.bs[1] + 1) * (B.bs[2] + 1); }
+static int64_t unpack_rows(const BlockDev &B) { return (int64_t)(B.bs[0] + 1) * (B.bs[1] + 1); }
+static int64_t unpack_e3(const BlockDev &B) { return B.bs[2] + 1; }
 
 int hamr_rdump_block_size(const hamr_ctx *c, int n, int red1, int red2, int red3, long long *count)
 {
@@ -1171,7 +1173,7 @@ int hamr_rdump_block_read(hamr_ctx *c, int n, const double *rec, int red1, int r
     if (io_reserve(c, bytes)) return -1;
     if (h2d(c, c->io, rec, bytes)) return -1;
     launch_rdump_unpack(c->P, c->io_blist, c->io_roff, c->roff.data(), n, 1, unpack_cells(B), rdump_desc(c, red),
-                        c->io, c->p, c->ps, c->s);
+                        c->io, c->p, c->ps, B.bs[2] + 1, unpack_rows(B), c->s);
     if (check_launch(c, "hamr_rdump_block_read")) return -1;
     HIPCHK(c, hipStreamSynchronize(c->s));
     return 0;
@@ -1199,7 +1201,8 @@ int hamr_rdump_read_dev(hamr_ctx *c, const double *drec)
     if (!c || !drec) return fail(c, "hamr_rdump_read_dev: bad arguments");
     const int red[3] = {1, 1, 1};
     launch_rdump_unpack(c->P, c->io_blist, c->io_roff, c->roff.data(), 0, nblk_of(c),
-                        max_over_blocks(c, unpack_cells), rdump_desc(c, red), drec, c->p, c->ps, c->s);
+                        max_over_blocks(c, unpack_cells), rdump_desc(c, red), drec, c->p, c->ps,
+                        (int)max_over_blocks(c, unpack_e3), max_over_blocks(c, unpack_rows), c->s);
     if (check_launch(c, "hamr_rdump_read_dev")) return -1;
     HIPCHK(c, hipStreamSynchronize(c->s));
     return 0;

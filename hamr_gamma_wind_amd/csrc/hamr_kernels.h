@@ -123,9 +123,11 @@ struct RdumpRead {
 };
 void launch_rdump_pack(const KParams &P, const int *blist, const int64_t *roff, const int64_t *roff_host, int first,
                        int count, int64_t max_cells, const double *p, const double *ps, double *out, hipStream_t s);
+// max_e3 / max_rows: the largest bs3 + 1 and (bs1 + 1)(bs2 + 1) of the items
+// (records of the blocks' own size take a staged kernel when max_e3 <= 256).
 void launch_rdump_unpack(const KParams &P, const int *blist, const int64_t *roff, const int64_t *roff_host, int first,
                          int count, int64_t max_cells, const RdumpRead &R, const double *rec, double *p, double *ps,
-                         hipStream_t s);
+                         int max_e3, int64_t max_rows, hipStream_t s);
 // Dump records (new_dump files): DREC floats per interior cell; item q's
 // record starts at out + doff[q] - doff_host[first]; max_cells bounds the items'
 // interior cell counts.

@@ -112,6 +112,60 @@ __global__ void k_rdump_unpack(const KParams P, const int *blist, const int64_t 
     sc[2 * (int64_t)B.ks] = s3;
 }
 
+// The unpack of a record of the block's own size (red = 1), staged: a
+// workgroup takes whole z-rows (i, j) of the destination range, copies their
+// records (contiguous runs of (BS3+1) RREC doubles) into LDS with coalesced
+// loads, then each thread forms its cell as k_rdump_unpack does (the same
+// operations: sums of one term from 0, weights 1).  Needs BS3 + 1 <= RPACK_T.
+__global__ __launch_bounds__(RPACK_T) void k_rdump_unpack1(const KParams P, const int *blist, const int64_t *roff,
+                                                           int first, int64_t base, RdumpRead R, const double *rec,
+                                                           double *p, double *ps)
+{
+    __shared__ double s[RPACK_T * RREC];
+    const int item = first + (int)blockIdx.y;
+    const BlockDev &B = P.blk[blist[item]];
+    const int e2 = B.bs[1] + 1, e3 = B.bs[2] + 1;
+    const int rows = RPACK_T / e3, nrows = (B.bs[0] + 1) * e2;
+    const int row0 = (int)blockIdx.x * rows;
+    if (row0 >= nrows) return;  // the whole workgroup leaves together
+    const int nr = min(rows, nrows - row0);
+    const int run = e3 * RREC;
+    const double *r0 = rec + (roff[item] - base);
+    for (int e = threadIdx.x; e < nr * run; e += RPACK_T) {
+        const int rr = e / run, off = e - rr * run, row = row0 + rr;
+        const int a = row / e2, b = row - a * e2;
+        s[e] = r0[((((int64_t)(NG + a) * B.D[1] + NG + b) * B.D[2]) + NG) * RREC + off];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t >= nr * e3) return;
+    const int rr = t / e3, c = t - rr * e3, row = row0 + rr;
+    const int a = row / e2, b = row - a * e2;
+    const double *v = s + t * RREC;
+    const double rf = 1.0 / (double)(1 * 1 * 1);
+    const int i = NG + a, j = NG + b, z = NG + c;
+    const int64_t col = (int64_t)i * B.D[1] + j;
+    const double g1 = P.gdet[mg1(B, FACE1) + col], g2 = P.gdet[mg1(B, FACE2) + col];
+    const double g3 = P.gdet[mg1(B, FACE3) + col];
+    const int64_t cell = ((int64_t)i * B.D[1] + j) * B.D[2] + z;
+    double *pc = p + NPR * B.off + cell;
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        double acc = 0.;
+        acc += v[k] * rf;
+        pc[(int64_t)k * B.ks] = acc;
+    }
+    double s1 = 0., s2 = 0., s3 = 0.;
+    s1 += v[NPR + 1] * rf / g1;
+    const double v2 = v[NPR + 2] * rf / g2;
+    s2 += R.spherical ? v2 * R.fractheta / R.fractheta_old : v2;
+    s3 += v[NPR + 3] * rf / g3;
+    double *sc = ps + 3 * B.off + cell;
+    sc[0] = s1;
+    sc[B.ks] = s2;
+    sc[2 * (int64_t)B.ks] = s3;
+}
+
 // Dump record of a block (dump_block, dump.c:276-356, ideal gas, no optional
 // physics): the interior cells in index_3D order, DREC = 9 floats each: rho,
 // u, u^0..u^3 (ucon_calc at the cell centre, phys.c:309) and the cell-centred
@@ -178,8 +232,18 @@ void launch_rdump_pack(const KParams &P, const int *blist, const int64_t *roff, 
 
 void launch_rdump_unpack(const KParams &P, const int *blist, const int64_t *roff, const int64_t *roff_host, int first,
                          int count, int64_t max_cells, const RdumpRead &R, const double *rec, double *p, double *ps,
-                         hipStream_t s)
+                         int max_e3, int64_t max_rows, hipStream_t s)
 {
+    if (R.red[0] == 1 && R.red[1] == 1 && R.red[2] == 1 && max_e3 <= RPACK_T) {
+        // the staged form: max_rows z-rows per block at most, RPACK_T / e3 rows per workgroup
+        const unsigned gx = (unsigned)((max_rows + (RPACK_T / max_e3) - 1) / (RPACK_T / max_e3));
+        for (int f = first; f < first + count; f += 65535) {
+            const int cnt = min(65535, first + count - f);
+            hipLaunchKernelGGL(k_rdump_unpack1, dim3(gx, (unsigned)cnt), dim3(RPACK_T), 0, s, P, blist, roff, f,
+                               roff_host[first], R, rec, p, ps);
+        }
+        return;
+    }
     const int T = 256;
     const unsigned gx = (unsigned)((max_cells + T - 1) / T);
     for (int f = first; f < first + count; f += 65535) {

@@ -194,6 +194,13 @@ def test_graph_tuned_and_per_block_calls_equal_batched_step():
         H.advance_step(ctx, dt)
         got = _state(ctx)
         assert _same(got[0], one[0]) and _same(got[1], one[1])
+        # the same sequence driven by the C host (libhamr_host.so), flux and EMF
+        # corrections at the level jumps included
+        ctx.set(H.HAMR_P, start[0])
+        ctx.set(H.HAMR_PS, start[1])
+        H.host_advance(ctx, dt)
+        got = _state(ctx)
+        assert _same(got[0], one[0]) and _same(got[1], one[1])
     finally:
         ctx.close()
 

@@ -184,6 +184,12 @@ def test_per_block_sequence_equals_batched_step(setup):
     pb, psb = ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS)
     assert ndt_a == ndt_b
     assert np.array_equal(pa, pb) and np.array_equal(psa, psb)
+    # the same sequence driven by the C host (libhamr_host.so)
+    ctx.set(H.HAMR_P, p0)
+    ctx.set(H.HAMR_PS, ps0)
+    ndt_c = H.host_advance(ctx, dt)
+    assert ndt_c == ndt_b
+    assert np.array_equal(ctx.get(H.HAMR_P), pb) and np.array_equal(ctx.get(H.HAMR_PS), psb)
 
 
 def test_graph_replay_equals_stream(setup):
