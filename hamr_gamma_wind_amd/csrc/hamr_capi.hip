@@ -33,6 +33,7 @@ struct TuneCfg {
     int tile = kDefaultTile;  // index into kTiles
     int grid = 0;             // workgroups; 0: one per item
     int swz = 0;              // XCD-contiguous item order
+    int march = 0;            // TS_CONS only: the x1-marching form (tile, grid, swz unused)
     float ms = -1.f;          // measured time of the chosen configuration
 };
 
@@ -354,13 +355,40 @@ static void emf_all(hamr_ctx *c, const double *p)
                c->prec, c->s);
 }
 
+static int max_plane(const hamr_ctx *c)
+{
+    int m = 0;
+    for (const BlockDev &B : c->blocks) m = std::max(m, B.bs[1] * B.bs[2]);
+    return m;
+}
+
+static int min_bs3(const hamr_ctx *c)
+{
+    int m = 1 << 30;
+    for (const BlockDev &B : c->blocks) m = std::min(m, B.bs[2]);
+    return m;
+}
+
+// The conserved update of every block in the form the TS_CONS tuning chose.
+static void cons_all(hamr_ctx *c, const Launch &L, int full, double Dt_val, const double *dt_ptr, double dt_mult,
+                     const double *pi, const double *pb, const double *psf)
+{
+    if (c->tune[TS_CONS].march)
+        launch_cons_march(c->P, nblk_of(c), max_plane(c), min_bs3(c), full, Dt_val, dt_ptr, dt_mult, pi, pb, c->U0,
+                          c->Un, psf, c->F[0], c->F[1], c->F[2], c->prec, c->s);
+    else
+        launch_update_part(c->P, 0, L, full, Dt_val, dt_ptr, dt_mult, pi, pb, nullptr, c->U0, c->Un, psf, c->F[0],
+                           c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, c->s);
+}
+
 static void update_all(hamr_ctx *c, int full, double Dt_val, const double *dt_ptr, double dt_mult, const double *pi,
                        const double *pb, double *pf, const double *psf)
 {
-    const int slots[3] = {TS_CONS, TS_INVERT, TS_FALLBACK};
-    for (int part = 0; part < 3; part++)
-        launch_update_part(c->P, part, all_launch(c, slots[part]), full, Dt_val, dt_ptr, dt_mult, pi, pb, pf, c->U0,
-                           c->Un, psf, c->F[0], c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, c->s);
+    cons_all(c, all_launch(c, TS_CONS), full, Dt_val, dt_ptr, dt_mult, pi, pb, psf);
+    const int slots[2] = {TS_INVERT, TS_FALLBACK};
+    for (int part = 1; part < 3; part++)
+        launch_update_part(c->P, part, all_launch(c, slots[part - 1]), full, Dt_val, dt_ptr, dt_mult, pi, pb, pf,
+                           c->U0, c->Un, psf, c->F[0], c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, c->s);
 }
 
 extern "C" {
@@ -1448,7 +1476,7 @@ static void run_slot(hamr_ctx *c, int slot, const Launch &L, int T_ex, int G_ex)
     case TS_ECENT: launch_emf_centre(P, L, c->p, c->Ec, c->prec, s); break;
     case TS_ECORNER: launch_emf_corner(P, L, c->Ec, c->F[0], c->F[1], c->F[2], c->E, c->prec, s); break;
     case TS_CT: launch_ct_update(P, L, 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, s); break;
-    case TS_CONS:
+    case TS_CONS: cons_all(c, L, 0, 0., c->dt, 0.5, c->p, c->p, c->psh); break;
     case TS_INVERT:
     case TS_FALLBACK:
         launch_update_part(P, slot - TS_CONS, L, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0],
@@ -1517,6 +1545,18 @@ int hamr_autotune(hamr_ctx *c, int nrep)
                     cand.ms = ms;
                     if (best.ms < 0.f || ms < best.ms) best = cand;
                 }
+        if (slot == TS_CONS) {  // the x1-marching form of the update
+            TuneCfg cand;
+            cand.march = 1;
+            const TuneCfg keep = c->tune[slot];
+            c->tune[slot] = cand;
+            float ms;
+            const int r = time_launches(c, slot, all_launch(c, slot), 0, 0, nrep, &ms);
+            c->tune[slot] = keep;
+            if (r) return -1;
+            cand.ms = ms;
+            if (ms < best.ms) best = cand;
+        }
         c->tune[slot] = best;
     }
     if (h2d(c, c->failcnt, fc.data(), fc.size() * sizeof(int))) return -1;
@@ -1553,9 +1593,9 @@ int hamr_tune_get(const hamr_ctx *c, int slot, int *cfg, double *ms)
 {
     if (!c || slot < 0 || slot >= TS_N || !cfg) return -1;
     const TuneCfg &t = c->tune[slot];
-    cfg[0] = kTiles[t.tile];
-    cfg[1] = t.grid;
-    cfg[2] = t.swz;
+    cfg[0] = t.march ? 128 : kTiles[t.tile];
+    cfg[1] = t.march ? -1 : t.grid;
+    cfg[2] = t.march ? 0 : t.swz;
     if (ms) *ms = t.ms;
     return 0;
 }
@@ -1566,10 +1606,13 @@ int hamr_tune_set(hamr_ctx *c, int slot, const int *cfg)
     int ti = -1;
     for (int k = 0; k < 3; k++)
         if (kTiles[k] == cfg[0]) ti = k;
-    if (ti < 0 || cfg[1] < 0 || (cfg[2] != 0 && cfg[2] != 1)) return fail(c, "hamr_tune_set: tile 64/128/256, grid >= 0, swz 0/1");
-    c->tune[slot].tile = ti;
-    c->tune[slot].grid = cfg[1];
-    c->tune[slot].swz = cfg[2];
+    const bool march = slot == TS_CONS && cfg[1] == -1;
+    if (!march && (ti < 0 || cfg[1] < 0 || (cfg[2] != 0 && cfg[2] != 1)))
+        return fail(c, "hamr_tune_set: tile 64/128/256, grid >= 0 (-1 for the update's x1-marching form), swz 0/1");
+    c->tune[slot].tile = march ? kDefaultTile : ti;
+    c->tune[slot].grid = march ? 0 : cfg[1];
+    c->tune[slot].swz = march ? 0 : cfg[2];
+    c->tune[slot].march = march ? 1 : 0;
     c->tune[slot].ms = -1.f;
     if (c->graph) {
         HIPCHK(c, hipStreamSynchronize(c->s));

@@ -81,6 +81,13 @@ void launch_update(const KParams &P, const Launch &L, int full, double Dt_val, c
 // U(p) - Dt div F + Dt S(p) with the face average of psf -> Unew.
 void launch_cons(const KParams &P, const Launch &L, double Dt, const double *p, double *U0, double *Unew,
                  const double *psf, const void *F1, const void *F2, const void *F3, int mixed, hipStream_t s);
+// k_cons marching along x1 (one thread per (j, z) of each block's plane; grid
+// y = blocks, at most 65535): the same cells and arithmetic as part 0 below.
+// max_plane / min_bs3: the largest bs2 bs3 and the smallest bs3 of the blocks.
+void launch_cons_march(const KParams &P, int nblk, int max_plane, int min_bs3, int full, double Dt_val,
+                       const double *dt_ptr, double dt_mult, const double *pi, const double *pb, double *U0,
+                       double *Unew, const double *psf, const void *F1, const void *F2, const void *F3, int mixed,
+                       hipStream_t s);
 // The kernels of the update one at a time (k_cons, k_invert, k_fallback), for timing and tuning.
 void launch_update_part(const KParams &P, int part, const Launch &L, int full, double Dt_val, const double *dt_ptr,
                         double dt_mult, const double *pi, const double *pb, double *pf, double *U0, double *Unew,

@@ -184,8 +184,96 @@ __device__ __forceinline__ void comp_add(double &s, double &c, double t)
     s = u;
 }
 
-// R is the storage type of the fluxes: double (FP64 step, the reference's
-// arithmetic) or float (mixed step, compensated FP64 accumulation).
+// One cell of the conserved update (cell g of block B, metric column values
+// through mv(v), v one of MV_*): U^n (predictor: from pi, stored in U0;
+// corrector: read from U0), minus Dt times the flux divergence, plus Dt times
+// the source from the state (pi or pb), the face-field average into the B
+// slots; written to Unew.  f1l / f1h are the x1 fluxes at the cell's lower and
+// upper faces (the caller loads or carries them).  R is the storage type of
+// the fluxes: double (FP64 step, the reference's arithmetic) or float (mixed
+// step, compensated FP64 accumulation).
+template <class R, class MV>
+__device__ __forceinline__ void cons_cell(const KParams &P, const BlockDev &B, int full, double Dt, int64_t g,
+                                          const double *__restrict__ pi, const double *__restrict__ pb,
+                                          double *__restrict__ U0, double *__restrict__ Unew,
+                                          const double *__restrict__ psf, const R *f1l, const R *f1h,
+                                          const R *__restrict__ F2, const R *__restrict__ F3, MV mv)
+{
+    const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2], sj = B.D[2];
+    const int64_t base = NPR * B.off + g;
+    Geom<double> G;
+#pragma unroll
+    for (int s = 0; s < 10; s++) {
+        G.gcov[s] = mv(MV_GCOV + s);
+        G.gcon[s] = mv(MV_GCON + s);
+    }
+    G.g = mv(MV_G);
+    double pf[NPR], U[NPR], dU[NPR];
+    State<double> q;
+    if (!full) {
+        double d1, d2;
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pf[k] = pi[base + k * K];
+        get_state(pf, G, q);
+        primtoflux(pf, q, 0, G, P.gam, U, &d1, &d2);
+#pragma unroll
+        for (int k = 0; k < NPR; k++) U0[base + k * K] = U[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) U[k] = U0[base + k * K];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pf[k] = pb[base + k * K];
+        get_state(pf, G, q);
+    }
+    // fluxes kept in their storage type until used (half the registers in the mixed step)
+    R f2l[NPR], f2h[NPR], f3l[NPR], f3h[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        const R *f2 = F2 + base + k * K, *f3 = F3 + base + k * K;
+        f2l[k] = f2[0];
+        f2h[k] = f2[sj];
+        f3l[k] = f3[0];
+        f3h[k] = f3[1];
+    }
+    source(pf, q, [&](int lam, int s) { return mv(MV_CONN + lam * 10 + s); }, G.g, P.gam, dU);
+    if (sizeof(R) == sizeof(double)) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            U[k] -= Dt * (f1h[k] - f1l[k]) / (B.dx[0] * 1.0);
+            U[k] -= Dt * (f2h[k] - f2l[k]) / (B.dx[1] * 1.0);
+            U[k] -= Dt * (f3h[k] - f3l[k]) / (B.dx[2] * 1.0);
+        }
+#pragma unroll
+        for (int k = 0; k < NPR; k++) U[k] += Dt * (dU[k]);
+    } else {
+        // Dt/dx once per direction (27 FP64 divisions per cell fewer):
+        // the FP32 flux differences are exact in FP64, so each term
+        // differs from the reference's Dt*dF/dx by FP64 roundings only
+        const double c1 = Dt / B.dx[0], c2 = Dt / B.dx[1], c3 = Dt / B.dx[2];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) {
+            double sum = U[k], cmp = 0.;
+            comp_add(sum, cmp, -(c1 * ((double)f1h[k] - (double)f1l[k])));
+            comp_add(sum, cmp, -(c2 * ((double)f2h[k] - (double)f2l[k])));
+            comp_add(sum, cmp, -(c3 * ((double)f3h[k] - (double)f3l[k])));
+            comp_add(sum, cmp, Dt * dU[k]);
+            U[k] = sum + cmp;
+        }
+    }
+    {
+        const double *sp = psf + 3 * B.off + g;
+        U[B1] = 0.0;
+        U[B2] = 0.0;
+        U[B1] += (sp[0] * mv(MV_GF1) + sp[si] * mv(MV_GF1P)) / (2.0 * 1.0);
+        U[B2] += (sp[K] * mv(MV_GF2) + sp[K + sj] * mv(MV_GF2P)) / (2.0 * 1.0);
+        U[B3] = (sp[2 * K] * mv(MV_GF3) + sp[2 * K + 1] * mv(MV_GF3)) / 2.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) Unew[base + k * K] = U[k];
+}
+
+// One thread per interior cell, tiles of a block's cells z fastest; the
+// metric columns a tile touches are staged in LDS.
 template <class R>
 __global__ void __launch_bounds__(256) k_cons(KParams P, WorkList wl, int full, double Dt_val,
                                               const double *__restrict__ dt_ptr, double dt_mult,
@@ -216,86 +304,92 @@ __global__ void __launch_bounds__(256) k_cons(KParams P, WorkList wl, int full, 
             int a, b, c;
             region_index(t, bs2, bs3, a, b, c);
             const int i = a + NG, j = b + NG, z = c + NG;
-            const int qq = t / bs3 - q0;
-            const double *mv = smem + qq;
+            const double *mvp = smem + (t / bs3 - q0);
             const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
-            const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2], sj = B.D[2];
+            const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2];
             const int64_t g = ((int64_t)i * B.D[1] + j) * B.D[2] + z;
             const int64_t base = NPR * B.off + g;
-            Geom<double> G;
-#pragma unroll
-            for (int s = 0; s < 10; s++) {
-                G.gcov[s] = mv[(MV_GCOV + s) * ncw];
-                G.gcon[s] = mv[(MV_GCON + s) * ncw];
-            }
-            G.g = mv[MV_G * ncw];
-            double pf[NPR], U[NPR], dU[NPR];
-            State<double> q;
-            if (!full) {
-                double d1, d2;
-#pragma unroll
-                for (int k = 0; k < NPR; k++) pf[k] = pi[base + k * K];
-                get_state(pf, G, q);
-                primtoflux(pf, q, 0, G, P.gam, U, &d1, &d2);
-#pragma unroll
-                for (int k = 0; k < NPR; k++) U0[base + k * K] = U[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < NPR; k++) U[k] = U0[base + k * K];
-#pragma unroll
-                for (int k = 0; k < NPR; k++) pf[k] = pb[base + k * K];
-                get_state(pf, G, q);
-            }
-            // fluxes kept in their storage type until used (half the registers in the mixed step)
-            R f1l[NPR], f1h[NPR], f2l[NPR], f2h[NPR], f3l[NPR], f3h[NPR];
+            R f1l[NPR], f1h[NPR];
 #pragma unroll
             for (int k = 0; k < NPR; k++) {
-                const R *f1 = F1 + base + k * K, *f2 = F2 + base + k * K, *f3 = F3 + base + k * K;
+                const R *f1 = F1 + base + k * K;
                 f1l[k] = f1[0];
                 f1h[k] = f1[si];
-                f2l[k] = f2[0];
-                f2h[k] = f2[sj];
-                f3l[k] = f3[0];
-                f3h[k] = f3[1];
             }
-            source(pf, q, [&](int lam, int s) { return mv[(MV_CONN + lam * 10 + s) * ncw]; }, G.g, P.gam, dU);
-            if (sizeof(R) == sizeof(double)) {
-#pragma unroll
-                for (int k = 0; k < NPR; k++) {
-                    U[k] -= Dt * (f1h[k] - f1l[k]) / (B.dx[0] * 1.0);
-                    U[k] -= Dt * (f2h[k] - f2l[k]) / (B.dx[1] * 1.0);
-                    U[k] -= Dt * (f3h[k] - f3l[k]) / (B.dx[2] * 1.0);
-                }
-#pragma unroll
-                for (int k = 0; k < NPR; k++) U[k] += Dt * (dU[k]);
-            } else {
-                // Dt/dx once per direction (27 FP64 divisions per cell fewer):
-                // the FP32 flux differences are exact in FP64, so each term
-                // differs from the reference's Dt*dF/dx by FP64 roundings only
-                const double c1 = Dt / B.dx[0], c2 = Dt / B.dx[1], c3 = Dt / B.dx[2];
-#pragma unroll
-                for (int k = 0; k < NPR; k++) {
-                    double sum = U[k], cmp = 0.;
-                    comp_add(sum, cmp, -(c1 * ((double)f1h[k] - (double)f1l[k])));
-                    comp_add(sum, cmp, -(c2 * ((double)f2h[k] - (double)f2l[k])));
-                    comp_add(sum, cmp, -(c3 * ((double)f3h[k] - (double)f3l[k])));
-                    comp_add(sum, cmp, Dt * dU[k]);
-                    U[k] = sum + cmp;
-                }
-            }
-            {
-                const double *sp = psf + 3 * B.off + g;
-                U[B1] = 0.0;
-                U[B2] = 0.0;
-                U[B1] += (sp[0] * mv[MV_GF1 * ncw] + sp[si] * mv[MV_GF1P * ncw]) / (2.0 * 1.0);
-                U[B2] += (sp[K] * mv[MV_GF2 * ncw] + sp[K + sj] * mv[MV_GF2P * ncw]) / (2.0 * 1.0);
-                U[B3] = (sp[2 * K] * mv[MV_GF3 * ncw] + sp[2 * K + 1] * mv[MV_GF3 * ncw]) / 2.0;
-            }
-#pragma unroll
-            for (int k = 0; k < NPR; k++) Unew[base + k * K] = U[k];
+            cons_cell<R>(P, B, full, Dt, g, pi, pb, U0, Unew, psf, f1l, f1h, F2, F3,
+                         [&](int v) { return mvp[v * ncw]; });
         }
         __syncthreads();
     }
+}
+
+// The same update marching along x1: one thread per (j, z) of a block's
+// interior x2-x3 plane walks i upward, so each x1 face flux is read once and
+// carried from one cell to the next in registers (the per-cell form reads
+// every interior x1 face twice, the second time from HBM unless the tile of
+// the next x1 plane ran recently on the same XCD).  At each i the workgroup
+// stages the metric of the columns its threads touch in LDS, as k_cons does
+// per tile.  Grid: x over the plane in RPLANE tiles, y over the blocks.
+constexpr int RPLANE = 128;
+template <class R>
+__global__ void __launch_bounds__(RPLANE) __attribute__((amdgpu_waves_per_eu(3)))
+k_cons_march(KParams P, int full, double Dt_val, const double *__restrict__ dt_ptr, double dt_mult,
+             const double *__restrict__ pi, const double *__restrict__ pb, double *__restrict__ U0,
+             double *__restrict__ Unew, const double *__restrict__ psf, const R *__restrict__ F1,
+             const R *__restrict__ F2, const R *__restrict__ F3)
+{
+    extern __shared__ double smem[];
+    const BlockDev &B = P.blk[blockIdx.y];
+    const int bs2 = B.bs[1], bs3 = B.bs[2], nplane = bs2 * bs3;
+    const int t0 = blockIdx.x * RPLANE, t = t0 + threadIdx.x;
+    if (t0 >= nplane) return;  // the whole workgroup leaves together
+    const bool act = t < nplane;
+    const int q0 = t0 / bs3, q1 = mymin(t0 + RPLANE - 1, nplane - 1) / bs3, ncw = q1 - q0 + 1;
+    const int j = (act ? t : t0) / bs3 + NG, z = (act ? t : t0) % bs3 + NG;
+    const double *mvp = smem + (j - NG - q0);
+    const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
+    const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2];
+    R fl[NPR], fh[NPR];
+    if (act) {
+        const int64_t base = NPR * B.off + ((int64_t)NG * B.D[1] + j) * B.D[2] + z;
+#pragma unroll
+        for (int k = 0; k < NPR; k++) fl[k] = F1[base + k * K];
+    }
+#pragma unroll 1
+    for (int i = NG; i < NG + B.bs[0]; i++) {
+        __syncthreads();  // the previous plane's metric is no longer read
+        for (int e = threadIdx.x; e < MV_N * ncw; e += RPLANE) {
+            const int v = e / ncw, qq = e - v * ncw;
+            smem[e] = metric_value(P, B, v, (int64_t)i * B.D[1] + (q0 + qq + NG));
+        }
+        __syncthreads();
+        if (act) {
+            const int64_t g = ((int64_t)i * B.D[1] + j) * B.D[2] + z;
+            const int64_t base = NPR * B.off + g;
+#pragma unroll
+            for (int k = 0; k < NPR; k++) fh[k] = F1[base + k * K + si];
+            cons_cell<R>(P, B, full, Dt, g, pi, pb, U0, Unew, psf, fl, fh, F2, F3,
+                         [&](int v) { return mvp[v * ncw]; });
+#pragma unroll
+            for (int k = 0; k < NPR; k++) fl[k] = fh[k];
+        }
+    }
+}
+
+void launch_cons_march(const KParams &P, int nblk, int max_plane, int min_bs3, int full, double Dt_val,
+                       const double *dt_ptr, double dt_mult, const double *pi, const double *pb, double *U0,
+                       double *Unew, const double *psf, const void *F1, const void *F2, const void *F3, int mixed,
+                       hipStream_t s)
+{
+    if (nblk <= 0 || min_bs3 <= 0) return;
+    const dim3 grid((unsigned)((max_plane + RPLANE - 1) / RPLANE), (unsigned)nblk);
+    const size_t lds = sizeof(double) * MV_N * (size_t)((RPLANE - 1) / min_bs3 + 2);
+    if (mixed)
+        hipLaunchKernelGGL((k_cons_march<float>), grid, dim3(RPLANE), lds, s, P, full, Dt_val, dt_ptr, dt_mult, pi,
+                           pb, U0, Unew, psf, (const float *)F1, (const float *)F2, (const float *)F3);
+    else
+        hipLaunchKernelGGL((k_cons_march<double>), grid, dim3(RPLANE), lds, s, P, full, Dt_val, dt_ptr, dt_mult, pi,
+                           pb, U0, Unew, psf, (const double *)F1, (const double *)F2, (const double *)F3);
 }
 
 // Adds the number of active lanes with `pred` set to *cnt with one atomic per

@@ -326,7 +326,10 @@ enum {
     HAMR_TUNE_EXCHANGE, HAMR_TUNE_N
 };
 int hamr_autotune(hamr_ctx *ctx, int nrep);
-/* cfg[3]: threads per workgroup, workgroups (0: one per item), XCD order; ms: tuned time (-1: not tuned). */
+/* cfg[3]: threads per workgroup, workgroups (0: one per item), XCD order; ms: tuned time (-1: not tuned).
+ * For HAMR_TUNE_CONS, workgroups = -1 selects the update's x1-marching form
+ * (one thread per (j, z) walking i; the same arithmetic), which the tuner
+ * also tries. */
 int hamr_tune_get(const hamr_ctx *ctx, int slot, int *cfg, double *ms);
 int hamr_tune_set(hamr_ctx *ctx, int slot, const int *cfg);
 /* Average device time (ms, HIP events on the context stream) of one launch of

@@ -184,6 +184,13 @@ def test_graph_tuned_and_per_block_calls_equal_batched_step():
         ctx.step(2, dt=dt)
         got = _state(ctx)
         assert _same(got[0], ref[0]) and _same(got[1], ref[1])
+        # the update's x1-marching form (blocks of three sizes)
+        ctx.tune_set(H.HAMR_TUNE_NAMES.index("cons"), 128, -1, 0)
+        ctx.set(H.HAMR_P, start[0])
+        ctx.set(H.HAMR_PS, start[1])
+        ctx.step(2, dt=dt)
+        got = _state(ctx)
+        assert _same(got[0], ref[0]) and _same(got[1], ref[1])
         # the reference-shaped per-block sequence (one step)
         ctx.set(H.HAMR_P, start[0])
         ctx.set(H.HAMR_PS, start[1])

@@ -248,3 +248,26 @@ def test_aos_staging_round_trip(setup):
         assert np.array_equal(back[0], qa) and np.array_equal(back[1][..., 1:], qsa[..., 1:])
     ctx.set(H.HAMR_P, p0)
     ctx.set(H.HAMR_PS, ps0)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_marching_update_equals_per_cell_update(setup, mode):
+    """The x1-marching form of the conserved update (HAMR_TUNE_CONS with
+    workgroups -1) gives the per-cell form's step bit for bit."""
+    desc, ctx, grid, p0, ps0 = setup
+    slot = H.HAMR_TUNE_NAMES.index("cons")
+    ctx.set_precision(H.HAMR_PREC_FP64 if mode == "fp64" else H.HAMR_PREC_MIXED)
+    try:
+        outs = []
+        for cfg in ((256, 0, 0), (128, -1, 0)):
+            ctx.tune_set(slot, *cfg)
+            ctx.set(H.HAMR_P, p0)
+            ctx.set(H.HAMR_PS, ps0)
+            dt = ctx.estimate_dt()
+            ndt = ctx.step(2, dt)
+            outs.append((ndt, ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS)))
+        assert outs[0][0] == outs[1][0]
+        assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
+    finally:
+        ctx.tune_set(slot, 256, 0, 0)
+        ctx.set_precision(H.HAMR_PREC_FP64)
