@@ -351,7 +351,7 @@ def main():
                     help="steps of the reference-structured per-block FP64 sequence (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (each GPU box holds one MI355X); "
+        print("bench.py: the step is measured on one MI355X (one process, one GPU); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -10,7 +10,7 @@
 // ring's polar E1 average and the pi-shifted partner exist as in the uniform
 // grid.
 //
-// Ghost exchange (this build's specification of the multi-level ghost fill;
+// Ghost exchange (this library's own specification of the multi-level ghost fill;
 // the reference exchanges through boundcomP.c / GPU_program2.cu packs):
 //   stage d = 1, 2, 3 fills the ghosts along d over the extent already filled
 //   along the earlier directions and the interior along the later ones (the

@@ -405,7 +405,7 @@ __device__ __forceinline__ void wave_count(int *cnt, bool pred)
     if (__lane_id() == __ffsll((unsigned long long)act) - 1) atomicAdd(cnt, __popcll(m));
 }
 
-// Floors (the build's specification: rho >= rho_floor r^-1.5, u >= u_floor
+// Floors (this library's own specification: rho >= rho_floor r^-1.5, u >= u_floor
 // r^-2.5, flag -333 as GPU_fixup.h:399-415) and the result of one cell.  A
 // cell among the NG lowest or highest interior cells of its z-row is also
 // written into the x3 ghost cell of the periodic neighbour that the ghost fill

@@ -2,7 +2,7 @@
  * hamr_amr_oracle.c -- FP64 CPU restatement of the step on a multi-level set
  * of blocks of uneven sizes (level ratio 2).
  *
- * TEST INFRASTRUCTURE ONLY (the checker for the HIP step; see hamr_oracle.h).
+ * TEST INFRASTRUCTURE ONLY (what the HIP step is tested against; see hamr_oracle.h).
  *
  * Each block is advanced by the single-grid stage functions of hamr_oracle.c on
  * its own ghost-inclusive arrays (its level's spacing, its metric); the polar

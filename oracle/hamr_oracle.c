@@ -1,7 +1,7 @@
 /*
  * hamr_oracle.c -- FP64 CPU restatement of H-AMR's GRMHD fluid step.
  *
- * TEST INFRASTRUCTURE ONLY (the checker for the HIP step; see hamr_oracle.h).
+ * TEST INFRASTRUCTURE ONLY (what the HIP step is tested against; see hamr_oracle.h).
  *
  * Every function follows the reference's GPU-path arithmetic, expression by
  * expression, so that the HIP kernels (built, like this file, without FMA
@@ -1351,7 +1351,7 @@ void oracle_update(const oracle_params *par, int full, double Dt, const double *
                         if (flag) fc2++;
                     }
                 }
-                /* floors (the build's specification; reference fixup_cell is a no-op) */
+                /* floors (the library's own specification; reference fixup_cell is a no-op) */
                 {
                     double r = m->rcen[col], sr = sqrt(r);
                     int hit = 0;

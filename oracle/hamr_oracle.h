@@ -1,8 +1,8 @@
 /*
  * hamr_oracle.h -- FP64 CPU restatement of H-AMR's GRMHD fluid step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP step in
- * hamr_gamma_wind_amd/: only tests/, __graft_entry__.smoke() and the
+ * TEST INFRASTRUCTURE ONLY: the HIP step in hamr_gamma_wind_amd/ is tested
+ * against this library.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it.  The product never links it.
  *
  * It works on ONE global grid with NG=3 ghost zones (whereas the reference and
@@ -116,7 +116,7 @@ void oracle_inflow_check(double *pr, const double *gcov, const double *gcon, int
 /* ---- multi-level block sets (hamr_amr_oracle.c) ----
  * Blocks of uneven sizes on levels of ratio 2, each its own ghost-inclusive
  * grid in the layout above (p[b], ps[b], pflag[b]; metric m[b] in the layout
- * above with ncol = D1*D2 of the block).  The ghost fill is the build's
+ * above with ncol = D1*D2 of the block).  The ghost fill is the library's
  * specification of the multi-level exchange (copies, 2x2x2 means, MC-limited
  * linear prolongation; see hamr_amr_oracle.c), found here by searching the
  * block list for the leaf that holds each position. */

@@ -3,7 +3,7 @@
  * (rdump_block_write / rdump_block_read, restart.c:52-67 and :178-269) and of
  * its dump record (dump_block, dump.c:276-356).
  *
- * TEST INFRASTRUCTURE ONLY (the checker for the HIP record kernels; see
+ * TEST INFRASTRUCTURE ONLY (what the HIP record kernels are tested against; see
  * hamr_oracle.h).
  *
  * Arrays are one block's ghost-inclusive grid in the layout of hamr_oracle.h
@@ -20,7 +20,7 @@
 #define RREC (ORC_NPR + 4)
 
 /* restart.c:55-63: p[0..8], ps[0], ps[1] gdet(FACE1), ps[2] gdet(FACE2),
- * ps[3] gdet(FACE3) per cell, cells in index_3D order.  This build keeps no
+ * ps[3] gdet(FACE3) per cell, cells in index_3D order.  This layout keeps no
  * face-field component 0: its slot is written as 0 (the reader skips it). */
 void oracle_rdump_block_write(const int bs[3], const double *p, const double *ps, const double *gdet, double *rec)
 {

@@ -1,6 +1,6 @@
 """ctypes bindings of the FP64 CPU oracle (oracle/liboracle.so).
 
-Test infrastructure: the oracle is the checker of the HIP step, never the thing
+Test infrastructure: the HIP step is tested against the oracle, which is never the thing
 measured or shipped.  Arrays follow the layout documented in oracle/hamr_oracle.h
 (structure of arrays over the ghost-inclusive global grid, z fastest).
 """
