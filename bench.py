@@ -168,20 +168,19 @@ def host_sequence(H, ctx, p0, ps0, nsteps):
     return time.perf_counter() - t0
 
 
-# The multi-level torus: base 128x96x32 (64x32x32 superblocks); level 1 over
-# x1 [0,160) x2 [32,160) of its 256x192x64 cells (r <= 80 M, 30-150 deg), level 2
-# over x1 [0,256) x2 [128,256) of its 512x384x128 cells (r <= 34.6 M, 60-120 deg);
-# blocks of 64x32x32, 32x16x16 and 16^3 by the seeded choice (seed 2).
-AMR_DESC = ((64, 32, 32), (2, 3, 1))
-AMR_BOXES = [((0, 160), (32, 160), (0, 64)), ((0, 256), (128, 256), (0, 128))]
+def _amr_torus():
+    from hamr_gamma_wind_amd import workloads as W
+
+    return W.AMR_TORUS
 
 
 def amr_bench(H, args):
     """Both arithmetics on the multi-level block set: tuned launches, graph replay."""
     import numpy as np
 
-    desc = H.make_desc(*AMR_DESC)
-    amr = H.make_amr_desc(AMR_BOXES, seed=2)
+    A = _amr_torus()
+    desc = H.make_desc(A["bs"], A["nb"])
+    amr = H.make_amr_desc(A["boxes"], seed=A["seed"])
     ctx = H.AmrContext(desc, amr)
     try:
         info = ctx.info()

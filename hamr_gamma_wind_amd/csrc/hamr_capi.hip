@@ -328,10 +328,10 @@ static void bound_all(hamr_ctx *c, double *p, double *ps, bool x3_rows_done = fa
         return;
     }
     launch_bound(c->P, 1, all_launch(c, TS_BOUND1), p, ps, c->pflag, 0, c->s);
-    launch_amr_exchange(c->P, c->ex[0], c->nex[0], c->pro[0], c->npro[0], p, ps, c->pflag, T, G, c->s);
+    launch_amr_exchange(c->P, 1, c->ex[0], c->nex[0], c->pro[0], c->npro[0], p, ps, c->pflag, T, G, c->s);
     launch_bound(c->P, 2, all_launch(c, TS_BOUND2), p, ps, c->pflag, 0, c->s);
-    launch_amr_exchange(c->P, c->ex[1], c->nex[1], c->pro[1], c->npro[1], p, ps, c->pflag, T, G, c->s);
-    launch_amr_exchange(c->P, c->ex[2], c->nex[2], c->pro[2], c->npro[2], p, ps, c->pflag, T, G, c->s);
+    launch_amr_exchange(c->P, 2, c->ex[1], c->nex[1], c->pro[1], c->npro[1], p, ps, c->pflag, T, G, c->s);
+    launch_amr_exchange(c->P, 3, c->ex[2], c->nex[2], c->pro[2], c->npro[2], p, ps, c->pflag, T, G, c->s);
     launch_bound(c->P, 1, all_launch(c, TS_BOUND1), p, ps, c->pflag, 1, c->s);
 }
 
@@ -1487,7 +1487,7 @@ static void run_slot(hamr_ctx *c, int slot, const Launch &L, int T_ex, int G_ex)
     case TS_BOUND3: launch_bound(P, slot - TS_BOUND1 + 1, L, c->p, c->ps, c->pflag, 0, s); break;
     default:
         for (int st = 0; st < 3; st++)
-            launch_amr_exchange(P, c->ex[st], c->nex[st], c->pro[st], c->npro[st], c->p, c->ps, c->pflag, T_ex, G_ex, s);
+            launch_amr_exchange(P, st + 1, c->ex[st], c->nex[st], c->pro[st], c->npro[st], c->p, c->ps, c->pflag, T_ex, G_ex, s);
         break;
     }
 }
@@ -1510,17 +1510,38 @@ static int time_launches(hamr_ctx *c, int slot, const Launch &L, int T_ex, int G
     return check_launch(c, "autotune");
 }
 
+// Timing and tuning launches run the stages on the current state: the update
+// kernels overwrite the flags and failure counters (and ph, U0, Un), the
+// face-field update psh.  These keep the flags and counters of the last step
+// and leave p/ps and ph/psh with consistent ghosts afterwards.
+struct FlagCopy {
+    std::vector<int> fc, fl;
+};
+static int save_flags(hamr_ctx *c, FlagCopy &k)
+{
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    k.fc.resize((size_t)nblk_of(c) * 4);
+    k.fl.resize((size_t)c->cells);
+    if (d2h(c, k.fc.data(), c->failcnt, k.fc.size() * sizeof(int))) return -1;
+    return d2h(c, k.fl.data(), c->pflag, k.fl.size() * sizeof(int));
+}
+static int restore_flags(hamr_ctx *c, const FlagCopy &k, const char *what)
+{
+    if (h2d(c, c->failcnt, k.fc.data(), k.fc.size() * sizeof(int))) return -1;
+    if (h2d(c, c->pflag, k.fl.data(), k.fl.size() * sizeof(int))) return -1;
+    bound_all(c, c->p, c->ps);
+    bound_all(c, c->ph, c->psh);
+    if (check_launch(c, what)) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
 int hamr_autotune(hamr_ctx *c, int nrep)
 {
     if (!c || nrep < 1) return fail(c, "hamr_autotune: bad arguments");
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    // the tuning launches run the update on the current state: keep the
-    // failure counters and flags of the last step
-    const size_t nb = (size_t)nblk_of(c);
-    std::vector<int> fc(nb * 4), fl((size_t)c->cells);
-    if (d2h(c, fc.data(), c->failcnt, fc.size() * sizeof(int))) return -1;
-    if (d2h(c, fl.data(), c->pflag, fl.size() * sizeof(int))) return -1;
+    FlagCopy keepf;
+    if (save_flags(c, keepf)) return -1;
     static const int grids[4] = {0, 1024, 2048, 4096};
     for (int slot = 0; slot < TS_N; slot++) {
         if (slot == TS_EXCHANGE && !c->amr) continue;
@@ -1559,13 +1580,7 @@ int hamr_autotune(hamr_ctx *c, int nrep)
         }
         c->tune[slot] = best;
     }
-    if (h2d(c, c->failcnt, fc.data(), fc.size() * sizeof(int))) return -1;
-    if (h2d(c, c->pflag, fl.data(), fl.size() * sizeof(int))) return -1;
-    // leave both states with consistent ghosts
-    bound_all(c, c->p, c->ps);
-    bound_all(c, c->ph, c->psh);
-    if (check_launch(c, "autotune")) return -1;
-    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (restore_flags(c, keepf, "autotune")) return -1;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
@@ -1581,10 +1596,13 @@ int hamr_time_kernel(hamr_ctx *c, int slot, int nrep, double *avg_ms)
         return 0;
     }
     HIPCHK(c, hipSetDevice(c->device));
+    FlagCopy keepf;
+    if (save_flags(c, keepf)) return -1;
     Launch L{};
     if (slot != TS_EXCHANGE) L = all_launch(c, slot);
     float ms = 0.f;
     if (time_launches(c, slot, L, kTiles[c->tune[TS_EXCHANGE].tile], c->tune[TS_EXCHANGE].grid, nrep, &ms)) return -1;
+    if (restore_flags(c, keepf, "hamr_time_kernel")) return -1;
     *avg_ms = ms;
     return 0;
 }

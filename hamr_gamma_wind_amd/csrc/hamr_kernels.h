@@ -145,10 +145,10 @@ void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStre
 void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s);
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
 
-// Multi-level ghost exchange of one stage: copy/restriction entries, then
-// prolongation entries (they write disjoint ghost cells and read only interior
-// cells and ghosts of earlier stages).  grid <= 0: one thread per entry.
-void launch_amr_exchange(const KParams &P, const AmrEntry *e, int64_t n, const AmrProlong *pe, int64_t npro,
+// Multi-level ghost exchange of stage dir (1..3): copy/restriction entries,
+// then prolongation entries (they write disjoint ghost cells and read only
+// interior cells and ghosts of earlier stages).  grid <= 0: one thread per entry.
+void launch_amr_exchange(const KParams &P, int dir, const AmrEntry *e, int64_t n, const AmrProlong *pe, int64_t npro,
                          double *p, double *ps, int *pflag, int threads, int grid, hipStream_t s);
 
 }  // namespace hamr

@@ -635,13 +635,37 @@ __global__ void __launch_bounds__(256) k_bound(KParams P, WorkList wl, double *p
 // and its source: a same-level cell (copy) or the 2x2x2 finer cells that tile
 // it (restriction, the mean of packsendaverage1, GPU_program2.cu:154-197); the
 // face fields of a restriction are the means of the four finer faces that tile
-// each coarse face.  A prolongation entry names the coarse cell that contains
-// the ghost and its six neighbours; the ghost gets the coarse value plus
-// a quarter of each direction's MC slope (slope_lim_2, GPU_program2.cu:10-45),
-// signed by the side of the coarse cell it lies on (unpackreceivecoarse1,
+// each coarse face, the component normal to the exchange direction weighted by
+// the finer faces' gdet and divided by the coarse face's (the flux mean that
+// packsendaverage1-3 pack, GPU_program2.cu:176-185, :231-236, :281-285, over
+// the receiver's gdet, as the disabled store of unpackreceive1 does,
+// :376-380).  A prolongation entry names the coarse cell that contains the
+// ghost and its six neighbours; the ghost gets the coarse value plus a quarter
+// of each direction's MC slope (slope_lim_2, GPU_program2.cu:10-45), signed by
+// the side of the coarse cell it lies on (unpackreceivecoarse1,
 // GPU_program2.cu:615-803).  A face field along direction a carries no slope
 // along a; a fine face that halves a coarse cell starts from the coarse
 // cell-centred field, one on a coarse face from that face's value.
+//
+// No entry of the exchange along d writes the d-component of the face field
+// at the block's own upper boundary face (ghost index NG+bs along d, interior
+// across): the block's constrained-transport update owns that face, as the
+// reference's receive never stores the normal component (unpackreceive1,
+// GPU_program2.cu:376-380; unpackreceivecoarse1 stores only the tangential
+// ones, :779-796), so the divergence of every interior cell's field stays at
+// its roundoff.  The same component in ghost rows beyond another direction's
+// edge is written (the fluxes of the transverse ghost faces read it).
+
+// Whether ghost cell (i, j, z) of block B holds B's own upper boundary face
+// along direction d0 (0..2): index NG+bs along d0, interior across.
+__device__ __forceinline__ bool own_upper_face(const BlockDev &B, int d0, const int loc[3])
+{
+    bool own = loc[d0] == NG + B.bs[d0];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (k != d0) own = own && loc[k] >= NG && loc[k] < NG + B.bs[k];
+    return own;
+}
 
 // value of source s (cell or 2x2x2 mean) for the 9 primitives and 3 face fields
 __device__ __forceinline__ void amr_source(const KParams &P, const double *__restrict__ p,
@@ -670,28 +694,55 @@ __device__ __forceinline__ void amr_source(const KParams &P, const double *__res
     v[NPR + 2] = 0.25 * (sp[2 * K] + sp[2 * K + sj] + sp[2 * K + si] + sp[2 * K + si + sj]);
 }
 
-__global__ void __launch_bounds__(256) k_amr_copy(KParams P, const AmrEntry *__restrict__ e, int64_t n, double *p,
-                                                  double *ps, int *pflag)
+// gdet-weighted mean of the four finer d0-faces of the 2x2x2 cells from cell c
+// of block S (term order of packsendaverage1-3), over the coarse face's gdet gc
+__device__ __forceinline__ double amr_flux_mean(const KParams &P, const BlockDev &S, const double *__restrict__ ps,
+                                                int d0, int64_t c, double gc)
 {
+    const int64_t K = S.ks, si = (int64_t)S.D[1] * S.D[2], sj = S.D[2];
+    const int64_t col = c / S.D[2];
+    const double *sp = ps + 3 * S.off + (int64_t)d0 * K + c;
+    const double *gf = P.gdet + mg1(S, d0 == 0 ? FACE1 : d0 == 1 ? FACE2 : FACE3) + col;
+    double m;
+    if (d0 == 0) m = 0.25 * (sp[0] * gf[0] + sp[1] * gf[0] + sp[sj] * gf[1] + sp[sj + 1] * gf[1]);
+    else if (d0 == 1) m = 0.25 * (sp[0] * gf[0] + sp[1] * gf[0] + sp[si] * gf[S.D[1]] + sp[si + 1] * gf[S.D[1]]);
+    else m = 0.25 * (sp[0] * gf[0] + sp[sj] * gf[1] + sp[si] * gf[S.D[1]] + sp[si + sj] * gf[S.D[1] + 1]);
+    return m / gc;
+}
+
+// dir: the exchange direction of the entries (1..3)
+__global__ void __launch_bounds__(256) k_amr_copy(KParams P, const AmrEntry *__restrict__ e, int64_t n, int dir,
+                                                  double *p, double *ps, int *pflag)
+{
+    const int d0 = dir - 1;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         const AmrEntry E = e[t];
         double v[NPR + 3];
         amr_source(P, p, ps, E.sblk, E.scode, v);
         const BlockDev &B = P.blk[E.dblk];
         const BlockDev &S = P.blk[E.sblk];
+        int loc[3];
+        region_index(E.dcell, B.D[1], B.D[2], loc[0], loc[1], loc[2]);
+        if ((E.scode >> 28) == AS_AVG8) {
+            const double gc = P.gdet[mg1(B, d0 == 0 ? FACE1 : d0 == 1 ? FACE2 : FACE3) + (int64_t)loc[0] * B.D[1] + loc[1]];
+            v[NPR + d0] = amr_flux_mean(P, S, ps, d0, E.scode & 0x0FFFFFFF, gc);
+        }
+        const bool own = own_upper_face(B, d0, loc);
         const int64_t K = B.ks;
         double *pd = p + NPR * B.off + E.dcell, *sd = ps + 3 * B.off + E.dcell;
 #pragma unroll
         for (int k = 0; k < NPR; k++) pd[k * K] = v[k];
 #pragma unroll
-        for (int a = 0; a < 3; a++) sd[a * K] = v[NPR + a];
+        for (int a = 0; a < 3; a++)
+            if (!(a == d0 && own)) sd[a * K] = v[NPR + a];
         if (pflag) pflag[B.off + E.dcell] = pflag[S.off + (E.scode & 0x0FFFFFFF)];
     }
 }
 
 __global__ void __launch_bounds__(256) k_amr_prolong(KParams P, const AmrProlong *__restrict__ e, int64_t n,
-                                                     double *p, double *ps, int *pflag)
+                                                     int dir, double *p, double *ps, int *pflag)
 {
+    const int d0 = dir - 1;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         const AmrProlong E = e[t];
         const BlockDev &B = P.blk[E.dblk];
@@ -699,6 +750,7 @@ __global__ void __launch_bounds__(256) k_amr_prolong(KParams P, const AmrProlong
         // side of the coarse cell along each direction (global fine index parity)
         int loc[3];
         region_index(E.dcell, B.D[1], B.D[2], loc[0], loc[1], loc[2]);
+        const bool own = own_upper_face(B, d0, loc);
         double sgn[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) sgn[d] = ((B.o[d] + loc[d] - NG) & 1) ? 1.0 : -1.0;
@@ -731,7 +783,7 @@ __global__ void __launch_bounds__(256) k_amr_prolong(KParams P, const AmrProlong
 #pragma unroll
             for (int d = 0; d < 3; d++)
                 if (d != a) v = v + 0.25 * sgn[d] * dq[d][kk];
-            sd[a * K] = v;
+            if (!(a == d0 && own)) sd[a * K] = v;
         }
         if (pflag) pflag[B.off + E.dcell] = pflag[C.off + E.ccell];
     }
@@ -946,18 +998,18 @@ void launch_bound(const KParams &P, int dir, const Launch &L, double *p, double 
     else hipLaunchKernelGGL(k_bound<3>, gdim(L), bdim(L), 0, s, P, L.wl, p, ps, pflag, outflow_only, x3_rows_done);
 }
 
-void launch_amr_exchange(const KParams &P, const AmrEntry *e, int64_t n, const AmrProlong *pe, int64_t npro,
+void launch_amr_exchange(const KParams &P, int dir, const AmrEntry *e, int64_t n, const AmrProlong *pe, int64_t npro,
                          double *p, double *ps, int *pflag, int threads, int grid, hipStream_t s)
 {
     if (n > 0) {
         const int64_t need = (n + threads - 1) / threads;
         const unsigned g = (unsigned)(grid > 0 && grid < need ? grid : need);
-        hipLaunchKernelGGL(k_amr_copy, dim3(g), dim3(threads), 0, s, P, e, n, p, ps, pflag);
+        hipLaunchKernelGGL(k_amr_copy, dim3(g), dim3(threads), 0, s, P, e, n, dir, p, ps, pflag);
     }
     if (npro > 0) {
         const int64_t need = (npro + threads - 1) / threads;
         const unsigned g = (unsigned)(grid > 0 && grid < need ? grid : need);
-        hipLaunchKernelGGL(k_amr_prolong, dim3(g), dim3(threads), 0, s, P, pe, npro, p, ps, pflag);
+        hipLaunchKernelGGL(k_amr_prolong, dim3(g), dim3(threads), 0, s, P, pe, npro, dir, p, ps, pflag);
     }
 }
 

@@ -9,6 +9,20 @@ import numpy as np
 
 NG = 3
 
+# The torus grids: (cells per block, blocks per direction).  r 1.2-1000 M
+# log-spaced, theta over (0, pi), phi over 2 pi; NG = 3 ghost zones per block.
+TORUS_64 = ((16, 8, 8), (4, 4, 4))        # 64x32x32 as 64 blocks of 16x8x8
+TORUS_256 = ((32, 32, 32), (8, 4, 4))     # 256x128x128 as 128 blocks of 32^3 (4.2 M zones)
+TORUS_384 = ((32, 32, 32), (12, 6, 6))    # 384x192x192 as 432 blocks of 32^3 (14.2 M zones)
+
+# The multi-level torus: base 128x96x32 (64x32x32 superblocks); level 1 over
+# x1 [0,160) x2 [32,160) of its 256x192x64 cells (r <= 80 M, 30-150 deg), level 2
+# over x1 [0,256) x2 [128,256) of its 512x384x128 cells (r <= 34.6 M, 60-120 deg);
+# blocks of 64x32x32, 32x16x16 and 16^3 by the seeded choice (722 blocks,
+# 5.21 M zones).
+AMR_TORUS = dict(bs=(64, 32, 32), nb=(2, 3, 1),
+                 boxes=[((0, 160), (32, 160), (0, 64)), ((0, 256), (128, 256), (0, 128))], seed=2)
+
 
 def curl_faces(A, dx):
     """Face fields (lower faces) B = curl A on a periodic grid with edge-centred

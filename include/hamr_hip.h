@@ -139,8 +139,13 @@ int hamr_create(const hamr_desc *desc, hamr_ctx **out);
  * `desc` (every block resident, one slab per block and array).  All entry
  * points below work on it; the ghost fill (hamr_boundprim) then also exchanges
  * across block sizes and levels: copies, restriction by 2x2x2 means
- * (packsendaverage1, GPU_program2.cu:154-197) and MC-limited linear
- * prolongation (unpackreceivecoarse1, GPU_program2.cu:615-803).  Host layout
+ * (packsendaverage1, GPU_program2.cu:154-197; the face field normal to the
+ * exchange direction as the gdet-weighted mean of the four finer faces over
+ * the coarse face's gdet) and MC-limited linear prolongation
+ * (unpackreceivecoarse1, GPU_program2.cu:615-803).  No exchange writes the
+ * normal face field of a block's own upper boundary face (its CT update owns
+ * it; the reference's receive stores no normal component,
+ * GPU_program2.cu:376-380).  Host layout
  * of whole arrays: the blocks' [component][cell] slabs one after another, each
  * with its own (bs+6)^3 cell count. */
 int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *amr, hamr_ctx **out);
@@ -334,7 +339,11 @@ int hamr_tune_get(const hamr_ctx *ctx, int slot, int *cfg, double *ms);
 int hamr_tune_set(hamr_ctx *ctx, int slot, const int *cfg);
 /* Average device time (ms, HIP events on the context stream) of one launch of
  * kernel `slot` over all blocks with its tuned shape and predictor-step
- * operands (HAMR_TUNE_EXCHANGE: the three exchange stages of a multi-level set). */
+ * operands (HAMR_TUNE_EXCHANGE: the three exchange stages of a multi-level set).
+ * Like hamr_autotune it keeps p/ps, the flags and the failure counters of the
+ * last step, and leaves p/ps and ph/psh with their ghosts filled; the
+ * intermediate arrays (fluxes, EMFs, U0, Un, ph, psh) hold the timed launches'
+ * results. */
 int hamr_time_kernel(hamr_ctx *ctx, int slot, int nrep, double *avg_ms);
 
 /* ---- one forward stage and its adjoint (no reference counterpart) ----

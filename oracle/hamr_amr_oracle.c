@@ -136,13 +136,54 @@ static double slope_lim(double y1, double y2, double y3)
     else return Dqc;
 }
 
+/* Test switch (planted-mistake tests only): 1 makes the exchange also write
+ * the normal face field of a block's own upper boundary face, which the
+ * exchange rule forbids. */
+int oracle_amr_plant_own_face = 0;
+
+/* write the ghost values; the component `skip` (0..2, or -1) of the face field is left as it is */
 static void write_ghost(const oracle_amr *g, int b, long c, double **p, double **ps, int **pflag, const double *v,
-                        int flag)
+                        int flag, int skip)
 {
     const long N = geo_of(&g->blk[b]).ncell;
     for (int k = 0; k < NPR; k++) p[b][k * N + c] = v[k];
-    for (int a = 0; a < 3; a++) ps[b][a * N + c] = v[NPR + a];
+    for (int a = 0; a < 3; a++)
+        if (a != skip) ps[b][a * N + c] = v[NPR + a];
     if (pflag) pflag[b][c] = flag;
+}
+
+/* gdet of face position pos of column (i, j) of block b */
+static double face_gdet(const oracle_amr *g, int b, int pos, int i, int j)
+{
+    const oracle_amr_block *B = &g->blk[b];
+    const long ncol = (long)(B->bs[0] + 2 * NG) * (B->bs[1] + 2 * NG);
+    return g->m[b].gdet[pos * ncol + (long)i * (B->bs[1] + 2 * NG) + j];
+}
+
+/* gdet-weighted mean of the four finer faces normal to direction dd of the
+ * 2x2x2 cells of block c from local cell (i, j, z) (the flux mean of
+ * packsendaverage1-3, GPU_program2.cu:176-185, :231-236, :281-285, same term
+ * order), divided by the coarse face's gdet gc */
+static double flux_mean(const oracle_amr *g, int c, int dd, int i, int j, int z, double **ps, double gc)
+{
+    const oracle_amr_block *B = &g->blk[c];
+    const long N = geo_of(B).ncell;
+    const double *f = ps[c] + dd * N;
+    double m;
+#define F(a, b2, e) f[cell_local(B, a, b2, e)]
+    if (dd == 0) {
+        const double g0 = face_gdet(g, c, FACE1, i, j), g1 = face_gdet(g, c, FACE1, i, j + 1);
+        m = 0.25 * (F(i, j, z) * g0 + F(i, j, z + 1) * g0 + F(i, j + 1, z) * g1 + F(i, j + 1, z + 1) * g1);
+    } else if (dd == 1) {
+        const double g0 = face_gdet(g, c, FACE2, i, j), g1 = face_gdet(g, c, FACE2, i + 1, j);
+        m = 0.25 * (F(i, j, z) * g0 + F(i, j, z + 1) * g0 + F(i + 1, j, z) * g1 + F(i + 1, j, z + 1) * g1);
+    } else {
+        m = 0.25 * (F(i, j, z) * face_gdet(g, c, FACE3, i, j) + F(i, j + 1, z) * face_gdet(g, c, FACE3, i, j + 1) +
+                    F(i + 1, j, z) * face_gdet(g, c, FACE3, i + 1, j) +
+                    F(i + 1, j + 1, z) * face_gdet(g, c, FACE3, i + 1, j + 1));
+    }
+#undef F
+    return m / gc;
 }
 
 /* value at position Q of level lq's coarse neighbour of a prolongation; 0 if none */
@@ -170,11 +211,22 @@ static int neighbour_values(const oracle_amr *g, int lq, int cblk, int Q[3], int
     return 1;
 }
 
-static void exchange_cell(const oracle_amr *g, int b, int i, int j, int z, double **p, double **ps, int **pflag)
+/* Exchange of ghost cell (i, j, z) of block b in the stage along dd (0..2).
+ * The dd-component of the face field at the block's own upper boundary face
+ * (ghost index NG+bs along dd, interior across) is never written: the block's
+ * CT update owns it (the reference's receive stores no normal component,
+ * unpackreceive1, GPU_program2.cu:376-380). */
+static void exchange_cell(const oracle_amr *g, int dd, int b, int i, int j, int z, double **p, double **ps,
+                          int **pflag)
 {
     const oracle_amr_block *B = &g->blk[b];
     const int l = B->level;
     const long dc = cell_local(B, i, j, z);
+    const int loc[3] = {i, j, z};
+    int own = loc[dd] == NG + B->bs[dd];
+    for (int k = 0; k < 3; k++)
+        if (k != dd && (loc[k] < NG || loc[k] >= NG + B->bs[k])) own = 0;
+    const int skip = (own && !oracle_amr_plant_own_face) ? dd : -1;
     int Y[3] = {B->o[0] + i - NG, B->o[1] + j - NG, B->o[2] + z - NG};
     Y[2] = wrap(Y[2], g->N[l][2]);
     const int out1 = Y[0] < 0 || Y[0] >= g->N[l][0], out2 = Y[1] < 0 || Y[1] >= g->N[l][1];
@@ -185,7 +237,7 @@ static void exchange_cell(const oracle_amr *g, int b, int i, int j, int z, doubl
         if (c < 0 || lc != l) abort();
         const long sc = cell_at(&g->blk[c], Y);
         cell_values(g, c, sc, p, ps, v);
-        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0, skip);
         return;
     }
     if (out1) return;
@@ -195,12 +247,15 @@ static void exchange_cell(const oracle_amr *g, int b, int i, int j, int z, doubl
     if (lc == l) {
         const long sc = cell_at(C, Y);
         cell_values(g, c, sc, p, ps, v);
-        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0, skip);
     } else if (lc == l + 1) {
         int F[3] = {2 * Y[0], 2 * Y[1], 2 * Y[2]};
         const long sc = cell_at(C, F);
         avg8_values(g, c, sc, p, ps, v);
-        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0);
+        const int pos = dd == 0 ? FACE1 : dd == 1 ? FACE2 : FACE3;
+        v[NPR + dd] = flux_mean(g, c, dd, F[0] - C->o[0] + NG, F[1] - C->o[1] + NG, F[2] - C->o[2] + NG, ps,
+                                face_gdet(g, b, pos, i, j));
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0, skip);
     } else if (lc == l - 1) {
         int P[3] = {Y[0] >> 1, Y[1] >> 1, Y[2] >> 1};
         const long pc = cell_at(C, P);
@@ -224,7 +279,7 @@ static void exchange_cell(const oracle_amr *g, int b, int i, int j, int z, doubl
                 if (d != a) w = w + 0.25 * sgn[d] * dq[d][kk];
             v[NPR + a] = w;
         }
-        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][pc] : 0);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][pc] : 0, skip);
     } else {
         abort();
     }
@@ -258,7 +313,7 @@ void oracle_amr_exchange(const oracle_amr *g, int stage, double **p, double **ps
             hi[dd] = side ? G.D[dd] : NG;
             for (int i = lo[0]; i < hi[0]; i++)
                 for (int j = lo[1]; j < hi[1]; j++)
-                    for (int z = lo[2]; z < hi[2]; z++) exchange_cell(g, b, i, j, z, p, ps, pflag);
+                    for (int z = lo[2]; z < hi[2]; z++) exchange_cell(g, dd, b, i, j, z, p, ps, pflag);
         }
     }
 }

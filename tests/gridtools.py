@@ -85,8 +85,34 @@ def block_view(desc, G):
     return global_to_blocks(desc, G[None])[:, 0]
 
 
+# smallest normal double: the floor of the pointwise relative error, so that
+# an exact zero on both sides is no error and a difference at a zero reference
+# value is a huge one
+DENORMAL_FLOOR = 2.2250738585072014e-308
+
+
+def pointwise_relerr(a, b, axis_comp=1, floor=DENORMAL_FLOOR):
+    """max over cells of |a - b| / max(|b|, floor) per component (component
+    axis `axis_comp`): the FP64 parity bar |a - b| <= tol |b| cell by cell."""
+    a = np.moveaxis(np.asarray(a), axis_comp, 0).reshape(np.shape(a)[axis_comp], -1)
+    b = np.moveaxis(np.asarray(b), axis_comp, 0).reshape(a.shape[0], -1)
+    d = np.abs(a - b)
+    return np.max(d / np.maximum(np.abs(b), floor), axis=1, initial=0.0)
+
+
+def pointwise_relerr_blocks(got, want, floor=DENORMAL_FLOOR):
+    """pointwise_relerr over lists of per-block [component][...] arrays."""
+    err = None
+    for a, b in zip(got, want):
+        e = pointwise_relerr(a[None], b[None], floor=floor)
+        err = e if err is None else np.maximum(err, e)
+    return err
+
+
 def relerr_by_var(a, b, axis_comp=1):
-    """max |a - b| / max |b| per component (component axis `axis_comp`)."""
+    """max |a - b| / max |b| per component (component axis `axis_comp`).
+    Normalised by the grid maximum: a loose check for the mixed arithmetic's
+    error; FP64 parity uses pointwise_relerr."""
     a = np.moveaxis(a, axis_comp, 0).reshape(a.shape[axis_comp], -1)
     b = np.moveaxis(b, axis_comp, 0).reshape(b.shape[axis_comp], -1)
     scale = np.max(np.abs(b), axis=1)
@@ -123,3 +149,27 @@ def amr_oracle(oracle, desc, amr, riemann=0):
     _, dx0 = H.desc_coords(desc)
     return O.AmrGrid(oracle, [r[:7] for r in rows], N, metrics, desc.gam, desc.cour, dx0, desc.rho_floor,
                      desc.u_floor, riemann=riemann), rows
+
+
+def amr_divb(metrics, rows, dx0, ps):
+    """Largest |div B| of any interior cell of a multi-level block set,
+    relative to its block's scale: div B = sum_d (gdet B^d at the upper d-face
+    - at the lower d-face) / dx_d (the block's level spacing), scale = sum_d
+    max |gdet B^d| / dx_d over the block's faces.  metrics: per block, the
+    oracle layout (gdet[pos][col], FACE1 = 0, FACE2 = 1, FACE3 = 4); rows:
+    level, origin, size; ps: per block [3][D1][D2][D3] face fields."""
+    worst = 0.0
+    for b, r in enumerate(rows):
+        bs = [int(k) for k in r[4:7]]
+        D = [k + 2 * NG for k in bs]
+        dx = [d * 2.0 ** -int(r[0]) for d in dx0]
+        gd = metrics[b]["gdet"]
+        f = [ps[b][a] * gd[pos].reshape(D[0], D[1])[:, :, None] for a, pos in enumerate((0, 1, 4))]
+        I = tuple(slice(NG, NG + k) for k in bs)
+        up = [tuple(slice(NG + (1 if k == a else 0), NG + bs[k] + (1 if k == a else 0)) for k in range(3))
+              for a in range(3)]
+        div = sum((f[a][up[a]] - f[a][I]) / dx[a] for a in range(3))
+        scale = sum(np.abs(f[a][up[a]]).max() / dx[a] + np.abs(f[a][I]).max() / dx[a] for a in range(3))
+        if scale > 0:
+            worst = max(worst, float(np.abs(div).max() / scale))
+    return worst
