@@ -1106,6 +1106,10 @@ double oracle_block_dt(const oracle_params *par, const double *dt1, const double
 }
 
 /* ------------------------------------------------------------------------ */
+/* Test switch (planted-mistake tests of the polar E1 ring average only; 0 in
+ * every other use): see oracle_emf. */
+int oracle_pole_plant = 0;
+
 /* CT: consttransport1 (GPU_const_trans.h:63-119, E at cell centres),
  * consttransport2 (:121-339, corner EMF upwinded on the mass-flux sign, with the
  * polar rule :233-236) and the polar E1 average of E_average / read_E_avg /
@@ -1195,16 +1199,20 @@ void oracle_emf(const oracle_params *par, const double *pb, const double *F1, co
      * in z; then the mean of those over the phi-blocks of the ring; then /factor. */
     {
         int nb3 = par->n3 / par->bs3;
+        /* planted mistakes (tests only): 1 divides by bs3 + 1, 2 also sums the
+         * block's duplicated upper corner, 3 averages one block, not the ring */
+        const int zn = par->bs3 + (oracle_pole_plant == 2), dn = par->bs3 + (oracle_pole_plant == 1 || oracle_pole_plant == 2);
+        const int nbr = oracle_pole_plant == 3 ? 1 : nb3;
         for (int side = 0; side < 2; side++) {
             int j = side ? NG + par->n2 : NG;
             for (int i = NG; i <= NG + par->n1; i++) {
                 double avg = 0.;
-                for (int b = 0; b < nb3; b++) {
+                for (int b = 0; b < nbr; b++) {
                     double s = 0.;
-                    for (int zz = 0; zz < par->bs3; zz++) s += dt_factor * E[0 * N + CELL(G, i, j, NG + b * par->bs3 + zz)];
-                    s /= (double)(par->bs3);
-                    if (b == 0) avg = s / ((double)nb3);
-                    else avg += s / ((double)nb3);
+                    for (int zz = 0; zz < zn; zz++) s += dt_factor * E[0 * N + CELL(G, i, j, NG + b * par->bs3 + zz)];
+                    s /= (double)(dn);
+                    if (b == 0) avg = s / ((double)nbr);
+                    else avg += s / ((double)nbr);
                 }
                 for (int z = NG; z <= NG + par->n3; z++) {
                     E[0 * N + CELL(G, i, j, z)] = avg / dt_factor;
