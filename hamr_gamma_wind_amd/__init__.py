@@ -169,6 +169,7 @@ def load_library(path=None):
         "hamr_ct_update": [ctxp, C.c_int, C.c_double, C.c_int],
         "hamr_update": [ctxp, C.c_int, C.c_int, C.c_double],
         "hamr_boundprim": [ctxp, C.c_int],
+        "hamr_invert": [ctxp, C.c_int],
         "hamr_flags": [ctxp, C.c_int, _ip, _ip],
         "hamr_clear_flags": [ctxp],
         "hamr_estimate_dt": [ctxp, _dp],
@@ -486,6 +487,11 @@ class Context:
 
     def update(self, flag, n, Dt):
         self._chk(self.L.hamr_update(self.h, flag, n, Dt), "hamr_update")
+
+    def invert(self, flag):
+        """hamr_invert: the inversion cascade on HAMR_UNEW (flag 0: guess p ->
+        ph; flag 1: guess ph -> p)."""
+        self._chk(self.L.hamr_invert(self.h, flag), "hamr_invert")
 
     def boundprim(self, flag):
         self._chk(self.L.hamr_boundprim(self.h, flag), "hamr_boundprim")

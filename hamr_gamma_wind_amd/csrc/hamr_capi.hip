@@ -1347,6 +1347,18 @@ int hamr_update(hamr_ctx *c, int flag, int n, double Dt)
     return check_launch(c, "hamr_update");
 }
 
+int hamr_invert(hamr_ctx *c, int flag)
+{
+    if (!c) return -1;
+    const Launch Li = all_launch(c, TS_INVERT), Lf = all_launch(c, TS_FALLBACK);
+    double *guess = flag ? c->ph : c->p, *out = flag ? c->p : c->ph;
+    launch_update_part(c->P, 1, Li, flag ? 1 : 0, 0., nullptr, 1.0, guess, guess, out, c->U0, c->Un, nullptr, nullptr,
+                       nullptr, nullptr, c->pflag, c->failcnt, c->prec, c->s);
+    launch_update_part(c->P, 2, Lf, flag ? 1 : 0, 0., nullptr, 1.0, guess, guess, out, c->U0, c->Un, nullptr, nullptr,
+                       nullptr, nullptr, c->pflag, c->failcnt, c->prec, c->s);
+    return check_launch(c, "hamr_invert");
+}
+
 int hamr_boundprim(hamr_ctx *c, int flag)
 {
     if (!c) return -1;

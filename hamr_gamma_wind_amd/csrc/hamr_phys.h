@@ -332,7 +332,7 @@ HD double vsq_calc(double W, double Bsq, double Qtsq, double QdotBsq)
 // GPU_utoprim_2d.h general_newton_raphson + func_vsq + validate_x
 HD int gnr_2d(double *x, const InvPre &s, double gam)
 {
-    double dx[2], x_old[2], errx = 1.;
+    double dx[2], x_old[2], errx = 1., f = 1., df = 1.;
     int n_iter = 0, i_extra = 0, doing_extra = 0, keep = 1;
     const double Bsq = s.Bsq, Qtsq = s.Qtsq, QdotBsq = s.QdotBsq, Qdotn = s.Qdotn, D = s.D;
     while (keep) {
@@ -358,6 +358,8 @@ HD int gnr_2d(double *x, const InvPre &s, double gam)
         dx[0] = -(t2 * t11 + t4 * t18) * t21 * t36;
         t40 = (x[1] + t24) * t3;
         dx[1] = -(-t25 * t11 - 2.0 * t40 * t18) * t21 * t36;
+        df = -t11 * t11 - t18 * t18;  // resid = (t11, t18); f = |resid|^2 / 2 (utoprim_2d.c:635-640)
+        f = -0.5 * df;
         x_old[0] = x[0];
         x_old[1] = x[1];
         x[0] += dx[0];
@@ -372,6 +374,8 @@ HD int gnr_2d(double *x, const InvPre &s, double gam)
         if (((fabs(errx) <= kNEWT_TOL) && (doing_extra == 0)) || (i_extra > kEXTRA_NEWT_ITER) || (n_iter >= (kMAX_NEWT_ITER - 1))) keep = 0;
         n_iter++;
     }
+    // untrapped divergences of the last residual (utoprim_2d.c:511)
+    if ((isfinite(f) == 0) || (isfinite(df) == 0)) return 2;
     if (fabs(errx) > mymin(kNEWT_TOL, kMIN_NEWT_TOL)) return 1;
     return 0;
 }
@@ -434,7 +438,7 @@ HD double W_of_vsq2(double vsq, double &p, double &rho, double &u, double D, dou
 // GPU_utoprim_1dvsq2fix1.h general_newton_raphson2 + func_1d_gnr2 + validate_x2
 HD int gnr_1dvsq2(double *x, const InvPre &s, double K, double gam)
 {
-    double dx, x_old, errx = 1., W = 0., W_old = 0., p, rho, u;
+    double dx, x_old, errx = 1., W = 0., W_old = 0., p, rho, u, f = 1., df = 1.;
     int n_iter = 0, i_extra = 0, doing_extra = 0, keep = 1;
     while (keep) {
         double Wf, Wsq, dWdvsq, fact_tmp, resid, jac;
@@ -445,6 +449,8 @@ HD int gnr_1dvsq2(double *x, const InvPre &s, double K, double gam)
         resid = s.Qtsq - x[0] * fact_tmp * fact_tmp + s.QdotBsq * (s.Bsq + 2. * Wf) / Wsq;
         jac = -fact_tmp * (fact_tmp + 2. * dWdvsq * (x[0] + s.QdotBsq / (Wf * Wsq)));
         dx = -resid / jac;
+        f = 0.5 * resid * resid;
+        df = -2. * f;
         x_old = x[0];
         x[0] += dx;
         x[0] = (x[0] >= 1.0) ? (0.5 * (x_old + 1.)) : x[0];
@@ -459,6 +465,8 @@ HD int gnr_1dvsq2(double *x, const InvPre &s, double K, double gam)
         if (((fabs(errx) <= kNEWT_TOL) && (doing_extra == 0)) || (i_extra > kEXTRA_NEWT_ITER) || (n_iter >= (kMAX_NEWT_ITER - 1))) keep = 0;
         n_iter++;
     }
+    // untrapped divergences of the last residual (utoprim_1dvsq2fix1.c:488)
+    if ((isfinite(f) == 0) || (isfinite(df) == 0)) return 2;
     if (fabs(errx) > mymin(kNEWT_TOL, kMIN_NEWT_TOL)) return 1;
     return 0;
 }

@@ -293,6 +293,13 @@ int hamr_ct_bound(hamr_ctx *ctx, double dt);
 int hamr_ct_update(hamr_ctx *ctx, int flag, double Dt, int n);
 /* GPU_fixup (GPU_main.cu:1740): conserved update + Noble cascade + floors. */
 int hamr_update(hamr_ctx *ctx, int flag, int n, double Dt);
+/* The inversion half of GPU_fixup alone (GPU_fixup.h:349-425; the cascade
+ * of step_ch.c:632-651): the Noble 2D inversion of every interior cell of
+ * every block from the conserved variables in HAMR_UNEW, the entropy-based 1D
+ * v^2 and the 1D inversions where it fails, the floors, the flags and the
+ * failimage counts.  flag 0: guess p, result in ph; flag 1: guess ph, result
+ * in p.  For callers that form U themselves, and to drive the failure paths. */
+int hamr_invert(hamr_ctx *ctx, int flag);
 /* GPU_boundprim (GPU_main.cu:2113): ghost fill of all blocks; flag 0 fills
  * ph/psh, flag 1 fills p/ps. */
 int hamr_boundprim(hamr_ctx *ctx, int flag);

@@ -569,8 +569,24 @@ static void inv_prepare(const double *Ut, const double *gcov, const double *gcon
     s->D = Ut[RHO];
 }
 
+/* The inversions follow the arithmetic of the reference's GPU path (the path
+ * the library replaces): u~^2 from the six packed metric terms
+ * (GPU_utoprim_2d.h:167-172) and fma in func_vsq's pressure terms
+ * (GPU_utoprim_2d.h, ideal-gas branch).  Built with ORACLE_CPU_ARITH (test
+ * infrastructure: liboracle_cpuarith.so) they follow the reference's CPU file
+ * instead: the nine-term double loop (utoprim_2d.c:250-253) and
+ * pressure_W_vsq / dpdvsq_calc without fma (utoprim_2d.c:662-686), so that the
+ * golden vectors of the compiled CPU file pin the restatement to its last bit,
+ * including the inputs on which the two reference paths part ways. */
 static double utsq_of(const double *prim, const double *gcov)
 {
+#ifdef ORACLE_CPU_ARITH
+    static const int pk[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+    double utsq = 0.;
+    for (int i = 1; i < 4; i++)
+        for (int j = 1; j < 4; j++) utsq += gcov[pk[i][j]] * prim[U1 + i - 1] * prim[U1 + j - 1];
+    return utsq;
+#else
     double utsq = gcov[4] * prim[U1] * prim[U1];
     utsq += 2. * gcov[5] * prim[U2] * prim[U1];
     utsq += 2. * gcov[6] * prim[U3] * prim[U1];
@@ -578,6 +594,7 @@ static double utsq_of(const double *prim, const double *gcov)
     utsq += 2 * gcov[8] * prim[U3] * prim[U2];
     utsq += gcov[9] * prim[U3] * prim[U3];
     return utsq;
+#endif
 }
 
 static void set_utcon(const inv_pre *s, double gamma, double W, double *prim)
@@ -601,16 +618,25 @@ static double x1_of_x0(double x0, double Bsq, double Qtsq, double QdotBsq)
     return ((vsq > 1.) ? (1.0 - 1.e-15) : vsq);
 }
 
-/* GPU_utoprim_2d.h func_vsq (ideal gas) */
-static void func_vsq(const double *x, double *dx, const inv_pre *s, double gam)
+/* GPU_utoprim_2d.h func_vsq (ideal gas); *df = -|resid|^2 and *f = -df/2
+ * (utoprim_2d.c:635-640) */
+static void func_vsq(const double *x, double *dx, const inv_pre *s, double gam, double *f, double *df)
 {
     double Wsq, p_tmp, dPdvsq, dPdW, gtmp, t11, t16, t18, t2, t21, t23, t24, t25, t3, t35, t36, t4, t40, t9;
     double Bsq = s->Bsq, Qtsq = s->Qtsq, QdotBsq = s->QdotBsq, Qdotn = s->Qdotn, D = s->D;
     Wsq = x[0] * x[0];
     gtmp = 1. - x[1];
+#ifdef ORACLE_CPU_ARITH
+    p_tmp = (gam - 1.) * (x[0] * gtmp - D * sqrt(gtmp)) / gam;
+#else
     p_tmp = (gam - 1.) * (fma(x[0], gtmp, -D * sqrt(gtmp))) / gam;
+#endif
     dPdW = (gam - 1.) * (1. - x[1]) / gam;
+    #ifdef ORACLE_CPU_ARITH
+    dPdvsq = (gam - 1.) * (0.5 * D / sqrt(1. - x[1]) - x[0]) / gam;
+#else
     dPdvsq = (gam - 1.) * (fma(0.5, D / sqrt(1. - x[1]), -x[0])) / gam;
+#endif
     t2 = -0.5 * Bsq + dPdvsq;
     t3 = Bsq + x[0];
     t4 = t3 * t3;
@@ -627,17 +653,19 @@ static void func_vsq(const double *x, double *dx, const inv_pre *s, double gam)
     dx[0] = -(t2 * t11 + t4 * t18) * t21 * t36;
     t40 = (x[1] + t24) * t3;
     dx[1] = -(-t25 * t11 - 2.0 * t40 * t18) * t21 * t36;
+    *df = -t11 * t11 - t18 * t18;
+    *f = -0.5 * (*df);
 }
 
 /* GPU_utoprim_2d.h general_newton_raphson */
 static int gnr_2d(double *x, const inv_pre *s, double gam, double tol)
 {
-    double dx[2], x_old[2], errx = 1.;
+    double dx[2], x_old[2], errx = 1., f = 1., df = 1.;
     int n_iter = 0, i_extra = 0, doing_extra = 0, keep = 1;
     x_old[0] = x[0];
     x_old[1] = x[1];
     while (keep) {
-        func_vsq(x, dx, s, gam);
+        func_vsq(x, dx, s, gam, &f, &df);
         x_old[0] = x[0];
         x_old[1] = x[1];
         x[0] += dx[0];
@@ -653,6 +681,8 @@ static int gnr_2d(double *x, const inv_pre *s, double gam, double tol)
         if (((fabs(errx) <= tol) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) || (n_iter >= (MAX_NEWT_ITER - 1))) keep = 0;
         n_iter++;
     }
+    /* untrapped divergences of the last residual (utoprim_2d.c:511) */
+    if ((isfinite(f) == 0) || (isfinite(df) == 0)) return 2;
     if (fabs(errx) > MYMIN(tol, MIN_NEWT_TOL)) return 1;
     return 0;
 }
@@ -713,7 +743,7 @@ static double W_of_vsq2(double vsq, double *p, double *rho, double *u, double D,
 
 static int gnr_1dvsq2(double *x, const inv_pre *s, double K, double gam, double tol)
 {
-    double dx, x_old, errx = 1., W = 0., W_old = 0., p, rho, u;
+    double dx, x_old, errx = 1., W = 0., W_old = 0., p, rho, u, f = 1., df = 1.;
     int n_iter = 0, i_extra = 0, doing_extra = 0, keep = 1;
     while (keep) {
         /* func_1d_gnr2 */
@@ -725,6 +755,8 @@ static int gnr_1dvsq2(double *x, const inv_pre *s, double K, double gam, double 
         resid = s->Qtsq - x[0] * fact_tmp * fact_tmp + s->QdotBsq * (s->Bsq + 2. * Wf) / Wsq;
         jac = -fact_tmp * (fact_tmp + 2. * dWdvsq * (x[0] + s->QdotBsq / (Wf * Wsq)));
         dx = -resid / jac;
+        f = 0.5 * resid * resid;
+        df = -2. * f;
         x_old = x[0];
         x[0] += dx;
         /* validate_x2 */
@@ -740,6 +772,8 @@ static int gnr_1dvsq2(double *x, const inv_pre *s, double K, double gam, double 
         if (((fabs(errx) <= tol) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) || (n_iter >= (MAX_NEWT_ITER - 1))) keep = 0;
         n_iter++;
     }
+    /* untrapped divergences of the last residual (utoprim_1dvsq2fix1.c:488) */
+    if ((isfinite(f) == 0) || (isfinite(df) == 0)) return 2;
     if (fabs(errx) > MYMIN(tol, MIN_NEWT_TOL)) return 1;
     return 0;
 }
@@ -1328,6 +1362,75 @@ void oracle_cons(const oracle_params *par, double Dt, const double *pi, double *
             }
 }
 
+/* The inversion cascade of one cell (GPU_fixup.h:349-425, step_ch.c:632-651):
+ * Noble 2D from the guess in pf, then the entropy-based 1D v^2 inversion, then
+ * the 1D inversion, each from what the previous one left in pf; then the
+ * floors (the library's own specification; the reference's fixup_cell is a
+ * no-op) with flag -333.  fc[0..3] count 2D failures, 1D v^2 failures, 1D
+ * failures and floor hits.  Returns the cell's flag. */
+static int invert_cell(const oracle_params *par, const double *U, const double *gcov, const double *gcon, double gd,
+                       double r, double *pf, long long fc[4])
+{
+    const double gam = par->gam;
+    int flag = oracle_utoprim(0, U, gcov, gcon, gd, gam, pf);
+    if (flag) {
+        fc[0]++;
+        flag = oracle_utoprim(1, U, gcov, gcon, gd, gam, pf);
+        if (flag) {
+            fc[1]++;
+            flag = oracle_utoprim(2, U, gcov, gcon, gd, gam, pf);
+            if (flag) fc[2]++;
+        }
+    }
+    double sr = sqrt(r);
+    int hit = 0;
+    if (par->rho_floor > 0.) {
+        double rhoflr = par->rho_floor / (r * sr);
+        if (pf[RHO] < rhoflr) { pf[RHO] = rhoflr; hit = 1; }
+    }
+    if (par->u_floor > 0.) {
+        double uflr = par->u_floor / (r * r * sr);
+        if (pf[UU] < uflr) { pf[UU] = uflr; hit = 1; }
+    }
+    if (hit) { flag = -333; fc[3]++; }
+    return flag;
+}
+
+/* The inversion half of the fixup kernel alone over the interior: the cascade
+ * of invert_cell on the conserved variables Unew with the guess pi (full = 0)
+ * or pb (full = 1); result into pf_out and pflag. */
+void oracle_invert(const oracle_params *par, int full, const double *pi, const double *pb, double *pf_out,
+                   const double *Unew, const oracle_metric *m, int *pflag, long long *failcount)
+{
+    ogrid G = mkgrid(par);
+    const int NG = ORC_NG;
+    const long N = G.ncell;
+    long long fc0 = 0, fc1 = 0, fc2 = 0, fc3 = 0;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4) reduction(+ : fc0, fc1, fc2, fc3)
+    for (int i = NG; i < NG + par->n1; i++)
+        for (int j = NG; j < NG + par->n2; j++)
+            for (int z = NG; z < NG + par->n3; z++) {
+                long g = CELL(G, i, j, z), col = COL(G, i, j);
+                const double *gcov, *gcon, *guess = full ? pb : pi;
+                double gd, pf[ORC_NPR], U[ORC_NPR];
+                long long fc[4] = {0, 0, 0, 0};
+                geom_at(m, &G, CENT, col, &gcov, &gcon, &gd);
+                for (int k = 0; k < ORC_NPR; k++) {
+                    U[k] = Unew[(long)k * N + g];
+                    pf[k] = guess[(long)k * N + g];
+                }
+                pflag[g] = invert_cell(par, U, gcov, gcon, gd, m->rcen[col], pf, fc);
+                for (int k = 0; k < ORC_NPR; k++) pf_out[(long)k * N + g] = pf[k];
+                fc0 += fc[0]; fc1 += fc[1]; fc2 += fc[2]; fc3 += fc[3];
+            }
+    if (failcount) {
+        failcount[0] += fc0;
+        failcount[1] += fc1;
+        failcount[2] += fc2;
+        failcount[3] += fc3;
+    }
+}
+
 /* fixup kernel, GPU_fixup.h:5-425: conserved update, source, face-B average,
  * Noble cascade, floors and flags. */
 void oracle_update(const oracle_params *par, int full, double Dt, const double *pi, const double *pb,
@@ -1347,32 +1450,11 @@ void oracle_update(const oracle_params *par, int full, double Dt, const double *
                 const double *gcov, *gcon;
                 double gd, pf[ORC_NPR], U[ORC_NPR];
                 int k, flag;
+                long long fc[4] = {0, 0, 0, 0};
                 geom_at(m, &G, CENT, col, &gcov, &gcon, &gd);
                 cons_cell(par, &G, full, Dt, g, col, pi, pb, U0, psf, F1, F2, F3, m, pf, U);
-                flag = oracle_utoprim(0, U, gcov, gcon, gd, gam, pf);
-                if (flag) {
-                    fc0++;
-                    flag = oracle_utoprim(1, U, gcov, gcon, gd, gam, pf);
-                    if (flag) {
-                        fc1++;
-                        flag = oracle_utoprim(2, U, gcov, gcon, gd, gam, pf);
-                        if (flag) fc2++;
-                    }
-                }
-                /* floors (the library's own specification; reference fixup_cell is a no-op) */
-                {
-                    double r = m->rcen[col], sr = sqrt(r);
-                    int hit = 0;
-                    if (par->rho_floor > 0.) {
-                        double rhoflr = par->rho_floor / (r * sr);
-                        if (pf[RHO] < rhoflr) { pf[RHO] = rhoflr; hit = 1; }
-                    }
-                    if (par->u_floor > 0.) {
-                        double uflr = par->u_floor / (r * r * sr);
-                        if (pf[UU] < uflr) { pf[UU] = uflr; hit = 1; }
-                    }
-                    if (hit) { flag = -333; fc3++; }
-                }
+                flag = invert_cell(par, U, gcov, gcon, gd, m->rcen[col], pf, fc);
+                fc0 += fc[0]; fc1 += fc[1]; fc2 += fc[2]; fc3 += fc[3];
                 pflag[g] = flag;
                 for (k = 0; k < ORC_NPR; k++) pf_out[(long)k * N + g] = pf[k];
             }

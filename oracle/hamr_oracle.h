@@ -99,6 +99,10 @@ void oracle_update(const oracle_params *par, int full, double Dt, const double *
                    const double *pb, double *pf, double *U0, const double *psf,
                    const double *F1, const double *F2, const double *F3,
                    const oracle_metric *m, int *pflag, long long *failcount);
+/* The inversion cascade, floors and flags alone over the interior, on the
+ * conserved variables Unew with the guess pi (full = 0) or pb (full = 1). */
+void oracle_invert(const oracle_params *par, int full, const double *pi, const double *pb, double *pf_out,
+                   const double *Unew, const oracle_metric *m, int *pflag, long long *failcount);
 void oracle_bound(const oracle_params *par, double *p, double *ps, const oracle_metric *m);
 /* Per-block harmonic dt from the three dtface arrays, minimum over blocks. */
 double oracle_block_dt(const oracle_params *par, const double *dt1, const double *dt2,

@@ -61,6 +61,7 @@ class Oracle:
         L.oracle_update.argtypes = [P, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, M,
                                     _ip, C.POINTER(C.c_longlong)]
         L.oracle_cons.argtypes = [P, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, M]
+        L.oracle_invert.argtypes = [P, C.c_int, _dp, _dp, _dp, _dp, M, _ip, C.POINTER(C.c_longlong)]
         L.oracle_bound.argtypes = [P, _dp, _dp, M]
         L.oracle_block_dt.argtypes = [P, _dp, _dp, _dp]
         L.oracle_block_dt.restype = C.c_double
@@ -216,6 +217,18 @@ class Grid:
                                  self.failcount)
         return pflag
 
+    def invert(self, full, guess, Unew):
+        """The inversion cascade, floors and flags over the interior on Unew
+        from the guess: (primitives, pflag); failure counts added to
+        self.failcount."""
+        pf = np.array(guess, dtype=float, copy=True)
+        pflag = np.zeros(self.D, dtype=np.int32)
+        g = np.ascontiguousarray(guess, dtype=float)
+        self.o.lib.oracle_invert(C.byref(self.par), full, _ptr(g), _ptr(g), _ptr(pf),
+                                 _ptr(np.ascontiguousarray(Unew, dtype=float)), C.byref(self.metric),
+                                 _ptr(pflag, _ip), self.failcount)
+        return pf, pflag
+
     def cons(self, Dt, pi, psf, F1, F2, F3):
         U0 = self.zeros(NPR)
         Un = self.zeros(NPR)
@@ -240,6 +253,16 @@ def load():
     if not os.path.exists(LIB):
         raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
     return Oracle(C.CDLL(LIB))
+
+
+def load_cpu_arith():
+    """The oracle built with the inversions in the arithmetic of the
+    reference's CPU file (ORACLE_CPU_ARITH): for pinning against the golden
+    vectors of that file to the last bit."""
+    path = os.path.join(os.path.dirname(LIB), "liboracle_cpuarith.so")
+    if not os.path.exists(path):
+        raise RuntimeError("oracle/liboracle_cpuarith.so is not built (run __graft_entry__.build())")
+    return Oracle(C.CDLL(path))
 
 
 # ---- multi-level block sets ----

@@ -1,0 +1,105 @@
+"""The failure paths of the inversion cascade on the GPU (hamr_invert) against
+the oracle, cell for cell.
+
+Workload: the 64x32x32 torus (as 4x4x4 blocks of 16x8x8; Kerr a = 0.9375,
+Gamma = 13/9, floors 1e-5 r^-1.5 and 1e-7 r^-2.5).  The conserved variables
+of every interior cell are U(P) with the face-averaged field (the oracle's
+conserved update with Dt = 0); in 4096 seeded cells they are distorted the way
+the crafted failure fixtures are (total energy scaled by a factor in [-3, 3],
+momentum or field scaled by up to 1e4, the guess's 4-velocity inflated by up
+to 1e11, a non-positive density), so that the 2D inversion fails (codes 2, 6,
+101, 201, -100) and the entropy-based 1D v^2 and the 1D inversions run on the
+failed cells.
+
+Bars: pflag of every interior cell, the four failimage counts and the
+primitives bit-exact (NaN matching NaN) against the oracle's cascade
+(oracle_invert) on the same U and guess.  Coverage: the final flags include
+-333 (floors), -100, 2 and 101 (all three inversions failed), and cells end on
+each fallback (2D failed, 1D v^2 failed, 1D failed).
+"""
+import numpy as np
+import pytest
+
+import hamr_gamma_wind_amd as H
+from hamr_gamma_wind_amd import workloads as W
+from tests import gridtools as GT
+
+pytestmark = pytest.mark.gpu
+
+NG = 3
+NPLANT = 4096
+
+
+def _planted(grid, desc, P, PS, seed=17):
+    """(guess, U) global arrays with NPLANT distorted interior cells."""
+    Z = grid.zeros(9)
+    U = grid.cons(0.0, P, PS, Z, Z, Z)
+    guess = P.copy()
+    rng = np.random.default_rng(seed)
+    # cells inside the torus (rho > 1e-3, far above the floors, which would
+    # overwrite the flag with -333)
+    I = (slice(NG, -NG),) * 3
+    cand = np.flatnonzero(P[0][I] > 1e-3)
+    flat = rng.choice(cand, size=min(NPLANT, cand.size), replace=False)
+    n = [desc.bs[k] * desc.nb[k] for k in range(3)]
+    ii, jj, zz = np.unravel_index(flat, n)
+    ii, jj, zz = ii + NG, jj + NG, zz + NG
+    mode = rng.integers(0, 6, size=flat.size)
+    for c in range(flat.size):
+        i, j, z, m = ii[c], jj[c], zz[c], mode[c]
+        u = U[:, i, j, z]
+        if m == 0:
+            u[1] = u[0] + rng.uniform(-3.0, 3.0) * (u[1] - u[0])
+        elif m == 1:
+            u[2:5] *= np.exp(rng.uniform(0.0, np.log(1e4)))
+        elif m == 2:
+            u[5:8] *= np.exp(rng.uniform(0.0, np.log(1e4)))
+        elif m == 3:
+            guess[2:5, i, j, z] *= np.exp(rng.uniform(0.0, np.log(1e11)))
+        elif m == 4:
+            u[0] = -abs(u[0]) * rng.uniform()
+        else:
+            u[1] = u[0] + rng.uniform(-2.0, 2.0) * (u[1] - u[0])
+            u[2:5] *= np.exp(rng.uniform(0.0, np.log(1e3)))
+        U[:, i, j, z] = u
+    return guess, U
+
+
+def test_inversion_failure_paths_match_oracle(oracle):
+    bs, nb = W.TORUS_64
+    desc = H.make_desc(bs, nb)
+    ctx = H.Context(desc)
+    try:
+        ctx.set_grid()
+        ctx.init_torus(H.torus_desc())
+        ctx.boundprim(1)
+        grid = GT.oracle_grid(oracle, desc)
+        P = GT.blocks_to_global(desc, ctx.get(H.HAMR_P))
+        PS = GT.blocks_to_global(desc, ctx.get(H.HAMR_PS))
+        grid.bound(P, PS)
+        guess, U = _planted(grid, desc, P, PS)
+        for k in range(4):
+            grid.failcount[k] = 0
+        want_p, want_flag = grid.invert(0, guess, U)
+        want_fc = [int(grid.failcount[k]) for k in range(4)]
+
+        print("planted cells: %d; device run: hamr_invert(0)" % NPLANT, flush=True)
+        ctx.set(H.HAMR_P, GT.global_to_blocks(desc, guess))
+        ctx.set(H.HAMR_UNEW, GT.global_to_blocks(desc, U))
+        ctx.clear_flags()
+        ctx.invert(0)
+        got_p = ctx.get(H.HAMR_PH)
+        pf, fi = ctx.all_flags()
+        I = slice(NG, -NG)
+        wf = GT.block_view(desc, want_flag)
+        codes, counts = np.unique(pf[:, I, I, I], return_counts=True)
+        print("flags:", dict(zip(codes.tolist(), counts.tolist())), "failimage", fi.sum(axis=0).tolist(),
+              "oracle", want_fc, flush=True)
+        assert np.array_equal(pf[:, I, I, I], wf[:, I, I, I])
+        assert fi.sum(axis=0).tolist() == want_fc
+        wp = GT.global_to_blocks(desc, want_p)
+        assert np.array_equal(got_p[:, :, I, I, I], wp[:, :, I, I, I], equal_nan=True)
+        assert {-333, -100, 2, 101} <= set(codes.tolist())
+        assert want_fc[0] > want_fc[1] > 0 and want_fc[2] > 0
+    finally:
+        ctx.close()

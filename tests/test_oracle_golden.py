@@ -170,3 +170,89 @@ def test_ppm_mutation_is_caught(oracle):
     y = [np.exp(0.1 * x) for x in (-2, -1, 0, 1, 2)]
     l, _ = oracle.para(y)
     assert abs(l - 0.5 * (y[1] + y[2])) > 1e-5
+
+
+@pytest.fixture(scope="module")
+def failures():
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ref_failures.json")) as fh:
+        return json.load(fh)
+
+
+@pytest.mark.parametrize("which,key", [(0, "2d"), (1, "1dvsq"), (2, "1dfix")])
+def test_inversions_in_cpu_arithmetic_match_reference_bitwise(failures, gold, which, key):
+    """With the reference CPU file's arithmetic (u~^2 as its nine-term loop, no
+    fma in func_vsq; ORACLE_CPU_ARITH) the restatement reproduces every return
+    code of the compiled reference on the regular and the crafted failure
+    cases, and its primitives to 1e-13 (the remaining differences: libm's pow
+    and sqrt against the same calls)."""
+    from tests import oracle_ctypes as O
+
+    cpu = O.load_cpu_arith()
+    worst = 0.0
+    for data in (gold, failures):
+        gam = data["gamma"]
+        for c in data["cases"]:
+            ret, prim = cpu.utoprim(which, c["U"], c["gcov"], c["gcon"], c["gdet"][0], gam, c["guess"])
+            assert ret == c["ret" + key], (which, ret, c["ret" + key])
+            ref = np.array(c["prim" + key])
+            if ret == 0:
+                worst = max(worst, relerr(prim[:5], ref[:5]))
+            elif ret != -100:
+                assert np.array_equal(prim[:5], ref[:5])
+    print("worst relative primitive error %.2e" % worst)
+    assert worst < 1e-13
+
+
+@pytest.mark.parametrize("which,key", [(0, "2d"), (1, "1dvsq"), (2, "1dfix")])
+def test_inversion_failure_cases_match_reference(oracle, failures, which, key):
+    """Crafted conserved states and guesses (oracle/ref/gen_golden.c failures:
+    energy, momentum, field or density distorted, guesses with huge Lorentz
+    factors) on which the reference's own Utoprim_2d, Utoprim_1dvsq2fix1 and
+    Utoprim_1dfix1 return their failure codes: the oracle returns the same
+    code and leaves the same primitives.  Codes covered for the 2D inversion:
+    0, 2 (guess's u^2 too big), 6 (negative u), 101 (Newton not converged),
+    201 (non-finite residual), -100 (D <= 0); cascades 2D fail -> 1D v^2 ok,
+    2D and 1D v^2 fail -> 1D ok, all three fail.  Codes 1, 3, 4 and 5 of the
+    2D inversion (utoprim_2d.c:308-356) are not reachable from finite inputs
+    in 400,000 draws: validate_x (utoprim_2d.c:417-427) clamps v^2 below 1
+    (code 4) and keeps W below W_TOO_BIG = 1e20 unless the start exceeds it,
+    which the Newton steps then leave (code 3); rho0 = D sqrt(1 - v^2) > 0
+    when D > 0 and v^2 < 1 (code 5); code 1 needs W == FAIL_VAL = 1e30.
+    One crafted input (a guess with Lorentz factor ~1e8) sits where the
+    reference's CPU and GPU paths part ways (the CPU file's residual
+    overflows, code 201; with the GPU path's u~^2 order and fma the 2D
+    Newton converges): the oracle, which follows the GPU path, is checked
+    there against the CPU-arithmetic build instead."""
+    from tests import oracle_ctypes as O
+
+    cpu = O.load_cpu_arith()
+    gam = failures["gamma"]
+    codes = set()
+    differ = 0
+    for c in failures["cases"]:
+        ret, prim = oracle.utoprim(which, c["U"], c["gcov"], c["gcon"], c["gdet"][0], gam, c["guess"])
+        codes.add(ret)
+        if ret != c["ret" + key]:
+            # the reference's two paths part ways on this input: the CPU file's
+            # arithmetic reproduces the golden code (checked bitwise in
+            # test_inversions_in_cpu_arithmetic_match_reference_bitwise), the
+            # GPU path's arithmetic, which the library follows, does not
+            assert cpu.utoprim(which, c["U"], c["gcov"], c["gcon"], c["gdet"][0], gam, c["guess"])[0] == \
+                c["ret" + key]
+            differ += 1
+            continue
+        ref = np.array(c["prim" + key])
+        if ret == 0:
+            assert relerr(prim[:5], ref[:5]) < 1e-10, (prim, ref)
+        else:
+            assert np.array_equal(prim[:5], ref[:5]), (prim, ref)
+        if ret != -100:
+            assert relerr(prim[5:8], ref[5:8], np.max(np.abs(ref[5:8])) + 1e-300) < 1e-14
+    assert differ <= 1, differ
+    if which == 0:
+        assert {0, 2, 6, 101, 201, -100} <= codes, codes
+    paths = {(c["ret2d"] != 0, c["ret1dvsq"] != 0, c["ret1dfix"] != 0) for c in failures["cases"]}
+    assert {(True, False, False), (True, True, False), (True, True, True)} <= paths
