@@ -562,18 +562,30 @@ class Context:
             for n in (range(self.nblk) if order is None else order):
                 self.dump_block_write(int(n)).tofile(fh)
 
+    def block_numbers(self):
+        """The reference's number of each block (AMR_set_coord, AMR.c:168-275):
+        number[n] of this context's block n (the inverse of hamr_block_order)."""
+        order = np.zeros(self.nblk, dtype=np.int32)
+        self._chk(self.L.hamr_block_order(self.h, _ptr(order, _ip)), "hamr_block_order")
+        number = np.empty_like(order)
+        number[order] = np.arange(self.nblk, dtype=np.int32)
+        return number
+
     def rdump_write(self, directory):
-        """restart_write's block files (restart.c:40-45): directory/rdump<n>
-        holds block n's record as raw doubles."""
+        """restart_write's block files (restart.c:40-45): directory/rdump<k>
+        holds the record, raw doubles, of the block the reference numbers k."""
         os.makedirs(directory, exist_ok=True)
+        number = self.block_numbers()
         for n in range(self.nblk):
-            self.rdump_block_write(n).tofile(os.path.join(directory, "rdump%d" % n))
+            self.rdump_block_write(n).tofile(os.path.join(directory, "rdump%d" % number[n]))
 
     def rdump_read(self, directory, red=(1, 1, 1)):
-        """restart_read (restart.c:94-175): every block from directory/rdump<n>
-        (records of blocks red times finer), then the ghost fill."""
+        """restart_read (restart.c:94-175): every block from directory/rdump<k>,
+        k its reference number (records of blocks red times finer), then the
+        ghost fill."""
+        number = self.block_numbers()
         for n in range(self.nblk):
-            path = os.path.join(directory, "rdump%d" % n)
+            path = os.path.join(directory, "rdump%d" % number[n])
             if not os.path.exists(path):
                 raise HamrError("cannot open restart file %s" % path)
             self.rdump_block_read(n, np.fromfile(path, dtype=np.float64), red)

@@ -435,6 +435,7 @@ int hamr_create(const hamr_desc *desc, hamr_ctx **out)
     int ngeo = 0;
     c->blocks = build_block_table(d, &ngeo);
     c->order = zorder(d);
+    if ((int)c->order.size() != (int)c->blocks.size()) return fail(c, "hamr_create: block numbering failed");
     const Layout &L = c->L;
     c->cells = (int64_t)L.nblk * L.ksize;
     c->cols = (int64_t)L.ngeo * L.kcol;
@@ -639,6 +640,7 @@ int hamr_desc_block_table(const hamr_desc *desc, int *rows, int *order)
     }
     if (order) {
         std::vector<int> z = zorder(d);
+        if (z.size() != t.size()) return -2;
         std::memcpy(order, z.data(), z.size() * sizeof(int));
     }
     return 0;

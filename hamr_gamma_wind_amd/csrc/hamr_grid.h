@@ -106,7 +106,9 @@ Layout make_layout(const GridDesc &d);
 // Neighbours: x1 faces outflow at the radial ends, x2 faces at the axis use the
 // pi-shifted partner block (c3 + NB3/2) with the polar mirror, x3 periodic.
 std::vector<BlockDev> build_block_table(const GridDesc &d, int *ngeo_out);
-// Z-order (Morton) traversal of the base-level blocks, the launch order.
+// The reference's block numbering (AMR_set_coord's level-0 Z-order,
+// AMR.c:188-275): element k is the natural index of the block the reference
+// numbers k; also the launch order.  Empty if the numbering is inconsistent.
 std::vector<int> zorder(const GridDesc &d);
 
 // Host metric generation for one geometry slab (block column c1, c2).

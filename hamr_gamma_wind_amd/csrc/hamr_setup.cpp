@@ -92,25 +92,56 @@ std::vector<BlockDev> build_block_table(const GridDesc &d, int *ngeo_out)
     return t;
 }
 
-static uint64_t spread3(uint64_t v)
+// The reference's block numbering of a single-level grid: the level-0 Z-order
+// of AMR_set_coord (AMR.c:188-275).  Block number k's coordinates follow from
+// k alone: walking the power-of-two scales p = 2^s from the largest down, the
+// counter is compared with the volume of the (clipped) p-cube at the current
+// corner, and each of the seven sub-cube moves of that scale (+x3, +x2, +x2x3,
+// +x1, +x1x3, +x1x2, +x1x2x3, each undoing the earlier moves it replaces) is
+// taken when the counter covers it, subtracting the blocks it skips.  A move
+// along a direction is possible only when a full p-step fits before the last
+// block along it.  The steps and their order are the reference's, so that the
+// numbers (and the rdump<k> file names they give) are the same.
+static void ref_block_coords(int k, const int nb[3], int c[3])
 {
-    uint64_t r = 0;
-    for (int b = 0; b < 21; b++) r |= ((v >> b) & 1ULL) << (3 * b);
-    return r;
+    const int maxs = (int)(log((double)std::max(nb[0], std::max(nb[1], nb[2]))) / log(2.));
+    int cnt = k;
+    c[0] = c[1] = c[2] = 0;
+    for (int s = maxs; s >= 0; s--) {
+        const int p = 1 << s;
+        const int in1 = std::min(p, nb[0] - c[0] - 1), in2 = std::min(p, nb[1] - c[1] - 1),
+                  in3 = std::min(p, nb[2] - c[2] - 1);
+        const bool f1 = in1 == p, f2 = in2 == p, f3 = in3 == p;
+        auto w = [&](int d) { return std::min(p, nb[d] - c[d]); };
+        auto cube = [&]() { return w(2) * w(1) * w(0); };
+        if (f3 && cnt >= cube()) { cnt -= in3 * w(1) * w(0); c[2] += in3; }
+        if (f2 && cnt >= cube()) { cnt -= w(2) * in2 * w(0); if (f3) c[2] -= in3; c[1] += in2; }
+        if (f3 && f2 && cnt >= cube()) { cnt -= in3 * w(1) * w(0); c[2] += in3; }
+        if (f1 && cnt >= cube()) {
+            cnt -= w(2) * w(1) * in1;
+            if (f3) c[2] -= in3;
+            if (f2) c[1] -= in2;
+            c[0] += in1;
+        }
+        if (f1 && f3 && cnt >= cube()) { cnt -= in3 * w(1) * w(0); c[2] += in3; }
+        if (f1 && f2 && cnt >= cube()) { cnt -= w(2) * in2 * w(0); if (f3) c[2] -= in3; c[1] += in2; }
+        if (f1 && f2 && f3 && cnt >= cube()) { cnt -= in3 * w(1) * w(0); c[2] += in3; }
+    }
 }
 
 std::vector<int> zorder(const GridDesc &d)
 {
-    std::vector<std::pair<uint64_t, int>> keys;
-    for (int c1 = 0; c1 < d.nb[0]; c1++)
-        for (int c2 = 0; c2 < d.nb[1]; c2++)
-            for (int c3 = 0; c3 < d.nb[2]; c3++) {
-                uint64_t k = (spread3(c1) << 2) | (spread3(c2) << 1) | spread3(c3);
-                keys.push_back({k, (c1 * d.nb[1] + c2) * d.nb[2] + c3});
-            }
-    std::sort(keys.begin(), keys.end());
-    std::vector<int> out;
-    for (auto &k : keys) out.push_back(k.second);
+    const int nblk = d.nb[0] * d.nb[1] * d.nb[2];
+    std::vector<int> out(nblk, -1), seen(nblk, 0);
+    for (int k = 0; k < nblk; k++) {
+        int c[3];
+        ref_block_coords(k, d.nb, c);
+        for (int a = 0; a < 3; a++)
+            if (c[a] < 0 || c[a] >= d.nb[a]) return {};
+        const int n = (c[0] * d.nb[1] + c[1]) * d.nb[2] + c[2];
+        if (seen[n]++) return {};
+        out[k] = n;
+    }
     return out;
 }
 

@@ -99,7 +99,9 @@ enum { HAMR_RIEMANN_HLL = 0, HAMR_RIEMANN_HLLC = 1 };
  * columns per slab and padded length, block and geometry counts. */
 int hamr_desc_layout(const hamr_desc *desc, int *D, long long *ncell, long long *ksize, long long *ncol,
                      long long *kcol, int *nblk, int *ngeo);
-/* Block table rows (17 ints per block, as hamr_block_row) and the Z-order. */
+/* Block table rows (17 ints per block, as hamr_block_row) and the
+ * reference's block numbering (as hamr_block_order); -2 if that numbering
+ * cannot be formed. */
 int hamr_desc_block_table(const hamr_desc *desc, int *rows, int *order);
 /* Grid coordinates startx[3], dx[3] (set_points / set_gridparam, coord.c:327-460). */
 int hamr_desc_coords(const hamr_desc *desc, double *startx, double *dx);
@@ -192,7 +194,11 @@ int hamr_layout(const hamr_ctx *ctx, int *D, long long *ncell, long long *ksize,
  * faces -x1 +x1 -x2 +x2 -x3 +x3 (-1: physical outflow edge), [11..16] face kinds
  * (0 copy, 1 outflow, 2 polar mirror). */
 int hamr_block_row(const hamr_ctx *ctx, int n, int *row17);
-/* Base-level Z-order of the blocks (n_ord). */
+/* The reference's block numbering (AMR_set_coord's level-0 Z-order,
+ * AMR.c:188-275; n_ord of a single-level grid): order[k] is this context's
+ * block (natural index (c1*NB2 + c2)*NB3 + c3) that the reference numbers k,
+ * the number its restart file rdump<k> carries (restart.c:41-42).  Also the
+ * launch order.  On a multi-level set: the identity. */
 int hamr_block_order(const hamr_ctx *ctx, int *order);
 /* set_grid (main.c:380) + GPU_write_metric (GPU_main.cu:139): compute the
  * Kerr-Schild metric in code coordinates on the host and upload it. */

@@ -26,6 +26,27 @@ def unpack(v):
     return g
 
 
+def test_block_numbering_equals_reference():
+    """The reference's own block numbering (AMR_set_coord, AMR.c:168-275, the
+    numbers of the rdump<k> restart files), written by the compiled AMR.c for
+    five block counts (oracle/ref/gen_order.c): hamr_desc_block_table's order
+    names the same block for every number k."""
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ref_block_order.json")) as fh:
+        geoms = json.load(fh)
+    assert {tuple(g["nb"]) for g in geoms} >= {(8, 4, 4), (12, 6, 6), (3, 4, 1)}
+    for g in geoms:
+        nb = tuple(g["nb"])
+        assert g["levels"] == 1 and g["n_max"] == nb[0] * nb[1] * nb[2] - 1
+        desc = H.make_desc((16, 16, 16), nb)
+        rows, order = H.desc_block_table(desc)
+        for k, (lev, c1, c2, c3) in enumerate(g["coords"]):
+            assert lev == 0
+            assert tuple(rows[order[k], 1:4]) == (c1, c2, c3), (nb, k)
+
+
 @pytest.fixture(scope="module")
 def gold():
     with open(GOLD) as f:
