@@ -1,33 +1,33 @@
 """Throughput of the MI355X GRMHD fluid step (zone-updates/s) on one GPU.
 
-Workload (default): the torus on a single-level 256x128x128 grid (r 1.2-1000 M
-log-spaced, Kerr a = 0.9375, Gamma = 13/9, FM torus r_in = 6, r_max = 12,
-poloidal loop normalised to min beta = 100, seeded +-4% perturbation of u,
-floors 1e-5 r^-1.5 and 1e-7 r^-2.5) as 8x4x4 blocks of 32^3 interior zones
-plus 3 ghost zones, all resident in HBM.  A step is one predictor + corrector
-over every block (PPM + HLL + CT + Noble 2D + ghost fill, Courant dt kept on the
+Workload of `value`: the mixed-precision step on the torus on a single-level
+384x192x192 grid (r 1.2-1000 M log-spaced, Kerr a = 0.9375, Gamma = 13/9, FM
+torus r_in = 6, r_max = 12, poloidal loop normalised to min beta = 100, seeded
++-4% perturbation of u, floors 1e-5 r^-1.5 and 1e-7 r^-2.5) as 12x6x6 blocks
+of 32^3 interior zones plus 3 ghost zones, all resident in HBM: FP32
+reconstruction, fluxes and EMFs; FP64 metric, face-field update, compensated
+conserved update and inversions.  A step is one predictor + corrector over
+every block (PPM + HLL + CT + Noble 2D + ghost fill, Courant dt kept on the
 device), replayed from a HIP graph.  A zone-update is one interior zone
 advanced one full step.
 
-Both arithmetics run from the same initial state for warmup + steps steps:
-the FP64 step and the mixed step (FP32 reconstruction, fluxes and EMFs; FP64
-metric, face-field update, compensated conserved update and inversions).
-`value` is the mixed step's throughput; the FP64 throughput, the speed-up and
-the error of the mixed state against the FP64 state after the same number of
-steps (max and L1, relative, for rho and the field) are reported beside it.
+Beside it, on the same grid and state: the FP64 step (the reference's
+arithmetic) and the speed-up; the error of the mixed state against the FP64
+state after 100 steps whatever --steps is (both advancing with the FP64 run's
+dt sequence; per variable max and L1, and the same figures for an FP64 run
+from the FP32-rounded initial state, the flow's own response to perturbations
+of FP32 size).  On the 256x128x128 grid (8x4x4 blocks of 32^3): the FP64
+batched step and the reference-structured FP64 step (the C host's per-block
+call sequence with a host dt read-back per block), and the FP64 step with
+HLLC fluxes.  Also the multi-level torus (3 levels, 722 blocks of uneven
+sizes, both arithmetics), the adjoint of one stage on the flat 128^3 box, and
+the restart records of the whole grid (pack and unpack, device to device).
 
 Launch shapes come from the autotuner (hamr_autotune) for each arithmetic.
-Beside the batched steps the line reports the reference-structured FP64 step
-(the C host's per-block call sequence with a host dt read-back per block), the
-multi-level torus (3 levels, ~700 blocks of uneven sizes, both arithmetics),
-the adjoint of one stage on the flat 128^3 box, the FP64 step with HLLC fluxes
-and the restart records of the whole grid (the rdump pack and unpack, device
-to device).
-
 Prints ONE JSON line with the contract's fields plus `roofline` (the mixed
-step's dominant stage, timed with HIP events on the step's stream) and
-`cpu_baseline` (the FP64 CPU oracle on the host cores, on a bounded sample of
-the same workload).
+step's dominant kernel: algorithmic bytes over its mean launch time, HIP
+events on the library's stream) and `cpu_baseline` (the FP64 CPU oracle on the
+host cores, on a bounded sample of the same workload).
 """
 import argparse
 import json
@@ -45,9 +45,9 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     # name: (block size, blocks per direction)
-    "c2": ((32, 32, 32), (8, 4, 4)),
-    "c3": ((32, 32, 32), (12, 6, 6)),
-    "c1": ((16, 8, 8), (4, 4, 4)),
+    "torus256": ((32, 32, 32), (8, 4, 4)),
+    "torus384": ((32, 32, 32), (12, 6, 6)),
+    "torus64": ((16, 8, 8), (4, 4, 4)),
 }
 
 STAGES = ["flux_x1", "flux_x2", "flux_x3", "emf", "pole", "ct_update", "update", "bound"]
@@ -102,7 +102,9 @@ def cpu_baseline(desc, nsteps):
     from tests import gridtools as GT
     from tests import oracle_ctypes as O
 
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    # the threads OpenMP runs with: OMP_NUM_THREADS where the host sets it (a
+    # GPU host's CPU share for one GPU), else every core
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(cores)
     grid = GT.oracle_grid(O.load(), desc)
     p, ps = H.desc_torus(desc, H.torus_desc())
@@ -328,14 +330,82 @@ def rdump_bench(H, ctx, bs, reps=5):
     return out
 
 
+def error_run(H, desc, p0, ps0, nsteps=100):
+    """The mixed step's error against the FP64 step after nsteps steps from the
+    same state (both with the FP64 run's dt sequence), and the same figures for
+    an FP64 run from the FP32-rounded state: per variable the max error
+    relative to the variable's largest magnitude and the L1 error (sum |d| /
+    sum |ref|) over the interior; for velocity and field the max error relative
+    to the vector's largest component."""
+    ctxs = {}
+    try:
+        for name, mode, q, qs in (("fp64", H.HAMR_PREC_FP64, p0, ps0), ("mixed", H.HAMR_PREC_MIXED, p0, ps0),
+                                  ("fp32_state", H.HAMR_PREC_FP64, p0.astype(np.float32).astype(np.float64),
+                                   ps0.astype(np.float32).astype(np.float64))):
+            c = H.Context(desc)
+            ctxs[name] = c
+            c.set_grid()
+            c.set(H.HAMR_P, q)
+            c.set(H.HAMR_PS, qs)
+            c.boundprim(1)
+            c.set_precision(mode)
+        dt = ctxs["fp64"].estimate_dt()
+        for _ in range(nsteps):
+            ndt = ctxs["fp64"].step(1, dt)
+            ctxs["mixed"].step(1, dt)
+            ctxs["fp32_state"].step(1, dt)
+            dt = ndt
+        I = slice(3, -3)
+        ref = ctxs["fp64"].get(H.HAMR_P)[:, :, I, I, I]
+        out = {"steps": nsteps}
+        for run in ("mixed", "fp32_state"):
+            a = ctxs[run].get(H.HAMR_P)[:, :, I, I, I]
+            e = {}
+            for name, k in (("rho", 0), ("u", 1), ("B1", 5), ("B2", 6), ("B3", 7)):
+                d = np.abs(a[:, k] - ref[:, k])
+                e[name + "_max"] = float(d.max() / np.abs(ref[:, k]).max())
+                e[name + "_l1"] = float(d.sum() / np.abs(ref[:, k]).sum())
+            for name, k0, k1 in (("velocity", 2, 5), ("field", 5, 8)):
+                d = np.abs(a[:, k0:k1] - ref[:, k0:k1])
+                e[name + "_max_of_vector"] = float(d.max() / np.abs(ref[:, k0:k1]).max())
+                e[name + "_l1"] = float(d.sum() / np.abs(ref[:, k0:k1]).sum())
+            del a
+            if run == "mixed":
+                out.update(e)
+            else:
+                out["fp64_from_fp32_rounded_state"] = e
+        return out
+    finally:
+        for c in ctxs.values():
+            c.close()
+
+
+def torus_context(H, config):
+    bs, nb = CONFIGS[config]
+    desc = H.make_desc(bs, nb)
+    ctx = H.Context(desc)
+    ctx.set_grid()
+    ctx.init_torus(H.torus_desc())
+    ctx.boundprim(1)
+    return desc, ctx, bs, nb, bs[0] * bs[1] * bs[2] * nb[0] * nb[1] * nb[2]
+
+
+def grid_name(bs, nb):
+    return "%dx%dx%d" % (bs[0] * nb[0], bs[1] * nb[1], bs[2] * nb[2])
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=97)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="torus384", choices=sorted(CONFIGS),
+                    help="grid of the headline (mixed) and FP64 batched steps")
+    ap.add_argument("--fp64-config", default="torus256", choices=sorted(CONFIGS),
+                    help="grid of the FP64 batched, per-block and HLLC steps beside them")
+    ap.add_argument("--error-steps", type=int, default=100, help="steps of the mixed-vs-FP64 error run (0: skip)")
     ap.add_argument("--no-graph", action="store_true", help="enqueue each step instead of replaying a HIP graph")
-    ap.add_argument("--cpu-steps", type=int, default=10, help="steps of the CPU oracle baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU oracle baseline (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one streaming copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None, help="run one arithmetic only")
@@ -343,6 +413,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--no-amr", action="store_true", help="skip the multi-level block set")
     ap.add_argument("--no-hllc", action="store_true", help="skip the FP64 step with HLLC fluxes")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the headline grid (no second grid, AMR, adjoint, HLLC, restart records)")
     ap.add_argument("--amr-steps", type=int, default=40)
     ap.add_argument("--amr-only", action="store_true",
                     help="measure only the multi-level block set (prints its JSON object; for profiling)")
@@ -353,21 +425,15 @@ def main():
         print("bench.py: the step is measured on one MI355X (one process, one GPU); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
+    if args.no_extra:
+        args.no_adjoint = args.no_amr = args.no_hllc = True
 
     import hamr_gamma_wind_amd as H
 
     if args.amr_only:
         print(json.dumps({"amr": amr_bench(H, args)}), flush=True)
         return
-    bs, nb = CONFIGS[args.config]
-    desc = H.make_desc(bs, nb)
-    nz = bs[0] * bs[1] * bs[2] * nb[0] * nb[1] * nb[2]
-    ctx = H.Context(desc)
-    ctx.set_grid()
-    ctx.init_torus(H.torus_desc())
-    ctx.boundprim(1)
+    desc, ctx, bs, nb, nz = torus_context(H, args.config)
     p0 = ctx.get(H.HAMR_P)
     ps0 = ctx.get(H.HAMR_PS)
 
@@ -377,13 +443,6 @@ def main():
     res = {name: run_mode(H, ctx, p0, ps0, m, args) for name, m in modes.items()}
     head = "mixed" if "mixed" in res else "fp64"
     el, p_head, stage_ms, dt, fi, tune, kern_ms = res[head]
-    host = None
-    if args.host_steps > 0:
-        el_host = host_sequence(H, ctx, p0, ps0, args.host_steps)
-        host = {"value": nz * args.host_steps / el_host, "ms_per_step": el_host / args.host_steps * 1e3,
-                "steps": args.host_steps,
-                "what": "FP64, the C host's per-block call sequence of advance_GPU (libhamr_host.so), "
-                        "each block's Courant dt read back to the host"}
     value = nz * args.steps / el
     mixed = head == "mixed"
 
@@ -393,34 +452,53 @@ def main():
     dom = max(kern_ms, key=kern_ms.get)
     bpz = kernel_bytes(dom, bs, mixed)
     achieved = nz * bpz / (kern_ms[dom] * 1e-3) / 1e9 if bpz > 0 else 0.0
-    traffic = read_traffic(dom, "%s_torus%dx%dx%d" % ("mixed" if mixed else "fp64", bs[0] * nb[0], bs[1] * nb[1],
-                                                       bs[2] * nb[2]))
-    rdump = rdump_bench(H, ctx, bs)
-    hllc = hllc_bench(H, ctx, p0, ps0, nz, args) if not args.no_hllc else None
+    traffic = read_traffic(dom, "%s_torus%s" % ("mixed" if mixed else "fp64", grid_name(bs, nb)))
     copy = None
     if args.copy_probe_mib > 0:
         nbytes = args.copy_probe_mib << 20
         ms = ctx.copy_probe(nbytes, 1)
         copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
+    rdump = rdump_bench(H, ctx, bs) if not args.no_extra else None
     ctx.close()
+    del p_head, res[head]
 
-    err = None
     fp64 = None
-    if "fp64" in res and "mixed" in res:
-        I = slice(3, -3)
-        a = res["mixed"][1][:, :, I, I, I]
-        b = res["fp64"][1][:, :, I, I, I]
-        err = {"steps": args.warmup + args.steps}
-        for name, k in (("rho", 0), ("B1", 5), ("B2", 6), ("B3", 7), ("u", 1)):
-            d = np.abs(a[:, k] - b[:, k])
-            err[name + "_max"] = float(d.max() / np.abs(b[:, k]).max())
-            err[name + "_l1"] = float(d.sum() / np.abs(b[:, k]).sum())
+    if "fp64" in res:
         el64 = res["fp64"][0]
         fp64 = {"value": nz * args.steps / el64, "ms_per_step": el64 / args.steps * 1e3,
-                "stages_ms_per_step": {k: round(2 * v, 4) for k, v in res["fp64"][2].items()}}
-        for name, k0, k1 in (("velocity", 2, 5), ("field", 5, 8)):
-            d = np.abs(a[:, k0:k1] - b[:, k0:k1]).max()
-            err[name + "_max_of_vector"] = float(d / np.abs(b[:, k0:k1]).max())
+                "stages_ms_per_step": {k: round(2 * v, 4) for k, v in res["fp64"][2].items()},
+                "kernels_ms_per_launch": {k: round(v, 4) for k, v in res["fp64"][6].items()},
+                "workload": "FP64 step on the same grid and state"}
+    res.clear()
+    err = error_run(H, desc, p0, ps0, args.error_steps) if (args.error_steps > 0 and not args.only) else None
+    del p0, ps0
+
+    second = None
+    if not args.no_extra:
+        # the FP64 GPU step on the second grid: batched (graph, tuned), the
+        # reference-structured per-block sequence and HLLC fluxes
+        desc2, ctx2, bs2, nb2, nz2 = torus_context(H, args.fp64_config)
+        try:
+            q0, qs0 = ctx2.get(H.HAMR_P), ctx2.get(H.HAMR_PS)
+            el2, _, st2, _, _, tune2, k2 = run_mode(H, ctx2, q0, qs0, H.HAMR_PREC_FP64, args)
+            second = {"workload": "FP64 GRMHD step, torus %s as %dx%dx%d blocks of %dx%dx%d + 3 ghosts"
+                                  % (grid_name(bs2, nb2), nb2[0], nb2[1], nb2[2], bs2[0], bs2[1], bs2[2]),
+                      "zones": nz2, "value": nz2 * args.steps / el2, "ms_per_step": el2 / args.steps * 1e3,
+                      "stages_ms_per_step": {k: round(2 * v, 4) for k, v in st2.items()},
+                      "kernels_ms_per_launch": {k: round(v, 4) for k, v in k2.items()}}
+            if args.host_steps > 0:
+                el_host = host_sequence(H, ctx2, q0, qs0, args.host_steps)
+                second["per_block_host"] = {
+                    "value": nz2 * args.host_steps / el_host, "ms_per_step": el_host / args.host_steps * 1e3,
+                    "steps": args.host_steps,
+                    "what": "FP64, the C host's per-block call sequence of advance_GPU (libhamr_host.so), "
+                            "each block's Courant dt read back to the host"}
+                second["speedup_batched_vs_per_block_host"] = second["value"] / second["per_block_host"]["value"]
+            if not args.no_hllc:
+                second["hllc"] = hllc_bench(H, ctx2, q0, qs0, nz2, args)
+            del q0, qs0
+        finally:
+            ctx2.close()
 
     adj = None
     if not args.no_adjoint:
@@ -433,8 +511,9 @@ def main():
     if args.cpu_steps > 0:
         v, cores, el_cpu = cpu_baseline(desc, args.cpu_steps)
         cpu = {"value": v, "unit": "zone-updates/s", "cores": cores, "kind": "port",
-               "sample": "%d FP64 steps of the same %dx%dx%d torus state by the oracle (OpenMP, %.1f s)"
-                         % (args.cpu_steps, bs[0] * nb[0], bs[1] * nb[1], bs[2] * nb[2], el_cpu)}
+               "sample": "%d FP64 steps of the same torus %s state by the oracle (OpenMP, %d threads, %.1f s; "
+                         "the host reports %d cores)" % (args.cpu_steps, grid_name(bs, nb), cores, el_cpu,
+                                                        os.cpu_count() or 0)}
 
     out = {
         "metric": METRIC,
@@ -449,11 +528,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64 (mixed)" if mixed else "f64",
         "data": "synthetic (seeded Fishbone-Moncrief torus initial state)",
-        "config": {"workload": "%s GRMHD step, torus %dx%dx%d as %dx%dx%d blocks of %dx%dx%d + 3 ghosts, "
+        "config": {"workload": "%s GRMHD step, torus %s as %dx%dx%d blocks of %dx%dx%d + 3 ghosts, "
                                "PPM+HLL+CT+Noble2D, %s"
-                               % ("mixed-precision" if mixed else "FP64", bs[0] * nb[0], bs[1] * nb[1],
-                                  bs[2] * nb[2], nb[0], nb[1], nb[2], bs[0], bs[1], bs[2],
-                                  "HIP graph replay" if not args.no_graph else "stream"),
+                               % ("mixed-precision" if mixed else "FP64", grid_name(bs, nb), nb[0], nb[1], nb[2],
+                                  bs[0], bs[1], bs[2], "HIP graph replay" if not args.no_graph else "stream"),
                    "zones": nz, "dt_final": dt, "inversion_failures_2d": int(fi[:, 0].sum()),
                    "floor_hits": int(fi[:, 3].sum())},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -465,15 +543,13 @@ def main():
         "stages_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
         "fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
-        "fp64_per_block_host": host,
-        "speedup_vs_fp64_per_block_host": (value / host["value"]) if host else None,
-        "tuned": {k: [v["threads"], v["grid"], v["xcd_order"]] for k, v in tune.items()} if tune else None,
-        "amr": amr_res,
         "error_vs_fp64": err,
+        "tuned": {k: [v["threads"], v["grid"], v["xcd_order"]] for k, v in tune.items()} if tune else None,
+        "fp64_second_grid": second,
+        "amr": amr_res,
         "adjoint": adj,
         "copy_probe": copy,
         "rdump": rdump,
-        "fp64_hllc": hllc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
