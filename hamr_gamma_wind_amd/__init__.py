@@ -664,7 +664,8 @@ class Context:
             cfg = np.zeros(3, dtype=np.int32)
             ms = C.c_double()
             self._chk(self.L.hamr_tune_get(self.h, k, _ptr(cfg, _ip), C.byref(ms)), "hamr_tune_get")
-            out[name] = dict(threads=int(cfg[0]), grid=int(cfg[1]), xcd_order=int(cfg[2]), ms=ms.value)
+            out[name] = dict(threads=int(cfg[0]), grid=int(cfg[1]), xcd_order=int(cfg[2]) & 1,
+                             second_form=(int(cfg[2]) >> 1) & 1, ms=ms.value)
         return out
 
     def time_kernel(self, slot, nrep=5):
@@ -672,8 +673,11 @@ class Context:
         self._chk(self.L.hamr_time_kernel(self.h, slot, nrep, C.byref(v)), "hamr_time_kernel")
         return v.value
 
-    def tune_set(self, slot, threads, grid, xcd_order):
-        cfg = np.array([threads, grid, xcd_order], dtype=np.int32)
+    def tune_set(self, slot, threads, grid, xcd_order, second_form=0):
+        """Launch shape of kernel `slot`; second_form = 1 selects the slot's
+        other form (flux slots: line-staged tiles of 256 threads instead of one
+        thread per face; cons: k_cons + k_invert instead of k_update)."""
+        cfg = np.array([threads, grid, (xcd_order & 1) | (second_form << 1)], dtype=np.int32)
         self._chk(self.L.hamr_tune_set(self.h, slot, _ptr(cfg, _ip)), "hamr_tune_set")
 
 

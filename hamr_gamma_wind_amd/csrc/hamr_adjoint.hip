@@ -367,6 +367,18 @@ __global__ void __launch_bounds__(256) k_adj_flux(AdjArgs A, const double *__res
 // ---------------------------------------------------------------------------
 static inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// Zeroes n doubles of each of a, b and c (the accumulators of the adjoint).  A
+// kernel rather than hipMemsetAsync, so that a caller may capture the adjoint
+// in a HIP graph.
+__global__ void __launch_bounds__(256) k_adj_zero3(double *a, double *b, double *c, int64_t n)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    a[t] = 0.;
+    b[t] = 0.;
+    c[t] = 0.;
+}
+
 void launch_adjoint(const KParams &P, const BlockDev &B, double Dt, const double *p, const double *ps,
                     const double *F1, const double *F2, const double *F3, const double *w, double *lP, double *lps,
                     double *lpsf, double *lE, double *lEc, double *lF1, double *lF2, double *lF3, hipStream_t s)
@@ -385,10 +397,7 @@ void launch_adjoint(const KParams &P, const BlockDev &B, double Dt, const double
     A.gcon = P.gcon + B.moff * NPOS * 10;
     A.gdet = P.gdet + B.moff * NPOS;
     const int64_t N = (int64_t)A.D[0] * A.D[1] * A.D[2];
-    const size_t bytes3 = sizeof(double) * 3 * (size_t)A.K;
-    (void)hipMemsetAsync(lps, 0, bytes3, s);
-    (void)hipMemsetAsync(lE, 0, bytes3, s);
-    (void)hipMemsetAsync(lEc, 0, bytes3, s);
+    hipLaunchKernelGGL(k_adj_zero3, g1(3 * A.K), dim3(256), 0, s, lps, lE, lEc, 3 * A.K);
     hipLaunchKernelGGL(k_adj_seed, g1(N), dim3(256), 0, s, A, w, lpsf);
     hipLaunchKernelGGL(k_adj_ct, g1(N), dim3(256), 0, s, A, lpsf, lE, lps);
     hipLaunchKernelGGL(k_adj_div, g1(N), dim3(256), 0, s, A, w, lF1, lF2, lF3);

@@ -20,6 +20,7 @@ namespace {
 
 constexpr int kTiles[3] = {64, 128, 256};   // tile sizes (threads per workgroup) the work lists exist for
 constexpr int kDefaultTile = 2;
+static_assert(kTiles[kDefaultTile] == kLineT, "line-staged flux tiles use the default tile size");
 
 // Kernels the autotuner configures (each walks the work list of one class).
 enum TuneSlot : int {
@@ -34,6 +35,8 @@ struct TuneCfg {
     int grid = 0;             // workgroups; 0: one per item
     int swz = 0;              // XCD-contiguous item order
     int march = 0;            // TS_CONS only: the x1-marching form (tile, grid, swz unused)
+    int alt = 0;              // the slot's second form: flux slots line-staged tiles (KC_LINE lists)
+                              // instead of one thread per face; TS_CONS k_cons + k_invert instead of k_update
     float ms = -1.f;          // measured time of the chosen configuration
 };
 
@@ -68,6 +71,7 @@ struct hamr_ctx {
     double *dt = nullptr;          // [0] dt of the next step, [1] previous
     double *gcov = nullptr, *gcon = nullptr, *gdet = nullptr, *conn = nullptr, *rcen = nullptr;
     double *tet = nullptr;         // face tetrads of the HLLC solver (96 per metric column)
+    float *g32 = nullptr;          // the metric in FP32 (KParams::g32)
     BlockDev *blk = nullptr;
     int *ring = nullptr, *rside = nullptr;  // polar rings
     int nring = 0, ring_len = 0, ni_max = 0;
@@ -98,6 +102,7 @@ struct hamr_ctx {
     double *io = nullptr;          // device record of one block (grown on demand)
     size_t io_bytes = 0;
 };
+
 
 static int fail(hamr_ctx *c, const std::string &m)
 {
@@ -198,12 +203,18 @@ static int build_class_lists(hamr_ctx *c)
             int cols = 1;
             for (int n : bl) {
                 const BlockDev &B = c->blocks[n];
-                const int64_t w = class_work(B, cls);
-                const int nt = (int)((w + T - 1) / T);
+                int nt;
+                if (cls >= KC_LINE1 && cls <= KC_LINE3) {  // kLineT threads per tile whatever T is
+                    const LineTiles lt = line_tiles(B.bs[0], B.bs[1], B.bs[2], cls - KC_LINE1 + 1);
+                    nt = lt.ntiles;
+                    cols = std::max(cols, lt.L * (lt.S + 5));
+                } else {
+                    nt = (int)((class_work(B, cls) + T - 1) / T);
+                    cols = std::max(cols, cons_max_cols(B.bs[2], T));
+                }
                 L.start[ti][n] = (int)items.size();
                 L.count[ti][n] = nt;
                 for (int t = 0; t < nt; t++) items.push_back(make_int2(n, t));
-                cols = std::max(cols, cons_max_cols(B.bs[2], T));
             }
             L.n[ti] = (int)items.size();
             L.lds_cols[ti] = cols;
@@ -216,18 +227,28 @@ static int build_class_lists(hamr_ctx *c)
     return 0;
 }
 
+// The class a tune slot's launches walk: the flux slots' per-face lists
+// unless their line-staged form is chosen.
+static bool line_flux(const hamr_ctx *c, int slot) { return slot <= TS_FLUX3 && c->tune[slot].alt; }
+static int slot_class(const hamr_ctx *c, int slot)
+{
+    return line_flux(c, slot) ? KC_LINE1 + slot - TS_FLUX1 : kSlotClass[slot];
+}
+
 // Launch of a tune slot over its whole block list with the tuned configuration.
 static Launch all_launch(const hamr_ctx *c, int slot)
 {
     const TuneCfg &t = c->tune[slot];
-    const ClassList &L = c->cl[kSlotClass[slot]];
+    const ClassList &L = c->cl[slot_class(c, slot)];
     Launch x;
-    x.threads = kTiles[t.tile];
-    x.wl.items = L.items[t.tile];
-    x.wl.n = L.n[t.tile];
+    x.line = line_flux(c, slot) ? 1 : 0;
+    const int ti = x.line ? kDefaultTile : t.tile;  // line-staged tiles are kLineT threads
+    x.threads = kTiles[ti];
+    x.wl.items = L.items[ti];
+    x.wl.n = L.n[ti];
     x.wl.swz = t.swz;
     x.grid = (t.grid > 0 && t.grid < x.wl.n) ? t.grid : x.wl.n;
-    x.lds_cols = L.lds_cols[t.tile];
+    x.lds_cols = L.lds_cols[ti];
     return x;
 }
 
@@ -242,7 +263,14 @@ static Launch block_launch(const hamr_ctx *c, int cls, int n)
     x.wl.swz = 0;
     x.grid = x.wl.n;
     x.lds_cols = L.lds_cols[kDefaultTile];
+    x.line = (cls >= KC_LINE1 && cls <= KC_LINE3) ? 1 : 0;
     return x;
+}
+
+// The flux launch of direction dir over block n, in the form the flux slot is tuned to.
+static Launch flux_block_launch(const hamr_ctx *c, int dir, int n)
+{
+    return block_launch(c, slot_class(c, TS_FLUX1 + dir - 1), n);
 }
 
 // Device geometry slabs and state arrays for the block table already in c.
@@ -258,7 +286,7 @@ static int init_device(hamr_ctx *c)
     const size_t KC = (size_t)c->cols;
     if (dalloc(c, &c->gcov, NPOS * 10 * KC) || dalloc(c, &c->gcon, NPOS * 10 * KC) || dalloc(c, &c->gdet, NPOS * KC) ||
         dalloc(c, &c->conn, 40 * KC) || dalloc(c, &c->rcen, KC) || dalloc(c, &c->tet, 96 * KC) ||
-        dalloc(c, &c->blk, nb))
+        dalloc(c, &c->g32, NPOS * G32N * KC) || dalloc(c, &c->blk, nb))
         return -1;
     if (h2d(c, c->blk, c->blocks.data(), nb * sizeof(BlockDev))) return -1;
     std::vector<int> bl(nb);
@@ -287,6 +315,7 @@ static int init_device(hamr_ctx *c)
     P.conn = c->conn;
     P.rcen = c->rcen;
     P.tet = c->tet;
+    P.g32 = c->g32;
     P.riemann = HAMR_RIEMANN_HLL;
     launch_dt_reset((int)nb, c->dtmin, c->s);
     if (check_launch(c, "dt reset")) return -1;
@@ -381,14 +410,28 @@ static void cons_all(hamr_ctx *c, const Launch &L, int full, double Dt_val, cons
                            c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, c->s);
 }
 
+// The update's conserved update and 2D inversion: k_update (one kernel), or
+// in TS_CONS's second form k_cons (or its x1-marching form) then k_invert.
+static bool fused_update(const hamr_ctx *c) { return !c->tune[TS_CONS].march && !c->tune[TS_CONS].alt; }
+static void update_front(hamr_ctx *c, const Launch &L, int full, double Dt_val, const double *dt_ptr,
+                         double dt_mult, const double *pi, const double *pb, double *pf, const double *psf)
+{
+    if (fused_update(c)) {
+        launch_update_part(c->P, 3, L, full, Dt_val, dt_ptr, dt_mult, pi, pb, pf, c->U0, c->Un, psf, c->F[0],
+                           c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, c->s);
+        return;
+    }
+    cons_all(c, L, full, Dt_val, dt_ptr, dt_mult, pi, pb, psf);
+    launch_update_part(c->P, 1, all_launch(c, TS_INVERT), full, Dt_val, dt_ptr, dt_mult, pi, pb, pf, c->U0, c->Un,
+                       psf, c->F[0], c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, c->s);
+}
+
 static void update_all(hamr_ctx *c, int full, double Dt_val, const double *dt_ptr, double dt_mult, const double *pi,
                        const double *pb, double *pf, const double *psf)
 {
-    cons_all(c, all_launch(c, TS_CONS), full, Dt_val, dt_ptr, dt_mult, pi, pb, psf);
-    const int slots[2] = {TS_INVERT, TS_FALLBACK};
-    for (int part = 1; part < 3; part++)
-        launch_update_part(c->P, part, all_launch(c, slots[part - 1]), full, Dt_val, dt_ptr, dt_mult, pi, pb, pf,
-                           c->U0, c->Un, psf, c->F[0], c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, c->s);
+    update_front(c, all_launch(c, TS_CONS), full, Dt_val, dt_ptr, dt_mult, pi, pb, pf, psf);
+    launch_update_part(c->P, 2, all_launch(c, TS_FALLBACK), full, Dt_val, dt_ptr, dt_mult, pi, pb, pf, c->U0, c->Un,
+                       psf, c->F[0], c->F[1], c->F[2], c->pflag, c->failcnt, c->prec, c->s);
 }
 
 extern "C" {
@@ -870,6 +913,9 @@ int hamr_upload_metric(hamr_ctx *c, int geo, const double *gcov, const double *g
     if (h2d(c, c->gcov + mo * NPOS * 10, gcov, NPOS * 10 * KC * sizeof(double))) return -1;
     if (h2d(c, c->gcon + mo * NPOS * 10, gcon, NPOS * 10 * KC * sizeof(double))) return -1;
     if (h2d(c, c->gdet + mo * NPOS, gdet, NPOS * KC * sizeof(double))) return -1;
+    launch_geom32(c->gcov + mo * NPOS * 10, c->gcon + mo * NPOS * 10, c->gdet + mo * NPOS, c->g32 + mo * NPOS * G32N,
+                  (int64_t)KC, c->s);
+    if (check_launch(c, "hamr_upload_metric")) return -1;
     if (h2d(c, c->conn + mo * 40, conn, 40 * KC * sizeof(double))) return -1;
     if (h2d(c, c->rcen + mo, rcen, KC * sizeof(double))) return -1;
     // the HLLC solver's face tetrads follow the metric (set_Mud after set_grid)
@@ -1280,7 +1326,7 @@ int hamr_dump_write_dev(hamr_ctx *c, float *drec)
 int hamr_fluxcalc(hamr_ctx *c, int dir, int flag, int n)
 {
     if (!c || !valid_block(c, n) || dir < 1 || dir > 3) return fail(c, "hamr_fluxcalc: bad arguments");
-    launch_flux(c->P, dir, block_launch(c, KC_FLUX1 + dir - 1, n), flag ? c->ph : c->p, flag ? c->psh : c->ps,
+    launch_flux(c->P, dir, flux_block_launch(c, dir, n), flag ? c->ph : c->p, flag ? c->psh : c->ps,
                 c->F[dir - 1], c->dtmin, c->prec, c->s);
     return check_launch(c, "hamr_fluxcalc");
 }
@@ -1490,7 +1536,7 @@ static void run_slot(hamr_ctx *c, int slot, const Launch &L, int T_ex, int G_ex)
     case TS_ECENT: launch_emf_centre(P, L, c->p, c->Ec, c->prec, s); break;
     case TS_ECORNER: launch_emf_corner(P, L, c->Ec, c->F[0], c->F[1], c->F[2], c->E, c->prec, s); break;
     case TS_CT: launch_ct_update(P, L, 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, s); break;
-    case TS_CONS: cons_all(c, L, 0, 0., c->dt, 0.5, c->p, c->p, c->psh); break;
+    case TS_CONS: update_front(c, L, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->psh); break;
     case TS_INVERT:
     case TS_FALLBACK:
         launch_update_part(P, slot - TS_CONS, L, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->U0, c->Un, c->psh, c->F[0],
@@ -1561,14 +1607,19 @@ int hamr_autotune(hamr_ctx *c, int nrep)
         if (slot == TS_EXCHANGE && !c->amr) continue;
         if (slot != TS_EXCHANGE && c->cl[kSlotClass[slot]].n[kDefaultTile] == 0) continue;
         TuneCfg best;
+        // second forms: the fluxes line-staged, the update k_cons + k_invert
+        const int nform = (slot <= TS_FLUX3 || slot == TS_CONS) ? 2 : 1;
+        for (int form = 0; form < nform; form++)
         for (int ti = 0; ti < 3; ti++)
             for (int gi = 0; gi < 4; gi++)
                 for (int swz = 0; swz < 2; swz++) {
                     if (slot == TS_EXCHANGE && swz) continue;
+                    if (slot <= TS_FLUX3 && form == 1 && kTiles[ti] != kLineT) continue;
                     TuneCfg cand;
                     cand.tile = ti;
                     cand.grid = grids[gi];
                     cand.swz = swz;
+                    cand.alt = form;
                     const TuneCfg keep = c->tune[slot];
                     c->tune[slot] = cand;
                     Launch L;
@@ -1627,7 +1678,7 @@ int hamr_tune_get(const hamr_ctx *c, int slot, int *cfg, double *ms)
     const TuneCfg &t = c->tune[slot];
     cfg[0] = t.march ? 128 : kTiles[t.tile];
     cfg[1] = t.march ? -1 : t.grid;
-    cfg[2] = t.march ? 0 : t.swz;
+    cfg[2] = t.march ? 0 : t.swz | (t.alt << 1);
     if (ms) *ms = t.ms;
     return 0;
 }
@@ -1639,11 +1690,15 @@ int hamr_tune_set(hamr_ctx *c, int slot, const int *cfg)
     for (int k = 0; k < 3; k++)
         if (kTiles[k] == cfg[0]) ti = k;
     const bool march = slot == TS_CONS && cfg[1] == -1;
-    if (!march && (ti < 0 || cfg[1] < 0 || (cfg[2] != 0 && cfg[2] != 1)))
-        return fail(c, "hamr_tune_set: tile 64/128/256, grid >= 0 (-1 for the update's x1-marching form), swz 0/1");
+    const int maxbits = (slot <= TS_FLUX3 || slot == TS_CONS) ? 3 : 1;
+    if (!march && (ti < 0 || cfg[1] < 0 || cfg[2] < 0 || cfg[2] > maxbits))
+        return fail(c, "hamr_tune_set: tile 64/128/256, grid >= 0 (-1 for the update's x1-marching form), "
+                       "cfg[2] bit 0 XCD order, bit 1 the second form (fluxes: line-staged tiles; "
+                       "update: k_cons + k_invert)");
     c->tune[slot].tile = march ? kDefaultTile : ti;
     c->tune[slot].grid = march ? 0 : cfg[1];
-    c->tune[slot].swz = march ? 0 : cfg[2];
+    c->tune[slot].swz = march ? 0 : (cfg[2] & 1);
+    c->tune[slot].alt = march ? 0 : (cfg[2] >> 1) & 1;
     c->tune[slot].march = march ? 1 : 0;
     c->tune[slot].ms = -1.f;
     if (c->graph) {
@@ -1707,44 +1762,42 @@ int hamr_stage_forward(hamr_ctx *c, double Dt)
     return check_launch(c, "hamr_stage_forward");
 }
 
+// The adjoint's work arrays: weights, lambda of P, ps, the updated face field,
+// the corner and centre EMFs and the three fluxes (57 doubles per cell of the
+// padded slab).
+struct AdjBufs {
+    double *w, *lP, *lps, *lpsf, *lE, *lEc, *lF[3];
+};
+constexpr int64_t kAdjDoublesPerCell = NPR + NPR + 3 + 3 + 3 + 3 + 3 * NPR;
+
 // w, grad_p, grad_ps: pointers of `kind` (host or device), unpadded layout.
 static int stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, double *gps, hipMemcpyKind in,
-                         hipMemcpyKind out)
+                         hipMemcpyKind out, const AdjBufs &A)
 {
-    if (!c || !w) return -1;
-    if (!c->d.minkowski || c->amr || nblk_of(c) != 1 || c->prec != HAMR_PREC_FP64 || c->P.riemann != HAMR_RIEMANN_HLL)
-        return fail(c, "hamr_stage_adjoint: needs a flat periodic box of one block in FP64 with HLL fluxes");
-    HIPCHK(c, hipSetDevice(c->device));
     const BlockDev &B = c->blocks[0];
     const size_t K = (size_t)B.ks, N = (size_t)ncell_of(B);
-    if (!c->aw) {
-        if (dalloc(c, &c->aw, NPR * K) || dalloc(c, &c->alP, NPR * K) || dalloc(c, &c->alps, 3 * K) ||
-            dalloc(c, &c->alpsf, 3 * K) || dalloc(c, &c->alE, 3 * K) || dalloc(c, &c->alEc, 3 * K) ||
-            dalloc(c, &c->alF[0], NPR * K) || dalloc(c, &c->alF[1], NPR * K) || dalloc(c, &c->alF[2], NPR * K))
-            return -1;
-    }
     HIPCHK(c, hipStreamSynchronize(c->s));
     // host buffers pass through the pinned staging buffer (w in, then the gradients out)
     const bool hin = in == hipMemcpyHostToDevice, hout = out == hipMemcpyDeviceToHost;
     if ((hin || hout) && stage_reserve(c, (NPR + 3) * N * sizeof(double))) return -1;
     double *sw = (double *)c->stage;
     if (hin) std::memcpy(sw, w, NPR * N * sizeof(double));
-    HIPCHK(c, hipMemcpy2DAsync(c->aw, K * sizeof(double), hin ? sw : w, N * sizeof(double), N * sizeof(double), NPR,
+    HIPCHK(c, hipMemcpy2DAsync(A.w, K * sizeof(double), hin ? sw : w, N * sizeof(double), N * sizeof(double), NPR,
                                in, c->s));
     if (hin) HIPCHK(c, hipStreamSynchronize(c->s));
     // the fluxes of the stage (their mass-flux signs select the EMF upwinding)
     bound_all(c, c->p, c->ps);
     for (int dir = 1; dir <= 3; dir++)
-        launch_flux(c->P, dir, block_launch(c, KC_FLUX1 + dir - 1, 0), c->p, c->ps, c->F[dir - 1], nullptr,
+        launch_flux(c->P, dir, flux_block_launch(c, dir, 0), c->p, c->ps, c->F[dir - 1], nullptr,
                     HAMR_PREC_FP64, c->s);
-    launch_adjoint(c->P, B, Dt, c->p, c->ps, c->F[0], c->F[1], c->F[2], c->aw, c->alP, c->alps, c->alpsf, c->alE,
-                   c->alEc, c->alF[0], c->alF[1], c->alF[2], c->s);
+    launch_adjoint(c->P, B, Dt, c->p, c->ps, c->F[0], c->F[1], c->F[2], A.w, A.lP, A.lps, A.lpsf, A.lE, A.lEc,
+                   A.lF[0], A.lF[1], A.lF[2], c->s);
     if (check_launch(c, "hamr_stage_adjoint")) return -1;
     double *sp = hout ? (double *)c->stage : gp, *sps = hout ? (double *)c->stage + NPR * N : gps;
     if (gp)
-        HIPCHK(c, hipMemcpy2DAsync(sp, N * sizeof(double), c->alP, K * sizeof(double), N * sizeof(double), NPR, out, c->s));
+        HIPCHK(c, hipMemcpy2DAsync(sp, N * sizeof(double), A.lP, K * sizeof(double), N * sizeof(double), NPR, out, c->s));
     if (gps)
-        HIPCHK(c, hipMemcpy2DAsync(sps, N * sizeof(double), c->alps, K * sizeof(double), N * sizeof(double), 3, out, c->s));
+        HIPCHK(c, hipMemcpy2DAsync(sps, N * sizeof(double), A.lps, K * sizeof(double), N * sizeof(double), 3, out, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (hout) {
         if (gp) std::memcpy(gp, sp, NPR * N * sizeof(double));
@@ -1753,14 +1806,43 @@ static int stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, do
     return 0;
 }
 
+static int adjoint_ok(hamr_ctx *c, const double *w, const char *what)
+{
+    if (!c) return -1;
+    if (!w) return fail(c, std::string(what) + ": no weights");
+    if (!c->d.minkowski || c->amr || nblk_of(c) != 1 || c->prec != HAMR_PREC_FP64 || c->P.riemann != HAMR_RIEMANN_HLL)
+        return fail(c, std::string(what) + ": needs a flat periodic box of one block in FP64 with HLL fluxes");
+    HIPCHK(c, hipSetDevice(c->device));
+    return 0;
+}
+
+// The work arrays of the allocating entry points: the context's own, allocated
+// by the first adjoint call.
+static int adjoint_bufs(hamr_ctx *c, AdjBufs &A)
+{
+    const size_t K = (size_t)c->blocks[0].ks;
+    if (!c->aw) {
+        if (dalloc(c, &c->aw, NPR * K) || dalloc(c, &c->alP, NPR * K) || dalloc(c, &c->alps, 3 * K) ||
+            dalloc(c, &c->alpsf, 3 * K) || dalloc(c, &c->alE, 3 * K) || dalloc(c, &c->alEc, 3 * K) ||
+            dalloc(c, &c->alF[0], NPR * K) || dalloc(c, &c->alF[1], NPR * K) || dalloc(c, &c->alF[2], NPR * K))
+            return -1;
+    }
+    A = AdjBufs{c->aw, c->alP, c->alps, c->alpsf, c->alE, c->alEc, {c->alF[0], c->alF[1], c->alF[2]}};
+    return 0;
+}
+
 int hamr_stage_adjoint(hamr_ctx *c, double Dt, const double *w, double *gp, double *gps)
 {
-    return stage_adjoint(c, Dt, w, gp, gps, hipMemcpyHostToDevice, hipMemcpyDeviceToHost);
+    AdjBufs A;
+    if (adjoint_ok(c, w, "hamr_stage_adjoint") || adjoint_bufs(c, A)) return -1;
+    return stage_adjoint(c, Dt, w, gp, gps, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, A);
 }
 
 int hamr_stage_adjoint_dev(hamr_ctx *c, double Dt, const double *w, double *gp, double *gps)
 {
-    return stage_adjoint(c, Dt, w, gp, gps, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice);
+    AdjBufs A;
+    if (adjoint_ok(c, w, "hamr_stage_adjoint_dev") || adjoint_bufs(c, A)) return -1;
+    return stage_adjoint(c, Dt, w, gp, gps, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice, A);
 }
 
 int hamr_copy_probe(hamr_ctx *c, long long bytes, int nrep, double *avg_ms)

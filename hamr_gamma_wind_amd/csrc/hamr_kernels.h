@@ -16,6 +16,10 @@ struct KParams {
     const BlockDev *blk;
     const double *gcov, *gcon, *gdet, *conn, *rcen;
     const double *tet;  // face tetrads: tet[moff*96 + ((dir-1)*32 + e)*kcol + col] (e < 16: e_(a)^mu, else e^(a)_mu)
+    // The metric rounded to FP32 for the mixed step's kernels, with 1 - alpha
+    // formed in FP64 first: g32[moff*NPOS*G32N + (pos*G32N + s)*kcol + col],
+    // s = 0-9 gcov, 10-19 gcon, 20 gdet, 21 1 - 1/sqrt(-g^00).
+    const float *g32;
     int riemann;        // HAMR_RIEMANN_HLL (0) or HAMR_RIEMANN_HLLC (1); HLLC in the FP64 step only
 };
 
@@ -38,8 +42,38 @@ struct Launch {
     WorkList wl;
     int threads;
     int grid;
-    int lds_cols;  // metric columns one tile of interior cells can touch (k_cons)
+    int lds_cols;  // LDS a tile stages: metric columns (k_cons), or values per staged array (line-staged flux)
+    int line = 0;  // flux launches: 1 = line-staged tiles (KC_LINE lists), 0 = one thread per face (KC_FLUX lists)
 };
+
+constexpr int G32N = 22;  // FP32 metric values per column and position (KParams::g32)
+
+// Tiles of the line-staged flux kernel (kLineT threads per workgroup) of
+// direction dir (1..3) for a block of b1 x b2 x b3 cells.  A line (one
+// transverse position of the face range, both transverse directions over
+// [NG-1, NG+BS]) has b_dir + 1 faces, cut into nseg segments of S faces (the
+// last one possibly shorter), at most kLineSeg each.  The lines form n1 rows
+// (the slower transverse direction) of n2 lines; a tile is one segment on L =
+// kLineT / S consecutive lines of one row (tpr tiles per row), and stages
+// L (S + 5) cells per primitive.
+constexpr int kLineT = 256;
+constexpr int kLineSeg = 33;
+struct LineTiles {
+    int S, nseg, L, n1, n2, tpr, ntiles;
+};
+__host__ __device__ inline LineTiles line_tiles(int b1, int b2, int b3, int dir)
+{
+    LineTiles x;
+    const int nf = (dir == 1 ? b1 : dir == 2 ? b2 : b3) + 1;
+    x.nseg = (nf + kLineSeg - 1) / kLineSeg;
+    x.S = (nf + x.nseg - 1) / x.nseg;
+    x.L = kLineT / x.S;
+    x.n1 = (dir == 1 ? b2 : b1) + 2;
+    x.n2 = (dir == 3 ? b2 : b3) + 2;
+    x.tpr = (x.n2 + x.L - 1) / x.L;
+    x.ntiles = x.nseg * x.n1 * x.tpr;
+    return x;
+}
 
 // Kernel classes: each has its own iteration space per block.
 enum KClass : int {
@@ -48,6 +82,7 @@ enum KClass : int {
     KC_CORNER,                         // corners/faces [NG, NG+BS]   (corner EMF, face-B update)
     KC_INTERIOR,                       // interior cells               (update, inversion, fallback)
     KC_BOUND1, KC_BOUND2, KC_BOUND3,   // ghost layers of one direction
+    KC_LINE1, KC_LINE2, KC_LINE3,      // line-staged flux tiles of one direction (line_tiles)
     KC_N
 };
 // Threads a block needs in a kernel class.
@@ -88,7 +123,9 @@ void launch_cons_march(const KParams &P, int nblk, int max_plane, int min_bs3, i
                        const double *dt_ptr, double dt_mult, const double *pi, const double *pb, double *U0,
                        double *Unew, const double *psf, const void *F1, const void *F2, const void *F3, int mixed,
                        hipStream_t s);
-// The kernels of the update one at a time (k_cons, k_invert, k_fallback), for timing and tuning.
+// The kernels of the update one at a time: part 0 k_cons, 1 k_invert, 2
+// k_fallback, 3 k_update (k_cons and k_invert in one kernel).  launch_update
+// runs parts 3 and 2.
 void launch_update_part(const KParams &P, int part, const Launch &L, int full, double Dt_val, const double *dt_ptr,
                         double dt_mult, const double *pi, const double *pb, double *pf, double *U0, double *Unew,
                         const double *psf, const void *F1, const void *F2, const void *F3, int *pflag, int *failcnt,
@@ -141,6 +178,10 @@ void launch_rdump_unpack(const KParams &P, const int *blist, const int64_t *roff
 constexpr int DREC = 9;
 void launch_dump_pack(const KParams &P, const int *blist, const int64_t *doff, const int64_t *doff_host, int first,
                       int count, int64_t max_cells, const double *p, float *out, hipStream_t s);
+// The FP32 metric (KParams::g32) of ncol columns of one geometry slab from its
+// FP64 metric (the slab's pointers into gcov, gcon, gdet and g32).
+void launch_geom32(const double *gcov, const double *gcon, const double *gdet, float *g32, int64_t kcol,
+                   hipStream_t s);
 void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);
 void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s);
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);

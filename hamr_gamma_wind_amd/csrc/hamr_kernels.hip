@@ -145,14 +145,16 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
 }
 
 // Conserved update, source, face-B average, Noble cascade, floors and flags:
-// the fixup kernel of GPU_fixup.h:5-425, as three kernels.  k_cons forms the
-// updated conserved variables (full = 0, predictor: U^n from pi is stored in
-// U0, source from pi; full = 1, corrector: U from U0, source from pb) and
-// writes them to Unew; k_invert runs the 2D inversion on Unew with the same
-// initial guess (pi or pb), applies the floors and sets the flags; k_fallback
-// finishes the cascade for the cells whose 2D inversion failed.  The split
-// keeps each kernel's registers low enough for several waves per SIMD, which
-// hides the latency of its loads; the arithmetic is that of one fused kernel.
+// the fixup kernel of GPU_fixup.h:5-425.  The step runs it as k_update (the
+// updated conserved variables and the 2D inversion of each cell in one pass,
+// U' kept in registers) and k_fallback (the 1D inversions of the few cells
+// whose 2D inversion failed, so that they do not set k_update's register
+// budget).  k_cons + k_invert are the same update in two kernels with U'
+// passed through memory (Unew): k_cons forms the updated conserved variables
+// (full = 0, predictor: U^n from pi is stored in U0, source from pi; full = 1,
+// corrector: U from U0, source from pb) and k_invert inverts them with the
+// same initial guess (pi or pb), applies the floors and sets the flags.  All
+// forms give the same result bit for bit.
 
 // Per-column metric values k_cons stages in LDS (every z-row of a block shares
 // one column): gcov, gcon, gdet at CENT, the connection, and the face gdets the
@@ -188,16 +190,17 @@ __device__ __forceinline__ void comp_add(double &s, double &c, double t)
 // through mv(v), v one of MV_*): U^n (predictor: from pi, stored in U0;
 // corrector: read from U0), minus Dt times the flux divergence, plus Dt times
 // the source from the state (pi or pb), the face-field average into the B
-// slots; written to Unew.  f1l / f1h are the x1 fluxes at the cell's lower and
-// upper faces (the caller loads or carries them).  R is the storage type of
-// the fluxes: double (FP64 step, the reference's arithmetic) or float (mixed
-// step, compensated FP64 accumulation).
+// slots; returned in U.  pf returns the state the source was formed from (pi
+// or pb at the cell), the inversion's initial guess.  f1l / f1h are the x1
+// fluxes at the cell's lower and upper faces (the caller loads or carries
+// them).  R is the storage type of the fluxes: double (FP64 step, the
+// reference's arithmetic) or float (mixed step, compensated FP64 accumulation).
 template <class R, class MV>
 __device__ __forceinline__ void cons_cell(const KParams &P, const BlockDev &B, int full, double Dt, int64_t g,
                                           const double *__restrict__ pi, const double *__restrict__ pb,
-                                          double *__restrict__ U0, double *__restrict__ Unew,
-                                          const double *__restrict__ psf, const R *f1l, const R *f1h,
-                                          const R *__restrict__ F2, const R *__restrict__ F3, MV mv)
+                                          double *__restrict__ U0, const double *__restrict__ psf, const R *f1l,
+                                          const R *f1h, const R *__restrict__ F2, const R *__restrict__ F3, MV mv,
+                                          double (&U)[NPR], double (&pf)[NPR])
 {
     const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2], sj = B.D[2];
     const int64_t base = NPR * B.off + g;
@@ -208,7 +211,7 @@ __device__ __forceinline__ void cons_cell(const KParams &P, const BlockDev &B, i
         G.gcon[s] = mv(MV_GCON + s);
     }
     G.g = mv(MV_G);
-    double pf[NPR], U[NPR], dU[NPR];
+    double dU[NPR];
     State<double> q;
     if (!full) {
         double d1, d2;
@@ -268,8 +271,6 @@ __device__ __forceinline__ void cons_cell(const KParams &P, const BlockDev &B, i
         U[B2] += (sp[K] * mv(MV_GF2) + sp[K + sj] * mv(MV_GF2P)) / (2.0 * 1.0);
         U[B3] = (sp[2 * K] * mv(MV_GF3) + sp[2 * K + 1] * mv(MV_GF3)) / 2.0;
     }
-#pragma unroll
-    for (int k = 0; k < NPR; k++) Unew[base + k * K] = U[k];
 }
 
 // One thread per interior cell, tiles of a block's cells z fastest; the
@@ -316,8 +317,11 @@ __global__ void __launch_bounds__(256) k_cons(KParams P, WorkList wl, int full, 
                 f1l[k] = f1[0];
                 f1h[k] = f1[si];
             }
-            cons_cell<R>(P, B, full, Dt, g, pi, pb, U0, Unew, psf, f1l, f1h, F2, F3,
-                         [&](int v) { return mvp[v * ncw]; });
+            double U[NPR], pf[NPR];
+            cons_cell<R>(P, B, full, Dt, g, pi, pb, U0, psf, f1l, f1h, F2, F3, [&](int v) { return mvp[v * ncw]; },
+                         U, pf);
+#pragma unroll
+            for (int k = 0; k < NPR; k++) Unew[base + k * K] = U[k];
         }
         __syncthreads();
     }
@@ -368,8 +372,11 @@ k_cons_march(KParams P, int full, double Dt_val, const double *__restrict__ dt_p
             const int64_t base = NPR * B.off + g;
 #pragma unroll
             for (int k = 0; k < NPR; k++) fh[k] = F1[base + k * K + si];
-            cons_cell<R>(P, B, full, Dt, g, pi, pb, U0, Unew, psf, fl, fh, F2, F3,
-                         [&](int v) { return mvp[v * ncw]; });
+            double U[NPR], pf[NPR];
+            cons_cell<R>(P, B, full, Dt, g, pi, pb, U0, psf, fl, fh, F2, F3, [&](int v) { return mvp[v * ncw]; },
+                         U, pf);
+#pragma unroll
+            for (int k = 0; k < NPR; k++) Unew[base + k * K] = U[k];
 #pragma unroll
             for (int k = 0; k < NPR; k++) fl[k] = fh[k];
         }
@@ -485,6 +492,85 @@ __global__ void __launch_bounds__(256) k_invert(KParams P, WorkList wl, int full
         pflag[B.off + g] = kFallbackPending;
 #pragma unroll
         for (int k = 0; k < NPR; k++) pf_out[base + k * K] = pf[k];
+    }
+}
+
+// The conserved update and the 2D inversion of each interior cell in one
+// pass (fixup, GPU_fixup.h:5-425, without the 1D fallbacks): k_cons's update
+// with the column metric staged in LDS, then k_invert's inversion on the
+// updated U in registers, from the state the source was formed from (pi or pb)
+// as the guess.  U' and the guess never go through memory, except for a cell
+// whose 2D inversion fails: its U' goes to Unew, its flag is kFallbackPending,
+// and k_fallback finishes it.  Same arithmetic as k_cons + k_invert, bit for
+// bit.  Two waves per SIMD (<= 256 VGPRs): held at three (168 VGPRs) it
+// spills 43-65 registers and runs 1.2x slower.
+template <class R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_update(KParams P, WorkList wl, int full, double Dt_val, const double *__restrict__ dt_ptr, double dt_mult,
+         const double *__restrict__ pi, const double *__restrict__ pb, double *__restrict__ pf_out,
+         double *__restrict__ U0, double *__restrict__ Unew, const double *__restrict__ psf, const R *__restrict__ F1,
+         const R *__restrict__ F2, const R *__restrict__ F3, int *__restrict__ pflag, int *__restrict__ failcnt)
+{
+    extern __shared__ double smem[];
+    FOR_ITEMS(wl)
+    {
+        const int2 it = wl_get(wl, w_);
+        const int n = it.x;
+        const BlockDev &B = P.blk[n];
+        const int bs2 = B.bs[1], bs3 = B.bs[2];
+        const int nint = B.bs[0] * bs2 * bs3;
+        const int t0 = it.y * blockDim.x;
+        const int t = t0 + threadIdx.x;
+        const int q0 = t0 / bs3, q1 = mymin(t0 + (int)blockDim.x - 1, nint - 1) / bs3;
+        const int ncw = q1 - q0 + 1;
+        for (int e = threadIdx.x; e < MV_N * ncw; e += blockDim.x) {
+            const int v = e / ncw, qq = e - v * ncw, q = q0 + qq;
+            const int64_t col = (int64_t)(q / bs2 + NG) * B.D[1] + (q % bs2 + NG);
+            smem[e] = metric_value(P, B, v, col);
+        }
+        __syncthreads();
+        if (t < nint) {
+            int a, b, c;
+            region_index(t, bs2, bs3, a, b, c);
+            const int i = a + NG, j = b + NG, z = c + NG;
+            const double *mvp = smem + (t / bs3 - q0);
+            const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
+            const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2];
+            const int64_t g = ((int64_t)i * B.D[1] + j) * B.D[2] + z, col = (int64_t)i * B.D[1] + j;
+            const int64_t base = NPR * B.off + g;
+            double U[NPR], pf[NPR];
+            {
+                R f1l[NPR], f1h[NPR];
+#pragma unroll
+                for (int k = 0; k < NPR; k++) {
+                    const R *f1 = F1 + base + k * K;
+                    f1l[k] = f1[0];
+                    f1h[k] = f1[si];
+                }
+                cons_cell<R>(P, B, full, Dt, g, pi, pb, U0, psf, f1l, f1h, F2, F3,
+                             [&](int v) { return mvp[v * ncw]; }, U, pf);
+            }
+            Geom<double> G;
+#pragma unroll
+            for (int s = 0; s < 10; s++) {
+                G.gcov[s] = mvp[(MV_GCOV + s) * ncw];
+                G.gcon[s] = mvp[(MV_GCON + s) * ncw];
+            }
+            G.g = mvp[MV_G * ncw];
+            const int flag = utoprim(0, U, G.gcov, G.gcon, G.g, P.gam, pf);
+            if (flag == 0) {
+                finish_cell(P, B, n, col, g, z, base, 0, pf, pf_out, pflag, failcnt);
+            } else {
+                atomicAdd(&failcnt[n * 4 + 0], 1);
+                pflag[B.off + g] = kFallbackPending;
+#pragma unroll
+                for (int k = 0; k < NPR; k++) {
+                    pf_out[base + k * K] = pf[k];
+                    Unew[base + k * K] = U[k];
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -840,6 +926,33 @@ __global__ void __launch_bounds__(256) k_amr_emf(KParams P, const AmrEmf *__rest
 // Courant dt of the step: per-block harmonic of the direction minima, minimum
 // over blocks (step_ch.c:1313-1322), clamped to 1.3 dt (step_ch.c:974).  Resets
 // the minima for the next corrector.  dt[0] <- new dt, dt[1] <- previous dt.
+// FP32 copy of one geometry slab's metric (KParams::g32): each value rounded
+// once, and 1 - alpha = 1 - 1/sqrt(-g^00) formed in FP64 before it is rounded
+// (the mixed step's FP32 energy flux needs 1 - alpha to its own precision).
+__global__ void __launch_bounds__(256) k_geom32(const double *__restrict__ gcov, const double *__restrict__ gcon,
+                                                const double *__restrict__ gdet, float *__restrict__ g32, int64_t kcol)
+{
+    const int64_t col = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (col >= kcol) return;
+    for (int pos = 0; pos < NPOS; pos++) {
+        float *o = g32 + (int64_t)pos * G32N * kcol + col;
+        const double *gc = gcov + (int64_t)pos * 10 * kcol + col, *gi = gcon + (int64_t)pos * 10 * kcol + col;
+        for (int s = 0; s < 10; s++) {
+            o[s * kcol] = (float)gc[s * kcol];
+            o[(10 + s) * kcol] = (float)gi[s * kcol];
+        }
+        o[20 * kcol] = (float)gdet[(int64_t)pos * kcol + col];
+        o[21 * kcol] = (float)(1. - 1. / sqrt(-gi[0]));
+    }
+}
+
+void launch_geom32(const double *gcov, const double *gcon, const double *gdet, float *g32, int64_t kcol,
+                   hipStream_t s)
+{
+    if (kcol <= 0) return;
+    hipLaunchKernelGGL(k_geom32, dim3((unsigned)((kcol + 255) / 256)), dim3(256), 0, s, gcov, gcon, gdet, g32, kcol);
+}
+
 __global__ void k_dt_finalize(int nblk, unsigned long long *__restrict__ dtmin, double *__restrict__ dt)
 {
     __shared__ double red[256];
@@ -958,6 +1071,18 @@ void launch_update_part(const KParams &P, int part, const Launch &L, int full, d
                         int mixed, hipStream_t s)
 {
     if (L.wl.n <= 0) return;
+    if (part == 3) {  // k_update: parts 0 and 1 in one kernel
+        const size_t lds = sizeof(double) * MV_N * (size_t)L.lds_cols;
+        if (mixed)
+            hipLaunchKernelGGL((k_update<float>), gdim(L), bdim(L), lds, s, P, L.wl, full, Dt_val, dt_ptr, dt_mult, pi,
+                               pb, pf, U0, Unew, psf, (const float *)F1, (const float *)F2, (const float *)F3, pflag,
+                               failcnt);
+        else
+            hipLaunchKernelGGL((k_update<double>), gdim(L), bdim(L), lds, s, P, L.wl, full, Dt_val, dt_ptr, dt_mult, pi,
+                               pb, pf, U0, Unew, psf, (const double *)F1, (const double *)F2, (const double *)F3, pflag,
+                               failcnt);
+        return;
+    }
     if (part == 0) {
         const size_t lds = sizeof(double) * MV_N * (size_t)L.lds_cols;
         if (mixed)
@@ -977,7 +1102,7 @@ void launch_update(const KParams &P, const Launch &L, int full, double Dt_val, c
                    const double *pi, const double *pb, double *pf, double *U0, double *Unew, const double *psf,
                    const void *F1, const void *F2, const void *F3, int *pflag, int *failcnt, int mixed, hipStream_t s)
 {
-    for (int part = 0; part < 3; part++)
+    for (int part : {3, 2})
         launch_update_part(P, part, L, full, Dt_val, dt_ptr, dt_mult, pi, pb, pf, U0, Unew, psf, F1, F2, F3, pflag,
                            failcnt, mixed, s);
 }

@@ -33,20 +33,32 @@ __device__ __forceinline__ int2 wl_get(const WorkList &wl, int w)
 __device__ __forceinline__ int64_t mg10(const BlockDev &B, int pos, int s) { return B.moff * (NPOS * 10) + ((int64_t)pos * 10 + s) * B.kcol; }
 __device__ __forceinline__ int64_t mg1(const BlockDev &B, int pos) { return B.moff * NPOS + (int64_t)pos * B.kcol; }
 
-// Metric at one position of one column, rounded to R (the metric is stored in FP64).
+// Metric at one position of one column in the arithmetic R: the FP64 metric,
+// or for R = float its FP32 copy (the FP64 values rounded, and 1 - alpha
+// formed in FP64 and rounded, by k_geom32 when the metric was uploaded).
 template <class R = double>
 __device__ __forceinline__ Geom<R> load_geom(const KParams &P, const BlockDev &B, int pos, int64_t col)
 {
     Geom<R> G;
-    const double *gc = P.gcov + mg10(B, pos, 0) + col;
-    const double *gi = P.gcon + mg10(B, pos, 0) + col;
+    if constexpr (sizeof(R) == 4) {
+        const float *g = P.g32 + B.moff * (NPOS * G32N) + (int64_t)pos * G32N * B.kcol + col;
 #pragma unroll
-    for (int s = 0; s < 10; s++) {
-        G.gcov[s] = (R)gc[s * B.kcol];
-        G.gcon[s] = (R)gi[s * B.kcol];
+        for (int s = 0; s < 10; s++) {
+            G.gcov[s] = g[s * B.kcol];
+            G.gcon[s] = g[(10 + s) * B.kcol];
+        }
+        G.g = g[20 * B.kcol];
+        G.oma = g[21 * B.kcol];
+    } else {
+        const double *gc = P.gcov + mg10(B, pos, 0) + col;
+        const double *gi = P.gcon + mg10(B, pos, 0) + col;
+#pragma unroll
+        for (int s = 0; s < 10; s++) {
+            G.gcov[s] = gc[s * B.kcol];
+            G.gcon[s] = gi[s * B.kcol];
+        }
+        G.g = P.gdet[mg1(B, pos) + col];
     }
-    G.g = (R)P.gdet[mg1(B, pos) + col];
-    if constexpr (sizeof(R) == 4) G.oma = (R)(1. - 1. / sqrt(-gi[0]));
     return G;
 }
 

@@ -344,10 +344,16 @@ enum {
     HAMR_TUNE_EXCHANGE, HAMR_TUNE_N
 };
 int hamr_autotune(hamr_ctx *ctx, int nrep);
-/* cfg[3]: threads per workgroup, workgroups (0: one per item), XCD order; ms: tuned time (-1: not tuned).
- * For HAMR_TUNE_CONS, workgroups = -1 selects the update's x1-marching form
- * (one thread per (j, z) walking i; the same arithmetic), which the tuner
- * also tries. */
+/* cfg[3]: threads per workgroup, workgroups (0: one per item), flags; ms:
+ * tuned time (-1: not tuned).  Flags bit 0: XCD-contiguous item order; bit 1,
+ * the slot's second form: for HAMR_TUNE_FLUX1-3 line-staged tiles of 256
+ * threads (each cell's slope, interface and limited parabola formed once per
+ * tile) instead of one thread per face; for HAMR_TUNE_CONS the conserved
+ * update and the 2D inversion in two kernels (k_cons, then HAMR_TUNE_INVERT's
+ * k_invert) instead of one (k_update).  For HAMR_TUNE_CONS, workgroups = -1
+ * selects the two-kernel form with the update's x1-marching k_cons (one thread
+ * per (j, z) walking i).  The tuner tries every form; all give the same result
+ * bit for bit. */
 int hamr_tune_get(const hamr_ctx *ctx, int slot, int *cfg, double *ms);
 int hamr_tune_set(hamr_ctx *ctx, int slot, const int *cfg);
 /* Average device time (ms, HIP events on the context stream) of one launch of

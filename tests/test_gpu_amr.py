@@ -191,6 +191,15 @@ def test_graph_tuned_and_per_block_calls_equal_batched_step():
         ctx.step(2, dt=dt)
         got = _state(ctx)
         assert _same(got[0], ref[0]) and _same(got[1], ref[1])
+        # the fluxes' line-staged tiles (blocks of 16, 32 and 64 cells along each
+        # direction: one and two segments per line) and the update in two kernels
+        for f in ("flux1", "flux2", "flux3", "cons"):
+            ctx.tune_set(H.HAMR_TUNE_NAMES.index(f), 256, 0, 0, 1)
+        ctx.set(H.HAMR_P, start[0])
+        ctx.set(H.HAMR_PS, start[1])
+        ctx.step(2, dt=dt)
+        got = _state(ctx)
+        assert _same(got[0], ref[0]) and _same(got[1], ref[1])
         # the update's x1-marching form (blocks of three sizes)
         ctx.tune_set(H.HAMR_TUNE_NAMES.index("cons"), 128, -1, 0)
         ctx.set(H.HAMR_P, start[0])

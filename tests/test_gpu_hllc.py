@@ -25,6 +25,12 @@ pytestmark = pytest.mark.gpu
 
 NG = 3
 TOL = 1e-12
+# Fluxes: 1e-12 cell by cell, but 1e-10 for the entropy flux (KTOT), whose
+# factor P rho^-Gamma comes from the device's pow and the oracle's glibc pow,
+# which may differ in the last ulp; where the HLL (or HLLC) combination of the
+# two sides nearly cancels, that ulp is a larger part of the face's flux
+# (observed 1.5e-12 and 2.3e-11; every other component agrees bit for bit).
+FLUX_TOL = np.array([TOL] * 8 + [1e-10])
 
 
 def _check(name, got, want, tol=TOL):
@@ -87,7 +93,7 @@ def test_hllc_fluxes_match_oracle(setup):
         want = GT.global_to_blocks(desc, Fo[d])
         sl = [slice(NG - 1, NG + desc.bs[k] + 1) for k in range(3)]
         sl[d - 1] = slice(NG, NG + desc.bs[d - 1] + 1)
-        _check("HLLC F%d" % d, got[:, :, sl[0], sl[1], sl[2]], want[:, :, sl[0], sl[1], sl[2]])
+        _check("HLLC F%d" % d, got[:, :, sl[0], sl[1], sl[2]], want[:, :, sl[0], sl[1], sl[2]], tol=FLUX_TOL)
     want_dt = np.zeros_like(bdt)
     dtb = {d: GT.block_view(desc, dto[d]) for d in (1, 2, 3)}
     for n in range(ctx.nblk):
