@@ -71,8 +71,9 @@ def kernel_bytes(kernel, bs, mixed=False):
         "emf_centre": 6 * 8 + 3 * w,        # U1-3, B1-3 read; 3 centre EMFs written
         "emf_corner": 15 * w,               # 3 centre EMFs, 6 flux B-components, 3 mass fluxes read; 3 EMFs written
         "ct_update": 3 * w + 6 * 8,         # 3 EMFs + 3 face B read, 3 face B written
-        "cons": 12 * 8 + 27 * w + 18 * 8,   # 9 prims + 3 face B, 27 fluxes read; U^n and U' written
-        "invert": 18 * 8 + 9 * 8 + 4,       # U' and the guess read; 9 prims and a flag written
+        # k_update (predictor form): 9 prims + 3 face B, 27 fluxes read; U^n, 9 prims and a flag written
+        "cons": 12 * 8 + 27 * w + 18 * 8 + 4,
+        "invert": 18 * 8 + 9 * 8 + 4,       # k_invert (two-kernel form): U' and the guess read; 9 prims, flag written
         "fallback": 4,                      # the flags (the few failed cells are negligible)
         "exchange": 0,
     }
@@ -207,7 +208,7 @@ def amr_bench(H, args):
                          "stages_ms_per_step": {k: round(2 * v, 4) for k, v in stage_ms.items()},
                          "kernels_ms_per_launch": {k: round(v, 4) for k, v in kern_ms.items()},
                          "inversion_failures_2d": int(fi[:, 0].sum()),
-                         "tuned": {k: [v["threads"], v["grid"], v["xcd_order"]] for k, v in tune.items()}
+                         "tuned": {k: [v["threads"], v["grid"], v["xcd_order"], v["second_form"]] for k, v in tune.items()}
                          if tune else None}
         out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
         err = {"steps": args.warmup + steps}
@@ -544,7 +545,7 @@ def main():
         "fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "error_vs_fp64": err,
-        "tuned": {k: [v["threads"], v["grid"], v["xcd_order"]] for k, v in tune.items()} if tune else None,
+        "tuned": {k: [v["threads"], v["grid"], v["xcd_order"], v["second_form"]] for k, v in tune.items()} if tune else None,
         "fp64_second_grid": second,
         "amr": amr_res,
         "adjoint": adj,
