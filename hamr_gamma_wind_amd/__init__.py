@@ -185,6 +185,7 @@ def load_library(path=None):
         "hamr_stage_forward": [ctxp, C.c_double],
         "hamr_stage_adjoint": [ctxp, C.c_double, _dp, _dp, _dp],
         "hamr_stage_adjoint_dev": [ctxp, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p],
+        "hamr_stage_adjoint_dev_noalloc": [ctxp, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p],
         "hamr_create_amr": [D, C.POINTER(AmrDescC), C.POINTER(ctxp)],
         "hamr_desc_amr": [D, C.POINTER(AmrDescC), _llp, _ip, _ip, _ip],
         "hamr_desc_amr_geometry": [D, C.POINTER(AmrDescC), C.c_int, _dp, _dp, _dp, _dp, _dp],
@@ -607,6 +608,12 @@ class Context:
     def stage_adjoint_dev(self, Dt, w_ptr, gp_ptr, gps_ptr):
         self._chk(self.L.hamr_stage_adjoint_dev(self.h, Dt, C.c_void_p(w_ptr), C.c_void_p(gp_ptr),
                                                 C.c_void_p(gps_ptr)), "hamr_stage_adjoint_dev")
+
+    def stage_adjoint_dev_noalloc(self, Dt, w_ptr, gp_ptr, gps_ptr):
+        """stage_adjoint_dev with the work arrays in the library's static device
+        scratch (no allocation; boxes up to the scratch's size)."""
+        self._chk(self.L.hamr_stage_adjoint_dev_noalloc(self.h, Dt, C.c_void_p(w_ptr), C.c_void_p(gp_ptr),
+                                                        C.c_void_p(gps_ptr)), "hamr_stage_adjoint_dev_noalloc")
 
     def set_precision(self, mode):
         """HAMR_PREC_FP64 or HAMR_PREC_MIXED (FP32 reconstruction, fluxes, EMFs)."""

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -101,8 +103,21 @@ struct hamr_ctx {
     int64_t *io_doff = nullptr;
     double *io = nullptr;          // device record of one block (grown on demand)
     size_t io_bytes = 0;
+    double *scratch = nullptr;     // this device's address of the file-scope `scratch` array
 };
 
+// ---- the device scratch of hamr_stage_adjoint_dev_noalloc ----
+// The adjoint's work arrays for the entry point that allocates nothing, in one
+// file-scope device array that every GPU the library runs on holds from the
+// code object's load to the end of the process (kScratchBytes each).  On the
+// host the array's name only stands for the symbol: kernels and copies get
+// the device address that hipGetSymbolAddress returns for the current device,
+// taken once per device (scratch_address) and kept in hamr_ctx::scratch.
+constexpr size_t kScratchBytes = (size_t)HAMR_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;  // device -> address of `scratch` on that device
 
 static int fail(hamr_ctx *c, const std::string &m)
 {
@@ -274,9 +289,26 @@ static Launch flux_block_launch(const hamr_ctx *c, int dir, int n)
 }
 
 // Device geometry slabs and state arrays for the block table already in c.
+// c->scratch: the address of `scratch` on the context's device (current here).
+static int scratch_address(hamr_ctx *c)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(c->device);
+    if (it == g_scratch_ptr.end()) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(c, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        it = g_scratch_ptr.emplace(c->device, (double *)ptr).first;
+    }
+    c->scratch = it->second;
+    return 0;
+}
+
 static int init_device(hamr_ctx *c)
 {
     const size_t nb = (size_t)nblk_of(c), K = (size_t)c->cells;
+    if (scratch_address(c)) return -1;
     if (dalloc(c, &c->p, NPR * K) || dalloc(c, &c->ph, NPR * K) || dalloc(c, &c->ps, 3 * K) ||
         dalloc(c, &c->psh, 3 * K) || dalloc(c, &c->F[0], NPR * K) || dalloc(c, &c->F[1], NPR * K) ||
         dalloc(c, &c->F[2], NPR * K) || dalloc(c, &c->E, 3 * K) || dalloc(c, &c->Ec, 3 * K) ||
@@ -1842,6 +1874,36 @@ int hamr_stage_adjoint_dev(hamr_ctx *c, double Dt, const double *w, double *gp, 
 {
     AdjBufs A;
     if (adjoint_ok(c, w, "hamr_stage_adjoint_dev") || adjoint_bufs(c, A)) return -1;
+    return stage_adjoint(c, Dt, w, gp, gps, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice, A);
+}
+
+// The work arrays carved from this device's `scratch` (c->scratch, resolved at
+// hamr_create): no allocation.  Every accumulator is cleared by a fill kernel
+// inside launch_adjoint; the rest is written before it is read.
+int hamr_stage_adjoint_dev_noalloc(hamr_ctx *c, double Dt, const double *w, double *gp, double *gps)
+{
+    if (adjoint_ok(c, w, "hamr_stage_adjoint_dev_noalloc")) return -1;
+    const int64_t K = c->blocks[0].ks;
+    const int64_t need = kAdjDoublesPerCell * K * (int64_t)sizeof(double);
+    if (need > (int64_t)kScratchBytes)
+        return fail(c, "hamr_stage_adjoint_dev_noalloc: the box needs " + std::to_string(need) +
+                           " bytes of work arrays, above the " + std::to_string(kScratchBytes) +
+                           "-byte scratch (HAMR_NOALLOC_SCRATCH_BYTES)");
+    if (!c->scratch) return fail(c, "hamr_stage_adjoint_dev_noalloc: no scratch address on this device");
+    double *q = c->scratch;
+    auto take = [&](int64_t ncomp) {
+        double *r = q;
+        q += ncomp * K;
+        return r;
+    };
+    AdjBufs A;
+    A.w = take(NPR);
+    A.lP = take(NPR);
+    A.lps = take(3);
+    A.lpsf = take(3);
+    A.lE = take(3);
+    A.lEc = take(3);
+    for (int d = 0; d < 3; d++) A.lF[d] = take(NPR);
     return stage_adjoint(c, Dt, w, gp, gps, hipMemcpyDeviceToDevice, hipMemcpyDeviceToDevice, A);
 }
 

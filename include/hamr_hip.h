@@ -379,6 +379,24 @@ int hamr_stage_forward(hamr_ctx *ctx, double Dt);
 int hamr_stage_adjoint(hamr_ctx *ctx, double Dt, const double *w, double *grad_p, double *grad_ps);
 int hamr_stage_adjoint_dev(hamr_ctx *ctx, double Dt, const double *w_dev, double *grad_p_dev, double *grad_ps_dev);
 
+/* hamr_stage_adjoint_dev without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, training or serving loops that forbid hipMalloc).  The
+ * adjoint's work arrays (57 doubles per cell of the block's padded slab) live
+ * in a file-scope device array of HAMR_NOALLOC_SCRATCH_BYTES bytes that each
+ * GPU the library runs on holds until the process ends; a box whose work
+ * arrays do not fit is refused (-1, hamr_last_error says so): at 128 MiB,
+ * boxes of up to 294,337 ghost-inclusive cells, e.g. 60^3 interior zones.
+ * Inputs and outputs as hamr_stage_adjoint_dev (device pointers).  The call
+ * makes no hipMalloc/hipFree or other allocating call; it clears the
+ * accumulators it needs with a fill kernel and synchronises the context's
+ * stream before returning.  All calls on one GPU share that GPU's array, so
+ * calls that overlap on one GPU (several contexts on several host threads or
+ * streams) race; calls in sequence do not. */
+#define HAMR_NOALLOC_SCRATCH_BYTES (128ll << 20)
+int hamr_stage_adjoint_dev_noalloc(hamr_ctx *ctx, double Dt, const double *w_dev, double *grad_p_dev,
+                                   double *grad_ps_dev);
+
 /* ---- libhamr_host.so: the reference's host-side step in C ----
  * advance_GPU / GPU_step_ch (step_ch.c:916-1336) over the per-block entry
  * points above for blocks 0..nblk-1 (every block at timelevel 1), each block's
