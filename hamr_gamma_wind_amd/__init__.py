@@ -211,6 +211,13 @@ def load_library(path=None):
         "hamr_dump_block_write": [ctxp, C.c_int, C.POINTER(C.c_float)],
         "hamr_dump_offsets": [ctxp, _llp],
         "hamr_dump_write_dev": [ctxp, C.c_void_p],
+        "hamr_set_timelevels": [ctxp, _ip],
+        "hamr_get_timelevels": [ctxp, _ip],
+        "hamr_block_dts": [ctxp, _dp],
+        "hamr_timelevels_from_dt": [ctxp, C.c_int, _ip],
+        "hamr_las_rule": [C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _ip, _ip],
+        "hamr_step_las": [ctxp, C.c_int, C.c_int, _dp],
+        "hamr_las_info": [ctxp, _llp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -757,6 +764,58 @@ class AmrContext(Context):
 
     def all_flags(self):
         return [self.flags(n) for n in range(self.nblk)]
+
+    # local adaptive time stepping
+    def set_timelevels(self, tl):
+        v = np.ascontiguousarray(tl, dtype=np.int32)
+        assert v.size == self.nblk
+        self._chk(self.L.hamr_set_timelevels(self.h, _ptr(v, _ip)), "hamr_set_timelevels")
+
+    def timelevels(self):
+        v = np.zeros(self.nblk, dtype=np.int32)
+        self._chk(self.L.hamr_get_timelevels(self.h, _ptr(v, _ip)), "hamr_get_timelevels")
+        return v
+
+    def block_dts(self):
+        v = np.zeros(self.nblk)
+        self._chk(self.L.hamr_block_dts(self.h, _ptr(v)), "hamr_block_dts")
+        return v
+
+    def timelevels_from_dt(self, maxtl):
+        v = np.zeros(self.nblk, dtype=np.int32)
+        self._chk(self.L.hamr_timelevels_from_dt(self.h, maxtl, _ptr(v, _ip)), "hamr_timelevels_from_dt")
+        return v
+
+    def step_las(self, maxtl, ncycles=1, dt=None):
+        """ncycles cycles of 2 maxtl sub-iterations (each advances every block
+        by maxtl dt); returns the next dt."""
+        v = C.c_double(dt if dt is not None else -1.0)
+        self._chk(self.L.hamr_step_las(self.h, maxtl, ncycles, C.byref(v)), "hamr_step_las")
+        return v.value
+
+    def las_info(self):
+        v = np.zeros(6, dtype=np.int64)
+        self._chk(self.L.hamr_las_info(self.h, _ptr(v, _llp)), "hamr_las_info")
+        return dict(flux1=int(v[0]), flux2=int(v[1]), flux3=int(v[2]), emf_same=int(v[3]), emf_jump=int(v[4]),
+                    tlmax=int(v[5]))
+
+
+def las_rule(bdt, dt, maxtl, rings=(), pairs=()):
+    """set_timelevel's rule (hamr_las_rule) on plain arrays: rings a list of
+    equal-length block-id lists, pairs (coarse, finer) block pairs."""
+    L = load_library()
+    bdt = np.ascontiguousarray(bdt, dtype=float)
+    n = bdt.size
+    r = np.ascontiguousarray(np.array(rings, dtype=np.int32).reshape(len(rings), -1) if len(rings) else
+                             np.zeros((0, 0), dtype=np.int32))
+    pr = np.ascontiguousarray(np.array(pairs, dtype=np.int32).reshape(-1, 2) if len(pairs) else
+                              np.zeros((0, 2), dtype=np.int32))
+    tl = np.zeros(n, dtype=np.int32)
+    rc = L.hamr_las_rule(n, _ptr(bdt), float(dt), int(maxtl), r.shape[0], r.shape[1] if r.size else 0,
+                         _ptr(r, _ip), pr.shape[0], _ptr(pr, _ip), _ptr(tl, _ip))
+    if rc:
+        raise HamrError("hamr_las_rule: bad arguments")
+    return tl
 
 
 def advance_step(ctx, dt):

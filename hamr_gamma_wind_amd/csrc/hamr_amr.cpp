@@ -21,8 +21,14 @@
 //   - in a polar ghost row (x2 beyond the axis, stage 3): a copy of the
 //     same-level leaf that holds the row's clamped position, at Y itself (that
 //     leaf's mirror ghost);
-//   - beyond the radial boundary along an earlier direction: left to the final
-//     outflow pass, which applies the outflow rule to the block's own edge cell;
+//   - beyond the block along an earlier direction (a corner ghost) when the leaf
+//     across the face that holds the position clamped into the block's range
+//     is on level l: a copy of that leaf's storage cell at Y (its ghost of the
+//     earlier stage), so that face fields there are the neighbour's own, as the
+//     reference's face exchanges carry the neighbour's ghost rows;
+//   - otherwise beyond the radial boundary along an earlier direction: left to
+//     the final outflow pass, which applies the outflow rule to the block's own
+//     edge cell;
 //   - otherwise (x3 periodic wrap): the leaf L containing Y gives a copy
 //     (level l), the mean of the 2x2x2 children (level l+1), or the
 //     MC-limited linear prolongation of its cell (level l-1), whose neighbour
@@ -390,6 +396,26 @@ std::string build_amr(const GridDesc &d, const AmrDesc &a, AmrGrid &g)
                                 g.ex[d0].push_back({n, dcell, c, (AS_CELL << 28) | cell_in(b.blk[c], Y)});
                                 continue;
                             }
+                            // a corner ghost (beyond the block along an earlier
+                            // direction): the same-level leaf across this face that
+                            // holds the clamped position has it in its storage (its
+                            // ghost of the earlier stage), face fields included
+                            bool corner = false;
+                            int Yk[3] = {Y[0], Y[1], Y[2]};
+                            const int loc[3] = {i, j, z};
+                            for (int e = 0; e < d0; e++)
+                                if (loc[e] < NG || loc[e] >= NG + B.bs[e]) {
+                                    corner = true;
+                                    Yk[e] = std::min(std::max(Y[e], B.o[e]), B.o[e] + B.bs[e] - 1);
+                                }
+                            if (corner) {
+                                int lk, k = b.find(l, Yk, &lk);
+                                if (k >= 0 && lk == l) {
+                                    if (!storage_of(b.blk[k], Y)) return "amr: corner ghost outside its source";
+                                    g.ex[d0].push_back({n, dcell, k, (AS_CELL << 28) | cell_in(b.blk[k], Y)});
+                                    continue;
+                                }
+                            }
                             if (out1) continue;  // final outflow pass
                             int lc, c = b.find(l, Y, &lc);
                             if (c < 0) return "amr: neighbouring blocks more than one level apart";
@@ -580,6 +606,192 @@ static bool src_ok(const std::vector<BlockDev> &blk, int b, int code)
     if (kind != AS_AVG8) return false;
     const int64_t last = (int64_t)c + (int64_t)S.D[1] * S.D[2] + S.D[2] + 1;
     return c >= 0 && last < ncell;
+}
+
+// ---- local adaptive time stepping ----
+
+namespace {
+
+// Leaf lookup over a finished block set: owner of each 16^3 unit per level.
+struct LeafMap {
+    int nlev = 0;
+    int N[kAmrMaxLevels][3] = {};
+    std::vector<int> own[kAmrMaxLevels];
+    int U(int l, int k) const { return N[l][k] / UNIT; }
+    int &at(int l, int a, int b, int c) { return own[l][((size_t)a * U(l, 1) + b) * U(l, 2) + c]; }
+    int get(int l, int a, int b, int c) const { return own[l][((size_t)a * U(l, 1) + b) * U(l, 2) + c]; }
+    explicit LeafMap(const AmrGrid &g)
+    {
+        nlev = g.nlev;
+        for (int l = 0; l < nlev; l++) {
+            for (int k = 0; k < 3; k++) N[l][k] = g.N[l][k];
+            own[l].assign((size_t)U(l, 0) * U(l, 1) * U(l, 2), -1);
+        }
+        for (int n = 0; n < (int)g.blocks.size(); n++) {
+            const BlockDev &B = g.blocks[n];
+            for (int a = B.o[0] / UNIT; a < (B.o[0] + B.bs[0]) / UNIT; a++)
+                for (int b = B.o[1] / UNIT; b < (B.o[1] + B.bs[1]) / UNIT; b++)
+                    for (int c = B.o[2] / UNIT; c < (B.o[2] + B.bs[2]) / UNIT; c++) at(B.level, a, b, c) = n;
+        }
+    }
+    // leaf holding cell Y of level l (inside the grid), *lc its level; -1 if none
+    int find(int l, const int Y[3], int *lc) const
+    {
+        int o = get(l, Y[0] / UNIT, Y[1] / UNIT, Y[2] / UNIT);
+        if (o >= 0) { *lc = l; return o; }
+        if (l + 1 < nlev) {
+            o = get(l + 1, 2 * Y[0] / UNIT, 2 * Y[1] / UNIT, 2 * Y[2] / UNIT);
+            if (o >= 0) { *lc = l + 1; return o; }
+        }
+        for (int c = l - 1; c >= 0; c--) {
+            const int sh = l - c;
+            o = get(c, (Y[0] >> sh) / UNIT, (Y[1] >> sh) / UNIT, (Y[2] >> sh) / UNIT);
+            if (o >= 0) { *lc = c; return o; }
+        }
+        return -1;
+    }
+};
+
+}  // namespace
+
+std::vector<std::pair<int, int>> jump_pairs(const AmrGrid &g)
+{
+    std::vector<std::pair<int, int>> v;
+    for (int d = 0; d < 3; d++)
+        for (const AmrReflux &e : g.rfx[d]) v.emplace_back(e.cb, e.fb);
+    for (const AmrEmf &e : g.emf) v.emplace_back(e.cb, e.fb);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return v;
+}
+
+void las_rule(int nblk, const double *bdt, double dt, int maxtl, const std::vector<int> &ring, int ring_len,
+              const std::vector<std::pair<int, int>> &pairs, int *tl)
+{
+    int imax = 0;
+    while ((2 << imax) <= maxtl) imax++;
+    for (int n = 0; n < nblk; n++) {
+        tl[n] = 1;
+        for (int i = imax; i >= 0; i--)
+            if (bdt[n] / dt >= 1.05 * std::pow(2., i)) {
+                tl[n] = 1 << i;
+                break;
+            }
+    }
+    const int nring = ring_len > 0 ? (int)ring.size() / ring_len : 0;
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (int r = 0; r < nring; r++) {
+            int m = kMaxTimeLevel;
+            for (int k = 0; k < ring_len; k++) m = std::min(m, tl[ring[(size_t)r * ring_len + k]]);
+            for (int k = 0; k < ring_len; k++) {
+                int &x = tl[ring[(size_t)r * ring_len + k]];
+                if (x != m) { x = m; changed = true; }
+            }
+        }
+        for (const auto &pr : pairs)
+            if (tl[pr.second] > tl[pr.first]) {
+                tl[pr.second] = tl[pr.first];
+                changed = true;
+            }
+    }
+}
+
+std::string build_las_tables(const AmrGrid &g, const std::vector<int> &tl, LasTables &t)
+{
+    const auto &blk = g.blocks;
+    const int nblk = (int)blk.size();
+    if ((int)tl.size() != nblk) return "las: one time level per block";
+    for (int n = 0; n < nblk; n++)
+        if (tl[n] < 1 || tl[n] > kMaxTimeLevel || (tl[n] & (tl[n] - 1)))
+            return "las: time levels are powers of 2 from 1 to " + std::to_string(kMaxTimeLevel);
+    if (g.ring_len > 0)
+        for (size_t r = 0; r < g.ring.size(); r += g.ring_len)
+            for (int k = 1; k < g.ring_len; k++)
+                if (tl[g.ring[r + k]] != tl[g.ring[r]]) return "las: the blocks of a polar ring share one time level";
+    for (const auto &pr : jump_pairs(g))
+        if (tl[pr.second] > tl[pr.first])
+            return "las: a finer block at a level jump has a larger time level than its coarser neighbour";
+    const LeafMap M(g);
+    auto cell = [](const BlockDev &B, const int K[3]) {
+        return ((K[0] - B.o[0] + NG) * B.D[1] + (K[1] - B.o[1] + NG)) * B.D[2] + (K[2] - B.o[2] + NG);
+    };
+    // fluxes: level jumps, then same-level faces whose neighbour is faster
+    for (int d = 0; d < 3; d++) {
+        auto &v = t.flux[d];
+        v.clear();
+        for (const AmrReflux &e : g.rfx[d]) v.push_back({e.cb, e.cf, e.fb, e.ff | LAS_MEAN});
+        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        for (int n = 0; n < nblk; n++) {
+            const BlockDev &B = blk[n];
+            const int l = B.level;
+            for (int side = 0; side < 2; side++) {
+                int Y[3];
+                Y[d] = side ? B.o[d] + B.bs[d] : B.o[d] - 1;
+                if (d < 2 && (Y[d] < 0 || Y[d] >= M.N[l][d])) continue;
+                if (d == 2) Y[2] = ((Y[2] % M.N[l][2]) + M.N[l][2]) % M.N[l][2];
+                for (int ya = B.o[ta]; ya < B.o[ta] + B.bs[ta]; ya++)
+                    for (int yb = B.o[tb]; yb < B.o[tb] + B.bs[tb]; yb++) {
+                        Y[ta] = ya;
+                        Y[tb] = yb;
+                        int lc = -1;
+                        const int m = M.find(l, Y, &lc);
+                        if (m < 0 || lc != l || m == n || tl[m] >= tl[n]) continue;
+                        const BlockDev &S = blk[m];
+                        int K[3] = {Y[0], Y[1], Y[2]}, Ks[3] = {Y[0], Y[1], Y[2]};
+                        K[d] = side ? B.o[d] + B.bs[d] : B.o[d];
+                        Ks[d] = side ? S.o[d] : S.o[d] + S.bs[d];
+                        v.push_back({n, cell(B, K), m, cell(S, Ks)});
+                    }
+            }
+        }
+    }
+    // EMFs: level-jump edges as g.emf; other boundary edges take the EMF of
+    // the fastest same-level leaf around them (first in the quadrant order
+    // (-,-), (-,+), (+,-), (+,+) on ties) when it is faster than the block
+    t.emf_jump.clear();
+    t.emf_same.clear();
+    for (const AmrEmf &e : g.emf) t.emf_jump.push_back({e.cb, e.cc, e.fb, e.fc | LAS_MEAN});
+    for (int n = 0; n < nblk; n++) {
+        const BlockDev &B = blk[n];
+        const int l = B.level;
+        for (int tt = 0; tt < 3; tt++) {
+            const int p = (tt + 1) % 3, q = (tt + 2) % 3;
+            int K[3];
+            for (K[tt] = B.o[tt]; K[tt] < B.o[tt] + B.bs[tt]; K[tt]++)
+                for (K[p] = B.o[p]; K[p] <= B.o[p] + B.bs[p]; K[p]++) {
+                    const bool pb = (K[p] == B.o[p] || K[p] == B.o[p] + B.bs[p]);
+                    for (K[q] = B.o[q]; K[q] <= B.o[q] + B.bs[q]; K[q] += (pb ? 1 : B.bs[q])) {
+                        int best = -1, finer = 0;
+                        for (int qd = 0; qd < 4; qd++) {
+                            int Y[3] = {K[0], K[1], K[2]}, in = 1;
+                            Y[p] += (qd >> 1) - 1;
+                            Y[q] += (qd & 1) - 1;
+                            for (int x = 0; x < 2; x++)
+                                if (Y[x] < 0 || Y[x] >= M.N[l][x]) in = 0;
+                            Y[2] = ((Y[2] % M.N[l][2]) + M.N[l][2]) % M.N[l][2];
+                            if (!in) continue;
+                            int lc = -1;
+                            const int m = M.find(l, Y, &lc);
+                            if (m < 0) continue;
+                            if (lc == l + 1) finer = 1;
+                            if (lc == l && (best < 0 || tl[m] < tl[best])) best = m;
+                        }
+                        if (finer || best < 0 || tl[best] >= tl[n]) continue;
+                        const BlockDev &S = blk[best];
+                        int Ks[3] = {K[0], K[1], K[2]};
+                        if (Ks[2] < S.o[2]) Ks[2] += M.N[l][2];
+                        if (Ks[2] > S.o[2] + S.bs[2]) Ks[2] -= M.N[l][2];
+                        for (int x = 0; x < 3; x++) {
+                            const int hi = S.o[x] + S.bs[x] - (x == tt ? 1 : 0);
+                            if (Ks[x] < S.o[x] || Ks[x] > hi) return "las: EMF correction source outside its block";
+                        }
+                        t.emf_same.push_back({n, cell(B, K) | (tt << 28), best, cell(S, Ks)});
+                    }
+                }
+        }
+    }
+    return "";
 }
 
 std::string validate_amr(const AmrGrid &g)

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "hamr_grid.h"
@@ -45,5 +46,27 @@ struct AmrGrid {
 std::string build_amr(const GridDesc &d, const AmrDesc &a, AmrGrid &g);
 // Range checks of every table index (run by build_amr).
 std::string validate_amr(const AmrGrid &g);
+
+// Local adaptive time stepping on a multi-level set (time level tl[n] of block
+// n, powers of 2 up to kMaxTimeLevel).
+constexpr int kMaxTimeLevel = 16;
+struct LasTables {
+    std::vector<LasCorr> flux[3];   // per direction: level jumps (g.rfx) and same-level faces, tl_s < tl_r
+    std::vector<LasCorr> emf_same;  // same-level edges taking the fastest neighbour's EMF
+    std::vector<LasCorr> emf_jump;  // level-jump edges (g.emf)
+};
+// The time levels are checked first: powers of 2 up to kMaxTimeLevel, equal
+// over each polar ring, and no finer block at a level jump slower than its
+// coarser neighbour.  Returns "" or an error.
+std::string build_las_tables(const AmrGrid &g, const std::vector<int> &tl, LasTables &t);
+// The time-level rule of set_timelevel (LAS.c:5-144, PRESTEP = 0): block n gets
+// the largest 2^i <= maxtl with bdt[n] / dt >= 1.05 2^i (else 1), each polar
+// ring the minimum of its blocks; then, for the level-jump rule above, a finer
+// block at a jump takes at most its coarser neighbour's level (pairs: (coarse,
+// finer) block pairs), repeated with the ring minimum until nothing changes.
+void las_rule(int nblk, const double *bdt, double dt, int maxtl, const std::vector<int> &ring, int ring_len,
+              const std::vector<std::pair<int, int>> &pairs, int *tl);
+// (coarse, finer) block pairs of the level-jump corrections of g, sorted and unique.
+std::vector<std::pair<int, int>> jump_pairs(const AmrGrid &g);
 
 }  // namespace hamr

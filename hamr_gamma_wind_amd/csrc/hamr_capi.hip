@@ -104,6 +104,17 @@ struct hamr_ctx {
     double *io = nullptr;          // device record of one block (grown on demand)
     size_t io_bytes = 0;
     double *scratch = nullptr;     // this device's address of the file-scope `scratch` array
+    // local adaptive time stepping (multi-level sets)
+    AmrGrid grid;                  // the set's blocks, rings, levels and correction tables (no exchange tables)
+    std::vector<int> tl;           // time level per block (empty: not set)
+    int tlmax = 1;                 // largest time level
+    double *bdt = nullptr;         // per-block Courant dt of the last corrector that recorded them
+    std::vector<void *> las_allocs;
+    int2 *las_items = nullptr;     // filtered work lists of every (sub-iteration, phase) set of blocks
+    std::vector<int64_t> las_off;  // [(s * 2 + phase - 1) * KC_N + cls] -> {start, count} pairs
+    LasCorr *las_f[3] = {nullptr, nullptr, nullptr}, *las_es = nullptr, *las_ej = nullptr;
+    int64_t nlas_f[3] = {0, 0, 0}, nlas_es = 0, nlas_ej = 0;
+    double *acc_f[3] = {nullptr, nullptr, nullptr}, *acc_es = nullptr, *acc_ej = nullptr;
 };
 
 // ---- the device scratch of hamr_stage_adjoint_dev_noalloc ----
@@ -330,7 +341,9 @@ static int init_device(hamr_ctx *c)
         c->roff[n + 1] = c->roff[n] + RREC * ncell_of(B);
         c->doff[n + 1] = c->doff[n] + DREC * (int64_t)B.bs[0] * B.bs[1] * B.bs[2];
     }
-    if (dalloc(c, &c->io_blist, nb) || dalloc(c, &c->io_roff, nb + 1) || dalloc(c, &c->io_doff, nb + 1)) return -1;
+    if (dalloc(c, &c->io_blist, nb) || dalloc(c, &c->io_roff, nb + 1) || dalloc(c, &c->io_doff, nb + 1) ||
+        dalloc(c, &c->bdt, nb))
+        return -1;
     if (h2d(c, c->io_blist, bl.data(), nb * sizeof(int))) return -1;
     if (h2d(c, c->io_roff, c->roff.data(), (nb + 1) * sizeof(int64_t))) return -1;
     if (h2d(c, c->io_doff, c->doff.data(), (nb + 1) * sizeof(int64_t))) return -1;
@@ -349,6 +362,8 @@ static int init_device(hamr_ctx *c)
     P.tet = c->tet;
     P.g32 = c->g32;
     P.riemann = HAMR_RIEMANN_HLL;
+    P.las_s = -1;
+    P.las_want = LAS_IDLE;
     launch_dt_reset((int)nb, c->dtmin, c->s);
     if (check_launch(c, "dt reset")) return -1;
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -589,6 +604,9 @@ int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *a, hamr_ctx **ou
             if (h2d(c, c->emfx, g.emf.data(), g.emf.size() * sizeof(AmrEmf))) return -1;
         }
     }
+    g.ex[0].clear(); g.ex[1].clear(); g.ex[2].clear();
+    g.pro[0].clear(); g.pro[1].clear(); g.pro[2].clear();
+    c->grid = std::move(g);
     c->err.clear();
     return 0;
 }
@@ -635,6 +653,7 @@ int hamr_destroy(hamr_ctx *c)
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->s) (void)hipStreamSynchronize(c->s);
     for (void *v : c->allocs) (void)hipFree(v);
+    for (void *v : c->las_allocs) (void)hipFree(v);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->io) (void)hipFree(c->io);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -1488,7 +1507,7 @@ static void enqueue_step(hamr_ctx *c)
     launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 1.0, c->ps, c->ps, c->E, c->prec, s);
     update_all(c, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->ps);
     bound_all(c, c->p, c->ps, true);
-    launch_dt_finalize(nb, c->dtmin, c->dt, s);
+    launch_dt_finalize(nb, c->dtmin, c->dt, s, c->bdt);
 }
 
 int hamr_estimate_dt(hamr_ctx *c, double *dt)
@@ -1501,7 +1520,7 @@ int hamr_estimate_dt(hamr_ctx *c, double *dt)
     launch_dt_reset(nblk_of(c), c->dtmin, c->s);
     for (int dir = 1; dir <= 3; dir++)
         launch_flux(c->P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->p, c->ps, c->F[dir - 1], c->dtmin, c->prec, c->s);
-    launch_dt_finalize(nblk_of(c), c->dtmin, c->dt, c->s);
+    launch_dt_finalize(nblk_of(c), c->dtmin, c->dt, c->s, c->bdt);
     if (check_launch(c, "hamr_estimate_dt")) return -1;
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (d2h(c, dt, c->dt, sizeof(double))) return -1;
@@ -1547,6 +1566,315 @@ int hamr_step_graph(hamr_ctx *c, int nsteps, double *dt_inout)
         (void)hipGraphDestroy(g);
     }
     for (int k = 0; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->s));
+    if (dt_inout) {
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (d2h(c, dt_inout, c->dt, sizeof(double))) return -1;
+    }
+    return 0;
+}
+
+// ---- local adaptive time stepping (multi-level sets) ----
+//
+// A cycle of 2 maxtl sub-iterations s (GPU_step_ch, step_ch.c:916-979, with
+// PRESTEP = PRESTEP2 = 0 and AMR_SWITCHTIMELEVEL = AMR_MAXTIMELEVEL = maxtl):
+// a block of time level tl takes its predictor (Dt = tl dt / 2, p -> ph) when
+// s % tl == tl - 1 and its corrector (Dt = tl dt, ph -> p) when s % 2tl ==
+// 2tl - 1 (advance_GPU, step_ch.c:997-1262).  Each stage runs twice per
+// sub-iteration, over the work lists of the blocks in each phase; fluxes and
+// EMFs are corrected across time levels and level jumps (k_las_corr); the
+// ghost fill exchanges between blocks that both step and writes both array
+// pairs (k_amr_copy / k_amr_prolong); the Courant minima of the last
+// sub-iteration, where every block takes its corrector, give the next dt
+// (min over blocks of the harmonic, clamped to 1.3 dt).
+
+}  // extern "C"
+
+template <class T> static int las_alloc(hamr_ctx *c, T **ptr, size_t n)
+{
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, n * sizeof(T) + 256);
+    if (e != hipSuccess) return fail(c, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->las_allocs.push_back(v);
+    e = hipMemsetAsync(v, 0, n * sizeof(T) + 256, c->s);
+    if (e != hipSuccess) return fail(c, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    *ptr = (T *)v;
+    return 0;
+}
+
+extern "C" {
+
+static void las_release(hamr_ctx *c)
+{
+    (void)hipStreamSynchronize(c->s);
+    for (void *v : c->las_allocs) (void)hipFree(v);
+    c->las_allocs.clear();
+    c->las_items = nullptr;
+    c->las_off.clear();
+    for (int d = 0; d < 3; d++) {
+        c->las_f[d] = nullptr;
+        c->acc_f[d] = nullptr;
+        c->nlas_f[d] = 0;
+    }
+    c->las_es = c->las_ej = nullptr;
+    c->acc_es = c->acc_ej = nullptr;
+    c->nlas_es = c->nlas_ej = 0;
+}
+
+// the kernel classes a cycle launches over filtered lists
+constexpr int kLasClasses[] = {KC_FLUX1, KC_FLUX2, KC_FLUX3, KC_ECENT, KC_CORNER, KC_INTERIOR, KC_BOUND1, KC_BOUND2};
+
+static int log2i(int v)
+{
+    int i = 0;
+    while ((1 << (i + 1)) <= v) i++;
+    return i;
+}
+
+// Time levels tl (validated by build_las_tables) into the block table, the
+// filtered work lists and the correction tables.
+static int set_timelevels(hamr_ctx *c, const std::vector<int> &tl)
+{
+    LasTables t;
+    const std::string e = build_las_tables(c->grid, tl, t);
+    if (!e.empty()) return fail(c, "hamr_set_timelevels: " + e);
+    las_release(c);
+    const int nb = nblk_of(c);
+    c->tl = tl;
+    c->tlmax = 1;
+    for (int n = 0; n < nb; n++) {
+        c->blocks[n].tl = tl[n];
+        c->tlmax = std::max(c->tlmax, tl[n]);
+    }
+    if (h2d(c, c->blk, c->blocks.data(), (size_t)nb * sizeof(BlockDev))) return -1;
+    // one list per class for each set of time levels in one phase; the lists of
+    // sub-iteration s and phase ph are those of the levels in phase ph at s
+    std::map<int, std::vector<int64_t>> built;
+    std::vector<int2> items;
+    const int nsub = 2 * c->tlmax;
+    c->las_off.assign((size_t)nsub * 2 * KC_N * 2, 0);
+    for (int sub = 0; sub < nsub; sub++)
+        for (int ph = LAS_HALF; ph <= LAS_FULL; ph++) {
+            int mask = 0;
+            for (int i = 0; (1 << i) <= kMaxTimeLevel; i++)
+                if (las_phase(1 << i, sub) == ph) mask |= 1 << i;
+            auto it = built.find(mask);
+            if (it == built.end()) {
+                std::vector<int64_t> v((size_t)KC_N * 2, 0);
+                for (int cls : kLasClasses) {
+                    const ClassList &L = c->cl[cls];
+                    v[(size_t)cls * 2] = (int64_t)items.size();
+                    for (int n : c->order) {
+                        if (!((mask >> log2i(tl[n])) & 1)) continue;
+                        for (int k = 0; k < L.count[kDefaultTile][n]; k++) items.push_back(make_int2(n, k));
+                    }
+                    v[(size_t)cls * 2 + 1] = (int64_t)items.size() - v[(size_t)cls * 2];
+                }
+                it = built.emplace(mask, std::move(v)).first;
+            }
+            std::copy(it->second.begin(), it->second.end(),
+                      c->las_off.begin() + ((size_t)sub * 2 + ph - 1) * KC_N * 2);
+        }
+    if (!items.empty()) {
+        if (las_alloc(c, &c->las_items, items.size())) return -1;
+        if (h2d(c, c->las_items, items.data(), items.size() * sizeof(int2))) return -1;
+    }
+    auto table = [&](const std::vector<LasCorr> &v, LasCorr **dst, int64_t *n, double **acc, int ncomp) {
+        *n = (int64_t)v.size();
+        if (v.empty()) return 0;
+        if (las_alloc(c, dst, v.size()) || las_alloc(c, acc, v.size() * ncomp)) return -1;
+        return h2d(c, *dst, v.data(), v.size() * sizeof(LasCorr));
+    };
+    for (int d = 0; d < 3; d++)
+        if (table(t.flux[d], &c->las_f[d], &c->nlas_f[d], &c->acc_f[d], NPR)) return -1;
+    if (table(t.emf_same, &c->las_es, &c->nlas_es, &c->acc_es, 1) ||
+        table(t.emf_jump, &c->las_ej, &c->nlas_ej, &c->acc_ej, 1))
+        return -1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+// Launch of class cls over the blocks in phase ph at sub-iteration s.
+static Launch las_launch(const hamr_ctx *c, int cls, int s, int ph)
+{
+    const int sub = s % (2 * c->tlmax);
+    const int64_t *o = &c->las_off[(((size_t)sub * 2 + ph - 1) * KC_N + cls) * 2];
+    Launch x;
+    x.threads = kTiles[kDefaultTile];
+    x.wl.items = c->las_items ? c->las_items + o[0] : nullptr;
+    x.wl.n = (int)o[1];
+    x.wl.swz = 0;
+    x.grid = x.wl.n;
+    x.lds_cols = c->cl[cls].lds_cols[kDefaultTile];
+    x.line = 0;
+    return x;
+}
+
+// Arrays of the blocks in phase ph: predictor p/ps -> ph/psh, corrector ph/psh -> p/ps.
+static double *las_in(hamr_ctx *c, int ph) { return ph == LAS_HALF ? c->p : c->ph; }
+static double *las_ins(hamr_ctx *c, int ph) { return ph == LAS_HALF ? c->ps : c->psh; }
+static double *las_out(hamr_ctx *c, int ph) { return ph == LAS_HALF ? c->ph : c->p; }
+static double *las_outs(hamr_ctx *c, int ph) { return ph == LAS_HALF ? c->psh : c->ps; }
+
+// Ghost fill at the end of sub-iteration s: physical faces of the stepping
+// blocks on their updated arrays, exchanges between blocks that both step.
+static void las_bound(hamr_ctx *c, int s)
+{
+    const KParams &P = c->P;
+    const int T = kTiles[c->tune[TS_EXCHANGE].tile], G = c->tune[TS_EXCHANGE].grid;
+    auto phys = [&](int cls, int dir, int final_pass) {
+        for (int ph = LAS_HALF; ph <= LAS_FULL; ph++) {
+            const Launch L = las_launch(c, cls, s, ph);
+            if (L.wl.n) launch_bound(P, dir, L, las_out(c, ph), las_outs(c, ph), c->pflag, final_pass, c->s);
+        }
+    };
+    phys(KC_BOUND1, 1, 0);
+    launch_amr_exchange(P, 1, c->ex[0], c->nex[0], c->pro[0], c->npro[0], c->p, c->ps, c->pflag, T, G, c->s, c->ph,
+                        c->psh);
+    phys(KC_BOUND2, 2, 0);
+    launch_amr_exchange(P, 2, c->ex[1], c->nex[1], c->pro[1], c->npro[1], c->p, c->ps, c->pflag, T, G, c->s, c->ph,
+                        c->psh);
+    launch_amr_exchange(P, 3, c->ex[2], c->nex[2], c->pro[2], c->npro[2], c->p, c->ps, c->pflag, T, G, c->s, c->ph,
+                        c->psh);
+    phys(KC_BOUND1, 1, 1);
+}
+
+static void enqueue_las_cycle(hamr_ctx *c, int maxtl)
+{
+    KParams &P = c->P;
+    hipStream_t st = c->s;
+    const int mixed = c->prec;
+    for (int s = 0; s < 2 * maxtl; s++) {
+        P.las_s = s;
+        P.las_want = LAS_IDLE;
+        const bool last = s == 2 * maxtl - 1;
+        for (int ph = LAS_HALF; ph <= LAS_FULL; ph++)
+            for (int dir = 1; dir <= 3; dir++) {
+                const Launch L = las_launch(c, KC_FLUX1 + dir - 1, s, ph);
+                if (L.wl.n)
+                    launch_flux(P, dir, L, las_in(c, ph), las_ins(c, ph), c->F[dir - 1], last ? c->dtmin : nullptr,
+                                mixed, st);
+            }
+        for (int d = 0; d < 3; d++)
+            for (int stage = 0; stage < 2; stage++)
+                launch_las_corr(P, c->las_f[d], c->nlas_f[d], d + 1, stage, c->F[d], c->acc_f[d], c->dt, mixed, st);
+        for (int ph = LAS_HALF; ph <= LAS_FULL; ph++) {
+            const Launch Lc = las_launch(c, KC_ECENT, s, ph), Le = las_launch(c, KC_CORNER, s, ph);
+            if (Lc.wl.n && Le.wl.n)
+                launch_emf(P, Lc, Le, las_in(c, ph), c->F[0], c->F[1], c->F[2], c->Ec, c->E, mixed, st);
+        }
+        for (int ph = LAS_HALF; ph <= LAS_FULL; ph++) {
+            P.las_want = ph;
+            pole(c, 0., c->dt);
+        }
+        P.las_want = LAS_IDLE;
+        for (int stage = 0; stage < 2; stage++)
+            launch_las_corr(P, c->las_es, c->nlas_es, 0, stage, c->E, c->acc_es, c->dt, mixed, st);
+        for (int stage = 0; stage < 2; stage++)
+            launch_las_corr(P, c->las_ej, c->nlas_ej, 0, stage, c->E, c->acc_ej, c->dt, mixed, st);
+        for (int ph = LAS_HALF; ph <= LAS_FULL; ph++) {
+            const Launch L = las_launch(c, KC_CORNER, s, ph);
+            if (L.wl.n)
+                launch_ct_update(P, L, 0., c->dt, ph == LAS_HALF ? 0.5 : 1.0, c->ps, las_outs(c, ph), c->E, mixed, st);
+        }
+        for (int ph = LAS_HALF; ph <= LAS_FULL; ph++) {
+            const Launch L = las_launch(c, KC_INTERIOR, s, ph);
+            if (!L.wl.n) continue;
+            const int full = ph == LAS_FULL;
+            for (int part : {3, 2})
+                launch_update_part(P, part, L, full, 0., c->dt, full ? 1.0 : 0.5, las_in(c, ph), las_in(c, ph),
+                                   las_out(c, ph), c->U0, c->Un, las_outs(c, ph), c->F[0], c->F[1], c->F[2], c->pflag,
+                                   c->failcnt, mixed, st);
+        }
+        las_bound(c, s);
+        if (last) launch_dt_finalize(nblk_of(c), c->dtmin, c->dt, st, c->bdt);
+    }
+    P.las_s = -1;
+    P.las_want = LAS_IDLE;
+}
+
+int hamr_las_rule(int nblk, const double *bdt, double dt, int maxtl, int nring, int ring_len, const int *ring,
+                  int npair, const int *pairs, int *tl)
+{
+    if (nblk < 0 || (nblk && (!bdt || !tl)) || !(dt > 0.) || maxtl < 1 || maxtl > kMaxTimeLevel ||
+        (maxtl & (maxtl - 1)) || nring < 0 || ring_len < 0 || npair < 0 || (nring && ring_len && !ring) ||
+        (npair && !pairs))
+        return -1;
+    std::vector<int> r(ring, ring + (size_t)nring * ring_len);
+    std::vector<std::pair<int, int>> pr;
+    for (int k = 0; k < npair; k++) pr.emplace_back(pairs[2 * k], pairs[2 * k + 1]);
+    for (int x : r)
+        if (x < 0 || x >= nblk) return -1;
+    for (const auto &q : pr)
+        if (q.first < 0 || q.first >= nblk || q.second < 0 || q.second >= nblk) return -1;
+    las_rule(nblk, bdt, dt, maxtl, r, ring_len, pr, tl);
+    return 0;
+}
+
+int hamr_set_timelevels(hamr_ctx *c, const int *tl)
+{
+    if (!c || !tl) return -1;
+    if (!c->amr) return fail(c, "hamr_set_timelevels: local adaptive time stepping runs on multi-level sets");
+    HIPCHK(c, hipSetDevice(c->device));
+    return set_timelevels(c, std::vector<int>(tl, tl + nblk_of(c)));
+}
+
+int hamr_get_timelevels(const hamr_ctx *c, int *tl)
+{
+    if (!c || !tl) return -1;
+    for (int n = 0; n < nblk_of(c); n++) tl[n] = c->tl.empty() ? 1 : c->tl[n];
+    return 0;
+}
+
+int hamr_block_dts(hamr_ctx *c, double *bdt)
+{
+    if (!c || !bdt) return -1;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return d2h(c, bdt, c->bdt, (size_t)nblk_of(c) * sizeof(double));
+}
+
+int hamr_timelevels_from_dt(hamr_ctx *c, int maxtl, int *tl_out)
+{
+    if (!c) return -1;
+    if (!c->amr) return fail(c, "hamr_timelevels_from_dt: local adaptive time stepping runs on multi-level sets");
+    if (maxtl < 1 || maxtl > kMaxTimeLevel || (maxtl & (maxtl - 1)))
+        return fail(c, "hamr_timelevels_from_dt: maxtl is a power of 2 from 1 to 16");
+    const int nb = nblk_of(c);
+    std::vector<double> bdt((size_t)nb);
+    double dt = 0.;
+    if (hamr_block_dts(c, bdt.data())) return -1;
+    if (d2h(c, &dt, c->dt, sizeof dt)) return -1;
+    if (!(dt > 0.) || dt > 1e299) return fail(c, "hamr_timelevels_from_dt: no step has set dt yet");
+    std::vector<int> tl((size_t)nb);
+    las_rule(nb, bdt.data(), dt, maxtl, c->grid.ring, c->grid.ring_len, jump_pairs(c->grid), tl.data());
+    if (set_timelevels(c, tl)) return -1;
+    if (tl_out) std::copy(tl.begin(), tl.end(), tl_out);
+    return 0;
+}
+
+int hamr_las_info(const hamr_ctx *c, long long *info)
+{
+    if (!c || !info) return -1;
+    for (int d = 0; d < 3; d++) info[d] = c->nlas_f[d];
+    info[3] = c->nlas_es;
+    info[4] = c->nlas_ej;
+    info[5] = c->tl.empty() ? 0 : c->tlmax;
+    return 0;
+}
+
+int hamr_step_las(hamr_ctx *c, int maxtl, int ncycles, double *dt_inout)
+{
+    if (!c) return -1;
+    if (!c->amr || c->tl.empty())
+        return fail(c, "hamr_step_las: set the time levels of a multi-level set first (hamr_set_timelevels)");
+    if (maxtl < c->tlmax || maxtl > kMaxTimeLevel || (maxtl & (maxtl - 1)))
+        return fail(c, "hamr_step_las: maxtl is a power of 2 from the largest time level to 16");
+    if (ncycles < 0) return fail(c, "hamr_step_las: ncycles < 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (set_dt(c, dt_inout)) return -1;
+    launch_dt_reset(nblk_of(c), c->dtmin, c->s);
+    for (int k = 0; k < ncycles; k++) enqueue_las_cycle(c, maxtl);
+    if (check_launch(c, "hamr_step_las")) return -1;
     if (dt_inout) {
         HIPCHK(c, hipStreamSynchronize(c->s));
         if (d2h(c, dt_inout, c->dt, sizeof(double))) return -1;

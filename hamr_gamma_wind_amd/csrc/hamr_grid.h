@@ -37,7 +37,7 @@ struct BlockDev {
     int D[3];       // bs + 2 NG
     int ks;         // cells per component slab (padded to 64)
     int kcol;       // columns per metric slab (padded to 64)
-    int pad2;
+    int tl;         // time level of local adaptive time stepping (a power of 2; 0 until set)
     int64_t off;    // cells per component of all earlier blocks' slabs
     int64_t moff;   // metric columns of all earlier geometry slabs
     double dx[3];   // code-coordinate spacing of the block's level
@@ -74,8 +74,23 @@ struct AmrEmf {
     int cb, cc;
     int fb, fc;
 };
+// Flux or EMF correction of a local-adaptive-time-stepping cycle: the value at
+// cell rcell of block rb (the receiver; for an EMF, rcell & 0x0FFFFFFF with the
+// component in bits 28-29, as AmrEmf) is replaced, at the receiver's corrector,
+// by the time-weighted sum of the sender sb's values over the sender's
+// correctors since the receiver's last one, divided by the receiver's step.
+// The sender's value at scell & 0x3FFFFFFF is one face / edge (a same-level
+// neighbour of a smaller time level), or with bit 30 (LAS_MEAN) the mean of the
+// four faces / two edges of a one level finer block that tile the receiver's
+// (the strides of AmrReflux / AmrEmf).  acc: one accumulator per entry and
+// component.
+constexpr int LAS_MEAN = 1 << 30;
+struct LasCorr {
+    int rb, rcell;
+    int sb, scell;
+};
 static_assert(sizeof(AmrEntry) == 16 && sizeof(AmrProlong) == 64 && sizeof(AmrReflux) == 16 &&
-                  sizeof(AmrEmf) == 16,
+                  sizeof(AmrEmf) == 16 && sizeof(LasCorr) == 16,
               "table entry layout");
 
 // Static description of a run (mirrors the reference's compile-time config).

@@ -21,7 +21,24 @@ struct KParams {
     // s = 0-9 gcov, 10-19 gcon, 20 gdet, 21 1 - 1/sqrt(-g^00).
     const float *g32;
     int riemann;        // HAMR_RIEMANN_HLL (0) or HAMR_RIEMANN_HLLC (1); HLLC in the FP64 step only
+    // Local adaptive time stepping: the sub-iteration s of a cycle (-1 outside
+    // one) and the phase a launch serves (LAS_HALF or LAS_FULL).  In a cycle a
+    // kernel scales its step by the block's time level (las_phase).
+    int las_s;
+    int las_want;
 };
+
+// Phase of block B in sub-iteration s of a cycle (advance_GPU, step_ch.c:1003-1010,
+// with PRESTEP = PRESTEP2 = 0): a block of time level tl takes its predictor
+// when s % tl == tl - 1 and its corrector when s % 2tl == 2tl - 1.
+enum { LAS_IDLE = 0, LAS_HALF = 1, LAS_FULL = 2 };
+__host__ __device__ inline int las_phase(int tl, int s)
+{
+    if (tl < 1) tl = 1;
+    if (s % (2 * tl) == 2 * tl - 1) return LAS_FULL;
+    if (s % tl == tl - 1) return LAS_HALF;
+    return LAS_IDLE;
+}
 
 // Work list of one launch: `items` are (block, tile) pairs, a tile being
 // blockDim.x consecutive threads of the block's iteration space.  A grid of
@@ -182,14 +199,28 @@ void launch_dump_pack(const KParams &P, const int *blist, const int64_t *doff, c
 // FP64 metric (the slab's pointers into gcov, gcon, gdet and g32).
 void launch_geom32(const double *gcov, const double *gcon, const double *gdet, float *g32, int64_t kcol,
                    hipStream_t s);
-void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s);
+// dt of the next step from the Courant minima (bdt, when non-null, receives
+// each block's harmonic 1/(1/dt1 + 1/dt2 + 1/dt3)); the minima are reset.
+void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s, double *bdt = nullptr);
+// Corrections of a local-adaptive-time-stepping cycle at sub-iteration P.las_s
+// (LasCorr entries of one direction of fluxes, dir 1..3, or of EMFs, dir 0):
+// stage 0 adds every sender's values at its corrector into the accumulators
+// (restarting them at the first corrector after the receiver's last), stage 1
+// writes acc / (dt tl) into every receiver at its corrector.
+void launch_las_corr(const KParams &P, const LasCorr *e, int64_t n, int dir, int stage, void *FE, double *acc,
+                     const double *dt_ptr, int mixed, hipStream_t s);
 void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s);
 void launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
 
 // Multi-level ghost exchange of stage dir (1..3): copy/restriction entries,
 // then prolongation entries (they write disjoint ghost cells and read only
 // interior cells and ghosts of earlier stages).  grid <= 0: one thread per entry.
+// In a local-time-stepping cycle (P.las_s >= 0) ph/psh are the half-step
+// arrays: ghosts are exchanged between blocks that both step in the
+// sub-iteration, read from each source's arrays of its phase and written into
+// both array pairs of the ghost's block.  Outside a cycle ph = psh = nullptr.
 void launch_amr_exchange(const KParams &P, int dir, const AmrEntry *e, int64_t n, const AmrProlong *pe, int64_t npro,
-                         double *p, double *ps, int *pflag, int threads, int grid, hipStream_t s);
+                         double *p, double *ps, int *pflag, int threads, int grid, hipStream_t s,
+                         double *ph = nullptr, double *psh = nullptr);
 
 }  // namespace hamr

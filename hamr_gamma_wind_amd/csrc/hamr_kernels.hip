@@ -45,9 +45,12 @@ __global__ void k_pole_average(KParams P, const int *__restrict__ ring, const in
     const int *rb = ring + (int64_t)rg * len;
     const BlockDev &B0 = P.blk[rb[0]];
     if (il > B0.bs[0]) return;
+    // in a local-time-stepping cycle: the rings of this launch's phase, each
+    // weighted by its time level's step (a ring's blocks share one time level)
+    if (P.las_s >= 0 && las_phase(B0.tl, P.las_s) != P.las_want) return;
     const int side = rside[rg];
     const int i = il + NG, j = side ? NG + B0.bs[1] : NG;
-    const double dt = dt_ptr ? dt_ptr[0] : dt_val;
+    const double dt = block_Dt(P, B0, dt_val, dt_ptr, 1.0);
     double avg = 0.;
     for (int b = 0; b < len; b++) {
         const BlockDev &B = P.blk[rb[b]];
@@ -97,7 +100,7 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
         int a, b, c;
         region_index(t, e2, e3, a, b, c);
         const int i = a + NG, j = b + NG, z = c + NG;
-        const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
+        const double Dt = block_Dt(P, B, Dt_val, dt_ptr, dt_mult);
         const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2], sj = B.D[2];
         const int64_t g = ((int64_t)i * B.D[1] + j) * B.D[2] + z, col = (int64_t)i * B.D[1] + j;
         const double gF1 = P.gdet[mg1(B, FACE1) + col], gF2 = P.gdet[mg1(B, FACE2) + col],
@@ -306,7 +309,7 @@ __global__ void __launch_bounds__(256) k_cons(KParams P, WorkList wl, int full, 
             region_index(t, bs2, bs3, a, b, c);
             const int i = a + NG, j = b + NG, z = c + NG;
             const double *mvp = smem + (t / bs3 - q0);
-            const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
+            const double Dt = block_Dt(P, B, Dt_val, dt_ptr, dt_mult);
             const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2];
             const int64_t g = ((int64_t)i * B.D[1] + j) * B.D[2] + z;
             const int64_t base = NPR * B.off + g;
@@ -351,7 +354,7 @@ k_cons_march(KParams P, int full, double Dt_val, const double *__restrict__ dt_p
     const int q0 = t0 / bs3, q1 = mymin(t0 + RPLANE - 1, nplane - 1) / bs3, ncw = q1 - q0 + 1;
     const int j = (act ? t : t0) / bs3 + NG, z = (act ? t : t0) % bs3 + NG;
     const double *mvp = smem + (j - NG - q0);
-    const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
+    const double Dt = block_Dt(P, B, Dt_val, dt_ptr, dt_mult);
     const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2];
     R fl[NPR], fh[NPR];
     if (act) {
@@ -534,7 +537,7 @@ k_update(KParams P, WorkList wl, int full, double Dt_val, const double *__restri
             region_index(t, bs2, bs3, a, b, c);
             const int i = a + NG, j = b + NG, z = c + NG;
             const double *mvp = smem + (t / bs3 - q0);
-            const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
+            const double Dt = block_Dt(P, B, Dt_val, dt_ptr, dt_mult);
             const int64_t K = B.ks, si = (int64_t)B.D[1] * B.D[2];
             const int64_t g = ((int64_t)i * B.D[1] + j) * B.D[2] + z, col = (int64_t)i * B.D[1] + j;
             const int64_t base = NPR * B.off + g;
@@ -797,40 +800,66 @@ __device__ __forceinline__ double amr_flux_mean(const KParams &P, const BlockDev
 }
 
 // dir: the exchange direction of the entries (1..3)
+// Exchange arrays of a local-time-stepping cycle (P.las_s >= 0; ph non-null):
+// a ghost is filled only when its block and its source block both step in
+// this sub-iteration; the source is read from the arrays of its own phase
+// (its predictor's ph/psh, or p/ps after its corrector; a block that does not
+// step here, a slope neighbour of a prolongation, from p/ps), and the ghost is
+// written into both of its block's array pairs, as the reference's receive
+// sets p and ph (unpackreceive1, GPU_program2.cu:345-388).
+__device__ __forceinline__ void las_arrays(const KParams &P, int blk, double *p, double *ps, double *ph,
+                                           double *psh, const double *&sp, const double *&sps)
+{
+    const bool half = ph && las_phase(P.blk[blk].tl, P.las_s) == LAS_HALF;
+    sp = half ? ph : p;
+    sps = half ? psh : ps;
+}
+__device__ __forceinline__ bool las_steps(const KParams &P, int blk)
+{
+    return las_phase(P.blk[blk].tl, P.las_s) != LAS_IDLE;
+}
+
 __global__ void __launch_bounds__(256) k_amr_copy(KParams P, const AmrEntry *__restrict__ e, int64_t n, int dir,
-                                                  double *p, double *ps, int *pflag)
+                                                  double *p, double *ps, double *ph, double *psh, int *pflag)
 {
     const int d0 = dir - 1;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         const AmrEntry E = e[t];
+        if (ph && (!las_steps(P, E.dblk) || !las_steps(P, E.sblk))) continue;
+        const double *sp, *sps;
+        las_arrays(P, E.sblk, p, ps, ph, psh, sp, sps);
         double v[NPR + 3];
-        amr_source(P, p, ps, E.sblk, E.scode, v);
+        amr_source(P, sp, sps, E.sblk, E.scode, v);
         const BlockDev &B = P.blk[E.dblk];
         const BlockDev &S = P.blk[E.sblk];
         int loc[3];
         region_index(E.dcell, B.D[1], B.D[2], loc[0], loc[1], loc[2]);
         if ((E.scode >> 28) == AS_AVG8) {
             const double gc = P.gdet[mg1(B, d0 == 0 ? FACE1 : d0 == 1 ? FACE2 : FACE3) + (int64_t)loc[0] * B.D[1] + loc[1]];
-            v[NPR + d0] = amr_flux_mean(P, S, ps, d0, E.scode & 0x0FFFFFFF, gc);
+            v[NPR + d0] = amr_flux_mean(P, S, sps, d0, E.scode & 0x0FFFFFFF, gc);
         }
         const bool own = own_upper_face(B, d0, loc);
         const int64_t K = B.ks;
-        double *pd = p + NPR * B.off + E.dcell, *sd = ps + 3 * B.off + E.dcell;
+        for (int pass = 0; pass < (ph ? 2 : 1); pass++) {
+            double *pd = (pass ? ph : p) + NPR * B.off + E.dcell, *sd = (pass ? psh : ps) + 3 * B.off + E.dcell;
 #pragma unroll
-        for (int k = 0; k < NPR; k++) pd[k * K] = v[k];
+            for (int k = 0; k < NPR; k++) pd[k * K] = v[k];
 #pragma unroll
-        for (int a = 0; a < 3; a++)
-            if (!(a == d0 && own)) sd[a * K] = v[NPR + a];
+            for (int a = 0; a < 3; a++)
+                if (!(a == d0 && own)) sd[a * K] = v[NPR + a];
+        }
         if (pflag) pflag[B.off + E.dcell] = pflag[S.off + (E.scode & 0x0FFFFFFF)];
     }
 }
 
 __global__ void __launch_bounds__(256) k_amr_prolong(KParams P, const AmrProlong *__restrict__ e, int64_t n,
-                                                     int dir, double *p, double *ps, int *pflag)
+                                                     int dir, double *p, double *ps, double *ph, double *psh,
+                                                     int *pflag)
 {
     const int d0 = dir - 1;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
         const AmrProlong E = e[t];
+        if (ph && (!las_steps(P, E.dblk) || !las_steps(P, E.cblk))) continue;
         const BlockDev &B = P.blk[E.dblk];
         const BlockDev &C = P.blk[E.cblk];
         // side of the coarse cell along each direction (global fine index parity)
@@ -841,7 +870,9 @@ __global__ void __launch_bounds__(256) k_amr_prolong(KParams P, const AmrProlong
 #pragma unroll
         for (int d = 0; d < 3; d++) sgn[d] = ((B.o[d] + loc[d] - NG) & 1) ? 1.0 : -1.0;
         double c0[NPR + 3], dq[3][NPR + 3];
-        amr_source(P, p, ps, E.cblk, (AS_CELL << 28) | E.ccell, c0);
+        const double *sp, *sps;
+        las_arrays(P, E.cblk, p, ps, ph, psh, sp, sps);
+        amr_source(P, sp, sps, E.cblk, (AS_CELL << 28) | E.ccell, c0);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             const int cm = E.nb[2 * d][1], cp = E.nb[2 * d + 1][1];
@@ -851,16 +882,18 @@ __global__ void __launch_bounds__(256) k_amr_prolong(KParams P, const AmrProlong
                 continue;
             }
             double vm[NPR + 3], vp[NPR + 3];
-            amr_source(P, p, ps, E.nb[2 * d][0], cm, vm);
-            amr_source(P, p, ps, E.nb[2 * d + 1][0], cp, vp);
+            las_arrays(P, E.nb[2 * d][0], p, ps, ph, psh, sp, sps);
+            amr_source(P, sp, sps, E.nb[2 * d][0], cm, vm);
+            las_arrays(P, E.nb[2 * d + 1][0], p, ps, ph, psh, sp, sps);
+            amr_source(P, sp, sps, E.nb[2 * d + 1][0], cp, vp);
 #pragma unroll
             for (int k = 0; k < NPR + 3; k++) dq[d][k] = mc_slope(vm[k], c0[k], vp[k]);
         }
         const int64_t K = B.ks;
-        double *pd = p + NPR * B.off + E.dcell, *sd = ps + 3 * B.off + E.dcell;
+        double pv[NPR], fv[3];
 #pragma unroll
         for (int k = 0; k < NPR; k++)
-            pd[k * K] = c0[k] + 0.25 * sgn[0] * dq[0][k] + 0.25 * sgn[1] * dq[1][k] + 0.25 * sgn[2] * dq[2][k];
+            pv[k] = c0[k] + 0.25 * sgn[0] * dq[0][k] + 0.25 * sgn[1] * dq[1][k] + 0.25 * sgn[2] * dq[2][k];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             // odd fine face along a: halves the coarse cell -> cell-centred field
@@ -869,7 +902,15 @@ __global__ void __launch_bounds__(256) k_amr_prolong(KParams P, const AmrProlong
 #pragma unroll
             for (int d = 0; d < 3; d++)
                 if (d != a) v = v + 0.25 * sgn[d] * dq[d][kk];
-            if (!(a == d0 && own)) sd[a * K] = v;
+            fv[a] = v;
+        }
+        for (int pass = 0; pass < (ph ? 2 : 1); pass++) {
+            double *pd = (pass ? ph : p) + NPR * B.off + E.dcell, *sd = (pass ? psh : ps) + 3 * B.off + E.dcell;
+#pragma unroll
+            for (int k = 0; k < NPR; k++) pd[k * K] = pv[k];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                if (!(a == d0 && own)) sd[a * K] = fv[a];
         }
         if (pflag) pflag[B.off + E.dcell] = pflag[C.off + E.ccell];
     }
@@ -923,9 +964,74 @@ __global__ void __launch_bounds__(256) k_amr_emf(KParams P, const AmrEmf *__rest
     }
 }
 
-// Courant dt of the step: per-block harmonic of the direction minima, minimum
-// over blocks (step_ch.c:1313-1322), clamped to 1.3 dt (step_ch.c:974).  Resets
-// the minima for the next corrector.  dt[0] <- new dt, dt[1] <- previous dt.
+// Corrections of a local-adaptive-time-stepping cycle (flux_send / flux_rec
+// and E_send / E_rec with calc_corr = 1 and PRESTEP = PRESTEP2 = 0,
+// boundcomF.c:5-1315, boundcomE.c; packsendfluxaverage1-3 and
+// unpackreceive1flux-3flux, GPU_program2.cu:1258-1771, the factor and
+// first-step rule of GPU_boundcomF.cu:7-8 and :109-110).  One thread per entry
+// and, for fluxes, all NPR components.  Stage 0 (senders at their corrector):
+// acc = c sum on the first corrector after the receiver's last (s % 2tl_r ==
+// 2tl_s - 1), acc += c sum after, with c = w (dt tl_s), w = 1/4 (four finer
+// faces), 1/2 (two finer edges) or 1, and sum the sender's values in the term
+// order of k_amr_reflux / k_amr_emf.  Stage 1 (receivers at their corrector):
+// value = acc / (dt tl_r).
+template <class R>
+__global__ void __launch_bounds__(256) k_las_corr(KParams P, const LasCorr *__restrict__ e, int64_t n, int dir,
+                                                  int stage, R *__restrict__ FE, double *__restrict__ acc,
+                                                  const double *__restrict__ dt_ptr)
+{
+    const double dt = dt_ptr[0];
+    const int s = P.las_s;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const LasCorr x = e[t];
+        const BlockDev &Rb = P.blk[x.rb], &Sb = P.blk[x.sb];
+        const int tr = Rb.tl > 0 ? Rb.tl : 1, ts = Sb.tl > 0 ? Sb.tl : 1;
+        const bool mean = (x.scell & LAS_MEAN) != 0;
+        const int sc = x.scell & 0x3FFFFFFF;
+        if (dir == 0) {  // EMF: component in bits 28-29 of rcell
+            const int comp = x.rcell >> 28, rc = x.rcell & 0x0FFFFFFF;
+            if (stage == 0) {
+                if (las_phase(ts, s) != LAS_FULL) continue;
+                const R *f = FE + 3 * Sb.off + (int64_t)comp * Sb.ks + sc;
+                double sum, c;
+                if (mean) {
+                    const int64_t st = comp == 0 ? (int64_t)Sb.D[1] * Sb.D[2] : comp == 1 ? (int64_t)Sb.D[2] : 1;
+                    sum = (double)f[0] + (double)f[st];
+                    c = 0.5 * (dt * (double)ts);
+                } else {
+                    sum = (double)f[0];
+                    c = 1.0 * (dt * (double)ts);
+                }
+                const bool first = s % (2 * tr) == 2 * ts - 1;
+                acc[t] = first ? c * sum : acc[t] + c * sum;
+            } else {
+                if (las_phase(tr, s) != LAS_FULL) continue;
+                FE[3 * Rb.off + (int64_t)comp * Rb.ks + rc] = (R)(acc[t] / (dt * (double)tr));
+            }
+            continue;
+        }
+        if (stage == 0) {
+            if (las_phase(ts, s) != LAS_FULL) continue;
+            const bool first = s % (2 * tr) == 2 * ts - 1;
+            const int64_t sa = dir == 1 ? Sb.D[2] : (int64_t)Sb.D[1] * Sb.D[2], sb = dir == 3 ? Sb.D[2] : 1;
+            const double c = (mean ? 0.25 : 1.0) * (dt * (double)ts);
+            const R *f = FE + NPR * Sb.off + sc;
+#pragma unroll
+            for (int k = 0; k < NPR; k++) {
+                const R *q = f + (int64_t)k * Sb.ks;
+                const double sum = mean ? (double)q[0] + (double)q[sb] + (double)q[sa] + (double)q[sa + sb] : (double)q[0];
+                double &a = acc[t * NPR + k];
+                a = first ? c * sum : a + c * sum;
+            }
+        } else {
+            if (las_phase(tr, s) != LAS_FULL) continue;
+            R *o = FE + NPR * Rb.off + x.rcell;
+#pragma unroll
+            for (int k = 0; k < NPR; k++) o[(int64_t)k * Rb.ks] = (R)(acc[t * NPR + k] / (dt * (double)tr));
+        }
+    }
+}
+
 // FP32 copy of one geometry slab's metric (KParams::g32): each value rounded
 // once, and 1 - alpha = 1 - 1/sqrt(-g^00) formed in FP64 before it is rounded
 // (the mixed step's FP32 energy flux needs 1 - alpha to its own precision).
@@ -953,7 +1059,12 @@ void launch_geom32(const double *gcov, const double *gcon, const double *gdet, f
     hipLaunchKernelGGL(k_geom32, dim3((unsigned)((kcol + 255) / 256)), dim3(256), 0, s, gcov, gcon, gdet, g32, kcol);
 }
 
-__global__ void k_dt_finalize(int nblk, unsigned long long *__restrict__ dtmin, double *__restrict__ dt)
+// Courant dt of the step: per-block harmonic of the direction minima, minimum
+// over blocks (step_ch.c:1313-1322), clamped to 1.3 dt (step_ch.c:974).  Resets
+// the minima for the next corrector.  dt[0] <- new dt, dt[1] <- previous dt;
+// bdt[n] <- block n's harmonic (bdt[][0] of the reference) when bdt is set.
+__global__ void k_dt_finalize(int nblk, unsigned long long *__restrict__ dtmin, double *__restrict__ dt,
+                              double *__restrict__ bdt)
 {
     __shared__ double red[256];
     double m = 1.e9;
@@ -963,6 +1074,7 @@ __global__ void k_dt_finalize(int nblk, unsigned long long *__restrict__ dtmin, 
         const double d3 = __longlong_as_double((long long)dtmin[n * 3 + 2]);
         const double b0 = 1. / (1. / d1 + 1. / d2 + 1. / d3);
         m = mymin(m, b0);
+        if (bdt) bdt[n] = b0;
         const unsigned long long init = (unsigned long long)__double_as_longlong(1.e9);
         dtmin[n * 3 + 0] = init;
         dtmin[n * 3 + 1] = init;
@@ -1124,17 +1236,18 @@ void launch_bound(const KParams &P, int dir, const Launch &L, double *p, double 
 }
 
 void launch_amr_exchange(const KParams &P, int dir, const AmrEntry *e, int64_t n, const AmrProlong *pe, int64_t npro,
-                         double *p, double *ps, int *pflag, int threads, int grid, hipStream_t s)
+                         double *p, double *ps, int *pflag, int threads, int grid, hipStream_t s, double *ph,
+                         double *psh)
 {
     if (n > 0) {
         const int64_t need = (n + threads - 1) / threads;
         const unsigned g = (unsigned)(grid > 0 && grid < need ? grid : need);
-        hipLaunchKernelGGL(k_amr_copy, dim3(g), dim3(threads), 0, s, P, e, n, dir, p, ps, pflag);
+        hipLaunchKernelGGL(k_amr_copy, dim3(g), dim3(threads), 0, s, P, e, n, dir, p, ps, ph, psh, pflag);
     }
     if (npro > 0) {
         const int64_t need = (npro + threads - 1) / threads;
         const unsigned g = (unsigned)(grid > 0 && grid < need ? grid : need);
-        hipLaunchKernelGGL(k_amr_prolong, dim3(g), dim3(threads), 0, s, P, pe, npro, dir, p, ps, pflag);
+        hipLaunchKernelGGL(k_amr_prolong, dim3(g), dim3(threads), 0, s, P, pe, npro, dir, p, ps, ph, psh, pflag);
     }
 }
 
@@ -1160,9 +1273,21 @@ void launch_amr_emf(const KParams &P, const AmrEmf *e, int64_t n, void *E, doubl
         hipLaunchKernelGGL(k_amr_emf<double>, dim3(g), dim3(256), 0, s, P, e, n, (double *)E, dt_val, dt_ptr);
 }
 
-void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s)
+void launch_dt_finalize(int nblk, unsigned long long *dtmin, double *dt, hipStream_t s, double *bdt)
 {
-    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s, nblk, dtmin, dt);
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s, nblk, dtmin, dt, bdt);
+}
+
+void launch_las_corr(const KParams &P, const LasCorr *e, int64_t n, int dir, int stage, void *FE, double *acc,
+                     const double *dt_ptr, int mixed, hipStream_t s)
+{
+    if (n <= 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+    if (mixed)
+        hipLaunchKernelGGL(k_las_corr<float>, dim3(g), dim3(256), 0, s, P, e, n, dir, stage, (float *)FE, acc, dt_ptr);
+    else
+        hipLaunchKernelGGL(k_las_corr<double>, dim3(g), dim3(256), 0, s, P, e, n, dir, stage, (double *)FE, acc,
+                           dt_ptr);
 }
 
 void launch_dt_reset(int nblk, unsigned long long *dtmin, hipStream_t s)

@@ -62,6 +62,17 @@ __device__ __forceinline__ Geom<R> load_geom(const KParams &P, const BlockDev &B
     return G;
 }
 
+// The step Dt of block B: the launch's Dt (dt_ptr[0] dt_mult when the launch
+// reads dt from the device, else Dt_val), scaled by the block's time level in
+// a local-adaptive-time-stepping cycle (a block of time level tl advances by
+// tl dt per full step, advance_GPU, step_ch.c:1184-1262).
+__device__ __forceinline__ double block_Dt(const KParams &P, const BlockDev &B, double Dt_val,
+                                           const double *__restrict__ dt_ptr, double dt_mult)
+{
+    const double Dt = dt_ptr ? dt_ptr[0] * dt_mult : Dt_val;
+    return P.las_s >= 0 ? Dt * (double)(B.tl > 0 ? B.tl : 1) : Dt;
+}
+
 __device__ __forceinline__ double wave_min(double v)
 {
 #pragma unroll

@@ -326,6 +326,47 @@ int hamr_step(hamr_ctx *ctx, int nsteps, double *dt_inout);
  * hamr_step). */
 int hamr_step_graph(hamr_ctx *ctx, int nsteps, double *dt_inout);
 int hamr_sync(hamr_ctx *ctx);
+
+/* ---- local adaptive time stepping (multi-level sets; LAS.c, advance_GPU) ----
+ * Replaces the timelevel machinery of GPU_step_ch / advance_GPU
+ * (step_ch.c:916-1336, PRESTEP = PRESTEP2 = 0, AMR_SWITCHTIMELEVEL =
+ * AMR_MAXTIMELEVEL), set_timelevel (LAS.c:5-236), the flux and EMF
+ * corrections across time levels (boundcomF.c / boundcomE.c with calc_corr 1,
+ * GPU_boundcomF.cu:5-330, GPU_program2.cu:1168-2704) and the ghost exchange
+ * between blocks that step together (unpackreceive1-3, GPU_program2.cu:292-390).
+ *
+ * Time levels of every block (nblk values, powers of 2 from 1 to 16, equal over
+ * each polar ring; a finer block at a level jump may not have a larger level
+ * than its coarser neighbour).  Builds the cycle's work lists and correction
+ * tables; <0 and a message on a violation.  Multi-level contexts only. */
+int hamr_set_timelevels(hamr_ctx *ctx, const int *tl);
+/* The current time levels (1 for every block until set). */
+int hamr_get_timelevels(const hamr_ctx *ctx, int *tl);
+/* Per-block Courant dt (the harmonic 1/(1/dt1 + 1/dt2 + 1/dt3), bdt[][0] of
+ * step_ch.c:1323) recorded by the last corrector of hamr_step,
+ * hamr_step_graph, hamr_step_las or hamr_estimate_dt. */
+int hamr_block_dts(hamr_ctx *ctx, double *bdt);
+/* set_timelevel's rule on those block dts and the current dt, then
+ * hamr_set_timelevels: block n gets the largest 2^i <= maxtl with bdt[n]/dt >=
+ * 1.05 2^i (else 1), each polar ring the minimum of its blocks; a finer block
+ * at a level jump is lowered to its coarser neighbour's level (with the ring
+ * minimum again until nothing changes).  tl_out (nblk values, may be NULL)
+ * receives the result. */
+int hamr_timelevels_from_dt(hamr_ctx *ctx, int maxtl, int *tl_out);
+/* The same rule on plain arrays (no context): ring is nring x ring_len block
+ * ids, pairs npair (coarse, finer) block pairs. */
+int hamr_las_rule(int nblk, const double *bdt, double dt, int maxtl, int nring, int ring_len, const int *ring,
+                  int npair, const int *pairs, int *tl);
+/* ncycles cycles of 2 maxtl sub-iterations (maxtl: a power of 2, at least the
+ * largest time level, at most 16), each advancing every block by maxtl dt: a
+ * block of level tl takes its predictor (tl dt / 2) at sub-iterations s with
+ * s % tl == tl - 1 and its corrector (tl dt) at s % 2tl == 2tl - 1.  dt is kept
+ * on the device as in hamr_step and is updated at each cycle's end from the
+ * Courant minima of its last sub-iteration. */
+int hamr_step_las(hamr_ctx *ctx, int maxtl, int ncycles, double *dt_inout);
+/* info[0..2]: flux-correction entries per direction, [3]: same-level EMF
+ * entries, [4]: level-jump EMF entries, [5]: the largest time level (0 until set). */
+int hamr_las_info(const hamr_ctx *ctx, long long *info);
 /* Average device time (ms, HIP events on the context stream) of one launch of
  * a stage over all blocks, with predictor-step operands (p and ps are not
  * modified): 0-2 flux x1..x3, 3 EMF, 4 polar average, 5 face-B update,

@@ -20,6 +20,11 @@
  *   2. polar faces (mirror from the pi-shifted partner), then the exchange of
  *      the internal x2 faces;
  *   3. the exchange of the x3 faces (periodic);
+ *   (in stages 2 and 3 a corner ghost, beyond the block along an earlier
+ *   direction, is a copy of the storage cell at Y of the same-level leaf
+ *   across the face that holds the position clamped into the block's range,
+ *   when there is one: that leaf's ghost of the earlier stage, its own
+ *   boundary faces included);
  *   4. radial outflow again over every row except the polar ghost rows (the
  *      radial ghosts of stages 2 and 3 are the outflow image of the block's own
  *      filled edge cells).
@@ -142,13 +147,25 @@ static double slope_lim(double y1, double y2, double y3)
 int oracle_amr_plant_own_face = 0;
 
 /* write the ghost values; the component `skip` (0..2, or -1) of the face field is left as it is */
+/* Local-adaptive-time-stepping exchange (NULL members: the plain exchange):
+ * only blocks with act[b] != 0 (stepping in the sub-iteration) exchange, with
+ * sources read from the arrays passed (each block's arrays of its phase) and
+ * ghosts written into those and into p2/ps2 (each block's other array pair). */
+typedef struct {
+    const int *act;
+    double **p2, **ps2;
+} las_x;
+
 static void write_ghost(const oracle_amr *g, int b, long c, double **p, double **ps, int **pflag, const double *v,
-                        int flag, int skip)
+                        int flag, int skip, const las_x *lx)
 {
     const long N = geo_of(&g->blk[b]).ncell;
-    for (int k = 0; k < NPR; k++) p[b][k * N + c] = v[k];
-    for (int a = 0; a < 3; a++)
-        if (a != skip) ps[b][a * N + c] = v[NPR + a];
+    for (int pass = 0; pass < (lx && lx->p2 ? 2 : 1); pass++) {
+        double *pp = pass ? lx->p2[b] : p[b], *qq = pass ? lx->ps2[b] : ps[b];
+        for (int k = 0; k < NPR; k++) pp[k * N + c] = v[k];
+        for (int a = 0; a < 3; a++)
+            if (a != skip) qq[a * N + c] = v[NPR + a];
+    }
     if (pflag) pflag[b][c] = flag;
 }
 
@@ -217,7 +234,7 @@ static int neighbour_values(const oracle_amr *g, int lq, int cblk, int Q[3], int
  * CT update owns it (the reference's receive stores no normal component,
  * unpackreceive1, GPU_program2.cu:376-380). */
 static void exchange_cell(const oracle_amr *g, int dd, int b, int i, int j, int z, double **p, double **ps,
-                          int **pflag)
+                          int **pflag, const las_x *lx)
 {
     const oracle_amr_block *B = &g->blk[b];
     const int l = B->level;
@@ -235,19 +252,42 @@ static void exchange_cell(const oracle_amr *g, int dd, int b, int i, int j, int 
         int Yc[3] = {Y[0] < 0 ? 0 : (Y[0] >= g->N[l][0] ? g->N[l][0] - 1 : Y[0]), Y[1] < 0 ? 0 : g->N[l][1] - 1, Y[2]};
         int lc, c = find_leaf(g, l, Yc, &lc);
         if (c < 0 || lc != l) abort();
+        if (lx && lx->act && !lx->act[c]) return;
         const long sc = cell_at(&g->blk[c], Y);
         cell_values(g, c, sc, p, ps, v);
-        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0, skip);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0, skip, lx);
         return;
+    }
+    /* a corner ghost (beyond the block along an earlier direction): the
+     * same-level leaf across the face that holds the position clamped into the
+     * block's range has it in its storage (its ghost of the earlier stage) */
+    {
+        int corner = 0, Yk[3] = {Y[0], Y[1], Y[2]};
+        for (int e = 0; e < dd; e++)
+            if (loc[e] < NG || loc[e] >= NG + B->bs[e]) {
+                corner = 1;
+                Yk[e] = Y[e] < B->o[e] ? B->o[e] : (Y[e] >= B->o[e] + B->bs[e] ? B->o[e] + B->bs[e] - 1 : Y[e]);
+            }
+        if (corner) {
+            int lk, k = find_leaf(g, l, Yk, &lk);
+            if (k >= 0 && lk == l) {
+                if (lx && lx->act && !lx->act[k]) return;
+                const long sc = cell_at(&g->blk[k], Y);
+                cell_values(g, k, sc, p, ps, v);
+                write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[k][sc] : 0, skip, lx);
+                return;
+            }
+        }
     }
     if (out1) return;
     int lc, c = find_leaf(g, l, Y, &lc);
     if (c < 0) abort();
+    if (lx && lx->act && !lx->act[c]) return;
     const oracle_amr_block *C = &g->blk[c];
     if (lc == l) {
         const long sc = cell_at(C, Y);
         cell_values(g, c, sc, p, ps, v);
-        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0, skip);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0, skip, lx);
     } else if (lc == l + 1) {
         int F[3] = {2 * Y[0], 2 * Y[1], 2 * Y[2]};
         const long sc = cell_at(C, F);
@@ -255,7 +295,7 @@ static void exchange_cell(const oracle_amr *g, int dd, int b, int i, int j, int 
         const int pos = dd == 0 ? FACE1 : dd == 1 ? FACE2 : FACE3;
         v[NPR + dd] = flux_mean(g, c, dd, F[0] - C->o[0] + NG, F[1] - C->o[1] + NG, F[2] - C->o[2] + NG, ps,
                                 face_gdet(g, b, pos, i, j));
-        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0, skip);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][sc] : 0, skip, lx);
     } else if (lc == l - 1) {
         int P[3] = {Y[0] >> 1, Y[1] >> 1, Y[2] >> 1};
         const long pc = cell_at(C, P);
@@ -279,7 +319,7 @@ static void exchange_cell(const oracle_amr *g, int dd, int b, int i, int j, int 
                 if (d != a) w = w + 0.25 * sgn[d] * dq[d][kk];
             v[NPR + a] = w;
         }
-        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][pc] : 0, skip);
+        write_ghost(g, b, dc, p, ps, pflag, v, pflag ? pflag[c][pc] : 0, skip, lx);
     } else {
         abort();
     }
@@ -295,11 +335,12 @@ static int polar_face(const oracle_amr *g, const oracle_amr_block *B, int side)
     return side ? (B->o[1] + B->bs[1] == g->N[B->level][1]) : (B->o[1] == 0);
 }
 
-void oracle_amr_exchange(const oracle_amr *g, int stage, double **p, double **ps, int **pflag)
+static void exchange_stage(const oracle_amr *g, int stage, double **p, double **ps, int **pflag, const las_x *lx)
 {
     const int dd = stage - 1;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int b = 0; b < g->nblk; b++) {
+        if (lx && lx->act && !lx->act[b]) continue;
         const oracle_amr_block *B = &g->blk[b];
         const bgeo G = geo_of(B);
         for (int side = 0; side < 2; side++) {
@@ -313,9 +354,14 @@ void oracle_amr_exchange(const oracle_amr *g, int stage, double **p, double **ps
             hi[dd] = side ? G.D[dd] : NG;
             for (int i = lo[0]; i < hi[0]; i++)
                 for (int j = lo[1]; j < hi[1]; j++)
-                    for (int z = lo[2]; z < hi[2]; z++) exchange_cell(g, dd, b, i, j, z, p, ps, pflag);
+                    for (int z = lo[2]; z < hi[2]; z++) exchange_cell(g, dd, b, i, j, z, p, ps, pflag, lx);
         }
     }
+}
+
+void oracle_amr_exchange(const oracle_amr *g, int stage, double **p, double **ps, int **pflag)
+{
+    exchange_stage(g, stage, p, ps, pflag, NULL);
 }
 
 /* metric of block b at position pos of column (i, j) */
@@ -439,7 +485,11 @@ static oracle_params block_params(const oracle_amr *g, int b)
 
 /* polar E1 average over each ring of blocks (same side, level, radial range and
  * shape, ordered in phi), as the single-grid oracle averages over its phi-blocks */
-static void ring_average(const oracle_amr *g, double **E, double dt)
+static void ring_average_las(const oracle_amr *g, double **E, double dt, const int *tl, int las_s);
+static void ring_average(const oracle_amr *g, double **E, double dt) { ring_average_las(g, E, dt, NULL, -1); }
+
+/* tl != NULL: the rings stepping in sub-iteration las_s, each weighted by tl dt */
+static void ring_average_las(const oracle_amr *g, double **E, double dt0, const int *tl, int las_s)
 {
     int *done = calloc((size_t)g->nblk * 2, sizeof(int));
     int *ring = malloc(sizeof(int) * (size_t)g->nblk);
@@ -462,6 +512,8 @@ static void ring_average(const oracle_amr *g, double **E, double dt)
                     ring[y - 1] = t;
                 }
             for (int x = 0; x < len; x++) done[ring[x] * 2 + side] = 1;
+            if (tl && oracle_las_phase(tl[ring[0]], las_s) == 0) continue;
+            const double dt = tl ? dt0 * (double)tl[ring[0]] : dt0;
             const int j = side ? NG + A->bs[1] : NG;
             for (int i = NG; i <= NG + A->bs[0]; i++) {
                 double avg = 0.;
@@ -734,5 +786,380 @@ double oracle_amr_step(const oracle_amr *g, double **p, double **ps, double dt, 
         if (d3) free(d3[b]);
     }
     free(ph); free(psh); free(F1); free(F2); free(F3); free(U0); free(E); free(Ec); free(d1); free(d2); free(d3);
+    return ndt;
+}
+
+/* ---- local adaptive time stepping ----
+ * One cycle of 2 maxtl sub-iterations (GPU_step_ch, step_ch.c:916-979, and
+ * advance_GPU, :997-1336, with PRESTEP = PRESTEP2 = 0 and AMR_SWITCHTIMELEVEL
+ * = AMR_MAXTIMELEVEL = maxtl).  In sub-iteration s a block of time level tl
+ * (oracle_las_phase) takes its predictor (p -> ph, Dt = tl dt / 2) or its
+ * corrector (ph -> p, Dt = tl dt) or rests.  Within a sub-iteration: fluxes of
+ * the stepping blocks (their phase's arrays); flux corrections; EMFs; the polar
+ * ring average of the stepping rings (weight tl dt); EMF corrections; CT and
+ * conserved updates; the ghost fill of the stepping blocks (physical faces on
+ * their updated arrays; exchanges only between stepping blocks, each source
+ * read from its phase's arrays, each ghost written into both array pairs of
+ * its block, as unpackreceive1 writes p and ph, GPU_program2.cu:345-388).
+ *
+ * Corrections (boundcomF.c / boundcomE.c with calc_corr = 1; the factor dt tl
+ * and first-step rule of GPU_boundcomF.cu:7-8 and :109-110; the receive
+ * acc / factor of unpackreceive1flux, GPU_program2.cu:1270-1277): a receiver
+ * face or edge takes, at its corrector, acc / (dt tl_r), where acc sums
+ * c (values) over the sender's correctors since the receiver's last one, c =
+ * w dt tl_s, restarted when s % 2tl_r == 2tl_s - 1.  Senders, found by
+ * searching the block list:
+ *   faces: the one level finer leaf across (w = 1/4, its four faces, as
+ *          oracle_amr_reflux), or a same-level leaf of smaller time level (w = 1);
+ *   edges: first the same-level rule: an edge on the block boundary that no
+ *          finer leaf touches takes the EMF of the same-level leaf around it
+ *          with the smallest time level (first in the quadrant order on ties)
+ *          when that is smaller than the block's (w = 1); then the level-jump
+ *          rule of oracle_amr_emf_correct (w = 1/2).
+ * The next dt comes from the last sub-iteration, where every block takes its
+ * corrector: min over blocks of the harmonic Courant dt, clamped to 1.3 dt. */
+
+/* Test switch (planted-mistake tests only): 1 leaves out the flux
+ * corrections of a cycle, 2 the same-level EMF corrections. */
+int oracle_las_plant = 0;
+
+int oracle_las_phase(int tl, int s)
+{
+    if (tl < 1) tl = 1;
+    if (s % (2 * tl) == 2 * tl - 1) return 2;
+    if (s % tl == tl - 1) return 1;
+    return 0;
+}
+
+/* same-level or finer sender of face cell (K: the face, Y: the cell across) of
+ * block b along d; *mean = 1 for a finer leaf; -1: none */
+static int face_sender(const oracle_amr *g, const int *tl, int b, int d, const int Y[3], int *mean)
+{
+    const oracle_amr_block *B = &g->blk[b];
+    int lc;
+    const int m = find_leaf(g, B->level, Y, &lc);
+    *mean = 0;
+    if (m < 0) return -1;
+    if (lc == B->level + 1) {
+        if (tl[m] > tl[b]) abort(); /* finer block slower than its coarser neighbour */
+        *mean = 1;
+        return m;
+    }
+    if (lc == B->level && m != b && tl[m] < tl[b]) return m;
+    return -1;
+}
+
+static void las_flux_corr(const oracle_amr *g, const int *tl, int s, double dt, double **F[3], double **acc[3])
+{
+    for (int stage = 0; stage < 2; stage++)
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int b = 0; b < g->nblk; b++) {
+            const oracle_amr_block *C = &g->blk[b];
+            const long NC = geo_of(C).ncell;
+            const int tr = tl[b];
+            if (stage == 1 && oracle_las_phase(tr, s) != 2) continue;
+            for (int d = 0; d < 3; d++) {
+                const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+                for (int side = 0; side < 2; side++) {
+                    int Y[3];
+                    Y[d] = side ? C->o[d] + C->bs[d] : C->o[d] - 1;
+                    if (d < 2 && (Y[d] < 0 || Y[d] >= g->N[C->level][d])) continue;
+                    if (d == 2) Y[2] = wrap(Y[2], g->N[C->level][2]);
+                    for (int ya = C->o[ta]; ya < C->o[ta] + C->bs[ta]; ya++)
+                        for (int yb = C->o[tb]; yb < C->o[tb] + C->bs[tb]; yb++) {
+                            Y[ta] = ya;
+                            Y[tb] = yb;
+                            int mean;
+                            const int m = face_sender(g, tl, b, d, Y, &mean);
+                            if (m < 0) continue;
+                            int K[3];
+                            for (int k = 0; k < 3; k++)
+                                K[k] = (k == d) ? (side ? NG + C->bs[d] : NG) : Y[k] - C->o[k] + NG;
+                            const long cc = cell_local(C, K[0], K[1], K[2]);
+                            if (stage == 1) {
+                                for (int k = 0; k < NPR; k++)
+                                    F[d][b][k * NC + cc] = acc[d][b][k * NC + cc] / (dt * (double)tr);
+                                continue;
+                            }
+                            const int ts = tl[m];
+                            if (oracle_las_phase(ts, s) != 2) continue;
+                            const int first = s % (2 * tr) == 2 * ts - 1;
+                            const oracle_amr_block *S = &g->blk[m];
+                            const long NS = geo_of(S).ncell;
+                            long f00, f01 = 0, f10 = 0, f11 = 0;
+                            if (mean) {
+                                int Z[3], L[3];
+                                Z[d] = side ? 2 * Y[d] : 2 * Y[d] + 1;
+                                Z[ta] = 2 * ya;
+                                Z[tb] = 2 * yb;
+                                for (int k = 0; k < 3; k++) L[k] = Z[k] - S->o[k] + NG;
+                                L[d] += side ? 0 : 1;
+                                int Lb[3] = {L[0], L[1], L[2]}, La[3] = {L[0], L[1], L[2]}, Lab[3] = {L[0], L[1], L[2]};
+                                Lb[tb] += 1;
+                                La[ta] += 1;
+                                Lab[ta] += 1;
+                                Lab[tb] += 1;
+                                f00 = cell_local(S, L[0], L[1], L[2]);
+                                f01 = cell_local(S, Lb[0], Lb[1], Lb[2]);
+                                f10 = cell_local(S, La[0], La[1], La[2]);
+                                f11 = cell_local(S, Lab[0], Lab[1], Lab[2]);
+                            } else {
+                                /* the same face in the neighbour: its lower face if it lies above */
+                                int L[3];
+                                for (int k = 0; k < 3; k++) L[k] = Y[k] - S->o[k] + NG;
+                                L[d] = side ? NG : NG + S->bs[d];
+                                f00 = cell_local(S, L[0], L[1], L[2]);
+                            }
+                            const double c = (mean ? 0.25 : 1.0) * (dt * (double)ts);
+                            for (int k = 0; k < NPR; k++) {
+                                const double *q = F[d][m] + k * NS;
+                                const double sum = mean ? q[f00] + q[f01] + q[f10] + q[f11] : q[f00];
+                                double *a = &acc[d][b][k * NC + cc];
+                                *a = first ? c * sum : *a + c * sum;
+                            }
+                        }
+                }
+            }
+        }
+}
+
+/* edge sender of boundary edge K (component t) of block b: rule 0 same-level,
+ * rule 1 level jump; -1: none.  Z receives the sender's edge (level of the sender). */
+static int edge_sender(const oracle_amr *g, const int *tl, int b, int t, const int K[3], int rule, int Z[3])
+{
+    const oracle_amr_block *C = &g->blk[b];
+    const int l = C->level, p = (t + 1) % 3, q = (t + 2) % 3;
+    int best = -1, finer = -1;
+    for (int dp = -1; dp <= 0; dp++)
+        for (int dq = -1; dq <= 0; dq++) {
+            int Y[3] = {K[0], K[1], K[2]}, ok = 1, lc;
+            Y[p] += dp;
+            Y[q] += dq;
+            for (int x = 0; x < 2; x++)
+                if (Y[x] < 0 || Y[x] >= g->N[l][x]) ok = 0;
+            Y[2] = wrap(Y[2], g->N[l][2]);
+            if (!ok) continue;
+            const int c = find_leaf(g, l, Y, &lc);
+            if (c < 0) continue;
+            if (lc == l + 1 && finer < 0) finer = c;
+            if (lc == l && (best < 0 || tl[c] < tl[best])) best = c;
+        }
+    if (rule == 1) {
+        if (finer < 0) return -1;
+        if (tl[finer] > tl[b]) abort();
+        const oracle_amr_block *Fb = &g->blk[finer];
+        for (int x = 0; x < 3; x++) Z[x] = 2 * K[x];
+        if (Z[2] < Fb->o[2]) Z[2] += g->N[l + 1][2];
+        if (Z[2] > Fb->o[2] + Fb->bs[2]) Z[2] -= g->N[l + 1][2];
+        return finer;
+    }
+    if (finer >= 0 || best < 0 || tl[best] >= tl[b]) return -1;
+    const oracle_amr_block *S = &g->blk[best];
+    for (int x = 0; x < 3; x++) Z[x] = K[x];
+    if (Z[2] < S->o[2]) Z[2] += g->N[l][2];
+    if (Z[2] > S->o[2] + S->bs[2]) Z[2] -= g->N[l][2];
+    return best;
+}
+
+static void las_emf_corr(const oracle_amr *g, const int *tl, int s, double dt, double **E, double **acc, int rule)
+{
+    for (int stage = 0; stage < 2; stage++)
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int b = 0; b < g->nblk; b++) {
+            const oracle_amr_block *C = &g->blk[b];
+            const long NC = geo_of(C).ncell;
+            const int tr = tl[b];
+            if (stage == 1 && oracle_las_phase(tr, s) != 2) continue;
+            if (rule == 1 && C->level + 1 >= g->nlev) continue;
+            for (int t = 0; t < 3; t++) {
+                const int p = (t + 1) % 3, q = (t + 2) % 3;
+                int K[3];
+                for (K[t] = C->o[t]; K[t] < C->o[t] + C->bs[t]; K[t]++)
+                    for (K[p] = C->o[p]; K[p] <= C->o[p] + C->bs[p]; K[p]++)
+                        for (K[q] = C->o[q]; K[q] <= C->o[q] + C->bs[q]; K[q]++) {
+                            if (K[p] != C->o[p] && K[p] != C->o[p] + C->bs[p] && K[q] != C->o[q] &&
+                                K[q] != C->o[q] + C->bs[q])
+                                continue;
+                            int Z[3];
+                            const int m = edge_sender(g, tl, b, t, K, rule, Z);
+                            if (m < 0) continue;
+                            const long cc = (long)t * NC + cell_at(C, K);
+                            if (stage == 1) {
+                                E[b][cc] = acc[b][cc] / (dt * (double)tr);
+                                continue;
+                            }
+                            const int ts = tl[m];
+                            if (oracle_las_phase(ts, s) != 2) continue;
+                            const int first = s % (2 * tr) == 2 * ts - 1;
+                            const oracle_amr_block *S = &g->blk[m];
+                            const double *e = E[m] + (long)t * geo_of(S).ncell;
+                            double sum, c;
+                            if (rule == 1) {
+                                int Z1[3] = {Z[0], Z[1], Z[2]};
+                                Z1[t] += 1;
+                                sum = e[cell_at(S, Z)] + e[cell_at(S, Z1)];
+                                c = 0.5 * (dt * (double)ts);
+                            } else {
+                                sum = e[cell_at(S, Z)];
+                                c = 1.0 * (dt * (double)ts);
+                            }
+                            acc[b][cc] = first ? c * sum : acc[b][cc] + c * sum;
+                        }
+            }
+        }
+}
+
+double oracle_amr_las_cycle(const oracle_amr *g, const int *tl, int maxtl, double **p, double **ps, double dt,
+                            int **pflag, long long *failcount, double *bdt_out, double **Fint, double **Eint)
+{
+    const int nb = g->nblk;
+    const int nsub = 2 * maxtl;
+    double **ph = calloc(nb, sizeof(double *)), **psh = calloc(nb, sizeof(double *));
+    double **F1 = calloc(nb, sizeof(double *)), **F2 = calloc(nb, sizeof(double *)), **F3 = calloc(nb, sizeof(double *));
+    double **U0 = calloc(nb, sizeof(double *)), **E = calloc(nb, sizeof(double *)), **Ec = calloc(nb, sizeof(double *));
+    double **d1 = calloc(nb, sizeof(double *)), **d2 = calloc(nb, sizeof(double *)), **d3 = calloc(nb, sizeof(double *));
+    double **a1 = calloc(nb, sizeof(double *)), **a2 = calloc(nb, sizeof(double *)), **a3 = calloc(nb, sizeof(double *));
+    double **aes = calloc(nb, sizeof(double *)), **aej = calloc(nb, sizeof(double *));
+    double **rp = calloc(nb, sizeof(double *)), **rps = calloc(nb, sizeof(double *));
+    double **op = calloc(nb, sizeof(double *)), **ops = calloc(nb, sizeof(double *));
+    int *act = calloc(nb, sizeof(int)), *phs = calloc(nb, sizeof(int));
+    double ndt = -1.;
+    int ok = ph && psh && F1 && F2 && F3 && U0 && E && Ec && d1 && d2 && d3 && a1 && a2 && a3 && aes && aej && rp &&
+             rps && op && ops && act && phs;
+    for (int b = 0; ok && b < nb; b++) {
+        const long N = geo_of(&g->blk[b]).ncell;
+        if (tl[b] < 1 || tl[b] > maxtl || (tl[b] & (tl[b] - 1))) ok = 0;
+        ph[b] = malloc(sizeof(double) * N * NPR);
+        psh[b] = malloc(sizeof(double) * N * 3);
+        F1[b] = calloc(N * NPR, sizeof(double));
+        F2[b] = calloc(N * NPR, sizeof(double));
+        F3[b] = calloc(N * NPR, sizeof(double));
+        U0[b] = calloc(N * NPR, sizeof(double));
+        E[b] = calloc(N * 3, sizeof(double));
+        Ec[b] = calloc(N * 3, sizeof(double));
+        d1[b] = malloc(sizeof(double) * N);
+        d2[b] = malloc(sizeof(double) * N);
+        d3[b] = malloc(sizeof(double) * N);
+        a1[b] = calloc(N * NPR, sizeof(double));
+        a2[b] = calloc(N * NPR, sizeof(double));
+        a3[b] = calloc(N * NPR, sizeof(double));
+        aes[b] = calloc(N * 3, sizeof(double));
+        aej[b] = calloc(N * 3, sizeof(double));
+        ok = ok && ph[b] && psh[b] && F1[b] && F2[b] && F3[b] && U0[b] && E[b] && Ec[b] && d1[b] && d2[b] && d3[b] &&
+             a1[b] && a2[b] && a3[b] && aes[b] && aej[b];
+        if (ok) {
+            memcpy(ph[b], p[b], sizeof(double) * N * NPR);
+            memcpy(psh[b], ps[b], sizeof(double) * N * 3);
+        }
+    }
+    if (ok) {
+        double **F[3] = {F1, F2, F3}, **acc[3] = {a1, a2, a3};
+        for (int s = 0; s < nsub; s++) {
+            const int last = s == nsub - 1;
+            for (int b = 0; b < nb; b++) {
+                phs[b] = oracle_las_phase(tl[b], s);
+                act[b] = phs[b] != 0;
+            }
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int b = 0; b < nb; b++) {
+                if (!act[b]) continue;
+                oracle_params par = block_params(g, b);
+                const double *pi = phs[b] == 1 ? p[b] : ph[b], *psi = phs[b] == 1 ? ps[b] : psh[b];
+                oracle_flux(&par, 1, pi, psi, &g->m[b], F1[b], last ? d1[b] : NULL);
+                oracle_flux(&par, 2, pi, psi, &g->m[b], F2[b], last ? d2[b] : NULL);
+                oracle_flux(&par, 3, pi, psi, &g->m[b], F3[b], last ? d3[b] : NULL);
+            }
+            if (oracle_las_plant != 1) las_flux_corr(g, tl, s, dt, F, acc);
+            if (Fint)
+                for (int b = 0; b < nb; b++) {
+                    if (phs[b] != 2) continue;
+                    const long n = geo_of(&g->blk[b]).ncell * NPR;
+                    const double Dt = dt * (double)tl[b];
+                    for (int d = 0; d < 3; d++)
+                        for (long k = 0; k < n; k++) Fint[3 * b + d][k] += Dt * F[d][b][k];
+                }
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int b = 0; b < nb; b++) {
+                if (!act[b]) continue;
+                oracle_params par = block_params(g, b);
+                oracle_emf(&par, phs[b] == 1 ? p[b] : ph[b], F1[b], F2[b], F3[b], &g->m[b], dt * (double)tl[b], E[b],
+                           Ec[b]);
+            }
+            ring_average_las(g, E, dt, tl, s);
+            if (oracle_las_plant != 2) las_emf_corr(g, tl, s, dt, E, aes, 0);
+            las_emf_corr(g, tl, s, dt, E, aej, 1);
+            if (Eint)
+                for (int b = 0; b < nb; b++) {
+                    if (phs[b] != 2) continue;
+                    const long n = geo_of(&g->blk[b]).ncell * 3;
+                    const double Dt = dt * (double)tl[b];
+                    for (long k = 0; k < n; k++) Eint[b][k] += Dt * E[b][k];
+                }
+            for (int b = 0; b < nb; b++) {
+                if (!act[b]) continue;
+                oracle_params par = block_params(g, b);
+                const double Dt = phs[b] == 1 ? 0.5 * dt * (double)tl[b] : dt * (double)tl[b];
+                if (phs[b] == 1) {
+                    oracle_ct_update(&par, Dt, ps[b], psh[b], E[b], &g->m[b]);
+                    oracle_update(&par, 0, Dt, p[b], p[b], ph[b], U0[b], psh[b], F1[b], F2[b], F3[b], &g->m[b],
+                                  pflag[b], failcount);
+                } else {
+                    oracle_ct_update(&par, Dt, ps[b], ps[b], E[b], &g->m[b]);
+                    oracle_update(&par, 1, Dt, p[b], ph[b], p[b], U0[b], ps[b], F1[b], F2[b], F3[b], &g->m[b],
+                                  pflag[b], failcount);
+                }
+            }
+            /* ghost fill of the stepping blocks */
+            for (int b = 0; b < nb; b++) {
+                rp[b] = phs[b] == 1 ? ph[b] : p[b];
+                rps[b] = phs[b] == 1 ? psh[b] : ps[b];
+                op[b] = phs[b] == 1 ? p[b] : ph[b];
+                ops[b] = phs[b] == 1 ? ps[b] : psh[b];
+            }
+            const las_x lx = {act, op, ops};
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int b = 0; b < nb; b++)
+                if (act[b]) outflow(g, b, rp, rps, pflag, 0);
+            exchange_stage(g, 1, rp, rps, pflag, &lx);
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int b = 0; b < nb; b++)
+                if (act[b]) polar(g, b, rp, rps, pflag);
+            exchange_stage(g, 2, rp, rps, pflag, &lx);
+            exchange_stage(g, 3, rp, rps, pflag, &lx);
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int b = 0; b < nb; b++)
+                if (act[b]) outflow(g, b, rp, rps, pflag, 1);
+            if (last) {
+                ndt = 1e9;
+                for (int b = 0; b < nb; b++) {
+                    oracle_params par = block_params(g, b);
+                    const double bd = oracle_block_dt(&par, d1[b], d2[b], d3[b]);
+                    if (bdt_out) bdt_out[b] = bd;
+                    if (bd < ndt) ndt = bd;
+                }
+                if (ndt > 1.3 * dt) ndt = 1.3 * dt;
+            }
+        }
+    }
+    for (int b = 0; b < nb; b++) {
+        if (ph) free(ph[b]);
+        if (psh) free(psh[b]);
+        if (F1) free(F1[b]);
+        if (F2) free(F2[b]);
+        if (F3) free(F3[b]);
+        if (U0) free(U0[b]);
+        if (E) free(E[b]);
+        if (Ec) free(Ec[b]);
+        if (d1) free(d1[b]);
+        if (d2) free(d2[b]);
+        if (d3) free(d3[b]);
+        if (a1) free(a1[b]);
+        if (a2) free(a2[b]);
+        if (a3) free(a3[b]);
+        if (aes) free(aes[b]);
+        if (aej) free(aej[b]);
+    }
+    free(ph); free(psh); free(F1); free(F2); free(F3); free(U0); free(E); free(Ec); free(d1); free(d2); free(d3);
+    free(a1); free(a2); free(a3); free(aes); free(aej); free(rp); free(rps); free(op); free(ops); free(act); free(phs);
     return ndt;
 }

@@ -162,6 +162,23 @@ void oracle_amr_fluxes(const oracle_amr *g, double **p, double **ps, double dt, 
  * returns the next dt (min over blocks, clamped to 1.3 dt), < 0 on allocation failure. */
 double oracle_amr_step(const oracle_amr *g, double **p, double **ps, double dt, int **pflag, long long *failcount);
 
+/* Phase of a block of time level tl in sub-iteration s of a local-adaptive-
+ * time-stepping cycle: 1 predictor (s % tl == tl - 1), 2 corrector (s % 2tl
+ * == 2tl - 1, checked first), 0 rest. */
+int oracle_las_phase(int tl, int s);
+/* One local-adaptive-time-stepping cycle of 2 maxtl sub-iterations (time
+ * levels tl[b]: powers of 2 up to maxtl; a finer block at a level jump no
+ * slower than its coarser neighbour), ghosts filled on input, pflag required.
+ * Returns the next dt (< 0 on a bad time level or allocation failure);
+ * bdt_out (may be NULL) receives each block's harmonic Courant dt of the last
+ * sub-iteration; Fint (may be NULL; 3 nblk arrays of NPR ncell, Fint[3 b + d]
+ * the fluxes of direction d + 1 of block b) accumulates Dt F of every
+ * corrector's (corrected) fluxes; Eint (may be NULL; nblk arrays of 3 ncell)
+ * the same of the corrector's (corrected) edge EMFs. */
+double oracle_amr_las_cycle(const oracle_amr *g, const int *tl, int maxtl, double **p, double **ps, double dt,
+                            int **pflag, long long *failcount, double *bdt_out, double **Fint, double **Eint);
+extern int oracle_las_plant;
+
 /* ---- restart record of one block (hamr_rdump_oracle.c) ----
  * One block's ghost-inclusive arrays in the layout above (bs: interior size;
  * gdet with ncol = D1*D2).  The record holds NPR+4 doubles per cell. */

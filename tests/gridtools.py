@@ -173,3 +173,53 @@ def amr_divb(metrics, rows, dx0, ps):
         if scale > 0:
             worst = max(worst, float(np.abs(div).max() / scale))
     return worst
+
+
+def las_levels(rows, seed=7):
+    """Time levels for the three-level torus set: level-0 blocks 4 (those on
+    the polar axis, which form the rings, always) or 2, level-1 blocks 2 or 1,
+    level-2 blocks 1, by a seeded choice.  Finer blocks at level jumps are
+    never slower than their coarser neighbours, and same-level neighbours of
+    different levels occur on levels 0 and 1."""
+    rng = np.random.default_rng(seed)
+    tl = np.zeros(len(rows), dtype=np.int32)
+    for b, r in enumerate(rows):
+        lev, pole = int(r[0]), int(r[8])
+        if lev == 0:
+            tl[b] = 4 if (pole or rng.random() < 0.5) else 2
+        elif lev == 1:
+            tl[b] = 2 if rng.random() < 0.5 else 1
+        else:
+            tl[b] = 1
+    return tl
+
+
+def las_rule_py(bdt, dt, maxtl, rings, pairs):
+    """set_timelevel's rule (LAS.c:5-144) restated: the largest power of 2 not
+    above maxtl with bdt / dt >= 1.05 times it (else 1), each ring its minimum,
+    finer blocks at level jumps lowered to their coarser neighbour's level,
+    repeated until nothing changes."""
+    tl = []
+    for b in bdt:
+        v = 1
+        k = maxtl
+        while k >= 1:
+            if b / dt >= 1.05 * k:
+                v = k
+                break
+            k //= 2
+        tl.append(v)
+    changed = True
+    while changed:
+        changed = False
+        for ring in rings:
+            m = min(tl[x] for x in ring)
+            for x in ring:
+                if tl[x] != m:
+                    tl[x] = m
+                    changed = True
+        for c, f in pairs:
+            if tl[f] > tl[c]:
+                tl[f] = tl[c]
+                changed = True
+    return np.array(tl, dtype=np.int32)

@@ -289,6 +289,9 @@ def _setup_amr(lib):
     lib.oracle_amr_reflux.argtypes = [G, pp, pp, pp, C.c_double]
     lib.oracle_amr_fluxes.argtypes = [G, pp, pp, C.c_double, pp, pp, pp]
     lib.oracle_amr_ct.argtypes = [G, pp, pp, C.c_double, pp]
+    lib.oracle_amr_las_cycle.argtypes = [G, _ip, C.c_int, pp, pp, C.c_double, ipp, C.POINTER(C.c_longlong), _dp,
+                                         pp, pp]
+    lib.oracle_amr_las_cycle.restype = C.c_double
     lib._amr_ready = True
 
 
@@ -360,3 +363,20 @@ class AmrGrid:
     def step(self, p, ps, pflag, dt):
         return self.o.lib.oracle_amr_step(C.byref(self.g), self._pp(p), self._pp(ps), dt, self._pp(pflag, _ip),
                                           self.failcount)
+
+    def las_cycle(self, tl, maxtl, p, ps, pflag, dt, fint=False):
+        """One local-adaptive-time-stepping cycle (oracle_amr_las_cycle);
+        returns (next dt, per-block Courant dt) and with fint also the
+        per-block [F1, F2, F3] and edge-EMF sums of Dt F and Dt E over each
+        block's correctors."""
+        tl = np.ascontiguousarray(tl, dtype=np.int32)
+        bdt = np.zeros(len(p))
+        F = [[np.zeros_like(a) for _ in range(3)] for a in p] if fint else None
+        E = [np.zeros((3,) + a.shape[1:]) for a in p] if fint else None
+        Fp = self._pp([x for blk in F for x in blk]) if fint else None
+        Ep = self._pp(E) if fint else None
+        ndt = self.o.lib.oracle_amr_las_cycle(C.byref(self.g), _ptr(tl, _ip), int(maxtl), self._pp(p), self._pp(ps), dt,
+                                              self._pp(pflag, _ip), self.failcount, _ptr(bdt), Fp, Ep)
+        if ndt < 0:
+            raise RuntimeError("oracle_amr_las_cycle: bad time levels or allocation failure")
+        return (ndt, bdt, F, E) if fint else (ndt, bdt)

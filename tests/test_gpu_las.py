@@ -1,0 +1,210 @@
+"""Local adaptive time stepping on the GPU (hamr_set_timelevels,
+hamr_step_las, hamr_timelevels_from_dt) against the oracle's cycle
+(oracle_amr_las_cycle) and the standard step.
+
+Workload: the three-level torus set of test_gpu_amr (base 64x64x32, level 1
+over x1 [0,64) x x2 [32,96), level 2 over x1 [0,64) x x2 [96,160), blocks of
+64x32x32, 32x16x16 and 16^3), FM torus with a uniform field (0.02, 0.01,
+0.005) added so that every interface sees nonzero EMFs; dt a quarter of the
+Courant dt.
+
+Bars:
+- every block at time level 1, maxtl 1: hamr_step_las equals hamr_step bit for
+  bit (state, flags, next dt); every block at time level 2, maxtl 2, with dt:
+  equal to hamr_step with 2 dt bit for bit;
+- mixed time levels (level-0 blocks 4 or 2, the polar rings 4, level-1 blocks
+  2 or 1, level-2 blocks 1; maxtl 4): one FP64 cycle within 1e-12 of the
+  oracle's cycle cell by cell (|a - b| <= 1e-12 |b|), flags equal, the next dt
+  and every block's Courant dt within 1e-12; the mixed-precision cycle within
+  1e-4 of the FP64 cycle (rho and u relative to their largest magnitude,
+  velocity and field to the largest component of their vector; the finest
+  blocks take four steps, each within 1e-5 in the single-step test);
+- the time levels from the block dts: powers of 2 up to maxtl, none above
+  set_timelevel's rule for its block, equal over each polar ring, accepted by
+  hamr_set_timelevels, and a cycle with them runs;
+- bad time levels (not a power of 2, unequal over a ring, a finer block at a
+  level jump slower than its coarser neighbour), a cycle before any time
+  levels or with maxtl below the largest, and a uniform context are refused
+  with a message.
+"""
+import numpy as np
+import pytest
+
+import hamr_gamma_wind_amd as H
+from tests import gridtools as GT
+
+pytestmark = pytest.mark.gpu
+
+DESC = dict(bs=(64, 32, 32), nb=(1, 2, 1))
+BOXES = [((0, 64), (32, 96), (0, 64)), ((0, 64), (96, 160), (0, 128))]
+BG = np.array([0.02, 0.01, 0.005])
+
+
+def _new_ctx():
+    desc = H.make_desc(DESC["bs"], DESC["nb"])
+    amr = H.make_amr_desc(BOXES, seed=5)
+    ctx = H.AmrContext(desc, amr)
+    ctx.set_grid()
+    ctx.init_torus(H.torus_desc())
+    p, ps = ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS)
+    for a, c in zip(p, ps):
+        a[5:8] += BG[:, None, None, None]
+        c += BG[:, None, None, None]
+    ctx.set(H.HAMR_P, p)
+    ctx.set(H.HAMR_PS, ps)
+    ctx.boundprim(1)
+    return desc, amr, ctx
+
+
+@pytest.fixture(scope="module")
+def las(oracle):
+    desc, amr, ctx = _new_ctx()
+    g, rows = GT.amr_oracle(oracle, desc, amr)
+    assert np.array_equal(ctx.layout, rows)
+    start = (ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS))
+    dt = 0.25 * ctx.estimate_dt()
+    yield ctx, g, rows, start, dt
+    ctx.close()
+
+
+def _reset(ctx, start):
+    ctx.set(H.HAMR_P, start[0])
+    ctx.set(H.HAMR_PS, start[1])
+    ctx.clear_flags()
+
+
+def _same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_unit_and_uniform_time_levels_equal_standard_steps(las):
+    ctx, g, rows, start, dt = las
+    ctx.set_precision(H.HAMR_PREC_FP64)
+    nb = ctx.nblk
+    print("unit time levels, maxtl 1", flush=True)
+    _reset(ctx, start)
+    want_dt = ctx.step(1, dt=dt)
+    want = (ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS), [f for f, _ in ctx.all_flags()])
+    _reset(ctx, start)
+    ctx.set_timelevels(np.ones(nb, dtype=np.int32))
+    got_dt = ctx.step_las(1, 1, dt=dt)
+    got = (ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS), [f for f, _ in ctx.all_flags()])
+    print("dt: step %.17g cycle %.17g" % (want_dt, got_dt), flush=True)
+    assert got_dt == want_dt
+    assert _same(got[0], want[0]) and _same(got[1], want[1]) and _same(got[2], want[2])
+    print("time level 2 everywhere, maxtl 2, against a step of 2 dt", flush=True)
+    _reset(ctx, start)
+    ctx.step(1, dt=2 * dt)
+    want = (ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS))
+    _reset(ctx, start)
+    ctx.set_timelevels(np.full(nb, 2, dtype=np.int32))
+    ctx.step_las(2, 1, dt=dt)
+    got = (ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS))
+    assert _same(got[0], want[0]) and _same(got[1], want[1])
+
+
+def test_mixed_time_levels_match_the_oracle(las):
+    ctx, g, rows, start, dt = las
+    ctx.set_precision(H.HAMR_PREC_FP64)
+    tl = GT.las_levels(rows)
+    _reset(ctx, start)
+    ctx.set_timelevels(tl)
+    info = ctx.las_info()
+    print("time levels per block count", np.bincount(tl).tolist(), "corrections", info, flush=True)
+    assert info["tlmax"] == 4 and info["flux1"] + info["flux2"] + info["flux3"] > 0 and info["emf_same"] > 0
+    ndt = ctx.step_las(4, 1, dt=dt)
+    bdt = ctx.block_dts()
+    gp, gps = ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS)
+    p = [np.ascontiguousarray(a) for a in start[0]]
+    ps = [np.ascontiguousarray(a) for a in start[1]]
+    flags = [np.zeros(a.shape[1:], dtype=np.int32) for a in p]
+    print("oracle cycle", flush=True)
+    want_dt, want_bdt = g.las_cycle(tl, 4, p, ps, flags, dt)
+    ep, es = GT.pointwise_relerr_blocks(gp, p), GT.pointwise_relerr_blocks(gps, ps)
+    print("dt %.17g oracle %.17g; block dt max rel %.2e" % (ndt, want_dt, np.max(np.abs(bdt - want_bdt) / want_bdt)),
+          flush=True)
+    print("P rel err per var", np.array2string(ep, precision=2), "face B", np.array2string(es, precision=2), flush=True)
+    assert abs(ndt - want_dt) <= 1e-12 * want_dt
+    assert np.all(np.abs(bdt - want_bdt) <= 1e-12 * want_bdt)
+    assert np.all(ep <= 1e-12) and np.all(es <= 1e-12)
+    for n in range(ctx.nblk):
+        assert np.array_equal(ctx.flags(n)[0], flags[n]), n
+
+
+def test_mixed_precision_cycle_close_to_fp64(las):
+    ctx, g, rows, start, dt = las
+    tl = GT.las_levels(rows)
+    ctx.set_timelevels(tl)
+    out = {}
+    for prec in (H.HAMR_PREC_FP64, H.HAMR_PREC_MIXED):
+        ctx.set_precision(prec)
+        _reset(ctx, start)
+        ctx.step_las(4, 1, dt=dt)
+        out[prec] = ctx.get(H.HAMR_P)
+    ctx.set_precision(H.HAMR_PREC_FP64)
+    a, b = out[H.HAMR_PREC_MIXED], out[H.HAMR_PREC_FP64]
+    err = {}
+    for name, sl in (("rho", [0]), ("u", [1]), ("v", [2, 3, 4]), ("B", [5, 6, 7])):
+        e = max(float(np.abs(x[sl] - y[sl]).max()) for x, y in zip(a, b))
+        s = max(float(np.abs(y[sl]).max()) for y in b)
+        err[name] = e / s
+    print("mixed vs FP64 cycle:", {k: "%.2e" % v for k, v in err.items()}, flush=True)
+    assert all(np.isfinite(x).all() for x in a)
+    assert max(err.values()) < 1e-4, err
+
+
+def test_time_levels_from_block_dts(las):
+    ctx, g, rows, start, dt = las
+    ctx.set_precision(H.HAMR_PREC_FP64)
+    _reset(ctx, start)
+    ndt = ctx.step(1, dt=dt)
+    bdt = ctx.block_dts()
+    tl = ctx.timelevels_from_dt(4)
+    raw = GT.las_rule_py(bdt, ndt, 4, [], [])
+    print("time levels from dt:", np.bincount(tl).tolist(), "unconstrained", np.bincount(raw).tolist(), flush=True)
+    assert np.array_equal(ctx.timelevels(), tl)
+    assert set(np.unique(tl).tolist()) <= {1, 2, 4} and np.all(tl <= raw)
+    # the polar rings (blocks on the axis: same level, side, radial range and shape)
+    rings = {}
+    for n, r in enumerate(rows):
+        if int(r[8]):
+            rings.setdefault((int(r[0]), int(r[8]), int(r[1]), int(r[4]), int(r[5]), int(r[6])), []).append(n)
+    assert rings
+    for members in rings.values():
+        assert len(set(tl[members].tolist())) == 1, members
+    ctx.step_las(4, 1)
+
+
+def test_bad_time_levels_are_refused(las):
+    ctx, g, rows, start, dt = las
+    nb = ctx.nblk
+    tl = GT.las_levels(rows)
+    bad = tl.copy()
+    bad[0] = 3
+    with pytest.raises(H.HamrError, match="powers of 2"):
+        ctx.set_timelevels(bad)
+    ring = [n for n, r in enumerate(rows) if int(r[8])]
+    bad = tl.copy()
+    bad[ring[0]] = 2
+    with pytest.raises(H.HamrError, match="ring"):
+        ctx.set_timelevels(bad)
+    bad = np.ones(nb, dtype=np.int32)
+    bad[[n for n, r in enumerate(rows) if int(r[0]) == 2][0]] = 2
+    with pytest.raises(H.HamrError, match="level jump"):
+        ctx.set_timelevels(bad)
+    ctx.set_timelevels(tl)
+    with pytest.raises(H.HamrError, match="maxtl"):
+        ctx.step_las(2, 1)
+    desc, amr, fresh = _new_ctx()
+    try:
+        with pytest.raises(H.HamrError, match="set the time levels"):
+            fresh.step_las(1, 1)
+    finally:
+        fresh.close()
+    uni = H.Context(H.make_desc((16, 8, 8), (2, 2, 2)))
+    try:
+        v = np.ones(8, dtype=np.int32)
+        assert uni.L.hamr_set_timelevels(uni.h, H._ptr(v, H._ip)) != 0
+        assert "multi-level" in uni.L.hamr_last_error(uni.h).decode()
+    finally:
+        uni.close()

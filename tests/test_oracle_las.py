@@ -1,0 +1,249 @@
+"""The oracle's local-adaptive-time-stepping cycle (oracle_amr_las_cycle),
+pinned on the CPU, and the time-level rule of the C ABI (hamr_las_rule).
+
+Workload: the three-level torus set of test_oracle_amr (base 64x64x32,
+level 1 over x1 [0,64) x x2 [32,96), level 2 over x1 [0,64) x x2 [96,160),
+blocks of 64x32x32, 32x16x16 and 16^3), FM torus with a uniform field
+(0.02, 0.01, 0.005) added so that every interface sees nonzero EMFs.
+
+Pins:
+- every block at time level 1 (maxtl 1): the cycle is the oracle's standard
+  multi-level step (itself pinned in test_oracle_amr), bit for bit;
+- every block at time level 2 (maxtl 2) with dt: the standard step with 2 dt,
+  bit for bit (the cycle's predictor and corrector steps are tl dt / 2 and
+  tl dt, and every scaling by 2 is exact);
+- mixed time levels (level-0 blocks 4 or 2, level-1 blocks 2 or 1, level-2
+  blocks 1; ratios up to 4 across level jumps and 2 between same-level
+  neighbours): at every corrected interface both sides integrate the same
+  flux and edge EMF over the cycle (Dt F and Dt E summed over each block's
+  correctors): equal across same-level neighbours of different time levels,
+  the mean of the four finer faces and of the two finer edges at level jumps.
+  That is conservation across the interface and agreement of the face fields
+  both sides evolve.  With the flux corrections or the same-level EMF
+  corrections planted out of the oracle the mismatch is of order one.
+  (Neighbours of equal time level are not corrected: each computes the shared
+  face from its own ghosts, and ghosts copied from a third block at different
+  sub-iterations of a cycle can differ, as in the reference's exchange, which
+  skips blocks that do not step);
+- the time-level rule against its restatement on random inputs.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import hamr_gamma_wind_amd as H
+from tests import gridtools as GT
+from tests.test_oracle_amr import _leaf_finder
+
+NG = 3
+DESC3 = dict(bs=(64, 32, 32), nb=(1, 2, 1))
+BOXES3 = [((0, 64), (32, 96), (0, 64)), ((0, 64), (96, 160), (0, 128))]
+BG = np.array([0.02, 0.01, 0.005])
+
+
+def _set(oracle):
+    desc = H.make_desc(DESC3["bs"], DESC3["nb"])
+    amr = H.make_amr_desc(BOXES3, seed=5)
+    g, rows = GT.amr_oracle(oracle, desc, amr)
+    p, ps = H.desc_amr_torus(desc, amr, H.torus_desc())
+    p = [np.ascontiguousarray(a) for a in p]
+    ps = [np.ascontiguousarray(a) for a in ps]
+    for a, c in zip(p, ps):
+        a[5:8] += BG[:, None, None, None]
+        c += BG[:, None, None, None]
+    flags = [np.zeros(a.shape[1:], dtype=np.int32) for a in p]
+    g.bound(p, ps, flags)
+    _, dx0 = H.desc_coords(desc)
+    return desc, g, rows, p, ps, flags, dx0
+
+
+def _copy(arrs):
+    return [np.array(a, copy=True) for a in arrs]
+
+
+@pytest.fixture(scope="module")
+def las_set(oracle):
+    return _set(oracle)
+
+
+def test_unit_time_levels_equal_the_standard_step(las_set):
+    desc, g, rows, p0, ps0, f0, dx0 = las_set
+    dt = 0.2 * min(dx0)
+    p, ps, fl = _copy(p0), _copy(ps0), _copy(f0)
+    want_dt = g.step(p, ps, fl, dt)
+    q, qs, ql = _copy(p0), _copy(ps0), _copy(f0)
+    ndt, _ = g.las_cycle(np.ones(len(rows), dtype=np.int32), 1, q, qs, ql, dt)
+    print("dt: step %.17g cycle %.17g" % (want_dt, ndt))
+    assert ndt == want_dt
+    for b in range(len(rows)):
+        assert np.array_equal(q[b], p[b]) and np.array_equal(qs[b], ps[b]) and np.array_equal(ql[b], fl[b]), b
+
+
+def test_uniform_time_level_two_equals_a_double_step(las_set):
+    desc, g, rows, p0, ps0, f0, dx0 = las_set
+    dt = 0.1 * min(dx0)
+    p, ps, fl = _copy(p0), _copy(ps0), _copy(f0)
+    g.step(p, ps, fl, 2 * dt)
+    q, qs, ql = _copy(p0), _copy(ps0), _copy(f0)
+    g.las_cycle(np.full(len(rows), 2, dtype=np.int32), 2, q, qs, ql, dt)
+    for b in range(len(rows)):
+        assert np.array_equal(q[b], p[b]) and np.array_equal(qs[b], ps[b]) and np.array_equal(ql[b], fl[b]), b
+
+
+def _levels(rows):
+    n0 = [DESC3["bs"][k] * DESC3["nb"][k] for k in range(3)]
+    return [[x << l for x in n0] for l in range(3)]
+
+
+def _interfaces(rows, N, tl):
+    """Every square of a block face whose other side is held by one leaf that
+    corrects it: (b, m, d, side, ua, ub, us, jump), m a same-level leaf of
+    smaller time level (squares of 16 cells) or a one level finer leaf
+    (squares of 8); ua, ub the square's first cells along the two transverse
+    directions (slower first), us its size."""
+    leaf = _leaf_finder(rows, N)
+    out = []
+    for b, r in enumerate(rows):
+        l, o, bs = int(r[0]), [int(x) for x in r[1:4]], [int(x) for x in r[4:7]]
+        for d in range(3):
+            ta, tb = [k for k in range(3) if k != d]
+            for side in (0, 1):
+                Y = [0, 0, 0]
+                Y[d] = o[d] + bs[d] if side else o[d] - 1
+                if d < 2 and not (0 <= Y[d] < N[l][d]):
+                    continue
+                Y[d] %= N[l][d]
+                for ua in range(o[ta], o[ta] + bs[ta], 16):
+                    for ub in range(o[tb], o[tb] + bs[tb], 16):
+                        Y[ta], Y[tb] = ua, ub
+                        m, lm = leaf(l, Y)
+                        if lm == l and m != b and tl[m] < tl[b]:
+                            out.append((b, m, d, side, ua, ub, 16, False))
+                        elif lm == l + 1:
+                            # a finer leaf holds 16 of its cells: 8 of this level
+                            for sa in (0, 8):
+                                for sb in (0, 8):
+                                    Y[ta], Y[tb] = ua + sa, ub + sb
+                                    m, lm = leaf(l, Y)
+                                    assert lm == l + 1
+                                    out.append((b, m, d, side, ua + sa, ub + sb, 8, True))
+    return out
+
+
+def _face_index(r, d, F, N):
+    """local index along d of global face coordinate F in block row r (x3 wrapped)"""
+    o, bs = int(r[1 + d]), int(r[4 + d])
+    if not (o <= F <= o + bs):
+        F += N if F < o else -N
+    return F - o + NG
+
+
+def _check_interfaces(rows, N, tl, Fint, Eint):
+    """worst relative mismatch of the time-integrated fluxes and in-plane edge
+    EMFs across the corrected interfaces: same level, equal; level jump, the
+    coarse face the mean of the four finer faces and the coarse edge the mean
+    of the two finer edges along it"""
+    wf = wE = 0.0
+    sf = sE = 0.0
+    nf = ne = 0
+    for b, m, d, side, ua, ub, us, jump in _interfaces(rows, N, tl):
+        rb, rm = rows[b], rows[m]
+        l = int(rb[0])
+        ta, tb = [k for k in range(3) if k != d]
+        ob, om = [int(x) for x in rb[1:4]], [int(x) for x in rm[1:4]]
+        Fg = ob[d] + (int(rb[4 + d]) if side else 0)
+        kb = Fg - ob[d] + NG
+        ib = [None] * 3
+        ib[d] = kb
+        ib[ta] = slice(ua - ob[ta] + NG, ua - ob[ta] + NG + us)
+        ib[tb] = slice(ub - ob[tb] + NG, ub - ob[tb] + NG + us)
+        fb = Fint[b][d][(slice(None),) + tuple(ib)]
+        c = 2 if jump else 1
+        km = _face_index(rm, d, c * Fg, N[l + (1 if jump else 0)][d])
+        im = [None] * 3
+        im[d] = km
+        im[ta] = slice(c * ua - om[ta] + NG, c * ua - om[ta] + NG + us * c)
+        im[tb] = slice(c * ub - om[tb] + NG, c * ub - om[tb] + NG + us * c)
+        fm = Fint[m][d][(slice(None),) + tuple(im)]
+        if jump:
+            fm = 0.25 * (fm[:, ::2, ::2] + fm[:, ::2, 1::2] + fm[:, 1::2, ::2] + fm[:, 1::2, 1::2])
+        wf = max(wf, float(np.abs(fb - fm).max()))
+        sf = max(sf, float(np.abs(fb).max()))
+        nf += 1
+        # in-plane edges strictly inside the unit (no third block around them)
+        for t in (ta, tb):
+            q = tb if t == ta else ta
+            u_t, u_q = (ua, ub) if t == ta else (ub, ua)
+            eb = [None] * 3
+            eb[d] = kb
+            eb[t] = slice(u_t - ob[t] + NG, u_t - ob[t] + NG + us)
+            eb[q] = slice(u_q - ob[q] + NG + 1, u_q - ob[q] + NG + us)
+            vb = Eint[b][t][tuple(eb)]
+            em = [None] * 3
+            em[d] = km
+            em[t] = slice(c * u_t - om[t] + NG, c * u_t - om[t] + NG + us * c)
+            em[q] = slice(c * u_q - om[q] + NG + c, c * u_q - om[q] + NG + us * c, c)
+            vm = Eint[m][t][tuple(em)]
+            if jump:
+                ax = 0 if t < q else 1
+                a0 = [slice(None)] * 2
+                a1 = [slice(None)] * 2
+                a0[ax], a1[ax] = slice(0, None, 2), slice(1, None, 2)
+                vm = 0.5 * (vm[tuple(a0)] + vm[tuple(a1)])
+            wE = max(wE, float(np.abs(vb - vm).max()))
+            sE = max(sE, float(np.abs(vb).max()))
+            ne += 1
+    return wf / sf, wE / sE, nf, ne
+
+
+@pytest.mark.parametrize("plant", [0, 1, 2])
+def test_mixed_time_levels_reconcile_fluxes_and_emfs(oracle, las_set, plant):
+    """At every corrected interface (a level jump, or same-level neighbours of
+    different time levels) both sides see the same time-integrated flux and
+    edge EMF over the cycle: the conservation of the corrected faces and the
+    agreement of the shared face fields.  Planted out of the oracle, the flux
+    corrections (1) or the same-level EMF corrections (2) leave them apart at
+    order one."""
+    desc, g, rows, p0, ps0, f0, dx0 = las_set
+    tl = GT.las_levels(rows)
+    dt = 0.1 * min(dx0)
+    N = _levels(rows)
+    p, ps, fl = _copy(p0), _copy(ps0), _copy(f0)
+    flag = ctypes.c_int.in_dll(oracle.lib, "oracle_las_plant")
+    flag.value = plant
+    try:
+        ndt, bdt, F, E = g.las_cycle(tl, 4, p, ps, fl, dt, fint=True)
+    finally:
+        flag.value = 0
+    rf, rE, nf, ne = _check_interfaces(rows, N, tl, F, E)
+    print("plant %d: blocks per time level %s; %d corrected face units: flux mismatch %.2e, edge EMF mismatch %.2e"
+          % (plant, np.bincount(tl).tolist(), nf, rf, rE))
+    assert nf > 0 and ndt > 0
+    if plant == 1:
+        assert rf > 1e-6, rf
+    else:
+        assert rf < 1e-13, rf
+    if plant == 2:
+        assert rE > 1e-6, rE
+    else:
+        assert rE < 1e-13, rE
+
+
+def test_time_level_rule_matches_its_restatement():
+    rng = np.random.default_rng(3)
+    for trial in range(40):
+        n = int(rng.integers(1, 60))
+        dt = float(rng.uniform(0.5, 2.0))
+        bdt = dt * np.exp(rng.uniform(-0.5, 3.5, size=n))
+        maxtl = int(2 ** rng.integers(0, 5))
+        perm = rng.permutation(n)
+        rl = int(rng.integers(1, 4))
+        rings = [list(perm[k:k + rl]) for k in range(0, (n // 2) // rl * rl, rl)]
+        pairs = [tuple(int(x) for x in rng.choice(n, 2, replace=False)) for _ in range(int(rng.integers(0, n)))]
+        got = H.las_rule(bdt, dt, maxtl, rings, pairs)
+        want = GT.las_rule_py(bdt, dt, maxtl, rings, pairs)
+        assert np.array_equal(got, want), (trial, got, want)
+    # the threshold: 1.05 * 2^i exactly qualifies, just below does not
+    got = H.las_rule([1.05 * 4, np.nextafter(1.05 * 4, 0), 100.0, 0.5], 1.0, 8)
+    assert got.tolist() == [4, 2, 8, 1]
