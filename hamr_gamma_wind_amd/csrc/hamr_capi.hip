@@ -1743,6 +1743,11 @@ static void enqueue_las_cycle(hamr_ctx *c, int maxtl)
     KParams &P = c->P;
     hipStream_t st = c->s;
     const int mixed = c->prec;
+    // the half-step arrays start as p/ps: their ghosts next to blocks that do
+    // not step with a block are the values of the last exchange, which the
+    // reference's receive writes into both array pairs
+    launch_copy(c->p, c->ph, (int64_t)(NPR * c->cells * sizeof(double)), st);
+    launch_copy(c->ps, c->psh, (int64_t)(3 * c->cells * sizeof(double)), st);
     for (int s = 0; s < 2 * maxtl; s++) {
         P.las_s = s;
         P.las_want = LAS_IDLE;

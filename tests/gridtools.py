@@ -223,3 +223,23 @@ def las_rule_py(bdt, dt, maxtl, rings, pairs):
                 tl[f] = tl[c]
                 changed = True
     return np.array(tl, dtype=np.int32)
+
+
+def floored_relerr_blocks(got, want, rel_floor=1e-10):
+    """Per component, max over every block's cells of |a - b| / max(|b|,
+    rel_floor * max |b| of that component over all blocks): pointwise relative
+    for every value above rel_floor of its component's largest, and relative
+    to that floor below it (values that rounding-level terms build up from
+    zero, such as a field component the step creates, are compared at the
+    floor)."""
+    nc = got[0].shape[0]
+    big = np.zeros(nc)
+    for b in want:
+        big = np.maximum(big, np.abs(b).reshape(nc, -1).max(axis=1))
+    fl = np.where(big > 0, rel_floor * big, DENORMAL_FLOOR)
+    err = np.zeros(nc)
+    for a, b in zip(got, want):
+        d = np.abs(a - b).reshape(nc, -1)
+        s = np.maximum(np.abs(b).reshape(nc, -1), fl[:, None])
+        err = np.maximum(err, (d / s).max(axis=1))
+    return err
