@@ -20,7 +20,8 @@ of FP32 size).  On the 256x128x128 grid (8x4x4 blocks of 32^3): the FP64
 batched step and the reference-structured FP64 step (the C host's per-block
 call sequence with a host dt read-back per block), and the FP64 step with
 HLLC fluxes.  Also the multi-level torus (3 levels, 722 blocks of uneven
-sizes, both arithmetics), the adjoint of one stage on the flat 128^3 box, and
+sizes, both arithmetics, and its local adaptive time stepping: time levels
+from the block dts, cycles against single-level steps), the adjoint of one stage on the flat 128^3 box, and
 the restart records of the whole grid (pack and unpack, device to device).
 
 Launch shapes come from the autotuner (hamr_autotune) for each arithmetic.
@@ -217,9 +218,42 @@ def amr_bench(H, args):
             m = max(float(np.abs(b[k])[3:-3, 3:-3, 3:-3].max()) for b in res["fp64"])
             err[name + "_max"] = d / m if m > 0 else d
         out["error_vs_fp64"] = err
+        out["local_time_stepping"] = las_bench(H, ctx, p0, ps0, out)
         return out
     finally:
         ctx.close()
+
+
+def las_bench(H, ctx, p0, ps0, amr_out, maxtl=4, cycles=10):
+    """Local adaptive time stepping on the multi-level set: time levels from
+    the block dts of one step (set_timelevel's rule, maxtl 4), then cycles of
+    2 maxtl sub-iterations replayed from a captured graph; each cycle advances
+    every block by maxtl dt, which the single-level step needs maxtl steps
+    for."""
+    import numpy as np
+
+    zones = np.array([int(np.prod(r[4:7])) for r in ctx.layout], dtype=np.int64)
+    out = {"maxtl": maxtl, "cycles": cycles}
+    for name, mode in (("fp64", H.HAMR_PREC_FP64), ("mixed", H.HAMR_PREC_MIXED)):
+        ctx.set_precision(mode)
+        ctx.set(H.HAMR_P, p0)
+        ctx.set(H.HAMR_PS, ps0)
+        dt = ctx.step(1)
+        tl = ctx.timelevels_from_dt(maxtl)
+        dt = ctx.step_las(maxtl, 1, dt=dt, graph=True)  # capture + warm-up
+        t0 = time.perf_counter()
+        ctx.step_las(maxtl, cycles, graph=True)
+        el = time.perf_counter() - t0
+        ms_cycle = el / cycles * 1e3
+        updates = int((zones * (maxtl // tl)).sum())
+        out[name] = {"ms_per_cycle": ms_cycle,
+                     "blocks_per_time_level": {str(k): int((tl == k).sum()) for k in (1, 2, 4, 8, 16) if (tl == k).any()},
+                     "zone_updates_per_cycle": updates,
+                     "zone_updates_single_level": int(zones.sum()) * maxtl,
+                     "zone_updates_per_s": updates * cycles / el,
+                     "speedup_vs_single_level_steps": maxtl * amr_out[name]["ms_per_step"] / ms_cycle}
+    ctx.set_precision(H.HAMR_PREC_FP64)
+    return out
 
 
 def adjoint_bench(H, n=128, reps=5):

@@ -217,6 +217,7 @@ def load_library(path=None):
         "hamr_timelevels_from_dt": [ctxp, C.c_int, _ip],
         "hamr_las_rule": [C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _ip, _ip],
         "hamr_step_las": [ctxp, C.c_int, C.c_int, _dp],
+        "hamr_step_las_graph": [ctxp, C.c_int, C.c_int, _dp],
         "hamr_las_info": [ctxp, _llp],
     }
     for name, args in sig.items():
@@ -786,11 +787,13 @@ class AmrContext(Context):
         self._chk(self.L.hamr_timelevels_from_dt(self.h, maxtl, _ptr(v, _ip)), "hamr_timelevels_from_dt")
         return v
 
-    def step_las(self, maxtl, ncycles=1, dt=None):
+    def step_las(self, maxtl, ncycles=1, dt=None, graph=False):
         """ncycles cycles of 2 maxtl sub-iterations (each advances every block
-        by maxtl dt); returns the next dt."""
+        by maxtl dt), launched one by one or replayed from a captured graph;
+        returns the next dt."""
         v = C.c_double(dt if dt is not None else -1.0)
-        self._chk(self.L.hamr_step_las(self.h, maxtl, ncycles, C.byref(v)), "hamr_step_las")
+        f = self.L.hamr_step_las_graph if graph else self.L.hamr_step_las
+        self._chk(f(self.h, maxtl, ncycles, C.byref(v)), "hamr_step_las")
         return v.value
 
     def las_info(self):

@@ -88,6 +88,8 @@ struct hamr_ctx {
     AmrEmf *emfx = nullptr;        // EMF correction at level jumps
     int64_t nemfx = 0;
     hipGraphExec_t graph = nullptr;
+    hipGraphExec_t las_graph = nullptr;  // one time-level cycle (hamr_step_las_graph)
+    int las_graph_maxtl = 0;
     int prec = HAMR_PREC_FP64;     // arithmetic of reconstruction, fluxes and EMFs
     // adjoint scratch (allocated by the first adjoint call)
     double *aw = nullptr, *alP = nullptr, *alps = nullptr, *alpsf = nullptr, *alE = nullptr, *alEc = nullptr;
@@ -165,6 +167,16 @@ template <class T> static int dalloc(hamr_ctx *c, T **ptr, size_t n)
 }
 
 static int nblk_of(const hamr_ctx *c) { return (int)c->blocks.size(); }
+
+// Captured graphs are dropped when what they captured changes (arithmetic,
+// Riemann solver, launch shapes, time levels).
+static void drop_graphs(hamr_ctx *c)
+{
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->las_graph) (void)hipGraphExecDestroy(c->las_graph);
+    c->graph = nullptr;
+    c->las_graph = nullptr;
+}
 static bool valid_block(const hamr_ctx *c, int n) { return n >= 0 && n < nblk_of(c); }
 static int64_t ncell_of(const BlockDev &B) { return (int64_t)B.D[0] * B.D[1] * B.D[2]; }
 
@@ -650,8 +662,8 @@ int hamr_destroy(hamr_ctx *c)
 {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->s) (void)hipStreamSynchronize(c->s);
+    drop_graphs(c);
     for (void *v : c->allocs) (void)hipFree(v);
     for (void *v : c->las_allocs) (void)hipFree(v);
     if (c->stage) (void)hipHostFree(c->stage);
@@ -998,10 +1010,7 @@ int hamr_set_riemann(hamr_ctx *c, int solver)
     if (solver == HAMR_RIEMANN_HLLC && c->prec != HAMR_PREC_FP64)
         return fail(c, "hamr_set_riemann: HLLC runs in the FP64 step only");
     HIPCHK(c, hipStreamSynchronize(c->s));
-    if (solver != c->P.riemann && c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    if (solver != c->P.riemann) drop_graphs(c);
     c->P.riemann = solver;
     return 0;
 }
@@ -1171,10 +1180,7 @@ int hamr_set_precision(hamr_ctx *c, int mode)
     if (mode == HAMR_PREC_MIXED && c->P.riemann != HAMR_RIEMANN_HLL)
         return fail(c, "hamr_set_precision: the mixed step uses HLL (HLLC runs in the FP64 step only)");
     HIPCHK(c, hipStreamSynchronize(c->s));
-    if (mode != c->prec && c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    if (mode != c->prec) drop_graphs(c);
     c->prec = mode;
     return 0;
 }
@@ -1638,6 +1644,7 @@ static int set_timelevels(hamr_ctx *c, const std::vector<int> &tl)
     const std::string e = build_las_tables(c->grid, tl, t);
     if (!e.empty()) return fail(c, "hamr_set_timelevels: " + e);
     las_release(c);
+    drop_graphs(c);
     const int nb = nblk_of(c);
     c->tl = tl;
     c->tlmax = 1;
@@ -1857,6 +1864,39 @@ int hamr_timelevels_from_dt(hamr_ctx *c, int maxtl, int *tl_out)
     return 0;
 }
 
+int hamr_step_las_graph(hamr_ctx *c, int maxtl, int ncycles, double *dt_inout)
+{
+    if (!c) return -1;
+    if (!c->amr || c->tl.empty())
+        return fail(c, "hamr_step_las_graph: set the time levels of a multi-level set first (hamr_set_timelevels)");
+    if (maxtl < c->tlmax || maxtl > kMaxTimeLevel || (maxtl & (maxtl - 1)))
+        return fail(c, "hamr_step_las_graph: maxtl is a power of 2 from the largest time level to 16");
+    if (ncycles < 0) return fail(c, "hamr_step_las_graph: ncycles < 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (set_dt(c, dt_inout)) return -1;
+    launch_dt_reset(nblk_of(c), c->dtmin, c->s);
+    if (c->las_graph && c->las_graph_maxtl != maxtl) {
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        (void)hipGraphExecDestroy(c->las_graph);
+        c->las_graph = nullptr;
+    }
+    if (!c->las_graph) {
+        hipGraph_t g;
+        HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        enqueue_las_cycle(c, maxtl);
+        HIPCHK(c, hipStreamEndCapture(c->s, &g));
+        HIPCHK(c, hipGraphInstantiate(&c->las_graph, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        c->las_graph_maxtl = maxtl;
+    }
+    for (int k = 0; k < ncycles; k++) HIPCHK(c, hipGraphLaunch(c->las_graph, c->s));
+    if (dt_inout) {
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (d2h(c, dt_inout, c->dt, sizeof(double))) return -1;
+    }
+    return 0;
+}
+
 int hamr_las_info(const hamr_ctx *c, long long *info)
 {
     if (!c || !info) return -1;
@@ -2011,10 +2051,7 @@ int hamr_autotune(hamr_ctx *c, int nrep)
         c->tune[slot] = best;
     }
     if (restore_flags(c, keepf, "autotune")) return -1;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    drop_graphs(c);
     return 0;
 }
 
@@ -2066,11 +2103,8 @@ int hamr_tune_set(hamr_ctx *c, int slot, const int *cfg)
     c->tune[slot].alt = march ? 0 : (cfg[2] >> 1) & 1;
     c->tune[slot].march = march ? 1 : 0;
     c->tune[slot].ms = -1.f;
-    if (c->graph) {
-        HIPCHK(c, hipStreamSynchronize(c->s));
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    drop_graphs(c);
     return 0;
 }
 

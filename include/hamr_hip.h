@@ -364,6 +364,10 @@ int hamr_las_rule(int nblk, const double *bdt, double dt, int maxtl, int nring, 
  * on the device as in hamr_step and is updated at each cycle's end from the
  * Courant minima of its last sub-iteration. */
 int hamr_step_las(hamr_ctx *ctx, int maxtl, int ncycles, double *dt_inout);
+/* Capture one cycle into a HIP graph (again when maxtl, the time levels, the
+ * arithmetic or the launch shapes change) and replay it ncycles times (same
+ * result as hamr_step_las). */
+int hamr_step_las_graph(hamr_ctx *ctx, int maxtl, int ncycles, double *dt_inout);
 /* info[0..2]: flux-correction entries per direction, [3]: same-level EMF
  * entries, [4]: level-jump EMF entries, [5]: the largest time level (0 until set). */
 int hamr_las_info(const hamr_ctx *ctx, long long *info);

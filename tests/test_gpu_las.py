@@ -27,6 +27,7 @@ Bars:
   and u relative to their largest magnitude, velocity and field to the
   largest component of their vector) within ten times that of four standard
   steps, rho within 1e-6 and the field within 1e-4;
+- three replays of the captured cycle equal three launched cycles bit for bit;
 - the time levels from the block dts: powers of 2 up to maxtl, none above
   set_timelevel's rule for its block, equal over each polar ring, accepted by
   hamr_set_timelevels, and a cycle with them runs;
@@ -109,6 +110,28 @@ def test_unit_and_uniform_time_levels_equal_standard_steps(las):
     ctx.step_las(2, 1, dt=dt)
     got = (ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS))
     assert _same(got[0], want[0]) and _same(got[1], want[1])
+
+
+def test_graph_replay_equals_launches(las):
+    """Three cycles launched one by one against three replays of the captured
+    cycle, each replay checked against the launched state after the same
+    number of cycles, with state reads in between."""
+    ctx, g, rows, start, dt = las
+    ctx.set_precision(H.HAMR_PREC_FP64)
+    ctx.set_timelevels(GT.las_levels(rows))
+    _reset(ctx, start)
+    want = []
+    d = dt
+    for k in range(3):
+        d = ctx.step_las(4, 1, dt=d)
+        want.append((ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS), d))
+    _reset(ctx, start)
+    d = dt
+    for k in range(3):
+        d = ctx.step_las(4, 1, dt=d, graph=True)
+        got = (ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS))
+        print("replay %d: dt %.17g launched %.17g" % (k, d, want[k][2]), flush=True)
+        assert d == want[k][2] and _same(got[0], want[k][0]) and _same(got[1], want[k][1]), k
 
 
 def test_mixed_time_levels_match_the_oracle(las):
