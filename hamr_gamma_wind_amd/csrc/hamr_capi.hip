@@ -117,6 +117,13 @@ struct hamr_ctx {
     LasCorr *las_f[3] = {nullptr, nullptr, nullptr}, *las_es = nullptr, *las_ej = nullptr;
     int64_t nlas_f[3] = {0, 0, 0}, nlas_es = 0, nlas_ej = 0;
     double *acc_f[3] = {nullptr, nullptr, nullptr}, *acc_es = nullptr, *acc_ej = nullptr;
+    // the exchange entries between blocks that both step, per set of stepping time levels
+    struct LasExchange {
+        AmrEntry *ex[3] = {nullptr, nullptr, nullptr};
+        AmrProlong *pro[3] = {nullptr, nullptr, nullptr};
+        int64_t nex[3] = {0, 0, 0}, npro[3] = {0, 0, 0};
+    };
+    std::map<int, LasExchange> las_ex;  // key: bit log2(tl) set for every stepping level
 };
 
 // ---- the device scratch of hamr_stage_adjoint_dev_noalloc ----
@@ -1624,6 +1631,16 @@ static void las_release(hamr_ctx *c)
     c->las_es = c->las_ej = nullptr;
     c->acc_es = c->acc_ej = nullptr;
     c->nlas_es = c->nlas_ej = 0;
+    c->las_ex.clear();
+}
+
+// Bit log2(tl) of every time level that steps (predictor or corrector) in sub-iteration s.
+static int stepping_levels(int s)
+{
+    int m = 0;
+    for (int i = 0; (1 << i) <= kMaxTimeLevel; i++)
+        if (las_phase(1 << i, s) != LAS_IDLE) m |= 1 << i;
+    return m;
 }
 
 // the kernel classes a cycle launches over filtered lists
@@ -1685,6 +1702,41 @@ static int set_timelevels(hamr_ctx *c, const std::vector<int> &tl)
         if (las_alloc(c, &c->las_items, items.size())) return -1;
         if (h2d(c, c->las_items, items.data(), items.size() * sizeof(int2))) return -1;
     }
+    // exchange entries per set of stepping levels (a sub-iteration exchanges
+    // only between blocks that both step)
+    {
+        std::vector<AmrEntry> ex[3];
+        std::vector<AmrProlong> pro[3];
+        for (int k = 0; k < 3; k++) {
+            ex[k].resize((size_t)c->nex[k]);
+            pro[k].resize((size_t)c->npro[k]);
+            if (c->nex[k] && d2h(c, ex[k].data(), c->ex[k], ex[k].size() * sizeof(AmrEntry))) return -1;
+            if (c->npro[k] && d2h(c, pro[k].data(), c->pro[k], pro[k].size() * sizeof(AmrProlong))) return -1;
+        }
+        for (int sub = 0; sub < nsub; sub++) {
+            const int m = stepping_levels(sub);
+            if (c->las_ex.count(m)) continue;
+            auto on = [&](int b) { return (m >> log2i(tl[b])) & 1; };
+            hamr_ctx::LasExchange X;
+            for (int k = 0; k < 3; k++) {
+                std::vector<AmrEntry> fe;
+                std::vector<AmrProlong> fp;
+                for (const AmrEntry &e : ex[k])
+                    if (on(e.dblk) && on(e.sblk)) fe.push_back(e);
+                for (const AmrProlong &e : pro[k])
+                    if (on(e.dblk) && on(e.cblk)) fp.push_back(e);
+                X.nex[k] = (int64_t)fe.size();
+                X.npro[k] = (int64_t)fp.size();
+                if (!fe.empty() && (las_alloc(c, &X.ex[k], fe.size()) ||
+                                    h2d(c, X.ex[k], fe.data(), fe.size() * sizeof(AmrEntry))))
+                    return -1;
+                if (!fp.empty() && (las_alloc(c, &X.pro[k], fp.size()) ||
+                                    h2d(c, X.pro[k], fp.data(), fp.size() * sizeof(AmrProlong))))
+                    return -1;
+            }
+            c->las_ex[m] = X;
+        }
+    }
     auto table = [&](const std::vector<LasCorr> &v, LasCorr **dst, int64_t *n, double **acc, int ncomp) {
         *n = (int64_t)v.size();
         if (v.empty()) return 0;
@@ -1728,6 +1780,7 @@ static void las_bound(hamr_ctx *c, int s)
 {
     const KParams &P = c->P;
     const int T = kTiles[c->tune[TS_EXCHANGE].tile], G = c->tune[TS_EXCHANGE].grid;
+    const hamr_ctx::LasExchange &X = c->las_ex.at(stepping_levels(s % (2 * c->tlmax)));
     auto phys = [&](int cls, int dir, int final_pass) {
         for (int ph = LAS_HALF; ph <= LAS_FULL; ph++) {
             const Launch L = las_launch(c, cls, s, ph);
@@ -1735,13 +1788,10 @@ static void las_bound(hamr_ctx *c, int s)
         }
     };
     phys(KC_BOUND1, 1, 0);
-    launch_amr_exchange(P, 1, c->ex[0], c->nex[0], c->pro[0], c->npro[0], c->p, c->ps, c->pflag, T, G, c->s, c->ph,
-                        c->psh);
+    launch_amr_exchange(P, 1, X.ex[0], X.nex[0], X.pro[0], X.npro[0], c->p, c->ps, c->pflag, T, G, c->s, c->ph, c->psh);
     phys(KC_BOUND2, 2, 0);
-    launch_amr_exchange(P, 2, c->ex[1], c->nex[1], c->pro[1], c->npro[1], c->p, c->ps, c->pflag, T, G, c->s, c->ph,
-                        c->psh);
-    launch_amr_exchange(P, 3, c->ex[2], c->nex[2], c->pro[2], c->npro[2], c->p, c->ps, c->pflag, T, G, c->s, c->ph,
-                        c->psh);
+    launch_amr_exchange(P, 2, X.ex[1], X.nex[1], X.pro[1], X.npro[1], c->p, c->ps, c->pflag, T, G, c->s, c->ph, c->psh);
+    launch_amr_exchange(P, 3, X.ex[2], X.nex[2], X.pro[2], X.npro[2], c->p, c->ps, c->pflag, T, G, c->s, c->ph, c->psh);
     phys(KC_BOUND1, 1, 1);
 }
 
