@@ -145,6 +145,9 @@ static double slope_lim(double y1, double y2, double y3)
  * the normal face field of a block's own upper boundary face, which the
  * exchange rule forbids. */
 int oracle_amr_plant_own_face = 0;
+/* Test switch (planted-mistake tests only): 1 drops the corner-ghost rule of
+ * stages 2 and 3 (corner ghosts then come from the leaf holding the corner). */
+int oracle_amr_plant_corner = 0;
 
 /* write the ghost values; the component `skip` (0..2, or -1) of the face field is left as it is */
 /* Local-adaptive-time-stepping exchange (NULL members: the plain exchange):
@@ -268,7 +271,7 @@ static void exchange_cell(const oracle_amr *g, int dd, int b, int i, int j, int 
                 corner = 1;
                 Yk[e] = Y[e] < B->o[e] ? B->o[e] : (Y[e] >= B->o[e] + B->bs[e] ? B->o[e] + B->bs[e] - 1 : Y[e]);
             }
-        if (corner) {
+        if (corner && !oracle_amr_plant_corner) {
             int lk, k = find_leaf(g, l, Yk, &lk);
             if (k >= 0 && lk == l) {
                 if (lx && lx->act && !lx->act[k]) return;

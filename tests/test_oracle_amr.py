@@ -12,7 +12,8 @@
   MC limiter on linear data and the face rule (odd fine faces from the coarse
   cell-centred field).
 - The block-set builder (hamr_desc_amr) gives the multi-level torus workload
-  its stated structure.
+  its stated structure, and its exchange tables let each stage's entries run
+  concurrently (no entry reads a cell its stage writes).
 """
 import numpy as np
 import pytest
@@ -518,3 +519,37 @@ def test_level_jumps_keep_div_b_at_roundoff(oracle):
     print("div B relative to the block scale: %.2e (planted overwrite: %.2e)" % (worst[0], worst[1]))
     assert worst[0] < 1e-12
     assert worst[1] > 1e-3
+
+
+@pytest.mark.parametrize("boxes", ["x1x2", "x1x2x3"])
+def test_exchange_entries_never_read_a_cell_their_stage_writes(boxes):
+    """The exchange kernels of one stage run their entries concurrently: no
+    entry may read (as a copied cell, one of the 2x2x2 children of a
+    restriction, a coarse cell or one of its six slope neighbours) a cell that
+    an entry of the same stage writes, and no ghost is written twice."""
+    desc = _desc(DESC3)
+    amr = H.make_amr_desc(BOXES3 if boxes == "x1x2" else BOXES3Z, seed=5)
+    info = H.desc_amr(desc, amr, tables=True)
+    rows = info["rows"]
+    D = np.array([[int(k) + 2 * NG for k in r[4:7]] for r in rows], dtype=np.int64)
+    for s in range(3):
+        ex, pro = info["ex"][s].astype(np.int64), info["pro"][s].astype(np.int64)
+        dst = np.concatenate([(ex[:, 0] << 32) | ex[:, 1], (pro[:, 0] << 32) | pro[:, 1]])
+        assert len(np.unique(dst)) == len(dst), s
+        kind, cell = ex[:, 3] >> 28, ex[:, 3] & 0x0FFFFFFF
+        srcs = [(ex[:, 2] << 32) | cell]
+        avg = kind == 2
+        b = ex[avg, 2]
+        sj, si = D[b, 2], D[b, 1] * D[b, 2]
+        c = cell[avg]
+        for off in (1, sj, sj + 1, si, si + 1, si + sj, si + sj + 1):
+            srcs.append((b << 32) | (c + off))
+        srcs.append((pro[:, 2] << 32) | pro[:, 3])
+        for k in range(6):
+            code = pro[:, 5 + 2 * k]
+            has = (code >> 28) != 0
+            srcs.append((pro[has, 4 + 2 * k] << 32) | (code[has] & 0x0FFFFFFF))
+        src = np.concatenate(srcs)
+        bad = np.isin(src, dst)
+        print("stage %d: %d entries, %d source reads, %d of written cells" % (s + 1, len(dst), len(src), bad.sum()))
+        assert not bad.any(), s

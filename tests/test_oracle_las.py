@@ -27,7 +27,10 @@ Pins:
   face from its own ghosts, and ghosts copied from a third block at different
   sub-iterations of a cycle can differ, as in the reference's exchange, which
   skips blocks that do not step);
-- the time-level rule against its restatement on random inputs.
+- the time-level rule against its restatement on random inputs;
+- the multi-level exchange's corner rule: after one standard step every face
+  two same-level blocks share has the same value in both; with the rule
+  planted out of the oracle some part.
 """
 import ctypes
 
@@ -250,3 +253,58 @@ def test_time_level_rule_matches_its_restatement():
     # the threshold: 1.05 * 2^i exactly qualifies, just below does not
     got = H.las_rule([1.05 * 4, np.nextafter(1.05 * 4, 0), 100.0, 0.5], 1.0, 8)
     assert got.tolist() == [4, 2, 8, 1]
+
+
+@pytest.mark.parametrize("plant", [0, 1])
+def test_same_level_shared_faces_after_a_step(oracle, las_set, plant):
+    """After one standard step every face that two same-level blocks share has
+    the same value in both (each evolves its own copy with its own edge EMFs):
+    corner ghosts come from the face neighbour's storage, so both blocks see
+    the same data around the shared edges (differences could remain only where
+    the position a corner ghost is clamped to lies on another level; bar 1e-7
+    of the face value).  With the corner rule planted out of the oracle the
+    corner ghosts come from the leaf holding the corner and shared faces part."""
+    desc, g, rows, p0, ps0, f0, dx0 = las_set
+    p, ps, fl = _copy(p0), _copy(ps0), _copy(f0)
+    flag = ctypes.c_int.in_dll(oracle.lib, "oracle_amr_plant_corner")
+    flag.value = plant
+    try:
+        g.step(p, ps, fl, 0.1 * min(dx0))
+    finally:
+        flag.value = 0
+    N = _levels(rows)
+    leaf = _leaf_finder(rows, N)
+    worst, count, differ = 0.0, 0, 0
+    for a, r in enumerate(rows):
+        l, o, bs = int(r[0]), [int(x) for x in r[1:4]], [int(x) for x in r[4:7]]
+        for d in range(3):
+            ta, tb = [k for k in range(3) if k != d]
+            Y = [0, 0, 0]
+            Y[d] = o[d] + bs[d]
+            if d < 2 and Y[d] >= N[l][d]:
+                continue
+            Y[d] %= N[l][d]
+            for ua in range(o[ta], o[ta] + bs[ta], 16):
+                for ub in range(o[tb], o[tb] + bs[tb], 16):
+                    Y[ta], Y[tb] = ua, ub
+                    m, lm = leaf(l, Y)
+                    if lm != l or m == a:
+                        continue
+                    mo = [int(x) for x in rows[m][1:4]]
+                    ia, ib = [None] * 3, [None] * 3
+                    ia[d], ib[d] = NG + bs[d], NG
+                    for t, u in ((ta, ua), (tb, ub)):
+                        ia[t] = slice(u - o[t] + NG, u - o[t] + NG + 16)
+                        ib[t] = slice(u - mo[t] + NG, u - mo[t] + NG + 16)
+                    va, vb = ps[a][d][tuple(ia)], ps[m][d][tuple(ib)]
+                    rel = np.abs(va - vb) / np.maximum(np.abs(vb), 1e-300)
+                    worst = max(worst, float(rel.max()))
+                    count += rel.size
+                    differ += int((va != vb).sum())
+    print("plant %d: same-level shared faces: %d, differing %d, worst relative difference %.2e"
+          % (plant, count, differ, worst))
+    assert count > 0
+    if plant:
+        assert worst > 1e-6, worst
+    else:
+        assert worst < 1e-7, worst
