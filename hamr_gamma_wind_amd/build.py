@@ -61,7 +61,8 @@ def build(verbose=False):
                 print(out)
     objs = [obj for _, obj in jobs]
     tmp = OUT + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, OUT)
     tmp = HOST_OUT + ".tmp"
     _run(["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-o", tmp, os.path.join(CSRC, "hamr_host.c"), "-L" + HERE,

@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/hamr_hip.h"
 #include "hamr_amr.h"
 #include "hamr_grid.h"
@@ -174,6 +176,16 @@ template <class T> static int dalloc(hamr_ctx *c, T **ptr, size_t n)
 }
 
 static int nblk_of(const hamr_ctx *c) { return (int)c->blocks.size(); }
+
+// A named roctx range around the host-side enqueue of a stage (rocprofv3
+// --marker-trace shows the ranges; the kernels they launch carry the same
+// correlation).  Inside a graph capture the ranges mark the capture.
+struct StageRange {
+    explicit StageRange(const char *name) { roctxRangePushA(name); }
+    ~StageRange() { roctxRangePop(); }
+    StageRange(const StageRange &) = delete;
+    StageRange &operator=(const StageRange &) = delete;
+};
 
 // Captured graphs are dropped when what they captured changes (arithmetic,
 // Riemann solver, launch shapes, time levels).
@@ -1502,25 +1514,51 @@ static void enqueue_step(hamr_ctx *c)
     const KParams &P = c->P;
     const int nb = nblk_of(c);
     hipStream_t s = c->s;
+    StageRange step("hamr_step");
     // predictor: p/ps -> ph/psh with Dt = dt/2
-    for (int dir = 1; dir <= 3; dir++)
-        launch_flux(P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->p, c->ps, c->F[dir - 1], nullptr, c->prec, s);
-    emf_all(c, c->p);
-    pole(c, 0., c->dt);
-    launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, s);
-    update_all(c, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->psh);
-    bound_all(c, c->ph, c->psh, true);
+    {
+        StageRange r("predictor: fluxes");
+        for (int dir = 1; dir <= 3; dir++)
+            launch_flux(P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->p, c->ps, c->F[dir - 1], nullptr, c->prec, s);
+    }
+    {
+        StageRange r("predictor: EMFs and face field");
+        emf_all(c, c->p);
+        pole(c, 0., c->dt);
+        launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, s);
+    }
+    {
+        StageRange r("predictor: update");
+        update_all(c, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->psh);
+    }
+    {
+        StageRange r("predictor: ghost fill");
+        bound_all(c, c->ph, c->psh, true);
+    }
     // corrector: U0 -> p/ps with Dt = dt, fluxes from ph/psh, Courant minima recorded
-    for (int dir = 1; dir <= 3; dir++)
-        launch_flux(P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->ph, c->psh, c->F[dir - 1], c->dtmin, c->prec, s);
-    reflux_all(c, 0., c->dt);
-    emf_all(c, c->ph);
-    pole(c, 0., c->dt);
-    emf_correct_all(c, 0., c->dt);
-    launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 1.0, c->ps, c->ps, c->E, c->prec, s);
-    update_all(c, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->ps);
-    bound_all(c, c->p, c->ps, true);
-    launch_dt_finalize(nb, c->dtmin, c->dt, s, c->bdt);
+    {
+        StageRange r("corrector: fluxes and corrections");
+        for (int dir = 1; dir <= 3; dir++)
+            launch_flux(P, dir, all_launch(c, TS_FLUX1 + dir - 1), c->ph, c->psh, c->F[dir - 1], c->dtmin, c->prec,
+                        s);
+        reflux_all(c, 0., c->dt);
+    }
+    {
+        StageRange r("corrector: EMFs and face field");
+        emf_all(c, c->ph);
+        pole(c, 0., c->dt);
+        emf_correct_all(c, 0., c->dt);
+        launch_ct_update(P, all_launch(c, TS_CT), 0., c->dt, 1.0, c->ps, c->ps, c->E, c->prec, s);
+    }
+    {
+        StageRange r("corrector: update");
+        update_all(c, 1, 0., c->dt, 1.0, c->ph, c->ph, c->p, c->ps);
+    }
+    {
+        StageRange r("corrector: ghost fill and dt");
+        bound_all(c, c->p, c->ps, true);
+        launch_dt_finalize(nb, c->dtmin, c->dt, s, c->bdt);
+    }
 }
 
 int hamr_estimate_dt(hamr_ctx *c, double *dt)
@@ -1803,9 +1841,11 @@ static void enqueue_las_cycle(hamr_ctx *c, int maxtl)
     // the half-step arrays start as p/ps: their ghosts next to blocks that do
     // not step with a block are the values of the last exchange, which the
     // reference's receive writes into both array pairs
+    StageRange cycle("hamr_step_las: cycle");
     launch_copy(c->p, c->ph, (int64_t)(NPR * c->cells * sizeof(double)), st);
     launch_copy(c->ps, c->psh, (int64_t)(3 * c->cells * sizeof(double)), st);
     for (int s = 0; s < 2 * maxtl; s++) {
+        StageRange sub("hamr_step_las: sub-iteration");
         P.las_s = s;
         P.las_want = LAS_IDLE;
         const bool last = s == 2 * maxtl - 1;
