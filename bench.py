@@ -200,6 +200,8 @@ def amr_bench(H, args):
                            % info["blocks"],
                "blocks": info["blocks"], "zones": info["zones"],
                "exchange_entries": sum(info[k] for k in ("ex1", "ex2", "ex3", "pro1", "pro2", "pro3")),
+               "exchange_entries_after_update": sum(info[k] for k in ("ex1", "ex2", "ex3", "pro1", "pro2", "pro3"))
+               - info["push3"],
                "steps": steps}
         res = {}
         for name, mode in (("fp64", H.HAMR_PREC_FP64), ("mixed", H.HAMR_PREC_MIXED)):

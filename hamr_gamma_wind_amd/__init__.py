@@ -309,7 +309,7 @@ def desc_torus(desc, tdesc):
 
 
 AMR_INFO_KEYS = ("blocks", "geometries", "cells", "zones", "ex1", "ex2", "ex3", "pro1", "pro2", "pro3", "rings", "amr",
-                 "reflux1", "reflux2", "reflux3", "emf")
+                 "reflux1", "reflux2", "reflux3", "emf", "push3")
 
 
 def desc_amr(desc, amr, tables=False):

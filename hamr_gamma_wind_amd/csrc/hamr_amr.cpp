@@ -331,6 +331,35 @@ std::string build_amr(const GridDesc &d, const AmrDesc &a, AmrGrid &g)
             }
         }
     }
+    // push table: for each x3 face and each 16x16 tile of the block's columns,
+    // the same-level leaf across the face that holds the tile (its ghost rows
+    // there then copy this block's cells; the update writes them)
+    g.push3tab.clear();
+    for (int n = 0; n < nblk; n++) {
+        BlockDev &B = b.blk[n];
+        const int l = B.level;
+        const Level &L = b.lev[l];
+        const int t1n = (B.bs[0] + kPushTile - 1) / kPushTile, t2n = (B.bs[1] + kPushTile - 1) / kPushTile;
+        for (int side = 0; side < 2; side++) {
+            B.push3[side] = (int)g.push3tab.size();
+            const int z = ((B.o[2] + (side ? B.bs[2] : -1)) % L.N[2] + L.N[2]) % L.N[2];
+            bool any = false;
+            for (int t1 = 0; t1 < t1n; t1++)
+                for (int t2 = 0; t2 < t2n; t2++) {
+                    const int Y0[3] = {B.o[0] + t1 * kPushTile, B.o[1] + t2 * kPushTile, z};
+                    const int Y1[3] = {std::min(Y0[0] + kPushTile, B.o[0] + B.bs[0]) - 1,
+                                       std::min(Y0[1] + kPushTile, B.o[1] + B.bs[1]) - 1, z};
+                    int l0, l1, m = b.find(l, Y0, &l0), m1 = b.find(l, Y1, &l1);
+                    if (m < 0 || l0 != l || m1 != m || l1 != l || B.bs[2] < NG) m = -1;
+                    any = any || m >= 0;
+                    g.push3tab.push_back(m);
+                }
+            if (!any) {
+                g.push3tab.resize(B.push3[side]);
+                B.push3[side] = -1;
+            }
+        }
+    }
     // pole rings: same side, level, radial range and size, in phi order
     g.ring.clear();
     g.rside.clear();
@@ -475,6 +504,45 @@ std::string build_amr(const GridDesc &d, const AmrDesc &a, AmrGrid &g)
             std::sort(g.pro[d0].begin() + pro0, g.pro[d0].end(),
                       [](const AmrProlong &x, const AmrProlong &y) { return x.dcell < y.dcell; });
         }
+    }
+    // stage 3: the ghost rows that the update writes (from the push table)
+    // go last, so that a fill right after the update launches only the entries
+    // before them; each must copy the cell that the update writes there
+    {
+        // the cell of block s that the push table sends to ghost cell dc of
+        // block r, or -1
+        auto pushed_from = [&](int r, int dc, int s) {
+            const BlockDev &R = b.blk[r], &S = b.blk[s];
+            const int i = dc / (R.D[1] * R.D[2]), j = (dc / R.D[2]) % R.D[1], z = dc % R.D[2];
+            if (i < NG || i >= NG + R.bs[0] || j < NG || j >= NG + R.bs[1] || (z >= NG && z < NG + R.bs[2]))
+                return -1;
+            const int side = z < NG ? 1 : 0;  // the sender's face: its upper one for the receiver's lower ghosts
+            if (S.level != R.level || S.push3[side] < 0) return -1;
+            const int is = i + R.o[0] - S.o[0], js = j + R.o[1] - S.o[1];
+            if (is < NG || is >= NG + S.bs[0] || js < NG || js >= NG + S.bs[1]) return -1;
+            const int t2n = (S.bs[1] + kPushTile - 1) / kPushTile;
+            if (g.push3tab[S.push3[side] + ((is - NG) / kPushTile) * t2n + (js - NG) / kPushTile] != r) return -1;
+            const int zs = side ? z + S.bs[2] : z - R.bs[2];
+            return (is * S.D[1] + js) * S.D[2] + zs;
+        };
+        auto pushed = [&](const AmrEntry &e) { return pushed_from(e.dblk, e.dcell, e.sblk) >= 0; };
+        auto mid = std::stable_partition(g.ex[2].begin(), g.ex[2].end(), [&](const AmrEntry &e) { return !pushed(e); });
+        g.npush3 = (int64_t)(g.ex[2].end() - mid);
+        int64_t sent = 0;
+        for (int n = 0; n < nblk; n++)
+            for (int side = 0; side < 2; side++) {
+                const BlockDev &B = b.blk[n];
+                if (B.push3[side] < 0) continue;
+                const int t1n = (B.bs[0] + kPushTile - 1) / kPushTile, t2n = (B.bs[1] + kPushTile - 1) / kPushTile;
+                for (int t = 0; t < t1n * t2n; t++)
+                    if (g.push3tab[B.push3[side] + t] >= 0)
+                        sent += (int64_t)NG * (std::min(kPushTile, B.bs[0] - (t / t2n) * kPushTile)) *
+                                std::min(kPushTile, B.bs[1] - (t % t2n) * kPushTile);
+            }
+        if (sent != g.npush3) return "amr: the push table writes ghosts that the exchange fills otherwise";
+        for (auto it = mid; it != g.ex[2].end(); ++it)
+            if (it->scode != ((AS_CELL << 28) | pushed_from(it->dblk, it->dcell, it->sblk)))
+                return "amr: an x3 ghost row of the push table has another source";
     }
     // flux correction at level jumps (flux_rec1-3 with calc_corr = 1,
     // boundcomF.c:583-1315): every interior face cell of a block face whose
@@ -834,7 +902,24 @@ std::string validate_amr(const AmrGrid &g)
         const BlockDev &B = blk[n];
         for (int f = 0; f < 6; f++)
             if (B.kind[f] == FK_POLE && (B.nbr[f] < 0 || B.nbr[f] >= (int)blk.size())) return "amr: bad pole partner";
+        // every receiver holds the tile's ghost rows (the kernels index its slab with them)
+        for (int side = 0; side < 2; side++) {
+            if (B.push3[side] < 0) continue;
+            const int t1n = (B.bs[0] + kPushTile - 1) / kPushTile, t2n = (B.bs[1] + kPushTile - 1) / kPushTile;
+            if (B.bs[2] < 2 * NG || B.push3[side] + t1n * t2n > (int)g.push3tab.size()) return "amr: bad push table";
+            for (int t = 0; t < t1n * t2n; t++) {
+                const int m = g.push3tab[B.push3[side] + t];
+                if (m < 0) continue;
+                if (m >= (int)blk.size()) return "amr: bad push table";
+                const BlockDev &M = blk[m];
+                const int lo[2] = {B.o[0] + (t / t2n) * kPushTile, B.o[1] + (t % t2n) * kPushTile};
+                const int hi[2] = {std::min(lo[0] + kPushTile, B.o[0] + B.bs[0]), std::min(lo[1] + kPushTile, B.o[1] + B.bs[1])};
+                for (int k = 0; k < 2; k++)
+                    if (lo[k] < M.o[k] || hi[k] > M.o[k] + M.bs[k]) return "amr: bad push table";
+            }
+        }
     }
+    if (g.npush3 < 0 || g.npush3 > (int64_t)g.ex[2].size()) return "amr: bad push3 count";
     for (int r : g.ring)
         if (r < 0 || r >= (int)blk.size()) return "amr: bad ring entry";
     return "";

@@ -32,6 +32,8 @@ struct AmrGrid {
     int64_t cells = 0;              // sum of the blocks' padded slab lengths
     int64_t cols = 0;               // sum of the geometry slabs' padded column counts
     std::vector<AmrEntry> ex[3];    // per stage: copies and restrictions
+    int64_t npush3 = 0;             // the last npush3 entries of stage 3: ghost rows that the update writes
+    std::vector<int> push3tab;      // push table (BlockDev::push3 rows)
     std::vector<AmrProlong> pro[3]; // per stage: prolongations
     std::vector<AmrReflux> rfx[3];  // per direction: coarse faces at level jumps
     std::vector<AmrEmf> emf;        // coarse edges at level jumps

@@ -85,6 +85,9 @@ struct hamr_ctx {
     AmrEntry *ex[3] = {nullptr, nullptr, nullptr};
     AmrProlong *pro[3] = {nullptr, nullptr, nullptr};
     int64_t nex[3] = {0, 0, 0}, npro[3] = {0, 0, 0};
+    int64_t npush3 = 0;            // stage 3's last entries: the rows that the update writes
+    std::vector<int> push3tab;     // push table (host copy)
+    int *push3 = nullptr;
     AmrReflux *rfx[3] = {nullptr, nullptr, nullptr};  // flux correction at level jumps, per direction
     int64_t nrfx[3] = {0, 0, 0};
     AmrEmf *emfx = nullptr;        // EMF correction at level jumps
@@ -363,6 +366,8 @@ static int init_device(hamr_ctx *c)
         dalloc(c, &c->g32, NPOS * G32N * KC) || dalloc(c, &c->blk, nb))
         return -1;
     if (h2d(c, c->blk, c->blocks.data(), nb * sizeof(BlockDev))) return -1;
+    if (dalloc(c, &c->push3, std::max<size_t>(c->push3tab.size(), 1))) return -1;
+    if (!c->push3tab.empty() && h2d(c, c->push3, c->push3tab.data(), c->push3tab.size() * sizeof(int))) return -1;
     std::vector<int> bl(nb);
     c->roff.assign(nb + 1, 0);
     c->doff.assign(nb + 1, 0);
@@ -392,6 +397,7 @@ static int init_device(hamr_ctx *c)
     P.rcen = c->rcen;
     P.tet = c->tet;
     P.g32 = c->g32;
+    P.push3 = c->push3;
     P.riemann = HAMR_RIEMANN_HLL;
     P.las_s = -1;
     P.las_want = LAS_IDLE;
@@ -424,8 +430,9 @@ static void pole(hamr_ctx *c, double dt_val, const double *dt_ptr)
 // later stages left (see hamr_amr.cpp).
 // Ghost fill of every block.  x3_rows_done: the update and face-field kernels
 // of the batched step already wrote the x3 ghosts of the rows inside each
-// block's x1 and x2 range from the periodic neighbour (only the remaining rows
-// are filled along x3; the result is the full fill's, cell for cell).
+// block's x1 and x2 range from its push3 neighbours (only the remaining rows
+// are filled along x3: on a multi-level set stage 3 leaves out its last
+// npush3 entries; the result is the full fill's, cell for cell).
 static void bound_all(hamr_ctx *c, double *p, double *ps, bool x3_rows_done = false)
 {
     const int T = kTiles[c->tune[TS_EXCHANGE].tile], G = c->tune[TS_EXCHANGE].grid;
@@ -438,7 +445,8 @@ static void bound_all(hamr_ctx *c, double *p, double *ps, bool x3_rows_done = fa
     launch_amr_exchange(c->P, 1, c->ex[0], c->nex[0], c->pro[0], c->npro[0], p, ps, c->pflag, T, G, c->s);
     launch_bound(c->P, 2, all_launch(c, TS_BOUND2), p, ps, c->pflag, 0, c->s);
     launch_amr_exchange(c->P, 2, c->ex[1], c->nex[1], c->pro[1], c->npro[1], p, ps, c->pflag, T, G, c->s);
-    launch_amr_exchange(c->P, 3, c->ex[2], c->nex[2], c->pro[2], c->npro[2], p, ps, c->pflag, T, G, c->s);
+    launch_amr_exchange(c->P, 3, c->ex[2], c->nex[2] - (x3_rows_done ? c->npush3 : 0), c->pro[2], c->npro[2], p, ps,
+                        c->pflag, T, G, c->s);
     launch_bound(c->P, 1, all_launch(c, TS_BOUND1), p, ps, c->pflag, 1, c->s);
 }
 
@@ -555,6 +563,7 @@ int hamr_create(const hamr_desc *desc, hamr_ctx **out)
     c->L = make_layout(d);
     int ngeo = 0;
     c->blocks = build_block_table(d, &ngeo);
+    c->push3tab = uniform_push_table(c->blocks);
     c->order = zorder(d);
     if ((int)c->order.size() != (int)c->blocks.size()) return fail(c, "hamr_create: block numbering failed");
     const Layout &L = c->L;
@@ -606,6 +615,7 @@ int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *a, hamr_ctx **ou
     c->amr = true;
     c->L = make_layout(c->d);
     c->blocks = g.blocks;
+    c->push3tab = g.push3tab;
     c->geos = g.geos;
     c->cells = g.cells;
     c->cols = g.cols;
@@ -613,6 +623,7 @@ int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *a, hamr_ctx **ou
     for (size_t n = 0; n < c->blocks.size(); n++) c->order[n] = (int)n;
     if (init_device(c)) return -1;
     if (upload_rings(c, g.ring, g.rside, g.ring_len, g.ni_max)) return -1;
+    c->npush3 = g.npush3;
     for (int s = 0; s < 3; s++) {
         c->nex[s] = (int64_t)g.ex[s].size();
         c->npro[s] = (int64_t)g.pro[s].size();
@@ -659,6 +670,7 @@ int hamr_amr_info(const hamr_ctx *c, long long *info)
     info[11] = c->amr ? 1 : 0;
     for (int d = 0; d < 3; d++) info[12 + d] = c->nrfx[d];
     info[15] = c->nemfx;
+    info[16] = c->npush3;
     return 0;
 }
 
@@ -875,6 +887,7 @@ int hamr_desc_amr(const hamr_desc *desc, const hamr_amr_desc *a, long long *info
     info[11] = 1;
     for (int d = 0; d < 3; d++) info[12 + d] = (long long)g.rfx[d].size();
     info[15] = (long long)g.emf.size();
+    info[16] = g.npush3;
     if (rows)
         for (size_t n = 0; n < g.blocks.size(); n++) {
             const BlockDev &B = g.blocks[n];
@@ -2217,7 +2230,7 @@ int hamr_time_stage(hamr_ctx *c, int stage, int nrep, double *avg_ms)
         case 4: pole(c, 0., c->dt); break;
         case 5: launch_ct_update(c->P, all_launch(c, TS_CT), 0., c->dt, 0.5, c->ps, c->psh, c->E, c->prec, c->s); break;
         case 6: update_all(c, 0, 0., c->dt, 0.5, c->p, c->p, c->ph, c->psh); break;
-        default: bound_all(c, c->ph, c->psh); break;
+        default: bound_all(c, c->ph, c->psh, true); break;
         }
     }
     HIPCHK(c, hipEventRecord(e1, c->s));

@@ -23,7 +23,7 @@ constexpr int NG = 3;
 // multi-level exchange tables (neighbours of other sizes or levels).
 enum FaceKind : int { FK_COPY = 0, FK_OUTFLOW = 1, FK_POLE = 2, FK_AMR = 3 };
 
-// Per-block record used by the kernels (plain data, 160 bytes).  Component k of
+// Per-block record used by the kernels (plain data, 168 bytes).  Component k of
 // an array with ncomp components starts at ncomp*off + k*ks for this block.
 struct BlockDev {
     int geo;        // geometry slab
@@ -38,11 +38,27 @@ struct BlockDev {
     int ks;         // cells per component slab (padded to 64)
     int kcol;       // columns per metric slab (padded to 64)
     int tl;         // time level of local adaptive time stepping (a power of 2; 0 until set)
+    int push3[2];   // lower / upper x3 face: first entry of the face's row of the push table (-1: none)
     int64_t off;    // cells per component of all earlier blocks' slabs
     int64_t moff;   // metric columns of all earlier geometry slabs
     double dx[3];   // code-coordinate spacing of the block's level
 };
-static_assert(sizeof(BlockDev) == 160, "BlockDev layout");
+static_assert(sizeof(BlockDev) == 168, "BlockDev layout");
+
+// Push table: the update and face-field kernels write the NG lowest and
+// highest cells of a block's z-rows also into the x3 ghost cells of the block
+// across that x3 face that holds them (same level; the ghost rows inside that
+// block's x1 and x2 range), so that the fill right after the update skips
+// them.  A face's row of the table (from BlockDev::push3) has one entry per
+// kPushTile x kPushTile tile of the block's (i, j) columns, tile (t1, t2) at
+// t1 * ceil(bs2 / kPushTile) + t2: the receiving block, or -1 (its ghosts come
+// from the exchange).  The receiver's ghost cell of the block's cell (i, j, z)
+// is (i + o1 - o1', j + o2 - o2', z - bs3) across the upper face and
+// (i + o1 - o1', j + o2 - o2', z + bs3') across the lower one.
+constexpr int kPushTile = 16;
+// The push table of a uniform grid (every x3 face: the periodic neighbour);
+// sets each block's push3 offsets.
+std::vector<int> uniform_push_table(std::vector<BlockDev> &t);
 
 // Multi-level exchange table entries (one per ghost cell).  A source code is
 // cell | kind << 28 with kind AS_CELL (that cell) or AS_AVG8 (the mean of the

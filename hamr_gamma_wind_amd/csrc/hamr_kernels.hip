@@ -111,16 +111,16 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
         double *sf = psf + 3 * B.off + g;
         const double dx1 = B.dx[0], dx2 = B.dx[1], dx3 = B.dx[2];
         double temp;
-        // the x3 ghosts of the periodic neighbour, as in finish_cell: B1 and B2
-        // of the NG lowest / highest cells of the z-row, B3 of the faces that
-        // are not the block's own x3 boundary faces (each block forms those)
+        // the x3 ghosts of the push table's receiver, as in finish_cell: B1 and
+        // B2 of the NG lowest / highest cells of the z-row, B3 of the faces
+        // that are not the block's own x3 boundary faces (each block forms those)
         const int bs3 = B.bs[2];
         const bool inner = i < NG + B.bs[0] && j < NG + B.bs[1];
-        const int m = !inner ? -1 : z < 2 * NG ? x3_copy_partner(P, B, it.x, 4)
-                                              : (z >= bs3 && z < NG + bs3) ? x3_copy_partner(P, B, it.x, 5) : -1;
+        PushTarget T;
         double *sm = nullptr;
-        if (m >= 0) sm = psf + 3 * P.blk[m].off + g + (z < 2 * NG ? bs3 : -bs3);
-        const int64_t KM = m >= 0 ? (int64_t)P.blk[m].ks : 0;
+        if (inner && (z < 2 * NG || (z >= bs3 && z < NG + bs3)) && x3_push_target(P, B, i, j, z, z < 2 * NG ? 0 : 1, T))
+            sm = psf + 3 * T.off + T.cell;
+        const int64_t KM = sm ? T.ks : 0;
         if (j < NG + B.bs[1] && z < NG + B.bs[2]) {
             temp = 0.;
             temp += 1.0 / 1.0 * si_[0] * gF1;
@@ -422,9 +422,10 @@ __device__ __forceinline__ void wave_count(int *cnt, bool pred)
 // would copy it to (the neighbour's upper ghosts from this block's lowest
 // cells, its lower ghosts from the highest), so that the ghost fill after the
 // update only has the rows beyond the block's x1 and x2 edges left along x3.
-__device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B, int n, int64_t col, int64_t g, int z,
-                                            int64_t base, int flag, double *pf, double *__restrict__ pf_out,
-                                            int *__restrict__ pflag, int *__restrict__ failcnt)
+__device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B, int n, int i, int j, int64_t col,
+                                            int64_t g, int z, int64_t base, int flag, double *pf,
+                                            double *__restrict__ pf_out, int *__restrict__ pflag,
+                                            int *__restrict__ failcnt)
 {
     const int64_t K = B.ks;
     const double r = P.rcen[B.moff + col], sr = sqrt(r);
@@ -443,13 +444,11 @@ __device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B,
 #pragma unroll
     for (int k = 0; k < NPR; k++) pf_out[base + k * K] = pf[k];
     const int bs3 = B.bs[2];
-    const int m = z < 2 * NG ? x3_copy_partner(P, B, n, 4) : z >= bs3 ? x3_copy_partner(P, B, n, 5) : -1;
-    if (m >= 0) {
-        const BlockDev &M = P.blk[m];
-        const int64_t gm = g + (z < 2 * NG ? bs3 : -bs3);
-        pflag[M.off + gm] = flag;
+    PushTarget T;
+    if ((z < 2 * NG || z >= bs3) && x3_push_target(P, B, i, j, z, z < 2 * NG ? 0 : 1, T)) {
+        pflag[T.off + T.cell] = flag;
 #pragma unroll
-        for (int k = 0; k < NPR; k++) pf_out[NPR * M.off + gm + k * M.ks] = pf[k];
+        for (int k = 0; k < NPR; k++) pf_out[NPR * T.off + T.cell + k * T.ks] = pf[k];
     }
 }
 
@@ -488,7 +487,7 @@ __global__ void __launch_bounds__(256) k_invert(KParams P, WorkList wl, int full
         }
         const int flag = utoprim(0, U, G.gcov, G.gcon, G.g, P.gam, pf);
         if (flag == 0) {
-            finish_cell(P, B, n, col, g, z, base, 0, pf, pf_out, pflag, failcnt);
+            finish_cell(P, B, n, i, j, col, g, z, base, 0, pf, pf_out, pflag, failcnt);
             continue;
         }
         atomicAdd(&failcnt[n * 4 + 0], 1);
@@ -562,7 +561,7 @@ k_update(KParams P, WorkList wl, int full, double Dt_val, const double *__restri
             G.g = mvp[MV_G * ncw];
             const int flag = utoprim(0, U, G.gcov, G.gcon, G.g, P.gam, pf);
             if (flag == 0) {
-                finish_cell(P, B, n, col, g, z, base, 0, pf, pf_out, pflag, failcnt);
+                finish_cell(P, B, n, i, j, col, g, z, base, 0, pf, pf_out, pflag, failcnt);
             } else {
                 atomicAdd(&failcnt[n * 4 + 0], 1);
                 pflag[B.off + g] = kFallbackPending;
@@ -610,7 +609,7 @@ __global__ void __launch_bounds__(256) k_fallback(KParams P, WorkList wl, const 
             flag = utoprim(2, U, G.gcov, G.gcon, G.g, P.gam, pf);
             if (flag) atomicAdd(&failcnt[n * 4 + 2], 1);
         }
-        finish_cell(P, B, n, col, g, z, base, flag, pf, pf_out, pflag, failcnt);
+        finish_cell(P, B, n, i, j, col, g, z, base, flag, pf, pf_out, pflag, failcnt);
     }
 }
 
@@ -660,7 +659,7 @@ __global__ void __launch_bounds__(256) k_bound(KParams P, WorkList wl, double *p
         // rows inside the block's x1 and x2 range whose x3 ghosts the update
         // and face-field kernels already wrote from the periodic neighbour
         if (DIR == 3 && x3_rows_done && aa >= NG && aa < NG + B.bs[0] && bb >= NG && bb < NG + B.bs[1] &&
-            x3_copy_partner(P, B, n, face) >= 0)
+            x3_row_pushed(P, B, n, face, aa, bb))
             continue;
         if (outflow_only) {
             if (kind != FK_OUTFLOW) continue;

@@ -80,15 +80,46 @@ __device__ __forceinline__ double wave_min(double v)
     return v;
 }
 
-// x3 neighbour m of block n across face f (4: lower, 5: upper) when it is a
-// same-level copy neighbour whose own face back is n: the blocks whose x3
-// ghost rows the step writes directly (the periodic x3 fill of a uniform grid).
-__device__ __forceinline__ int x3_copy_partner(const KParams &P, const BlockDev &B, int n, int f)
+// Where the update and face-field kernels also write cell (i, j, z) of block
+// B, one of the NG lowest (side 0) or highest (side 1) cells of its z-row: the
+// x3 ghost cell of the same-level block across that face that the push table
+// names for the cell's tile, as slab offset, component stride and cell index
+// there.  false: nothing (the exchange fills that ghost), and always in a
+// time-level cycle, whose exchange fills a ghost only when both blocks step.
+struct PushTarget {
+    int64_t off, ks, cell;
+};
+__device__ __forceinline__ bool x3_push_target(const KParams &P, const BlockDev &B, int i, int j, int z, int side,
+                                               PushTarget &T)
 {
-    if (B.kind[f] != FK_COPY) return -1;
-    const int m = B.nbr[f];
+    const int row = B.push3[side];
+    if (P.las_s >= 0 || row < 0) return false;
+    const int t2n = (B.bs[1] + kPushTile - 1) / kPushTile;
+    const int m = P.push3[row + ((i - NG) / kPushTile) * t2n + (j - NG) / kPushTile];
+    if (m < 0) return false;
     const BlockDev &M = P.blk[m];
-    return (M.kind[f ^ 1] == FK_COPY && M.nbr[f ^ 1] == n) ? m : -1;
+    const int im = i + B.o[0] - M.o[0], jm = j + B.o[1] - M.o[1];
+    const int zm = side ? z - B.bs[2] : z + M.bs[2];
+    T.off = M.off;
+    T.ks = M.ks;
+    T.cell = ((int64_t)im * M.D[1] + jm) * M.D[2] + zm;
+    return true;
+}
+
+// Whether the x3 ghosts of row (i, j) of block n (inside its x1 and x2 range)
+// beyond face f (4: lower, 5: upper) are written by the update of the copy
+// neighbour across f (its push table names n for that row's tile).
+__device__ __forceinline__ bool x3_row_pushed(const KParams &P, const BlockDev &B, int n, int f, int i, int j)
+{
+    const int s = B.nbr[f];
+    if (P.las_s >= 0 || s < 0) return false;
+    const BlockDev &S = P.blk[s];
+    const int row = S.push3[f == 4 ? 1 : 0];
+    if (row < 0) return false;
+    const int is = i + B.o[0] - S.o[0], js = j + B.o[1] - S.o[1];
+    if (is < NG || is >= NG + S.bs[0] || js < NG || js >= NG + S.bs[1]) return false;
+    const int t2n = (S.bs[1] + kPushTile - 1) / kPushTile;
+    return P.push3[row + ((is - NG) / kPushTile) * t2n + (js - NG) / kPushTile] == n;
 }
 
 // launch geometry of a work-list launch

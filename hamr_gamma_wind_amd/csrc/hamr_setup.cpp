@@ -68,6 +68,7 @@ std::vector<BlockDev> build_block_table(const GridDesc &d, int *ngeo_out)
                 b.kind[4] = FK_COPY;
                 b.nbr[5] = id(c1, c2, (c3 + 1) % nb3);
                 b.kind[5] = FK_COPY;
+                b.push3[0] = b.push3[1] = -1;
             }
     // per-block layout: uniform sizes, slabs in natural order
     const Layout L = make_layout(d);
@@ -90,6 +91,19 @@ std::vector<BlockDev> build_block_table(const GridDesc &d, int *ngeo_out)
     }
     if (ngeo_out) *ngeo_out = nb1 * nb2;
     return t;
+}
+
+std::vector<int> uniform_push_table(std::vector<BlockDev> &t)
+{
+    std::vector<int> tab;
+    for (BlockDev &b : t) {
+        const int ntile = ((b.bs[0] + kPushTile - 1) / kPushTile) * ((b.bs[1] + kPushTile - 1) / kPushTile);
+        for (int side = 0; side < 2; side++) {
+            b.push3[side] = (int)tab.size();
+            tab.insert(tab.end(), ntile, b.nbr[4 + side]);
+        }
+    }
+    return tab;
 }
 
 // The reference's block numbering of a single-level grid: the level-0 Z-order

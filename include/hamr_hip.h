@@ -111,7 +111,7 @@ int hamr_desc_coords(const hamr_desc *desc, double *startx, double *dx);
 int hamr_desc_metric_at(const hamr_desc *desc, const double *X, double *gcov, double *gcon, double *gdet,
                         double *conn, double *r_th);
 /* The multi-level block set of `amr` without a device (as hamr_create_amr
- * builds it): info[16] as hamr_amr_info; rows (15 ints per block, as
+ * builds it): info[17] as hamr_amr_info; rows (15 ints per block, as
  * hamr_block_layout), the copy/restriction entries of stages 1-3 one after
  * another (4 ints: ghost block, ghost cell, source block, source cell |
  * kind << 28 with kind 1 = cell, 2 = mean of the 2x2x2 cells from there) and
@@ -151,11 +151,14 @@ int hamr_create(const hamr_desc *desc, hamr_ctx **out);
  * of whole arrays: the blocks' [component][cell] slabs one after another, each
  * with its own (bs+6)^3 cell count. */
 int hamr_create_amr(const hamr_desc *desc, const hamr_amr_desc *amr, hamr_ctx **out);
-/* info[16]: blocks, geometry slabs, padded cells, interior zones, exchange
+/* info[17]: blocks, geometry slabs, padded cells, interior zones, exchange
  * entries (copy/restriction) of stages 1-3, prolongation entries of stages
  * 1-3, polar rings, 1 if multi-level, coarse faces at level jumps along x1,
  * x2, x3 (flux correction entries), coarse edges at level jumps (EMF
- * correction entries). */
+ * correction entries), the last stage-3 entries that the step's fill leaves
+ * out (x3 ghost rows inside the x1 and x2 range of a block whose x3
+ * neighbour has its level and shape: the step's update writes them from that
+ * neighbour's cells, as a uniform grid's periodic x3 fill). */
 int hamr_amr_info(const hamr_ctx *ctx, long long *info);
 /* row[15]: level, origin o1 o2 o3 (cells of its level), bs1 bs2 bs3, geometry
  * slab, pole bits, kinds of faces -x1..+x3 (0 copy, 1 outflow, 2 polar mirror,
@@ -374,7 +377,9 @@ int hamr_las_info(const hamr_ctx *ctx, long long *info);
 /* Average device time (ms, HIP events on the context stream) of one launch of
  * a stage over all blocks, with predictor-step operands (p and ps are not
  * modified): 0-2 flux x1..x3, 3 EMF, 4 polar average, 5 face-B update,
- * 6 conserved update + inversion, 7 ghost fill (3 directions). */
+ * 6 conserved update + inversion, 7 ghost fill (3 directions) as the step
+ * runs it after the update (the x3 ghost rows that the update writes left
+ * out). */
 int hamr_time_stage(hamr_ctx *ctx, int stage, int nrep, double *avg_ms);
 /* ---- launch autotuner (no reference counterpart) ----
  * Every kernel of the step walks a work list of (block, tile) items.  For each

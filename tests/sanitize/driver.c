@@ -166,7 +166,7 @@ static void multi_level(void)
     a.seed = 5;
     a.p_big = 0.35;
     a.p_mid = 0.35;
-    long long info[16];
+    long long info[17];
     CHECK(hamr_desc_amr(&d, &a, info, NULL, NULL, NULL) == 0);
     const int nblk = (int)info[0];
     long long nex = info[4] + info[5] + info[6], npro = info[7] + info[8] + info[9];

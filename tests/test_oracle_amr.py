@@ -553,3 +553,66 @@ def test_exchange_entries_never_read_a_cell_their_stage_writes(boxes):
         bad = np.isin(src, dst)
         print("stage %d: %d entries, %d source reads, %d of written cells" % (s + 1, len(dst), len(src), bad.sum()))
         assert not bad.any(), s
+
+
+@pytest.mark.parametrize("which", ["x1x2", "x1x2x3", "torus722"])
+def test_stage3_tail_is_the_rows_the_update_writes(which):
+    """Stage 3's last info['push3'] entries are exactly the x3 ghosts of the
+    rows inside a block's x1 and x2 range whose position (the cell across the
+    block's x3 face) lies in a block of the same level, each copying that
+    block's cell at the position (found here from the block rows: blocks tile
+    each level in 16-cell units); the step's fill leaves them out because that
+    block's update writes them.  No other entry has such a destination."""
+    from hamr_gamma_wind_amd import workloads as W
+    if which == "torus722":
+        A = W.AMR_TORUS
+        desc, amr = H.make_desc(A["bs"], A["nb"]), H.make_amr_desc(A["boxes"], seed=A["seed"])
+    else:
+        desc = _desc(DESC3)
+        amr = H.make_amr_desc(BOXES3 if which == "x1x2" else BOXES3Z, seed=5)
+    info = H.desc_amr(desc, amr, tables=True)
+    rows = info["rows"].astype(np.int64)
+    n0 = [desc.bs[k] * desc.nb[k] for k in range(3)]
+    U = 16
+    owner = {}
+    for b, r in enumerate(rows):
+        lev, o, bs = int(r[0]), r[1:4], r[4:7]
+        for u1 in range(o[0] // U, (o[0] + bs[0]) // U):
+            for u2 in range(o[1] // U, (o[1] + bs[1]) // U):
+                for u3 in range(o[2] // U, (o[2] + bs[2]) // U):
+                    owner[(lev, u1, u2, u3)] = b
+    want_dst, want_src = [], []
+    for b, r in enumerate(rows):
+        lev, o, bs = int(r[0]), r[1:4], r[4:7]
+        D = bs + 2 * NG
+        N3 = n0[2] << lev
+        for side in (0, 1):
+            zpos = (o[2] - 1) % N3 if side == 0 else (o[2] + bs[2]) % N3
+            for u1 in range(o[0] // U, (o[0] + bs[0]) // U):
+                for u2 in range(o[1] // U, (o[1] + bs[1]) // U):
+                    s_ = owner.get((lev, u1, u2, zpos // U))
+                    if s_ is None:
+                        continue
+                    S = rows[s_]
+                    DS = S[4:7] + 2 * NG
+                    i, j, g = np.meshgrid(np.arange(U), np.arange(U), np.arange(NG), indexing="ij")
+                    Y1, Y2 = u1 * U + i, u2 * U + j
+                    z = g if side == 0 else NG + bs[2] + g
+                    Y3 = (o[2] + z - NG) % N3
+                    dc = ((Y1 - o[0] + NG) * D[1] + (Y2 - o[1] + NG)) * D[2] + z
+                    sc = ((Y1 - S[1] + NG) * DS[1] + (Y2 - S[2] + NG)) * DS[2] + ((Y3 - S[3]) % N3 + NG)
+                    want_dst.append((b << 32) | dc.ravel())
+                    want_src.append((s_ << 32) | sc.ravel())
+    want_dst, want_src = np.concatenate(want_dst), np.concatenate(want_src)
+    ex3 = info["ex"][2].astype(np.int64)
+    npush = info["info"]["push3"]
+    tail, head = ex3[len(ex3) - npush:], ex3[:len(ex3) - npush]
+    print("%s: %d stage-3 entries, %d written by the update" % (which, len(ex3), npush))
+    assert npush == len(want_dst) and npush > 0
+    assert np.all((tail[:, 3] >> 28) == 1)
+    got_dst = (tail[:, 0] << 32) | tail[:, 1]
+    got_src = (tail[:, 2] << 32) | (tail[:, 3] & 0x0FFFFFFF)
+    a, b_ = np.argsort(got_dst), np.argsort(want_dst)
+    assert np.array_equal(got_dst[a], want_dst[b_])
+    assert np.array_equal(got_src[a], want_src[b_])
+    assert not np.isin((head[:, 0] << 32) | head[:, 1], want_dst).any()
