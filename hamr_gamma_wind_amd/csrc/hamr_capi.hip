@@ -267,7 +267,7 @@ static int build_class_lists(hamr_ctx *c)
                 if (cls >= KC_LINE1 && cls <= KC_LINE3) {  // kLineT threads per tile whatever T is
                     const LineTiles lt = line_tiles(B.bs[0], B.bs[1], B.bs[2], cls - KC_LINE1 + 1);
                     nt = lt.ntiles;
-                    cols = std::max(cols, lt.L * (lt.S + 5));
+                    cols = std::max(cols, lt.L * (lt.S + 3));
                 } else {
                     nt = (int)((class_work(B, cls) + T - 1) / T);
                     cols = std::max(cols, cons_max_cols(B.bs[2], T));

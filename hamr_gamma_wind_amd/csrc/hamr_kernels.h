@@ -72,8 +72,8 @@ constexpr int G32N = 22;  // FP32 metric values per column and position (KParams
 // [NG-1, NG+BS]) has b_dir + 1 faces, cut into nseg segments of S faces (the
 // last one possibly shorter), at most kLineSeg each.  The lines form n1 rows
 // (the slower transverse direction) of n2 lines; a tile is one segment on L =
-// kLineT / S consecutive lines of one row (tpr tiles per row), and stages
-// L (S + 5) cells per primitive.
+// kLineT / (S + 3) consecutive lines of one row (tpr tiles per row), one
+// thread per cell of the S + 3 cells whose slopes the segment's faces need.
 constexpr int kLineT = 256;
 constexpr int kLineSeg = 33;
 struct LineTiles {
@@ -85,7 +85,7 @@ __host__ __device__ inline LineTiles line_tiles(int b1, int b2, int b3, int dir)
     const int nf = (dir == 1 ? b1 : dir == 2 ? b2 : b3) + 1;
     x.nseg = (nf + kLineSeg - 1) / kLineSeg;
     x.S = (nf + x.nseg - 1) / x.nseg;
-    x.L = kLineT / x.S;
+    x.L = kLineT / (x.S + 3);
     x.n1 = (dir == 1 ? b2 : b1) + 2;
     x.n2 = (dir == 3 ? b2 : b3) + 2;
     x.tpr = (x.n2 + x.L - 1) / x.L;

@@ -165,122 +165,101 @@ k_flux(KParams P, WorkList wl, const double *__restrict__ p, const double *__res
 
 // ---------------------------------------------------------------------------
 // Line-staged reconstruction + Riemann flux: the faces, states and fluxes of
-// k_flux, with each cell's MC slope, each cell boundary's PPM interface value
-// and each cell's limited parabola formed once instead of once per face that
-// reaches it.
+// k_flux, with each cell's MC slope and limited parabola formed once instead
+// of once per face that reaches it.
 //
 // A work-list item is a tile (line_tiles): one segment of S consecutive faces
 // along DIR on L consecutive lines of one row, a line being one transverse
-// position of the face range.  For each reconstructed primitive the
-// workgroup stages the W = S + 5 cells of each line that the tile's faces
-// reach in LDS, rounded to R, and then forms, one cell or cell boundary per
-// thread with a barrier between the steps: the MC slopes of the S + 3 cells
-// inside them, the interface values of the S + 2 boundaries between those,
-// and the limited edges of the S + 1 cells next to the faces.  Each face
-// thread takes the upper edge of its lower cell and the lower edge of its
-// upper cell into its states.  Every value is para's expression on para's
-// operands, so the states and fluxes are k_flux's bit for bit.
-//
-// LDS slot of line q, staged cell c: c L + q, and work index u = t + r kLineT
-// of a step is its slot minus the step's first cell times L: consecutive
-// threads take consecutive lines (consecutive z for DIR = 1, 2; z rows for
-// DIR = 3, whose 38 contiguous cells are then read by 2 load instructions).
-// SC is the segment length when it is known at compile time (33: blocks of 32
-// or 64 cells along DIR; 17: 16 cells), 0 otherwise (a loop over the work).
-// A shorter last segment masks its extra faces and the cells past the block.
+// position of the face range, with one thread per cell of each line's S + 3
+// cells f0 - 2 .. f0 + S (f0: the segment's first face, as the index of its
+// upper cell).  In three steps, with a barrier between them:
+//   1. every cell's MC slope from its own three-point stencil (read from
+//      memory, rounded to R) into LDS;
+//   2. cells f0 - 1 .. f0 + S - 1: the interface values of both boundaries
+//      (the neighbours' slopes from LDS) and the limited parabola; the upper
+//      edge goes to LDS, the lower edge stays in registers;
+//   3. one thread per face (the face's upper cell): the lower cell's upper
+//      edge from LDS and its own lower edge are the face states.
+// Every value is para's expression on para's operands, so the states and
+// fluxes are k_flux's bit for bit.  Thread t of a tile: line q and cell c
+// (c = 0: cell f0 - 2) with t = c L + q for DIR = 1, 2 (consecutive threads
+// on consecutive lines, i.e. consecutive z) and t = q (S + 3) + c for DIR = 3
+// (consecutive threads along the z-row).  SC is the segment length when it is
+// known at compile time (33: blocks of 32 or 64 cells along DIR; 17: 16
+// cells), 0 otherwise.  A shorter last segment masks the cells past its faces.
 template <int DIR, class R, int SC>
 __device__ __forceinline__ void line_tile_states(const BlockDev &B, int item, const double *__restrict__ p, R *sm,
                                                  R *pl, R *pr, bool &fact, int64_t &cell, int &i, int &j)
 {
-    constexpr int T = kLineT;
+    constexpr int T = kLineT, NV = NPR - 1;
     const int t = threadIdx.x;
     const LineTiles lt = line_tiles(B.bs[0], B.bs[1], B.bs[2], DIR);
-    const int S = SC ? SC : lt.S, L = T / S, W = S + 5;
-    const int nseg = lt.nseg, tpr = lt.tpr;
-    const int seg = item % nseg, rest = item / nseg, row = rest / tpr, tb0 = (rest - row * tpr) * L;
+    const int S = SC ? SC : lt.S, W = S + 3, L = SC ? T / (SC + 3) : lt.L;
+    const int seg = item % lt.nseg, rest = item / lt.nseg, row = rest / lt.tpr, tb0 = (rest - row * lt.tpr) * L;
     const int nl = mymin(L, lt.n2 - tb0);
     const int nf = mymin(S, B.bs[DIR - 1] + 1 - seg * S);  // faces of this segment
-    const int f0 = NG + seg * S;                             // its first face (index of the upper cell along DIR)
-    const int dlen = B.D[DIR - 1];
-    R *const Y = sm, *const DQ = Y + L * W, *const A = DQ + L * W, *const LE = A + L * W, *const RE = LE + L * W;
-    // cells: line q of the tile is transverse (row, tb0 + q); position pos along DIR
+    const int f0 = NG + seg * S;
+    int q, c;
+    if (DIR == 3) { q = t / W; c = t - q * W; }
+    else { c = t / L; q = t - c * L; }
+    const int sn = DIR == 3 ? 1 : L;  // LDS distance to the next cell along DIR
+    const bool ok = q < nl && c < W;
+    const bool slope = ok && c <= nf + 2, edge = ok && c >= 1 && c < nf + 2;
+    fact = ok && c >= 2 && c < nf + 2;
     const int64_t D12 = (int64_t)B.D[1] * B.D[2];
     const int64_t sdir = DIR == 1 ? D12 : DIR == 2 ? (int64_t)B.D[2] : 1;
     const int64_t sq = DIR == 3 ? (int64_t)B.D[2] : 1;  // between consecutive lines of a row
     const int ta = row + NG - 1, tb = tb0 + NG - 1;
     const int64_t base = DIR == 1 ? (int64_t)ta * B.D[2] + tb : DIR == 2 ? (int64_t)ta * D12 + tb
                                                                          : (int64_t)ta * D12 + (int64_t)tb * B.D[2];
+    const int pos = f0 - 2 + c;  // this thread's cell along DIR
+    cell = base + q * sq + (int64_t)pos * sdir;
+    i = DIR == 1 ? pos : ta;  // the face's metric column (i, j)
+    j = DIR == 1 ? ta : DIR == 2 ? pos : tb + q;
+    R *const DQ = sm, *const RE = sm + NV * T;
     const int64_t K = B.ks;
-    const double *pb = p + NPR * B.off;
-    // work r of this thread (u = t + r T): line q[r], staged cell c[r] of the load step
-    constexpr int NR = SC ? (T / SC * (SC + 5) + T - 1) / T : 1;  // loads per thread (compile-time S)
-    int q[NR > 0 ? NR : 1];
-    int64_t goff[NR > 0 ? NR : 1];
-    bool gok[NR > 0 ? NR : 1];
-    if constexpr (SC) {
+    const double *pc = p + NPR * B.off + cell;
+    R ym[NV], y0[NV], yp[NV], dq[NV];
 #pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const int u = t + r * T, c = u / L;
-            q[r] = u - c * L;
-            gok[r] = u < L * W && q[r] < nl && f0 - 3 + c < dlen;
-            goff[r] = base + q[r] * sq + (int64_t)(f0 - 3 + c) * sdir;
-        }
-    }
-    const int ff = t / L, fq = t - ff * L;
-    fact = ff < nf && fq < nl;
-    const int lo = (ff + 2) * L + fq, hi = lo + L;  // the face's lower and upper cell
-#pragma unroll
-    for (int k = 0; k < NPR; k++) {
+    for (int k = 0, v = 0; k < NPR; k++) {
         if (k == B1 + DIR - 1) continue;  // the face-normal field comes from ps
-        const double *pk = pb + (int64_t)k * K;
-        if constexpr (SC) {
+        if (slope) {
+            const double *y = pc + (int64_t)k * K;
+            ym[v] = (R)y[-sdir];
+            y0[v] = (R)y[0];
+            yp[v] = (R)y[sdir];
+            dq[v] = mc_slope(ym[v], y0[v], yp[v]);
+            DQ[v * T + t] = dq[v];
+        }
+        v++;
+    }
+    __syncthreads();
+    R le[NV];
 #pragma unroll
-            for (int r = 0; r < NR; r++)
-                if (gok[r]) Y[t + r * T] = (R)pk[goff[r]];
-        } else {
-            for (int u = t; u < L * W; u += T) {
-                const int c = u / L, qq = u - c * L;
-                if (qq < nl && f0 - 3 + c < dlen) Y[u] = (R)pk[base + qq * sq + (int64_t)(f0 - 3 + c) * sdir];
-            }
-        }
-        __syncthreads();
-        // slopes of staged cells 1 .. S + 3
-        for (int u = t; u < L * (S + 3); u += T) {
-            if (SC && u >= L * (S + 3)) break;
-            const int c = u + L;
-            if ((SC ? q[(u - t) / T] : u % L) < nl) DQ[c] = mc_slope(Y[c - L], Y[c], Y[c + L]);
-        }
-        __syncthreads();
-        // interfaces between staged cells c - 1 and c, c = 2 .. S + 3
-        for (int u = t; u < L * (S + 2); u += T) {
-            const int c = u + 2 * L;
-            if ((SC ? q[(u - t) / T] : u % L) < nl) A[c] = ppm_face(Y[c - L], Y[c], DQ[c - L], DQ[c]);
-        }
-        __syncthreads();
-        // limited edges of staged cells 2 .. S + 2
-        for (int u = t; u < L * (S + 1); u += T) {
-            const int c = u + 2 * L;
-            if ((SC ? q[(u - t) / T] : u % L) < nl) {
-                R l, rr;
-                ppm_limit(Y[c], A[c], A[c + L], l, rr);
-                LE[c] = l;
-                RE[c] = rr;
-            }
-        }
-        __syncthreads();
-        if (fact) {
-            pl[k] = RE[lo];
-            pr[k] = LE[hi];
+    for (int v = 0; v < NV; v++) {
+        if (edge) {
+            R l, r;
+            ppm_limit(y0[v], ppm_face(ym[v], y0[v], DQ[v * T + t - sn], dq[v]),
+                      ppm_face(y0[v], yp[v], dq[v], DQ[v * T + t + sn]), l, r);
+            le[v] = l;
+            RE[v * T + t] = r;
         }
     }
-    const int pos = f0 + ff;
-    cell = base + fq * sq + (int64_t)pos * sdir;
-    i = DIR == 1 ? pos : ta;                            // the face's metric column (i, j)
-    j = DIR == 1 ? ta : DIR == 2 ? pos : tb + fq;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0, v = 0; k < NPR; k++) {
+        if (k == B1 + DIR - 1) continue;
+        if (fact) {
+            pl[k] = RE[v * T + t - sn];
+            pr[k] = le[v];
+        }
+        v++;
+    }
 }
 
-// Occupancy: four waves per SIMD in FP32 (<= 128 VGPRs, a few spilled), two
-// in FP64 (the FP64 states, speeds and fluxes need ~220 VGPRs).
+// Occupancy: four waves per SIMD in FP32 (<= 128 VGPRs), two in FP64 (the
+// FP64 states, speeds and fluxes need ~220 VGPRs).  The LDS of one tile:
+// 2 (NPR - 1) kLineT values of R.
 template <int DIR, class R, int RS>
 __global__ void __launch_bounds__(kLineT) __attribute__((amdgpu_waves_per_eu(RS || sizeof(R) == 8 ? 2 : 4)))
 k_flux_lines(KParams P, WorkList wl, const double *__restrict__ p, const double *__restrict__ ps, R *__restrict__ F,
@@ -307,7 +286,7 @@ k_flux_lines(KParams P, WorkList wl, const double *__restrict__ p, const double 
             else vdt = hll_face<DIR>(P, B, ps, pl, pr, i, j, cell, F);
         }
         if (dtmin) dt_reduce(vdt, dtmin, n * 3 + DIR - 1);
-        __syncthreads();  // the next item restages the LDS
+        // the next item's first barrier orders its LDS writes after these reads
     }
 }
 
@@ -427,7 +406,7 @@ template <int DIR, class R, int RS>
 static void launch_flux_lines(const KParams &P, const Launch &L, const double *p, const double *ps, R *F,
                               unsigned long long *dtmin, hipStream_t s)
 {
-    const size_t lds = (size_t)5 * L.lds_cols * sizeof(R);
+    const size_t lds = (size_t)2 * (NPR - 1) * kLineT * sizeof(R);
     hipLaunchKernelGGL((k_flux_lines<DIR, R, RS>), gdim(L), bdim(L), lds, s, P, L.wl, p, ps, F, dtmin);
 }
 

@@ -281,13 +281,7 @@ def test_marching_update_equals_per_cell_update(setup, mode):
         ctx.set_precision(H.HAMR_PREC_FP64)
 
 
-@pytest.mark.parametrize("mode", ["fp64", "mixed", "hllc"])
-def test_line_staged_fluxes_equal_per_face_fluxes(setup, mode):
-    """The flux kernels' two forms, one thread per face (the default) and
-    line-staged tiles (each cell's slope, interface value and limited parabola
-    formed once per tile), give the same fluxes over every face they write, the
-    same dt and the same state after two steps, bit for bit."""
-    desc, ctx, grid, p0, ps0 = setup
+def _compare_flux_forms(desc, ctx, p0, ps0, mode):
     ctx.set_precision(H.HAMR_PREC_MIXED if mode == "mixed" else H.HAMR_PREC_FP64)
     if mode == "hllc":
         ctx.set_riemann(H.HAMR_RIEMANN_HLLC)
@@ -316,6 +310,31 @@ def test_line_staged_fluxes_equal_per_face_fluxes(setup, mode):
             ctx.tune_set(slot, 256, 0, 0)
         ctx.set_riemann(H.HAMR_RIEMANN_HLL)
         ctx.set_precision(H.HAMR_PREC_FP64)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed", "hllc"])
+def test_line_staged_fluxes_equal_per_face_fluxes(setup, mode):
+    """The flux kernels' two forms, one thread per face (the default) and
+    line-staged tiles (each cell's slope and limited parabola formed once per
+    tile), give the same fluxes over every face they write, the same dt and
+    the same state after two steps, bit for bit (16 x 8 x 8 blocks: 17 faces
+    per line along x1, 9 along x2 and x3)."""
+    desc, ctx, grid, p0, ps0 = setup
+    _compare_flux_forms(desc, ctx, p0, ps0, mode)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_line_staged_fluxes_equal_per_face_fluxes_32(mode):
+    """The same on 32^3 blocks (33 faces per line, the headline grid's tiles)."""
+    desc = H.make_desc((32, 32, 32), (2, 2, 2))
+    ctx = H.Context(desc)
+    try:
+        ctx.set_grid()
+        ctx.init_torus(H.torus_desc())
+        ctx.boundprim(1)
+        _compare_flux_forms(desc, ctx, ctx.get(H.HAMR_P), ctx.get(H.HAMR_PS), mode)
+    finally:
+        ctx.close()
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
