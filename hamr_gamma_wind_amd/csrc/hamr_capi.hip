@@ -86,8 +86,8 @@ struct hamr_ctx {
     AmrProlong *pro[3] = {nullptr, nullptr, nullptr};
     int64_t nex[3] = {0, 0, 0}, npro[3] = {0, 0, 0};
     int64_t npush3 = 0;            // stage 3's last entries: the rows that the update writes
-    std::vector<int> push3tab;     // push table (host copy)
-    int *push3 = nullptr;
+    std::vector<int> push3tab;     // push table (host copy: receivers)
+    PushEntry *push3 = nullptr;
     AmrReflux *rfx[3] = {nullptr, nullptr, nullptr};  // flux correction at level jumps, per direction
     int64_t nrfx[3] = {0, 0, 0};
     AmrEmf *emfx = nullptr;        // EMF correction at level jumps
@@ -365,9 +365,13 @@ static int init_device(hamr_ctx *c)
         dalloc(c, &c->conn, 40 * KC) || dalloc(c, &c->rcen, KC) || dalloc(c, &c->tet, 96 * KC) ||
         dalloc(c, &c->g32, NPOS * G32N * KC) || dalloc(c, &c->blk, nb))
         return -1;
+    {
+        // (sets the blocks' kPushWhole flags before they are uploaded)
+        const std::vector<PushEntry> pe = push_entries(c->blocks, c->push3tab);
+        if (dalloc(c, &c->push3, std::max<size_t>(pe.size(), 1))) return -1;
+        if (!pe.empty() && h2d(c, c->push3, pe.data(), pe.size() * sizeof(PushEntry))) return -1;
+    }
     if (h2d(c, c->blk, c->blocks.data(), nb * sizeof(BlockDev))) return -1;
-    if (dalloc(c, &c->push3, std::max<size_t>(c->push3tab.size(), 1))) return -1;
-    if (!c->push3tab.empty() && h2d(c, c->push3, c->push3tab.data(), c->push3tab.size() * sizeof(int))) return -1;
     std::vector<int> bl(nb);
     c->roff.assign(nb + 1, 0);
     c->doff.assign(nb + 1, 0);
@@ -398,6 +402,7 @@ static int init_device(hamr_ctx *c)
     P.tet = c->tet;
     P.g32 = c->g32;
     P.push3 = c->push3;
+    P.tiled_push = c->amr ? 1 : 0;
     P.riemann = HAMR_RIEMANN_HLL;
     P.las_s = -1;
     P.las_want = LAS_IDLE;

@@ -38,7 +38,8 @@ struct BlockDev {
     int ks;         // cells per component slab (padded to 64)
     int kcol;       // columns per metric slab (padded to 64)
     int tl;         // time level of local adaptive time stepping (a power of 2; 0 until set)
-    int push3[2];   // lower / upper x3 face: first entry of the face's row of the push table (-1: none)
+    int push3[2];   // lower / upper x3 face: first entry of the face's row of the push table (-1: none),
+                    // | kPushWhole when one entry serves the whole face
     int64_t off;    // cells per component of all earlier blocks' slabs
     int64_t moff;   // metric columns of all earlier geometry slabs
     double dx[3];   // code-coordinate spacing of the block's level
@@ -59,6 +60,24 @@ constexpr int kPushTile = 16;
 // The push table of a uniform grid (every x3 face: the periodic neighbour);
 // sets each block's push3 offsets.
 std::vector<int> uniform_push_table(std::vector<BlockDev> &t);
+// Device form of a push-table entry: the receiver and where its ghost cell is.
+// With the receiver's D2 and D3 equal to the sender's the ghost cell is the
+// sender's cell index + delta; otherwise delta is kPushGeneral and the index is
+// formed from the two blocks' rows.
+constexpr int kPushGeneral = -2147483647 - 1;
+// BlockDev::push3 flag: every tile of the face has the same entry (the row's
+// first), so a launch reads it at one address for the whole face.
+constexpr int kPushWhole = 1 << 30;
+struct PushEntry {
+    int m;        // receiving block (-1: none)
+    int delta;
+    int ks;       // its component stride
+    int pad;
+    int64_t off;  // its slab offset
+};
+static_assert(sizeof(PushEntry) == 24, "PushEntry layout");
+// The device table of the receivers' table tab; sets kPushWhole in the blocks' push3.
+std::vector<PushEntry> push_entries(std::vector<BlockDev> &blk, const std::vector<int> &tab);
 
 // Multi-level exchange table entries (one per ghost cell).  A source code is
 // cell | kind << 28 with kind AS_CELL (that cell) or AS_AVG8 (the mean of the

@@ -85,7 +85,7 @@ __global__ void k_pole_average(KParams P, const int *__restrict__ ring, const in
 // Face-B update from the curl of the corner EMF, consttransport3
 // (GPU_const_trans.h:426-530); faces [NG, NG+BS] along their normal.  psi and
 // psf may alias (each thread reads only the faces it writes).
-template <class R>
+template <class R, bool TILED>
 __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, double Dt_val,
                                                    const double *__restrict__ dt_ptr, double dt_mult, const double *psi,
                                                    double *psf, const R *__restrict__ E)
@@ -118,7 +118,8 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
         const bool inner = i < NG + B.bs[0] && j < NG + B.bs[1];
         PushTarget T;
         double *sm = nullptr;
-        if (inner && (z < 2 * NG || (z >= bs3 && z < NG + bs3)) && x3_push_target(P, B, i, j, z, z < 2 * NG ? 0 : 1, T))
+        if (inner && (z < 2 * NG || (z >= bs3 && z < NG + bs3)) &&
+            x3_push_target<TILED>(P, B, i, j, z, g, z < 2 * NG ? 0 : 1, T))
             sm = psf + 3 * T.off + T.cell;
         const int64_t KM = sm ? T.ks : 0;
         if (j < NG + B.bs[1] && z < NG + B.bs[2]) {
@@ -422,6 +423,7 @@ __device__ __forceinline__ void wave_count(int *cnt, bool pred)
 // would copy it to (the neighbour's upper ghosts from this block's lowest
 // cells, its lower ghosts from the highest), so that the ghost fill after the
 // update only has the rows beyond the block's x1 and x2 edges left along x3.
+template <bool TILED = true>
 __device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B, int n, int i, int j, int64_t col,
                                             int64_t g, int z, int64_t base, int flag, double *pf,
                                             double *__restrict__ pf_out, int *__restrict__ pflag,
@@ -445,7 +447,7 @@ __device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B,
     for (int k = 0; k < NPR; k++) pf_out[base + k * K] = pf[k];
     const int bs3 = B.bs[2];
     PushTarget T;
-    if ((z < 2 * NG || z >= bs3) && x3_push_target(P, B, i, j, z, z < 2 * NG ? 0 : 1, T)) {
+    if ((z < 2 * NG || z >= bs3) && x3_push_target<TILED>(P, B, i, j, z, g, z < 2 * NG ? 0 : 1, T)) {
         pflag[T.off + T.cell] = flag;
 #pragma unroll
         for (int k = 0; k < NPR; k++) pf_out[NPR * T.off + T.cell + k * T.ks] = pf[k];
@@ -506,7 +508,7 @@ __global__ void __launch_bounds__(256) k_invert(KParams P, WorkList wl, int full
 // and k_fallback finishes it.  Same arithmetic as k_cons + k_invert, bit for
 // bit.  Two waves per SIMD (<= 256 VGPRs): held at three (168 VGPRs) it
 // spills 43-65 registers and runs 1.2x slower.
-template <class R>
+template <class R, bool TILED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_update(KParams P, WorkList wl, int full, double Dt_val, const double *__restrict__ dt_ptr, double dt_mult,
          const double *__restrict__ pi, const double *__restrict__ pb, double *__restrict__ pf_out,
@@ -561,7 +563,7 @@ k_update(KParams P, WorkList wl, int full, double Dt_val, const double *__restri
             G.g = mvp[MV_G * ncw];
             const int flag = utoprim(0, U, G.gcov, G.gcon, G.g, P.gam, pf);
             if (flag == 0) {
-                finish_cell(P, B, n, i, j, col, g, z, base, 0, pf, pf_out, pflag, failcnt);
+                finish_cell<TILED>(P, B, n, i, j, col, g, z, base, 0, pf, pf_out, pflag, failcnt);
             } else {
                 atomicAdd(&failcnt[n * 4 + 0], 1);
                 pflag[B.off + g] = kFallbackPending;
@@ -1168,12 +1170,21 @@ void launch_ct_update(const KParams &P, const Launch &L, double Dt_val, const do
                       const double *psi, double *psf, const void *E, int mixed, hipStream_t s)
 {
     if (L.wl.n <= 0) return;
-    if (mixed)
-        hipLaunchKernelGGL((k_ct_update<float>), gdim(L), bdim(L), 0, s, P, L.wl, Dt_val, dt_ptr, dt_mult, psi, psf,
-                           (const float *)E);
-    else
-        hipLaunchKernelGGL((k_ct_update<double>), gdim(L), bdim(L), 0, s, P, L.wl, Dt_val, dt_ptr, dt_mult, psi, psf,
-                           (const double *)E);
+    if (mixed) {
+        if (P.tiled_push)
+            hipLaunchKernelGGL((k_ct_update<float, true>), gdim(L), bdim(L), 0, s, P, L.wl, Dt_val, dt_ptr, dt_mult,
+                               psi, psf, (const float *)E);
+        else
+            hipLaunchKernelGGL((k_ct_update<float, false>), gdim(L), bdim(L), 0, s, P, L.wl, Dt_val, dt_ptr, dt_mult,
+                               psi, psf, (const float *)E);
+    } else {
+        if (P.tiled_push)
+            hipLaunchKernelGGL((k_ct_update<double, true>), gdim(L), bdim(L), 0, s, P, L.wl, Dt_val, dt_ptr, dt_mult,
+                               psi, psf, (const double *)E);
+        else
+            hipLaunchKernelGGL((k_ct_update<double, false>), gdim(L), bdim(L), 0, s, P, L.wl, Dt_val, dt_ptr, dt_mult,
+                               psi, psf, (const double *)E);
+    }
 }
 
 void launch_update_part(const KParams &P, int part, const Launch &L, int full, double Dt_val, const double *dt_ptr,
@@ -1184,14 +1195,15 @@ void launch_update_part(const KParams &P, int part, const Launch &L, int full, d
     if (L.wl.n <= 0) return;
     if (part == 3) {  // k_update: parts 0 and 1 in one kernel
         const size_t lds = sizeof(double) * MV_N * (size_t)L.lds_cols;
-        if (mixed)
-            hipLaunchKernelGGL((k_update<float>), gdim(L), bdim(L), lds, s, P, L.wl, full, Dt_val, dt_ptr, dt_mult, pi,
-                               pb, pf, U0, Unew, psf, (const float *)F1, (const float *)F2, (const float *)F3, pflag,
-                               failcnt);
-        else
-            hipLaunchKernelGGL((k_update<double>), gdim(L), bdim(L), lds, s, P, L.wl, full, Dt_val, dt_ptr, dt_mult, pi,
-                               pb, pf, U0, Unew, psf, (const double *)F1, (const double *)F2, (const double *)F3, pflag,
-                               failcnt);
+        if (mixed) {
+            auto k = P.tiled_push ? k_update<float, true> : k_update<float, false>;
+            hipLaunchKernelGGL(k, gdim(L), bdim(L), lds, s, P, L.wl, full, Dt_val, dt_ptr, dt_mult, pi, pb, pf, U0,
+                               Unew, psf, (const float *)F1, (const float *)F2, (const float *)F3, pflag, failcnt);
+        } else {
+            auto k = P.tiled_push ? k_update<double, true> : k_update<double, false>;
+            hipLaunchKernelGGL(k, gdim(L), bdim(L), lds, s, P, L.wl, full, Dt_val, dt_ptr, dt_mult, pi, pb, pf, U0,
+                               Unew, psf, (const double *)F1, (const double *)F2, (const double *)F3, pflag, failcnt);
+        }
         return;
     }
     if (part == 0) {

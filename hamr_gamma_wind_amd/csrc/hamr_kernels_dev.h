@@ -80,28 +80,57 @@ __device__ __forceinline__ double wave_min(double v)
     return v;
 }
 
-// Where the update and face-field kernels also write cell (i, j, z) of block
-// B, one of the NG lowest (side 0) or highest (side 1) cells of its z-row: the
-// x3 ghost cell of the same-level block across that face that the push table
-// names for the cell's tile, as slab offset, component stride and cell index
-// there.  false: nothing (the exchange fills that ghost), and always in a
-// time-level cycle, whose exchange fills a ghost only when both blocks step.
+// Where the update and face-field kernels also write cell (i, j, z) (index g)
+// of block B, one of the NG lowest (side 0) or highest (side 1) cells of its
+// z-row: the x3 ghost cell of the same-level block across that face that the
+// push table names for the cell's tile, as slab offset, component stride and
+// cell index there.  false: nothing (the exchange fills that ghost), and
+// always in a time-level cycle, whose exchange fills a ghost only when both
+// blocks step.  When one entry serves each face (kPushWhole: every face of a
+// uniform grid), both faces' entries are read at wave-uniform addresses and
+// each lane selects its side's.
 struct PushTarget {
     int64_t off, ks, cell;
 };
-__device__ __forceinline__ bool x3_push_target(const KParams &P, const BlockDev &B, int i, int j, int z, int side,
-                                               PushTarget &T)
+// TILED = false: the kernels' form for uniform grids, whose faces all have
+// one entry with a cell offset (no per-tile entries, no general index).
+template <bool TILED = true>
+__device__ __forceinline__ bool x3_push_target(const KParams &P, const BlockDev &B, int i, int j, int z, int64_t g,
+                                               int side, PushTarget &T)
 {
-    const int row = B.push3[side];
-    if (P.las_s >= 0 || row < 0) return false;
-    const int t2n = (B.bs[1] + kPushTile - 1) / kPushTile;
-    const int m = P.push3[row + ((i - NG) / kPushTile) * t2n + (j - NG) / kPushTile];
+    if (P.las_s >= 0) return false;
+    const int r0 = B.push3[0], r1 = B.push3[1];
+    int m, delta, ks;
+    int64_t off;
+    if (!TILED || ((r0 < 0 || (r0 & kPushWhole)) && (r1 < 0 || (r1 & kPushWhole)))) {
+        const PushEntry none{-1, 0, 0, 0, 0};
+        const PushEntry E0 = r0 >= 0 ? P.push3[r0 - kPushWhole] : none;
+        const PushEntry E1 = r1 >= 0 ? P.push3[r1 - kPushWhole] : none;
+        m = side ? E1.m : E0.m;
+        delta = side ? E1.delta : E0.delta;
+        ks = side ? E1.ks : E0.ks;
+        off = side ? E1.off : E0.off;
+    } else {
+        const int row = side ? r1 : r0;
+        if (row < 0) return false;
+        const int t2n = (B.bs[1] + kPushTile - 1) / kPushTile;
+        const PushEntry &E = P.push3[(row & kPushWhole) ? row - kPushWhole
+                                                        : row + ((i - NG) / kPushTile) * t2n + (j - NG) / kPushTile];
+        m = E.m;
+        delta = E.delta;
+        ks = E.ks;
+        off = E.off;
+    }
     if (m < 0) return false;
+    T.off = off;
+    T.ks = ks;
+    if (!TILED || delta != kPushGeneral) {
+        T.cell = g + delta;
+        return true;
+    }
     const BlockDev &M = P.blk[m];
     const int im = i + B.o[0] - M.o[0], jm = j + B.o[1] - M.o[1];
     const int zm = side ? z - B.bs[2] : z + M.bs[2];
-    T.off = M.off;
-    T.ks = M.ks;
     T.cell = ((int64_t)im * M.D[1] + jm) * M.D[2] + zm;
     return true;
 }
@@ -118,8 +147,9 @@ __device__ __forceinline__ bool x3_row_pushed(const KParams &P, const BlockDev &
     if (row < 0) return false;
     const int is = i + B.o[0] - S.o[0], js = j + B.o[1] - S.o[1];
     if (is < NG || is >= NG + S.bs[0] || js < NG || js >= NG + S.bs[1]) return false;
+    if (row & kPushWhole) return P.push3[row - kPushWhole].m == n;
     const int t2n = (S.bs[1] + kPushTile - 1) / kPushTile;
-    return P.push3[row + ((is - NG) / kPushTile) * t2n + (js - NG) / kPushTile] == n;
+    return P.push3[row + ((is - NG) / kPushTile) * t2n + (js - NG) / kPushTile].m == n;
 }
 
 // launch geometry of a work-list launch

@@ -106,6 +106,41 @@ std::vector<int> uniform_push_table(std::vector<BlockDev> &t)
     return tab;
 }
 
+std::vector<PushEntry> push_entries(std::vector<BlockDev> &blk, const std::vector<int> &tab)
+{
+    std::vector<PushEntry> out(tab.size(), PushEntry{-1, 0, 0, 0, 0});
+    for (BlockDev &B : blk) {
+        const int ntile = ((B.bs[0] + kPushTile - 1) / kPushTile) * ((B.bs[1] + kPushTile - 1) / kPushTile);
+        for (int side = 0; side < 2; side++) {
+            if (B.push3[side] < 0) continue;
+            B.push3[side] &= ~kPushWhole;
+            for (int t = 0; t < ntile; t++) {
+                const int m = tab[B.push3[side] + t];
+                if (m < 0) continue;
+                const BlockDev &M = blk[m];
+                PushEntry &E = out[B.push3[side] + t];
+                E.m = m;
+                E.ks = M.ks;
+                E.off = M.off;
+                E.delta = kPushGeneral;
+                if (M.D[1] == B.D[1] && M.D[2] == B.D[2]) {
+                    const int64_t d = ((int64_t)(B.o[0] - M.o[0]) * B.D[1] + (B.o[1] - M.o[1])) * B.D[2] +
+                                      (side ? -B.bs[2] : M.bs[2]);
+                    if (d > -2147483647 && d < 2147483647) E.delta = (int)d;
+                }
+            }
+            bool whole = true;
+            const PushEntry &F = out[B.push3[side]];
+            for (int t = 1; t < ntile; t++) {
+                const PushEntry &E = out[B.push3[side] + t];
+                whole = whole && E.m == F.m && E.delta == F.delta;
+            }
+            if (whole && F.delta != kPushGeneral) B.push3[side] |= kPushWhole;
+        }
+    }
+    return out;
+}
+
 // The reference's block numbering of a single-level grid: the level-0 Z-order
 // of AMR_set_coord (AMR.c:188-275).  Block number k's coordinates follow from
 // k alone: walking the power-of-two scales p = 2^s from the largest down, the
