@@ -403,6 +403,10 @@ static int init_device(hamr_ctx *c)
     P.g32 = c->g32;
     P.push3 = c->push3;
     P.tiled_push = c->amr ? 1 : 0;
+    P.xp = c->p;
+    P.xph = c->ph;
+    P.xps = c->ps;
+    P.xpsh = c->psh;
     P.riemann = HAMR_RIEMANN_HLL;
     P.las_s = -1;
     P.las_want = LAS_IDLE;
@@ -1777,8 +1781,12 @@ static int set_timelevels(hamr_ctx *c, const std::vector<int> &tl)
             for (int k = 0; k < 3; k++) {
                 std::vector<AmrEntry> fe;
                 std::vector<AmrProlong> fp;
-                for (const AmrEntry &e : ex[k])
+                // stage 3's last npush3 entries: the update writes them when both blocks step
+                const size_t nk = ex[k].size() - (k == 2 ? (size_t)c->npush3 : 0);
+                for (size_t q = 0; q < nk; q++) {
+                    const AmrEntry &e = ex[k][q];
                     if (on(e.dblk) && on(e.sblk)) fe.push_back(e);
+                }
                 for (const AmrProlong &e : pro[k])
                     if (on(e.dblk) && on(e.cblk)) fp.push_back(e);
                 X.nex[k] = (int64_t)fe.size();

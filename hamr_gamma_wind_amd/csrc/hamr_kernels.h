@@ -28,6 +28,9 @@ struct KParams {
     // kernel scales its step by the block's time level (las_phase).
     int las_s;
     int las_want;
+    // The state arrays (p, ph and their face fields): in a cycle the update
+    // writes a receiver's pushed ghosts into both pairs, as the exchange does.
+    double *xp, *xph, *xps, *xpsh;
 };
 
 // Phase of block B in sub-iteration s of a cycle (advance_GPU, step_ch.c:1003-1010,

@@ -117,10 +117,16 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
         const int bs3 = B.bs[2];
         const bool inner = i < NG + B.bs[0] && j < NG + B.bs[1];
         PushTarget T;
-        double *sm = nullptr;
+        double *sm = nullptr, *sm2 = nullptr;  // sm2: a time-level cycle's second array pair
         if (inner && (z < 2 * NG || (z >= bs3 && z < NG + bs3)) &&
-            x3_push_target<TILED>(P, B, i, j, z, g, z < 2 * NG ? 0 : 1, T))
-            sm = psf + 3 * T.off + T.cell;
+            x3_push_target<TILED>(P, B, i, j, z, g, z < 2 * NG ? 0 : 1, T)) {
+            if (TILED && P.las_s >= 0) {
+                sm = P.xps + 3 * T.off + T.cell;
+                sm2 = P.xpsh + 3 * T.off + T.cell;
+            } else {
+                sm = psf + 3 * T.off + T.cell;
+            }
+        }
         const int64_t KM = sm ? T.ks : 0;
         if (j < NG + B.bs[1] && z < NG + B.bs[2]) {
             temp = 0.;
@@ -129,6 +135,7 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
             temp += Dt / (1.0 * dx3) * (Eb(K + 1) - Eb(K));
             sf[0] = temp / gF1;
             if (sm) sm[0] = sf[0];
+            if (sm2) sm2[0] = sf[0];
         }
         if (i < NG + B.bs[0] && j < NG + B.bs[1]) {
             temp = si_[2 * K] * gF3;
@@ -136,6 +143,7 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
             temp += Dt / dx2 * (Eb(sj) - Eb(0));
             sf[2 * K] = temp / gF3;
             if (sm && z != NG) sm[2 * KM] = sf[2 * K];
+            if (sm2 && z != NG) sm2[2 * KM] = sf[2 * K];
         }
         if (i < NG + B.bs[0] && z < NG + B.bs[2]) {
             temp = 0.;
@@ -144,6 +152,7 @@ __global__ void __launch_bounds__(256) k_ct_update(KParams P, WorkList wl, doubl
             temp += -Dt / (1.0 * dx3) * (Eb(1) - Eb(0));
             sf[K] = temp / gF2;
             if (sm) sm[KM] = sf[K];
+            if (sm2) sm2[KM] = sf[K];
         }
     }
 }
@@ -449,8 +458,13 @@ __device__ __forceinline__ void finish_cell(const KParams &P, const BlockDev &B,
     PushTarget T;
     if ((z < 2 * NG || z >= bs3) && x3_push_target<TILED>(P, B, i, j, z, g, z < 2 * NG ? 0 : 1, T)) {
         pflag[T.off + T.cell] = flag;
+        // a time-level cycle writes the receiver's ghost into both of its arrays
+        const bool both = TILED && P.las_s >= 0;
+        for (int pass = 0; pass < (both ? 2 : 1); pass++) {
+            double *dst = both ? (pass ? P.xph : P.xp) : pf_out;
 #pragma unroll
-        for (int k = 0; k < NPR; k++) pf_out[NPR * T.off + T.cell + k * T.ks] = pf[k];
+            for (int k = 0; k < NPR; k++) dst[NPR * T.off + T.cell + k * T.ks] = pf[k];
+        }
     }
 }
 

@@ -84,11 +84,12 @@ __device__ __forceinline__ double wave_min(double v)
 // of block B, one of the NG lowest (side 0) or highest (side 1) cells of its
 // z-row: the x3 ghost cell of the same-level block across that face that the
 // push table names for the cell's tile, as slab offset, component stride and
-// cell index there.  false: nothing (the exchange fills that ghost), and
-// always in a time-level cycle, whose exchange fills a ghost only when both
-// blocks step.  When one entry serves each face (kPushWhole: every face of a
-// uniform grid), both faces' entries are read at wave-uniform addresses and
-// each lane selects its side's.
+// cell index there.  false: nothing (the exchange fills that ghost), and in
+// a time-level cycle when the receiver does not step in this sub-iteration
+// (its exchange fills a ghost only when both blocks step; when they do, the
+// caller writes both of the receiver's array pairs).  When one entry serves
+// each face (kPushWhole: every face of a uniform grid), both faces' entries
+// are read at wave-uniform addresses and each lane selects its side's.
 struct PushTarget {
     int64_t off, ks, cell;
 };
@@ -98,7 +99,7 @@ template <bool TILED = true>
 __device__ __forceinline__ bool x3_push_target(const KParams &P, const BlockDev &B, int i, int j, int z, int64_t g,
                                                int side, PushTarget &T)
 {
-    if (P.las_s >= 0) return false;
+    if (!TILED && P.las_s >= 0) return false;
     const int r0 = B.push3[0], r1 = B.push3[1];
     int m, delta, ks;
     int64_t off;
@@ -122,6 +123,7 @@ __device__ __forceinline__ bool x3_push_target(const KParams &P, const BlockDev 
         off = E.off;
     }
     if (m < 0) return false;
+    if (TILED && P.las_s >= 0 && las_phase(P.blk[m].tl, P.las_s) == LAS_IDLE) return false;
     T.off = off;
     T.ks = ks;
     if (!TILED || delta != kPushGeneral) {
