@@ -274,9 +274,10 @@ __global__ void __launch_bounds__(256) k_adj_ecent(AdjArgs A, const double *__re
 
 // One face of direction DIR: HLL, the two face states and the PPM
 // reconstruction of every primitive, transposed into lam_P (folded) and the
-// normal face field into lam_ps.
+// normal face field into lam_ps.  Held at two waves per SIMD (a few registers
+// spilled): 1.3x faster than one wave with none spilled.
 template <int DIR>
-__global__ void __launch_bounds__(256) k_adj_flux(AdjArgs A, const double *__restrict__ p, const double *__restrict__ ps,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_adj_flux(AdjArgs A, const double *__restrict__ p, const double *__restrict__ ps,
                                                   const double *__restrict__ lF, double *lP, double *lps)
 {
     const int e1 = A.bs[0] + (DIR == 1 ? 1 : 2), e2 = A.bs[1] + (DIR == 2 ? 1 : 2), e3 = A.bs[2] + (DIR == 3 ? 1 : 2);
@@ -294,16 +295,17 @@ __global__ void __launch_bounds__(256) k_adj_flux(AdjArgs A, const double *__res
     }
     if (!any) return;
     const int64_t sd = (DIR == 1) ? (int64_t)A.D[1] * A.D[2] : (DIR == 2) ? (int64_t)A.D[2] : 1;
-    // reconstruction (primal) with its Jacobian for the two cells of the face
-    double pl[NPR], pr[NPR], Jr[NPR][5], Jl[NPR][5];
+    // reconstruction (primal values only: the Jacobians are formed per
+    // primitive at the end, so that they are not live across the HLL part)
+    double pl[NPR], pr[NPR];
     for (int k = 0; k < NPR; k++) {
         const double *q = p + k * K + cell;
         const double xm[5] = {q[-3 * sd], q[-2 * sd], q[-sd], q[0], q[sd]};
         const double xc[5] = {q[-2 * sd], q[-sd], q[0], q[sd], q[2 * sd]};
-        double l, r, jl[5];
-        para_jacobian(xm, &l, &r, jl, Jr[k]);
+        double l, r, jl[5], jr[5];
+        para_jacobian(xm, &l, &r, jl, jr);
         pl[k] = r;
-        para_jacobian(xc, &l, &r, Jl[k], jl);
+        para_jacobian(xc, &l, &r, jl, jr);
         pr[k] = l;
     }
     const double bn = ps[(DIR - 1) * K + cell];
@@ -352,14 +354,22 @@ __global__ void __launch_bounds__(256) k_adj_flux(AdjArgs A, const double *__res
         lpl[B1 + DIR - 1] = lpr[B1 + DIR - 1] = 0.;
     }
     // reconstruction: pl from cells c-3..c+1, pr from c-2..c+2 along DIR
-    for (int m = -3; m <= 2; m++) {
-        const int ii = i + (DIR == 1 ? m : 0), jj = j + (DIR == 2 ? m : 0), zz = z + (DIR == 3 ? m : 0);
-        const int64_t fg = fold(A, ii, jj, zz);
-        for (int k = 0; k < NPR; k++) {
+    int64_t fg[6];
+    for (int m = -3; m <= 2; m++)
+        fg[m + 3] = fold(A, i + (DIR == 1 ? m : 0), j + (DIR == 2 ? m : 0), z + (DIR == 3 ? m : 0));
+    for (int k = 0; k < NPR; k++) {
+        if (lpl[k] == 0. && lpr[k] == 0.) continue;
+        const double *q = p + k * K + cell;
+        const double xm[5] = {q[-3 * sd], q[-2 * sd], q[-sd], q[0], q[sd]};
+        const double xc[5] = {q[-2 * sd], q[-sd], q[0], q[sd], q[2 * sd]};
+        double l, r, jl[5], Jr[5], Jl[5];
+        para_jacobian(xm, &l, &r, jl, Jr);
+        para_jacobian(xc, &l, &r, Jl, jl);
+        for (int m = -3; m <= 2; m++) {
             double v = 0.;
-            if (m <= 1) v += lpl[k] * Jr[k][m + 3];
-            if (m >= -2) v += lpr[k] * Jl[k][m + 2];
-            if (v != 0.) atomicAdd(&lP[k * K + fg], v);
+            if (m <= 1) v += lpl[k] * Jr[m + 3];
+            if (m >= -2) v += lpr[k] * Jl[m + 2];
+            if (v != 0.) atomicAdd(&lP[k * K + fg[m + 3]], v);
         }
     }
 }
