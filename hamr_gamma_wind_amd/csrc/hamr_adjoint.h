@@ -36,11 +36,14 @@ template <int N> HD Dual<N> operator-(const Dual<N> &a)
 { Dual<N> r; r.v = -a.v; for (int k = 0; k < N; k++) r.d[k] = -a.d[k]; return r; }
 template <int N> HD Dual<N> operator*(const Dual<N> &a, const Dual<N> &b)
 { Dual<N> r; r.v = a.v * b.v; for (int k = 0; k < N; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k]; return r; }
+// The value is the IEEE quotient; the tangents use one reciprocal of b (an
+// FP64 division per tangent would cost as much as the rest of the parabola).
 template <int N> HD Dual<N> operator/(const Dual<N> &a, const Dual<N> &b)
 {
     Dual<N> r;
     r.v = a.v / b.v;
-    for (int k = 0; k < N; k++) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+    const double ib = 1. / b.v;
+    for (int k = 0; k < N; k++) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
     return r;
 }
 template <int N> HD bool operator<(const Dual<N> &a, const Dual<N> &b) { return a.v < b.v; }
@@ -50,13 +53,14 @@ template <int N> HD Dual<N> fabs(const Dual<N> &a) { return (a.v < 0.) ? -a : a;
 template <int N> HD Dual<N> fmin(const Dual<N> &a, const Dual<N> &b) { return (b.v < a.v) ? b : a; }
 template <int N> HD Dual<N> copysign(const Dual<N> &a, const Dual<N> &s)
 {
-    const bool neg = signbit(s.v), aneg = signbit(a.v);
+    const bool neg = __builtin_signbit(s.v), aneg = __builtin_signbit(a.v);
     return (neg == aneg) ? a : -a;
 }
 template <int N> HD Dual<N> dv(Dual<N> a, Dual<N> b) { return a / b; }
 
-// d(l, r)/d(x1..x5) of para for one cell: Jl[5], Jr[5].
-HD void para_jacobian(const double *x, double *l, double *r, double *Jl, double *Jr)
+// d(l, r)/d(x1..x5) of para for one cell: Jl[5], Jr[5] (forward mode over
+// all five inputs; the reference form of para_jacobian below).
+HD void para_jacobian_dual(const double *x, double *l, double *r, double *Jl, double *Jr)
 {
     Dual<5> y[5];
     for (int k = 0; k < 5; k++) {
@@ -70,6 +74,79 @@ HD void para_jacobian(const double *x, double *l, double *r, double *Jl, double 
     for (int k = 0; k < 5; k++) {
         Jl[k] = lo.d[k];
         Jr[k] = ro.d[k];
+    }
+}
+
+// Tangent of mc_slope(ym, y0, yp) along the branch it takes (the choices of
+// para_jacobian_dual's fabs, fmin and copysign): s * dX, X the selected one of
+// Dqm = 2 (y0 - ym), Dqp = 2 (yp - y0), Dqc = 0.5 (yp - ym); t over (ym, y0, yp).
+HD void mc_slope_tangent(double ym, double y0, double yp, double t[3])
+{
+    const double Dqm = 2. * (y0 - ym), Dqp = 2. * (yp - y0), Dqc = 0.5 * (yp - ym);
+    t[0] = t[1] = t[2] = 0.;
+    if (!(Dqm * Dqp > 0.)) return;
+    const double am = fabs(Dqm), ap = fabs(Dqp), ac = fabs(Dqc);
+    const bool pick_p = ap < am;                // fmin(|Dqm|, |Dqp|)
+    const double m = pick_p ? ap : am;
+    const double sgn = __builtin_signbit(Dqc) ? -1. : 1.;  // copysign(., Dqc) of a positive value
+    if (m < ac) {
+        if (pick_p) {
+            const double f = sgn * (Dqp < 0. ? -2. : 2.);
+            t[1] = -f;
+            t[2] = f;
+        } else {
+            const double f = sgn * (Dqm < 0. ? -2. : 2.);
+            t[0] = -f;
+            t[1] = f;
+        }
+    } else {
+        const double f = sgn * (Dqc < 0. ? -0.5 : 0.5);
+        t[0] = -f;
+        t[2] = f;
+    }
+}
+
+// d(l, r)/d(x1..x5) of para for one cell: Jl[5], Jr[5], in closed form along
+// the limiter's active branches (the slopes' tangents have three entries, the
+// interface values' four): the values are para's; the tangents equal
+// para_jacobian_dual's up to the rounding of the division by 6.
+HD void para_jacobian(const double *x, double *l, double *r, double *Jl, double *Jr)
+{
+    double dq[3], t[3][5];
+    for (int s = 0; s < 3; s++) {
+        dq[s] = mc_slope(x[s], x[s + 1], x[s + 2]);
+        double ts[3];
+        mc_slope_tangent(x[s], x[s + 1], x[s + 2], ts);
+        for (int k = 0; k < 5; k++) t[s][k] = (k >= s && k < s + 3) ? ts[k - s] : 0.;
+    }
+    const double sixth = 1. / 6.;
+    // interface values below and above cell x3 (ppm_face) and their tangents
+    double lo = ppm_face(x[1], x[2], dq[0], dq[1]), hi = ppm_face(x[2], x[3], dq[1], dq[2]);
+    double tl[5], tr[5];
+    for (int k = 0; k < 5; k++) {
+        tl[k] = 0.5 * ((k == 2 ? 1. : 0.) + (k == 1 ? 1. : 0.)) - (t[1][k] - t[0][k]) * sixth;
+        tr[k] = 0.5 * ((k == 3 ? 1. : 0.) + (k == 2 ? 1. : 0.)) - (t[2][k] - t[1][k]) * sixth;
+    }
+    // ppm_limit on values and tangents
+    const double x3 = x[2];
+    const double qa = (hi - x3) * (x3 - lo), qd = hi - lo, qe = 6.0 * (x3 - 0.5 * (lo + hi));
+    if (qa <= 0.) {
+        lo = x3;
+        hi = x3;
+        for (int k = 0; k < 5; k++) tl[k] = tr[k] = (k == 2 ? 1. : 0.);
+    }
+    if (qd * (qd - qe) < 0.) {
+        lo = 3.0 * x3 - 2.0 * hi;
+        for (int k = 0; k < 5; k++) tl[k] = 3.0 * (k == 2 ? 1. : 0.) - 2.0 * tr[k];
+    } else if (qd * (qd + qe) < 0.) {
+        hi = 3.0 * x3 - 2.0 * lo;
+        for (int k = 0; k < 5; k++) tr[k] = 3.0 * (k == 2 ? 1. : 0.) - 2.0 * tl[k];
+    }
+    *l = lo;
+    *r = hi;
+    for (int k = 0; k < 5; k++) {
+        Jl[k] = tl[k];
+        Jr[k] = tr[k];
     }
 }
 
