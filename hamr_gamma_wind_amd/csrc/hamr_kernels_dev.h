@@ -105,8 +105,8 @@ __device__ __forceinline__ bool x3_push_target(const KParams &P, const BlockDev 
     int64_t off;
     if (!TILED || ((r0 < 0 || (r0 & kPushWhole)) && (r1 < 0 || (r1 & kPushWhole)))) {
         const PushEntry none{-1, 0, 0, 0, 0};
-        const PushEntry E0 = r0 >= 0 ? P.push3[r0 - kPushWhole] : none;
-        const PushEntry E1 = r1 >= 0 ? P.push3[r1 - kPushWhole] : none;
+        const PushEntry E0 = r0 >= 0 ? P.push3[r0 & ~kPushWhole] : none;
+        const PushEntry E1 = r1 >= 0 ? P.push3[r1 & ~kPushWhole] : none;
         m = side ? E1.m : E0.m;
         delta = side ? E1.delta : E0.delta;
         ks = side ? E1.ks : E0.ks;
@@ -115,7 +115,7 @@ __device__ __forceinline__ bool x3_push_target(const KParams &P, const BlockDev 
         const int row = side ? r1 : r0;
         if (row < 0) return false;
         const int t2n = (B.bs[1] + kPushTile - 1) / kPushTile;
-        const PushEntry &E = P.push3[(row & kPushWhole) ? row - kPushWhole
+        const PushEntry &E = P.push3[(row & kPushWhole) ? row & ~kPushWhole
                                                         : row + ((i - NG) / kPushTile) * t2n + (j - NG) / kPushTile];
         m = E.m;
         delta = E.delta;
@@ -149,7 +149,7 @@ __device__ __forceinline__ bool x3_row_pushed(const KParams &P, const BlockDev &
     if (row < 0) return false;
     const int is = i + B.o[0] - S.o[0], js = j + B.o[1] - S.o[1];
     if (is < NG || is >= NG + S.bs[0] || js < NG || js >= NG + S.bs[1]) return false;
-    if (row & kPushWhole) return P.push3[row - kPushWhole].m == n;
+    if (row & kPushWhole) return P.push3[row & ~kPushWhole].m == n;
     const int t2n = (S.bs[1] + kPushTile - 1) / kPushTile;
     return P.push3[row + ((is - NG) / kPushTile) * t2n + (js - NG) / kPushTile].m == n;
 }
