@@ -302,11 +302,11 @@ def adjoint_bench(H, n=128, reps=5):
         ctx.close()
 
 
-def hllc_bench(H, ctx, p0, ps0, nz, args, steps=20):
-    """The FP64 step with the HLLC Riemann solver (hamr_set_riemann) on the same
-    state: tuned launches, graph replay, 3 warm-up and `steps` timed steps, and
-    the flux kernels' ms per launch."""
-    ctx.set_precision(H.HAMR_PREC_FP64)
+def hllc_bench(H, ctx, p0, ps0, nz, args, steps=20, mixed=False):
+    """The step with the HLLC Riemann solver (hamr_set_riemann) on the same
+    state, FP64 or mixed (FP32 fluxes): tuned launches, graph replay, 3 warm-up
+    and `steps` timed steps, and the flux kernels' ms per launch."""
+    ctx.set_precision(H.HAMR_PREC_MIXED if mixed else H.HAMR_PREC_FP64)
     ctx.set_riemann(H.HAMR_RIEMANN_HLLC)
     try:
         ctx.set(H.HAMR_P, p0)
@@ -326,9 +326,11 @@ def hllc_bench(H, ctx, p0, ps0, nz, args, steps=20):
         kern = {k: round(ctx.time_kernel(i, 5), 4) for i, k in enumerate(KERNELS[:3])}
     finally:
         ctx.set_riemann(H.HAMR_RIEMANN_HLL)
+        ctx.set_precision(H.HAMR_PREC_FP64)
     return {"value": nz * steps / el, "unit": "zone-updates/s", "ms_per_step": el / steps * 1e3, "steps": steps,
             "flux_ms_per_launch": kern,
-            "what": "FP64 step with HLLC fluxes in the face's orthonormal frame (the reference's HLLC = 1)"}
+            "what": ("mixed step (FP32 fluxes)" if mixed else "FP64 step") +
+                    " with HLLC fluxes in the face's orthonormal frame (the reference's HLLC = 1)"}
 
 
 def rdump_bench(H, ctx, bs, reps=5):
@@ -533,6 +535,7 @@ def main():
                 second["speedup_batched_vs_per_block_host"] = second["value"] / second["per_block_host"]["value"]
             if not args.no_hllc:
                 second["hllc"] = hllc_bench(H, ctx2, q0, qs0, nz2, args)
+                second["hllc_mixed"] = hllc_bench(H, ctx2, q0, qs0, nz2, args, mixed=True)
             del q0, qs0
         finally:
             ctx2.close()

@@ -1048,8 +1048,6 @@ int hamr_set_riemann(hamr_ctx *c, int solver)
 {
     if (!c) return -1;
     if (solver != HAMR_RIEMANN_HLL && solver != HAMR_RIEMANN_HLLC) return fail(c, "hamr_set_riemann: unknown solver");
-    if (solver == HAMR_RIEMANN_HLLC && c->prec != HAMR_PREC_FP64)
-        return fail(c, "hamr_set_riemann: HLLC runs in the FP64 step only");
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (solver != c->P.riemann) drop_graphs(c);
     c->P.riemann = solver;
@@ -1218,8 +1216,6 @@ int hamr_set_precision(hamr_ctx *c, int mode)
 {
     if (!c) return -1;
     if (mode != HAMR_PREC_FP64 && mode != HAMR_PREC_MIXED) return fail(c, "hamr_set_precision: unknown mode");
-    if (mode == HAMR_PREC_MIXED && c->P.riemann != HAMR_RIEMANN_HLL)
-        return fail(c, "hamr_set_precision: the mixed step uses HLL (HLLC runs in the FP64 step only)");
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (mode != c->prec) drop_graphs(c);
     c->prec = mode;

@@ -20,7 +20,7 @@ struct KParams {
     // formed in FP64 first: g32[moff*NPOS*G32N + (pos*G32N + s)*kcol + col],
     // s = 0-9 gcov, 10-19 gcon, 20 gdet, 21 1 - 1/sqrt(-g^00).
     const float *g32;
-    int riemann;        // HAMR_RIEMANN_HLL (0) or HAMR_RIEMANN_HLLC (1); HLLC in the FP64 step only
+    int riemann;        // HAMR_RIEMANN_HLL (0) or HAMR_RIEMANN_HLLC (1), in either arithmetic
     const PushEntry *push3;  // push table (rows from BlockDev::push3)
     int tiled_push;          // 1: a multi-level set (per-tile push entries); 0: every face one entry
     // Local adaptive time stepping: the sub-iteration s of a cycle (-1 outside

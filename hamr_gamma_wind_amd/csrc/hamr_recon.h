@@ -97,7 +97,8 @@ __device__ __forceinline__ double hllc_face(const KParams &P, const BlockDev &B,
         mudi[e] = (R)T[(int64_t)(16 + e) * B.kcol];
     }
     R F1[NPR];
-    hllc_frame_flux<DIR>(pl, pr, ql.ucon, ql.bcon, qr.ucon, qr.bcon, mud, mudi, face_velocity<DIR>(G.gcon), gam, F1);
+    hllc_frame_flux<DIR>(pl, pr, ql.ucon, ql.bcon, qr.ucon, qr.bcon, mud, mudi, face_velocity<DIR>(G.gcon), gam,
+                         sizeof(R) == 4 ? G.oma : R(0.), F1);
     R *Fb = F + NPR * B.off + cell;
 #pragma unroll
     for (int k = 0; k < NPR; k++) Fb[(int64_t)k * K] = G.g * F1[k];
@@ -117,9 +118,10 @@ __device__ __forceinline__ void dt_reduce(double vdt, unsigned long long *__rest
 // each thread through a segment of faces along x1 or x2 and forms each cell's
 // parabola once issues 29% fewer FP32 VALU instructions per face, but its
 // carried state takes 157 registers (three waves) and it ran 7-35% slower.
-// RS selects the Riemann solver: 0 HLL, 1 HLLC.  The HLLC kernel (FP64 only)
-// is held at two waves per SIMD: 256 VGPRs with about 20 spilled to scratch
-// ran 1.4x faster than one wave with none spilled.
+// RS selects the Riemann solver: 0 HLL, 1 HLLC.  The HLLC kernel is held at
+// two waves per SIMD: in FP64, 256 VGPRs with about 20 spilled to scratch ran
+// 1.4x faster than one wave with none spilled (in FP32 it takes 139, three
+// waves, none spilled).
 template <int DIR, class R, int RS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RS ? 2 : 3)))
 k_flux(KParams P, WorkList wl, const double *__restrict__ p, const double *__restrict__ ps, R *__restrict__ F,
@@ -416,26 +418,22 @@ void recon_flux(const KParams &P, int dir, const Launch &L, const double *p, con
 {
     if (L.wl.n <= 0) return;
     if (L.line) {
-        if constexpr (sizeof(R) == 8) {
-            if (P.riemann == 1) {
-                if (dir == 1) launch_flux_lines<1, R, 1>(P, L, p, ps, F, dtmin, s);
-                else if (dir == 2) launch_flux_lines<2, R, 1>(P, L, p, ps, F, dtmin, s);
-                else launch_flux_lines<3, R, 1>(P, L, p, ps, F, dtmin, s);
-                return;
-            }
+        if (P.riemann == 1) {
+            if (dir == 1) launch_flux_lines<1, R, 1>(P, L, p, ps, F, dtmin, s);
+            else if (dir == 2) launch_flux_lines<2, R, 1>(P, L, p, ps, F, dtmin, s);
+            else launch_flux_lines<3, R, 1>(P, L, p, ps, F, dtmin, s);
+            return;
         }
         if (dir == 1) launch_flux_lines<1, R, 0>(P, L, p, ps, F, dtmin, s);
         else if (dir == 2) launch_flux_lines<2, R, 0>(P, L, p, ps, F, dtmin, s);
         else launch_flux_lines<3, R, 0>(P, L, p, ps, F, dtmin, s);
         return;
     }
-    if constexpr (sizeof(R) == 8) {
-        if (P.riemann == 1) {
-            if (dir == 1) hipLaunchKernelGGL((k_flux<1, R, 1>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
-            else if (dir == 2) hipLaunchKernelGGL((k_flux<2, R, 1>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
-            else hipLaunchKernelGGL((k_flux<3, R, 1>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
-            return;
-        }
+    if (P.riemann == 1) {
+        if (dir == 1) hipLaunchKernelGGL((k_flux<1, R, 1>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
+        else if (dir == 2) hipLaunchKernelGGL((k_flux<2, R, 1>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
+        else hipLaunchKernelGGL((k_flux<3, R, 1>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
+        return;
     }
     if (dir == 1) hipLaunchKernelGGL((k_flux<1, R, 0>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);
     else if (dir == 2) hipLaunchKernelGGL((k_flux<2, R, 0>), gdim(L), bdim(L), 0, s, P, L.wl, p, ps, F, dtmin);

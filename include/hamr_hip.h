@@ -176,8 +176,8 @@ int hamr_set_precision(hamr_ctx *ctx, int mode);
 int hamr_get_precision(const hamr_ctx *ctx, int *mode);
 /* Select the Riemann solver of every later flux stage and step (the
  * reference's HLLC / FRAME_TRANSFORM compile-time switch, config.h:480 and
- * :1434-1438).  HLLC runs in the FP64 step only: it is refused in the mixed
- * arithmetic, and the mixed arithmetic is refused while HLLC is selected. */
+ * :1434-1438).  Both solvers run in either arithmetic; the mixed step's HLLC
+ * carries the frame energy without its rest-mass part (FP32 would cancel it). */
 int hamr_set_riemann(hamr_ctx *ctx, int solver);
 int hamr_get_riemann(const hamr_ctx *ctx, int *solver);
 /* set_Mud (hllc.c:20-178): the orthonormal tetrad at a face of direction dir
