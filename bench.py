@@ -451,7 +451,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement (128^3 flat box)")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--no-amr", action="store_true", help="skip the multi-level block set")
-    ap.add_argument("--no-hllc", action="store_true", help="skip the FP64 step with HLLC fluxes")
+    ap.add_argument("--no-hllc", action="store_true", help="skip the steps with HLLC fluxes (FP64 and mixed)")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the headline grid (no second grid, AMR, adjoint, HLLC, restart records)")
     ap.add_argument("--amr-steps", type=int, default=40)
