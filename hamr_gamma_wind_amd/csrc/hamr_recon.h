@@ -118,12 +118,12 @@ __device__ __forceinline__ void dt_reduce(double vdt, unsigned long long *__rest
 // each thread through a segment of faces along x1 or x2 and forms each cell's
 // parabola once issues 29% fewer FP32 VALU instructions per face, but its
 // carried state takes 157 registers (three waves) and it ran 7-35% slower.
-// RS selects the Riemann solver: 0 HLL, 1 HLLC.  The HLLC kernel is held at
-// two waves per SIMD: in FP64, 256 VGPRs with about 20 spilled to scratch ran
-// 1.4x faster than one wave with none spilled (in FP32 it takes 139, three
-// waves, none spilled).
+// RS selects the Riemann solver: 0 HLL, 1 HLLC.  The FP64 HLLC kernel is held
+// at two waves per SIMD: 256 VGPRs with about 20 spilled to scratch ran 1.4x
+// faster than one wave with none spilled.  The FP32 HLLC kernel at four waves
+// (128 VGPRs, 9 spilled) ran 4% faster than at three (139, none spilled).
 template <int DIR, class R, int RS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RS ? 2 : 3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RS ? (sizeof(R) == 4 ? 4 : 2) : 3)))
 k_flux(KParams P, WorkList wl, const double *__restrict__ p, const double *__restrict__ ps, R *__restrict__ F,
        unsigned long long *__restrict__ dtmin)
 {
